@@ -1,0 +1,150 @@
+/*
+ * cfd2lcs_api.h -- flags and entry points of the MI355X-native libcfd2lcs
+ * FTLE path.  Include through cfd2lcs_inc_dp.h or cfd2lcs_inc_sp.h, which
+ * define lcsdata_t (double / float) exactly as the reference headers do.
+ *
+ * Part 1 is the reference interface, unchanged: a CFD solver built against
+ * the reference's cfd2lcs_inc.h links against this library without source
+ * changes.  Part 2 lists the additive entry points of this implementation;
+ * none of them alters the behaviour of part 1.
+ */
+#ifndef CFD2LCS_API_H
+#define CFD2LCS_API_H
+
+#if defined(__has_include)
+#if __has_include(<mpi.h>) && !defined(CFD2LCS_NO_MPI)
+#include <mpi.h>
+#define CFD2LCS_HAVE_MPI_H 1
+#endif
+#endif
+#ifndef CFD2LCS_HAVE_MPI_H
+/* No MPI on this host: the communicator is accepted and ignored (one GPU is
+ * one rank).  int matches MPICH's MPI_Comm; any handle type passed in a
+ * register is accepted by the library, which never dereferences it. */
+typedef int MPI_Comm;
+#endif
+
+/* The string namelength (cfd2lcs_inc_double.h:22) */
+#define LCS_NAMELEN 128
+
+/* Boundary condition flags (cfd2lcs_inc_double.h:27-33) */
+#define LCS_INTERNAL 0
+#define LCS_INFLOW 1
+#define LCS_OUTFLOW 2
+#define LCS_WALL 3
+#define LCS_SLIP 4
+#define LCS_2D 5
+#define LCS_MASK 6
+
+/* LCS diagnostics (cfd2lcs_inc_double.h:38-40) */
+#define FTLE_FWD 0
+#define FTLE_BKWD 1
+#define LP_TRACER 2
+
+/* Integration methods (cfd2lcs_inc_double.h:45-49) */
+#define EULER 0
+#define TRAPEZOIDAL 1
+#define RK2 2
+#define RK3 3
+#define RK4 4
+
+/* Interpolation methods (cfd2lcs_inc_double.h:54-59) */
+#define NEAREST_NBR 0
+#define LINEAR 1
+#define QUADRATIC 2
+#define CUBIC 3
+#define TSE 5
+#define TSE_LIMIT 6
+
+/* True/False indicators (cfd2lcs_inc_double.h:64-65) */
+#define LCS_FALSE 0
+#define LCS_TRUE 1
+
+/* Values of the additive "PRECISION" / "VELOCITY_STORAGE" options */
+#define LCS_FP64 8
+#define LCS_FP32 4
+#define LCS_FP16 2
+#define LCS_BF16 3
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------
+ * Part 1: the reference C API (cfd2lcs_inc_double.h:72-110,
+ * implemented in the reference by cfd2lcs_c.c:67-142 -> cfd2lcs.f90)
+ * --------------------------------------------------------------------- */
+
+/* replaces cfd2lcs_init_c (cfd2lcs_c.c:67-83 -> cfd2lcs.f90:2-53) */
+void cfd2lcs_init_c(MPI_Comm usercomm, int n[3], int offset[3], lcsdata_t *x,
+                    lcsdata_t *y, lcsdata_t *z, int *flag);
+
+/* replaces cfd2lcs_diagnostic_init_c (cfd2lcs_c.c:86-99 -> cfd2lcs.f90:252-358) */
+int cfd2lcs_diagnostic_init_c(int lcs_type, int resolution, lcsdata_t t,
+                              lcsdata_t h, char label[]);
+
+/* replaces cfd2lcs_update_c (cfd2lcs_c.c:102-115 -> cfd2lcs.f90:55-250) */
+void cfd2lcs_update_c(int n[3], lcsdata_t *u, lcsdata_t *v, lcsdata_t *w,
+                      lcsdata_t time);
+
+/* replaces cfd2lcs_diagnostic_destroy_c (cfd2lcs_c.c:118-122 -> cfd2lcs.f90:360-386) */
+void cfd2lcs_diagnostic_destroy_c(int lcs_handle);
+
+/* replaces cfd2lcs_set_option_c (cfd2lcs_c.c:125-132 -> cfd2lcs.f90:436-622) */
+void cfd2lcs_set_option_c(char option[], int val);
+
+/* replaces cfd2lcs_set_param_c (cfd2lcs_c.c:135-142 -> cfd2lcs.f90:624-682) */
+void cfd2lcs_set_param_c(char option[], lcsdata_t val);
+
+/* ---------------------------------------------------------------------
+ * Part 2: additive entry points (no counterpart in the reference C shim)
+ * --------------------------------------------------------------------- */
+
+/* C wrapper of the Fortran-only cfd2lcs_finalize (cfd2lcs.f90:388-434).
+ * Releases all device memory and resets the library, including the sticky
+ * error flag and the first-update state, so that init may be called again. */
+void cfd2lcs_finalize_c(void);
+
+/* Sticky error flag (CFD2LCS_ERROR, data_m.f90:381): 0 when healthy. */
+int cfd2lcs_error_c(void);
+
+/* Same as cfd2lcs_update_c, but u, v, w are device (HBM) pointers of n[0]*
+ * n[1]*n[2] lcsdata_t each, Fortran order; no host staging is involved. */
+void cfd2lcs_update_device_c(int n[3], const lcsdata_t *d_u,
+                             const lcsdata_t *d_v, const lcsdata_t *d_w,
+                             lcsdata_t time);
+
+/* Results kept in HBM by the library; copied to host arrays of
+ * n[0]*n[1]*n[2] values, Fortran order.  Return 0 on success.
+ *   flowmap : displacement over the current h interval (the BKWD
+ *             semi-Lagrangian field lp%fm, or the FWD particles' dx by
+ *             origin node)
+ *   composed: T-flowmap of the latest composition (what write_lcs writes
+ *             as the flowmap, io_nohdf5_m.f90:183-265)
+ *   ftle    : FTLE field of the latest composition (lcs_m.f90:66-193) */
+int cfd2lcs_get_flowmap_c(int lcs_handle, lcsdata_t *fx, lcsdata_t *fy,
+                          lcsdata_t *fz);
+int cfd2lcs_get_composed_c(int lcs_handle, lcsdata_t *fx, lcsdata_t *fy,
+                           lcsdata_t *fz);
+int cfd2lcs_get_ftle_c(int lcs_handle, lcsdata_t *ftle);
+/* device pointer (double, n[0]*n[1]*n[2], Fortran order) to the FTLE field;
+ * valid until the next update or finalize */
+const double *cfd2lcs_get_ftle_device_c(int lcs_handle);
+/* number of FTLE fields produced so far by a diagnostic, or -1 */
+int cfd2lcs_get_noutput_c(int lcs_handle);
+
+/* Counters (cfd2lcs_info, cfd2lcs.f90:708-810): fills out[8] with
+ *   [0] n_subcycle of the latest set_dt
+ *   [1] cumulative forward particle integrations (integrations_fwd_c)
+ *   [2] cumulative backward integrations (integrations_bkwd_c)
+ *   [3] seconds of GPU time in forward advection   (cpu_fwd)
+ *   [4] seconds of GPU time in backward advection  (cpu_bkwd)
+ *   [5] seconds of GPU time in composition          (cpu_reconstruct)
+ *   [6] seconds of GPU time in gradient/CG/FTLE     (cpu_ftle)
+ *   [7] spline evaluations outside the knot range (db3val returns 0) */
+void cfd2lcs_get_stats_c(double out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,977 @@
+// api.cpp -- the C ABI of the MI355X-native libcfd2lcs FTLE path and the
+// per-update orchestration of the HIP kernels.
+//
+// Library-global state with a sticky error flag, as the reference keeps
+// (data_m.f90:313-381, cfd2lcs.f90:684-706).  One process drives one GPU;
+// the MPI communicator is accepted and ignored.  Compiled twice: with
+// LCS_SINGLE defined the user-facing real type (lcsdata_t) is float
+// (libcfd2lcs_sp.so), otherwise double (libcfd2lcs_dp.so).  The device compute
+// precision is a separate run-time option ("PRECISION"), defaulting to the
+// library's type.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#ifdef LCS_SINGLE
+#define CFD2LCS_LCSDATA_DEFINED
+typedef float lcsdata_t;
+#else
+#define CFD2LCS_LCSDATA_DEFINED
+typedef double lcsdata_t;
+#endif
+#define CFD2LCS_NO_MPI
+#include "../../include/cfd2lcs_api.h"
+#include "grid_host.h"
+#include "lcs_device.h"
+
+using U = lcsdata_t;  // user-facing real type
+
+namespace lcs {
+
+// ---------------------------------------------------------------------------
+// options (data_m.f90:15-94) and sticky error
+// ---------------------------------------------------------------------------
+struct Options {
+  int integrator = kRK2;
+  int interpolator = kLinear;
+  int incompressible = 0;
+  int verbose = 0;
+  int sync_timer = 0;
+  U cfl_max = (U)0.5;
+  int precision = sizeof(U) == 8 ? LCS_FP64 : LCS_FP32;
+  bool warn_opt = false, warn_par = false;  // SAVEd warn flags (cfd2lcs.f90:444,632)
+};
+
+static Options g_opt;
+static int g_error = 0;
+
+static void fail(const std::string& msg) {
+  std::printf("ERROR: libcfd2lcs: %s\n", msg.c_str());
+  std::fflush(stdout);
+  g_error = 1;
+}
+
+#define HIPCHK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      fail(std::string("HIP error ") + hipGetErrorString(e_) + " at " #expr);           \
+      return;                                                                           \
+    }                                                                                   \
+  } while (0)
+#define HIPCHK_R(expr, rv)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      fail(std::string("HIP error ") + hipGetErrorString(e_) + " at " #expr);           \
+      return rv;                                                                        \
+    }                                                                                   \
+  } while (0)
+
+// device allocation owned by the engine
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  bool alloc(size_t b) {
+    release();
+    if (b == 0) return true;
+    if (hipMalloc(&p, b) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    bytes = b;
+    return true;
+  }
+};
+
+struct EngineBase {
+  virtual ~EngineBase() {}
+  virtual int add_diag(int type, U T, U h, const char* label) = 0;
+  virtual void update(const U* u, const U* v, const U* w, bool device, U time) = 0;
+  virtual int get_flowmap(int h, U* a, U* b, U* c) = 0;
+  virtual int get_composed(int h, U* a, U* b, U* c) = 0;
+  virtual int get_ftle(int h, U* out) = 0;
+  virtual const double* ftle_device(int h) = 0;
+  virtual int noutput(int h) = 0;
+  virtual void stats(double out[8]) = 0;
+  virtual int debug_grid(double* lines, int* flag, double* bn, double* ps, double* bb) = 0;
+  virtual bool fresh() const = 0;
+  virtual int ndiag() const = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Engine<T>: device state and orchestration in compute precision T
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Engine : EngineBase {
+  HostGrid hg;
+  GridDev<T> g{};
+  hipStream_t stream = nullptr;
+  DevBuf grid_mem;      // lines, flags, normals, deltas
+  DevBuf spline_mem[3]; // knots + band per axis
+  int spline_k = 0;     // order of the fitted spline state (0 = none)
+  DevBuf status_mem;
+  DevStatus* st = nullptr;
+  DevStatus* st_host = nullptr;
+  DevBuf vel[2];        // [u_n, u_np1] ghosted, 3 components each
+  DevBuf vcoef[2];      // spline coefficients of vel[]
+  bool vcoef_ok[2] = {false, false};
+  int cur = 1;          // index of u_np1 in vel[]
+  DevBuf stage_u;       // staging of user velocity (lcsdata_t), 3*np
+  DevBuf stage_out;     // staging for getters (lcsdata_t), 3*np
+  bool first_call = true;
+  U t_n = 0, t_np1 = 0;
+  int nsub_last = 0;
+  long long integ_fwd = 0, integ_bkwd = 0;
+  double time_fwd = 0, time_bkwd = 0, time_comp = 0, time_ftle = 0;
+  unsigned long long oob_total = 0;
+
+  struct Timed {
+    hipEvent_t a, b;
+    int cat;
+  };
+  std::vector<Timed> pending;
+
+  struct Diag {
+    int id, type, dir;
+    U Tint, h, dt_factor;
+    std::string label;
+    int nstep, nslot;
+    std::vector<char> slot_ok, slot_coef_ok;
+    DevBuf xp, dx, ns;     // FWD particles (origin order)
+    DevBuf fm[2];          // BKWD semi-Lagrangian flowmap ping-pong (ghosted)
+    int fcur = 0;
+    DevBuf fmcoef;         // spline coefficients of fm[fcur]
+    DevBuf slots, slotcoef;  // nslot * 3 ghosted fields
+    DevBuf order;          // slot order of the composition
+    DevBuf dxout;          // composed displacement by origin (3*np)
+    DevBuf compfm;         // composed flowmap, ghosted
+    DevBuf comp;           // composed flowmap interior (3*np)
+    DevBuf ftle;           // np doubles
+    int noutput = 0;
+  };
+  std::vector<std::unique_ptr<Diag>> diags;
+
+  ~Engine() override {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      for (auto& t : pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+      }
+      (void)hipStreamDestroy(stream);
+    }
+    if (st_host) (void)hipHostFree(st_host);
+  }
+
+  bool fresh() const override { return diags.empty() && first_call; }
+  int ndiag() const override { return (int)diags.size(); }
+
+  size_t fbytes() const { return sizeof(T) * (size_t)g.nt; }
+  F3<T> f3(DevBuf& b) {
+    T* p = (T*)b.p;
+    return F3<T>{{p, p + g.nt, p + 2 * g.nt}};
+  }
+  CF3<T> cf3(DevBuf& b) {
+    const T* p = (const T*)b.p;
+    return CF3<T>{{p, p + g.nt, p + 2 * g.nt}};
+  }
+  F3<T> p3(DevBuf& b) {
+    T* p = (T*)b.p;
+    return F3<T>{{p, p + g.np, p + 2 * g.np}};
+  }
+  CF3<T> cp3(DevBuf& b) {
+    const T* p = (const T*)b.p;
+    return CF3<T>{{p, p + g.np, p + 2 * g.np}};
+  }
+
+  bool init(const HostGrid& h) {
+    hg = h;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+      fail("cannot create a HIP stream (no GPU visible?)");
+      return false;
+    }
+    for (int d = 0; d < 3; d++) {
+      g.n[d] = hg.n[d];
+      g.N[d] = hg.N[d];
+      g.per[d] = hg.per[d];
+      g.kord[d] = 0;
+      g.kn[d] = nullptr;
+      g.band[d] = nullptr;
+    }
+    g.nt = hg.nt;
+    g.np = hg.np;
+    g.flat = hg.flat;
+    g.max_track = 2 * std::max(hg.n[0], std::max(hg.n[1], hg.n[2]));
+    for (int c = 0; c < 27; c++)
+      for (int d = 0; d < 3; d++) g.ps[c][d] = (T)hg.ps[c][d];
+    for (int d = 0; d < 6; d++) g.bb[d] = (T)hg.bb[d];
+    // one allocation: lines (2 per axis), normals, deltas, flags
+    const size_t nl = (size_t)hg.N[0] + hg.N[1] + hg.N[2];
+    const size_t reals = 2 * nl + 6 * (size_t)hg.np;
+    const size_t bytes = reals * sizeof(T) + (size_t)hg.nt + 256;
+    if (!grid_mem.alloc(bytes)) {
+      fail("hipMalloc failed for the grid");
+      return false;
+    }
+    std::vector<T> hr(reals);
+    size_t o = 0;
+    size_t off_gl[3], off_xl[3], off_bn[3], off_dl[3];
+    for (int d = 0; d < 3; d++) {
+      off_gl[d] = o;
+      for (int a = 0; a < hg.N[d]; a++) hr[o++] = (T)hg.line[d][a];
+      off_xl[d] = o;
+      for (int a = 0; a < hg.N[d]; a++) hr[o++] = (T)hg.xl[d][a];
+    }
+    for (int d = 0; d < 3; d++) {
+      off_bn[d] = o;
+      for (long long p = 0; p < hg.np; p++) hr[o++] = (T)hg.bn[d][p];
+      off_dl[d] = o;
+      for (long long p = 0; p < hg.np; p++) hr[o++] = (T)hg.dl[d][p];
+    }
+    T* base = (T*)grid_mem.p;
+    if (hipMemcpy(base, hr.data(), reals * sizeof(T), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((char*)(base + reals), hg.flag.data(), (size_t)hg.nt, hipMemcpyHostToDevice) != hipSuccess) {
+      fail("upload of the grid failed");
+      return false;
+    }
+    for (int d = 0; d < 3; d++) {
+      g.gl[d] = base + off_gl[d];
+      g.xl[d] = base + off_xl[d];
+      g.bn[d] = base + off_bn[d];
+      g.dl[d] = base + off_dl[d];
+    }
+    g.flag = (const int8_t*)(base + reals);
+    if (!status_mem.alloc(sizeof(DevStatus))) {
+      fail("hipMalloc failed (status)");
+      return false;
+    }
+    st = (DevStatus*)status_mem.p;
+    if (hipHostMalloc((void**)&st_host, sizeof(DevStatus), hipHostMallocDefault) != hipSuccess) {
+      fail("hipHostMalloc failed");
+      return false;
+    }
+    std::memset(st_host, 0, sizeof(DevStatus));
+    if (hipMemset(st, 0, sizeof(DevStatus)) != hipSuccess) return false;
+    for (int b = 0; b < 2; b++) {
+      if (!vel[b].alloc(3 * fbytes()) || hipMemset(vel[b].p, 0, 3 * fbytes()) != hipSuccess) {
+        fail("hipMalloc failed (velocity)");
+        return false;
+      }
+    }
+    if (!stage_u.alloc(3 * sizeof(U) * (size_t)g.np)) {
+      fail("hipMalloc failed (staging)");
+      return false;
+    }
+    return true;
+  }
+
+  // spline order of the active interpolator, with the order clamp of
+  // lp_tracking_m.f90:506-508
+  int spline_order_for(int interp) const { return interp == kQuadratic ? 3 : 4; }
+  bool ensure_spline(int interp) {
+    const int k = spline_order_for(interp);
+    if (spline_k == k) return true;
+    for (int d = 0; d < 3; d++) {
+      int kk = std::max(std::min(k, hg.N[d] - 1), 2);
+      SplineAxis ax;
+      std::string e = spline_axis(hg.xl[d], kk, ax);
+      if (!e.empty()) {
+        fail(e);
+        return false;
+      }
+      const size_t nr = ax.knots.size() + ax.band.size();
+      std::vector<T> hv(nr);
+      for (size_t a = 0; a < ax.knots.size(); a++) hv[a] = (T)ax.knots[a];
+      for (size_t a = 0; a < ax.band.size(); a++) hv[ax.knots.size() + a] = (T)ax.band[a];
+      if (!spline_mem[d].alloc(nr * sizeof(T)) ||
+          hipMemcpy(spline_mem[d].p, hv.data(), nr * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+        fail("upload of the spline factors failed");
+        return false;
+      }
+      g.kord[d] = kk;
+      g.kn[d] = (const T*)spline_mem[d].p;
+      g.band[d] = (const T*)spline_mem[d].p + ax.knots.size();
+    }
+    if (!(g.kord[0] == g.kord[1] && (g.kord[2] == g.kord[0] || g.kord[2] == 2))) {
+      fail("spline orders per axis not supported by this build (need equal x/y orders)");
+      return false;
+    }
+    spline_k = k;
+    vcoef_ok[0] = vcoef_ok[1] = false;
+    for (auto& dg : diags) std::fill(dg->slot_coef_ok.begin(), dg->slot_coef_ok.end(), 0);
+    return true;
+  }
+  bool is_spline(int interp) const { return interp == kQuadratic || interp == kCubic; }
+
+  void timed_begin(Timed& t, int cat) {
+    t.cat = cat;
+    (void)hipEventCreate(&t.a);
+    (void)hipEventCreate(&t.b);
+    (void)hipEventRecord(t.a, stream);
+  }
+  void timed_end(Timed& t) {
+    (void)hipEventRecord(t.b, stream);
+    pending.push_back(t);
+  }
+  void harvest_timers() {
+    for (auto& t : pending) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+        const double s = ms * 1e-3;
+        if (t.cat == 0) time_fwd += s;
+        if (t.cat == 1) time_bkwd += s;
+        if (t.cat == 2) time_comp += s;
+        if (t.cat == 3) time_ftle += s;
+      }
+      (void)hipEventDestroy(t.a);
+      (void)hipEventDestroy(t.b);
+    }
+    pending.clear();
+  }
+
+  int add_diag(int type, U T_, U h, const char* label) override {
+    auto dg = std::make_unique<Diag>();
+    dg->id = (int)diags.size() + 1;
+    dg->type = type;
+    dg->dir = type == 0 ? 1 : -1;
+    dg->Tint = T_;
+    dg->h = h;
+    dg->dt_factor = dg->dir < 0 ? (U)1 / (U)std::max(1 + 0, 1) : (U)1;
+    dg->label = label ? label : "";
+    // nstep = nint(T/h); slots 0..nstep (the write index may equal nstep)
+    dg->nstep = (int)std::lround((double)(T_ / h));
+    if (dg->nstep < 1 || dg->nstep > 100000) {
+      fail("T/h out of range");
+      return -1;
+    }
+    dg->nslot = dg->nstep + 1;
+    dg->slot_ok.assign(dg->nslot, 0);
+    dg->slot_coef_ok.assign(dg->nslot, 0);
+    const size_t fb = fbytes(), pb = sizeof(T) * (size_t)g.np;
+    bool ok = true;
+    if (dg->dir > 0) {
+      ok = ok && dg->xp.alloc(3 * pb) && dg->dx.alloc(3 * pb) && dg->ns.alloc(3 * sizeof(int16_t) * (size_t)g.np);
+    } else {
+      ok = ok && dg->fm[0].alloc(3 * fb) && dg->fm[1].alloc(3 * fb);
+    }
+    ok = ok && dg->slots.alloc((size_t)dg->nslot * 3 * fb) && dg->order.alloc(sizeof(int) * dg->nslot) &&
+         dg->dxout.alloc(3 * pb) && dg->compfm.alloc(3 * fb) && dg->comp.alloc(3 * pb) &&
+         dg->ftle.alloc(sizeof(double) * (size_t)g.np);
+    if (!ok) {
+      fail("hipMalloc failed for a diagnostic (out of device memory?)");
+      return -1;
+    }
+    if (hipMemsetAsync(dg->compfm.p, 0, 3 * fb, stream) != hipSuccess ||
+        hipMemsetAsync(dg->comp.p, 0, 3 * pb, stream) != hipSuccess ||
+        hipMemsetAsync(dg->ftle.p, 0, sizeof(double) * (size_t)g.np, stream) != hipSuccess) {
+      fail("hipMemset failed");
+      return -1;
+    }
+    reset_diag(*dg);
+    if (hipStreamSynchronize(stream) != hipSuccess) {
+      fail("diagnostic initialisation failed on the device");
+      return -1;
+    }
+    diags.push_back(std::move(dg));
+    return (int)diags.size();
+  }
+
+  // reset_lp (lp_m.f90:228-294): particles back on their nodes, fm = 0
+  void reset_diag(Diag& dg) {
+    if (dg.dir > 0) {
+      int16_t* ns = (int16_t*)dg.ns.p;
+      launch_fwd_reset(g, p3(dg.xp), p3(dg.dx), ns, ns + g.np, ns + 2 * g.np, stream);
+    } else {
+      (void)hipMemsetAsync(dg.fm[0].p, 0, 3 * fbytes(), stream);
+      (void)hipMemsetAsync(dg.fm[1].p, 0, 3 * fbytes(), stream);
+    }
+  }
+
+  // ct of a stage at time tt (lp_motion_m.f90:170 etc.); NaN (first update)
+  // clamps to 1 as with IEEE minNum/maxNum
+  static U ctf(U tt, U t0, U t1) { return std::fmax(std::fmin((tt - t0) / (t1 - t0), (U)1), (U)0); }
+
+  StageParams<T> stage_params(U t, U dt) const {
+    StageParams<T> sp{};
+    sp.integ = g_opt.integrator;
+    sp.dt = (T)dt;
+    const U half = (U)1 / (U)2;
+    U c[4] = {0, 0, 0, 0};
+    switch (g_opt.integrator) {
+      case kEuler: c[0] = ctf(t, t_n, t_np1); break;
+      case kTrapezoidal: c[0] = ctf(t + half * dt, t_n, t_np1); break;
+      case kRK2:
+        c[0] = ctf(t, t_n, t_np1);
+        c[1] = ctf(t + half * dt, t_n, t_np1);
+        break;
+      case kRK3:
+        c[0] = ctf(t, t_n, t_np1);
+        c[1] = ctf(t + half * dt, t_n, t_np1);
+        c[2] = ctf(t + dt, t_n, t_np1);
+        break;
+      default:
+        c[0] = ctf(t, t_n, t_np1);
+        c[1] = ctf(t + half * dt, t_n, t_np1);
+        c[2] = c[1];
+        c[3] = ctf(t + dt, t_n, t_np1);
+    }
+    for (int a = 0; a < 4; a++) sp.ct[a] = (T)c[a];
+    return sp;
+  }
+
+  void update(const U* u, const U* v, const U* w, bool device, U time) override {
+    const int interp = g_opt.interpolator;
+    if (interp != kNearest && interp != kLinear && interp != kQuadratic && interp != kCubic) {
+      fail("INTERPOLATOR TSE/TSE_LIMIT is not supported by this build");
+      return;
+    }
+    if (g_opt.integrator < kEuler || g_opt.integrator > kRK4) {
+      fail("unknown INTEGRATOR");
+      return;
+    }
+    const bool spl = is_spline(interp);
+    if (spl && !ensure_spline(interp)) return;
+    // time levels (cfd2lcs.f90:104-111) and u_n <- u_np1 (112)
+    if (first_call) {
+      t_n = time;
+      t_np1 = time;
+      first_call = false;
+    } else {
+      t_n = t_np1;
+      t_np1 = time;
+    }
+    const int old = cur;
+    cur = 1 - cur;
+    vcoef_ok[cur] = false;
+    // vel[cur] becomes u_np1; vel[old] (previous u_np1) is u_n; vcoef[b]
+    // holds the spline coefficients of vel[b]
+    const U *su = u, *sv = v, *sw = w;
+    if (!device) {
+      U* s = (U*)stage_u.p;
+      const size_t b = sizeof(U) * (size_t)g.np;
+      HIPCHK(hipMemcpyAsync(s, u, b, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(s + g.np, v, b, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(s + 2 * g.np, w, b, hipMemcpyHostToDevice, stream));
+      su = s;
+      sv = s + g.np;
+      sw = s + 2 * g.np;
+    }
+    F3<T> unp1 = f3(vel[cur]);
+    CF3<T> unm = cf3(vel[old]);
+    launch_ingest<U, T>(g, unp1, su, sv, sw, stream);
+    launch_vel_ghost(g, unp1, unm, st, stream);
+    HIPCHK(hipMemsetAsync(&st->cfl_bits, 0, sizeof(st->cfl_bits), stream));
+    launch_cfl(g, unp1, st, stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    harvest_timers();
+    oob_total = st_host->oob;
+    if (st_host->err) {
+      fail(st_host->err & kErrBcFlag ? "unknown bcflag" : (st_host->err & kErrTrack ? "particle tracking failed" : "device error"));
+      return;
+    }
+    double cfl_rate = 0;
+    {
+      unsigned long long bits = st_host->cfl_bits;
+      std::memcpy(&cfl_rate, &bits, sizeof(double));
+    }
+    if (spl) {
+      // spline coefficients of u_np1 (and of u_n when they are missing)
+      if (!fit_vel(cur)) return;
+      if (!vcoef_ok[old] && !fit_vel(old)) return;
+    }
+    const U dt_f = t_np1 - t_n;
+    for (auto& dgp : diags) {
+      Diag& dg = *dgp;
+      // set_dt (lp_motion_m.f90:56-127)
+      const U cmax = (U)((T)cfl_rate) * dt_f;
+      const double r = std::ceil((double)(cmax / (g_opt.cfl_max * dg.dt_factor)));
+      int nsub = 1;
+      if (r > 1e6) {
+        fail("CFL sub-cycle count above 1e6 (velocity or time step out of range)");
+        return;
+      }
+      if (r >= 1) nsub = (int)r;
+      nsub_last = nsub;
+      U dt = dt_f / (U)nsub;
+      if (dg.dir < 0) dt = (U)-1 * dt;
+      if (std::fabs(dt) > 0) {
+        if (dg.dir > 0) integ_fwd += (long long)g.np * nsub;
+        else integ_bkwd += (long long)g.np * nsub;
+      }
+      Timed tm;
+      timed_begin(tm, dg.dir > 0 ? 0 : 1);
+      CF3<T> cu_n = spl ? cf3(vcoef[old]) : cf3(vel[old]);
+      CF3<T> cu_np1 = spl ? cf3(vcoef[cur]) : cf3(vel[cur]);
+      if (dg.dir > 0) {
+        U t = t_n;
+        int16_t* ns = (int16_t*)dg.ns.p;
+        for (int s = 0; s < nsub; s++) {
+          launch_fwd_step(g, interp, stage_params(t, dt), cu_n, cu_np1, p3(dg.xp), p3(dg.dx), ns, ns + g.np,
+                          ns + 2 * g.np, st, stream);
+          t = t + dt;
+        }
+      } else {
+        U t = t_np1;
+        if (spl && !dg.fmcoef.p && !dg.fmcoef.alloc(3 * fbytes())) {
+          fail("hipMalloc failed (flowmap coefficients)");
+          return;
+        }
+        for (int s = 0; s < nsub; s++) {
+          DevBuf& a = dg.fm[dg.fcur];
+          DevBuf& b = dg.fm[1 - dg.fcur];
+          CF3<T> src = cf3(a);
+          if (spl) {
+            HIPCHK(hipMemcpyAsync(dg.fmcoef.p, a.p, 3 * fbytes(), hipMemcpyDeviceToDevice, stream));
+            launch_spline_fit(g, f3(dg.fmcoef), 3, stream);
+            src = cf3(dg.fmcoef);
+          }
+          launch_sl_step(g, interp, stage_params(t, dt), cu_n, cu_np1, src, f3(b), st, stream);
+          launch_fm_ghost(g, f3(b), cf3(a), st, stream);
+          dg.fcur = 1 - dg.fcur;
+          t = t + dt;
+        }
+      }
+      timed_end(tm);
+      HIPCHK(hipGetLastError());
+    }
+    // diagnostics at h boundaries (cfd2lcs.f90:144-224)
+    for (auto& dgp : diags) {
+      Diag& dg = *dgp;
+      if ((long long)(t_np1 / dg.h) == (long long)(t_n / dg.h)) continue;
+      output(dg, interp);
+      if (g_error) return;
+    }
+  }
+
+  bool fit_vel(int b) {
+    if (!vcoef[b].p && !vcoef[b].alloc(3 * fbytes())) {
+      fail("hipMalloc failed (velocity coefficients)");
+      return false;
+    }
+    if (hipMemcpyAsync(vcoef[b].p, vel[b].p, 3 * fbytes(), hipMemcpyDeviceToDevice, stream) != hipSuccess) {
+      fail("copy failed");
+      return false;
+    }
+    launch_spline_fit(g, f3(vcoef[b]), 3, stream);
+    vcoef_ok[b] = true;
+    return true;
+  }
+
+  T* slot_ptr(DevBuf& b, int s) { return (T*)b.p + (size_t)s * 3 * g.nt; }
+
+  // write_flowmap_substep + reconstruct_flowmap + compute_ftle + reset_lp
+  void output(Diag& dg, int interp) {
+    const double md = std::fmod((double)t_np1, (double)dg.Tint);
+    const int step = (int)std::lround((double)((U)md / dg.h));
+    if (step < 0 || step >= dg.nslot) {
+      fail("flowmap slot index out of range");
+      return;
+    }
+    const size_t fb = fbytes();
+    T* sp = slot_ptr(dg.slots, step);
+    F3<T> slot{{sp, sp + g.nt, sp + 2 * g.nt}};
+    if (dg.dir > 0) {
+      launch_field_from_origin(g, cp3(dg.dx), slot, stream);
+    } else {
+      HIPCHK(hipMemcpyAsync(sp, dg.fm[dg.fcur].p, 3 * fb, hipMemcpyDeviceToDevice, stream));
+    }
+    dg.slot_ok[step] = 1;
+    dg.slot_coef_ok[step] = 0;
+    const bool spl = is_spline(interp);
+    if (spl) {
+      if (!dg.slotcoef.p && !dg.slotcoef.alloc((size_t)dg.nslot * 3 * fb)) {
+        fail("hipMalloc failed (slot coefficients)");
+        return;
+      }
+      for (int s = 0; s < dg.nslot; s++) {
+        if (!dg.slot_ok[s] || dg.slot_coef_ok[s]) continue;
+        T* cp = slot_ptr(dg.slotcoef, s);
+        HIPCHK(hipMemcpyAsync(cp, slot_ptr(dg.slots, s), 3 * fb, hipMemcpyDeviceToDevice, stream));
+        launch_spline_fit(g, F3<T>{{cp, cp + g.nt, cp + 2 * g.nt}}, 3, stream);
+        dg.slot_coef_ok[s] = 1;
+      }
+    }
+    // slot order (flowmap_m.f90:202-268)
+    std::vector<int> order;
+    int s = step;
+    const int inc = dg.dir > 0 ? 1 : -1;
+    if (dg.dir > 0) s = s + inc;
+    for (int it = 0; it < dg.nstep; it++) {
+      if (s >= dg.nstep) s = 0;
+      if (s < 0) s = dg.nstep - 1;
+      if (dg.slot_ok[s]) order.push_back(s);
+      s += inc;
+    }
+    HIPCHK(hipMemcpyAsync(dg.order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream));
+    Timed tc;
+    timed_begin(tc, 2);
+    launch_compose(g, interp, (const T*)(spl ? dg.slotcoef.p : dg.slots.p), g.nt, (const int*)dg.order.p,
+                   (int)order.size(), p3(dg.dxout), st, stream);
+    launch_field_from_origin(g, cp3(dg.dxout), f3(dg.compfm), stream);
+    timed_end(tc);
+    Timed tf;
+    timed_begin(tf, 3);
+    const double tiny = sizeof(U) == 8 ? DBL_MIN : (double)FLT_MIN;
+    launch_ftle(g, cf3(dg.compfm), (double)dg.Tint, g_opt.incompressible, tiny, (double*)dg.ftle.p, p3(dg.comp),
+                stream);
+    timed_end(tf);
+    reset_diag(dg);
+    dg.noutput++;
+    HIPCHK(hipGetLastError());
+    // the order list must stay valid until the kernel ran: wait here
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost));
+    if (st_host->err & kErrTrack) fail("cannot locate all particles after the maximum number of tracking iterations");
+    else if (st_host->err) fail("device error during composition");
+  }
+
+  bool sync_status() {
+    if (hipStreamSynchronize(stream) != hipSuccess) {
+      fail("device synchronisation failed");
+      return false;
+    }
+    harvest_timers();
+    if (hipMemcpy(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    oob_total = st_host->oob;
+    if (st_host->err) {
+      fail("device error (bcflag/tracking)");
+      return false;
+    }
+    return true;
+  }
+
+  Diag* diag(int h) { return (h >= 1 && h <= (int)diags.size()) ? diags[h - 1].get() : nullptr; }
+
+  int copy_out(CF3<T> src, int ghosted, U* a, U* b, U* c) {
+    if (!stage_out.p && !stage_out.alloc(3 * sizeof(U) * (size_t)g.np)) return 1;
+    U* o = (U*)stage_out.p;
+    launch_interior_to_user<T, U>(g, src, ghosted, o, o + g.np, o + 2 * g.np, stream);
+    const size_t b1 = sizeof(U) * (size_t)g.np;
+    if (hipMemcpyAsync(a, o, b1, hipMemcpyDeviceToHost, stream) != hipSuccess) return 1;
+    if (b && hipMemcpyAsync(b, o + g.np, b1, hipMemcpyDeviceToHost, stream) != hipSuccess) return 1;
+    if (c && hipMemcpyAsync(c, o + 2 * g.np, b1, hipMemcpyDeviceToHost, stream) != hipSuccess) return 1;
+    return hipStreamSynchronize(stream) == hipSuccess ? 0 : 1;
+  }
+
+  int get_flowmap(int h, U* a, U* b, U* c) override {
+    Diag* dg = diag(h);
+    if (!dg) return 1;
+    if (dg->dir > 0) return copy_out(cp3(dg->dx), 0, a, b, c);
+    return copy_out(cf3(dg->fm[dg->fcur]), 1, a, b, c);
+  }
+  int get_composed(int h, U* a, U* b, U* c) override {
+    Diag* dg = diag(h);
+    if (!dg) return 1;
+    return copy_out(cp3(dg->comp), 0, a, b, c);
+  }
+  int get_ftle(int h, U* out) override {
+    Diag* dg = diag(h);
+    if (!dg) return 1;
+    std::vector<double> tmp(g.np);
+    if (hipStreamSynchronize(stream) != hipSuccess) return 1;
+    if (hipMemcpy(tmp.data(), dg->ftle.p, sizeof(double) * g.np, hipMemcpyDeviceToHost) != hipSuccess) return 1;
+    for (long long p = 0; p < g.np; p++) out[p] = (U)tmp[p];
+    return 0;
+  }
+  const double* ftle_device(int h) override {
+    Diag* dg = diag(h);
+    if (!dg) return nullptr;
+    (void)hipStreamSynchronize(stream);
+    return (const double*)dg->ftle.p;
+  }
+  int noutput(int h) override {
+    Diag* dg = diag(h);
+    return dg ? dg->noutput : -1;
+  }
+  void stats(double out[8]) override {
+    sync_status();
+    out[0] = nsub_last;
+    out[1] = (double)integ_fwd;
+    out[2] = (double)integ_bkwd;
+    out[3] = time_fwd;
+    out[4] = time_bkwd;
+    out[5] = time_comp;
+    out[6] = time_ftle;
+    out[7] = (double)oob_total;
+  }
+  int debug_grid(double* lines, int* flag, double* bn, double* ps, double* bb) override {
+    if (lines) {
+      size_t o = 0;
+      for (int d = 0; d < 3; d++)
+        for (int a = 0; a < hg.N[d]; a++) lines[o++] = hg.line[d][a];
+    }
+    if (flag)
+      for (long long l = 0; l < hg.nt; l++) flag[l] = hg.flag[l];
+    if (bn)
+      for (int d = 0; d < 3; d++)
+        for (long long p = 0; p < hg.np; p++) bn[d * hg.np + p] = hg.bn[d][p];
+    if (ps)
+      for (int c = 0; c < 27; c++)
+        for (int d = 0; d < 3; d++) ps[3 * c + d] = hg.ps[c][d];
+    if (bb)
+      for (int d = 0; d < 6; d++) bb[d] = hg.bb[d];
+    return 0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// library-global state
+// ---------------------------------------------------------------------------
+static std::unique_ptr<HostGrid> g_grid;
+static std::unique_ptr<EngineBase> g_eng;
+static int g_engine_prec = 0;
+
+static bool ensure_engine() {
+  if (g_eng) return true;
+  if (!g_grid) {
+    fail("cfd2lcs_init_c has not been called");
+    return false;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    fail("no HIP device visible: the FTLE path runs on the GPU only");
+    return false;
+  }
+  if (g_opt.precision == LCS_FP64) {
+    auto e = std::make_unique<Engine<double>>();
+    if (!e->init(*g_grid)) return false;
+    g_eng = std::move(e);
+  } else {
+    auto e = std::make_unique<Engine<float>>();
+    if (!e->init(*g_grid)) return false;
+    g_eng = std::move(e);
+  }
+  g_engine_prec = g_opt.precision;
+  return true;
+}
+
+}  // namespace lcs
+
+using namespace lcs;
+
+static std::string trim_label(const char* s, size_t n) {
+  std::string r(s, n);
+  while (!r.empty() && (r.back() == ' ' || r.back() == '\0')) r.pop_back();
+  return r;
+}
+
+extern "C" {
+
+void cfd2lcs_init_c(MPI_Comm usercomm, int n[3], int offset[3], lcsdata_t* x, lcsdata_t* y, lcsdata_t* z,
+                    int* flag) {
+  (void)usercomm;
+  (void)offset;  // one rank: the minimum offset is normalised to 0 (sgrid_m.f90:90-93)
+  if (g_error) return;
+  g_eng.reset();
+  const long long np = (long long)n[0] * n[1] * n[2];
+  if (n[0] < 1 || n[1] < 1 || n[2] < 1) {
+    fail("ng > ni, nj or nk");
+    return;
+  }
+  std::vector<double> X(np), Y(np), Z(np);
+  for (long long p = 0; p < np; p++) {
+    X[p] = x[p];
+    Y[p] = y[p];
+    Z[p] = z[p];
+  }
+  auto hg = std::make_unique<HostGrid>();
+  std::string e = hg->build(n, X.data(), Y.data(), Z.data(), flag);
+  if (!e.empty()) {
+    fail(e);
+    return;
+  }
+  g_grid = std::move(hg);
+  ensure_engine();
+}
+
+int cfd2lcs_diagnostic_init_c(int lcs_type, int resolution, lcsdata_t t, lcsdata_t h, char label[]) {
+  if (g_error) return 0;
+  if (!ensure_engine()) return 0;
+  if (resolution != 0) {
+    fail("resolution != 0 (refined LCS grids) is not supported by this build");
+    return 0;
+  }
+  if (lcs_type == LP_TRACER) {
+    fail("LP_TRACER diagnostics are not supported by this build");
+    return 0;
+  }
+  if (lcs_type != FTLE_FWD && lcs_type != FTLE_BKWD) {
+    fail("bad specification for lcs_type. Options are: FTLE_FWD, FTLE_BKWD, LP_TRACER");
+    return 0;
+  }
+  if (g_eng->ndiag() >= 1000) {
+    fail("NLCS exceeds NMAX_STRUCT");
+    return 0;
+  }
+  return g_eng->add_diag(lcs_type, t, h, label);
+}
+
+void cfd2lcs_update_c(int n[3], lcsdata_t* u, lcsdata_t* v, lcsdata_t* w, lcsdata_t time) {
+  if (g_error) return;
+  if (!ensure_engine()) return;
+  if (n[0] != g_grid->n[0] || n[1] != g_grid->n[1] || n[2] != g_grid->n[2]) {
+    fail("received velocity array of incorrect dimension");
+    return;
+  }
+  g_eng->update(u, v, w, false, time);
+}
+
+void cfd2lcs_update_device_c(int n[3], const lcsdata_t* d_u, const lcsdata_t* d_v, const lcsdata_t* d_w,
+                             lcsdata_t time) {
+  if (g_error) return;
+  if (!ensure_engine()) return;
+  if (n[0] != g_grid->n[0] || n[1] != g_grid->n[1] || n[2] != g_grid->n[2]) {
+    fail("received velocity array of incorrect dimension");
+    return;
+  }
+  g_eng->update(d_u, d_v, d_w, true, time);
+}
+
+void cfd2lcs_diagnostic_destroy_c(int lcs_handle) {
+  // the reference leaves this as a TODO (cfd2lcs.f90:372-379): nothing is freed
+  (void)lcs_handle;
+}
+
+void cfd2lcs_set_option_c(char option[], int val) {
+  if (g_error) return;
+  const std::string o = option ? option : "";
+  bool warn = false;
+  auto boolopt = [&](int& dst) {
+    if (val == LCS_TRUE) dst = 1;
+    else if (val == LCS_FALSE) dst = 0;
+    else warn = true;
+  };
+  int dummy = 0;
+  if (o == "SYNCTIMER") boolopt(g_opt.sync_timer);
+  else if (o == "DEBUG") boolopt(g_opt.verbose);
+  else if (o == "INCOMPRESSIBLE") boolopt(g_opt.incompressible);
+  else if (o == "WRITE_FLOWMAP" || o == "WRITE_VELOCITY" || o == "WRITE_BCFLAG") boolopt(dummy);
+  else if (o == "VELOCITY_INVARIANTS" || o == "AUX_GRID") {
+    boolopt(dummy);
+    if (val == LCS_TRUE) {
+      fail(o + " = LCS_TRUE is not supported by this build");
+      return;
+    }
+  } else if (o == "UPDATE_FREQ") {
+  } else if (o == "INTERPOLATOR") {
+    if (val != NEAREST_NBR && val != LINEAR && val != QUADRATIC && val != CUBIC && val != TSE && val != TSE_LIMIT)
+      warn = true;
+    g_opt.interpolator = val;
+  } else if (o == "INTEGRATOR") {
+    if (val < EULER || val > RK4) warn = true;
+    g_opt.integrator = val;
+  } else if (o == "PRECISION") {
+    // device compute precision; switching rebuilds the device state, which
+    // is allowed before the first diagnostic_init / update only
+    if (val != LCS_FP64 && val != LCS_FP32) {
+      warn = true;
+    } else {
+      g_opt.precision = val;
+      if (g_eng && g_engine_prec != val) {
+        if (!g_eng->fresh()) {
+          fail("PRECISION must be set before cfd2lcs_diagnostic_init_c and cfd2lcs_update_c");
+          return;
+        }
+        g_eng.reset();
+        ensure_engine();
+      }
+    }
+  } else {
+    warn = true;
+  }
+  if (warn) g_opt.warn_opt = true;
+  if (g_opt.warn_opt) fail("bad libcfd2lcs option/val combination: " + o + "," + std::to_string(val));
+}
+
+void cfd2lcs_set_param_c(char param[], lcsdata_t val) {
+  if (g_error) return;
+  const std::string p = param ? param : "";
+  bool warn = false;
+  if (p == "CFL") g_opt.cfl_max = val;
+  else if (p == "TRACER_INJECT_X" || p == "TRACER_INJECT_Y" || p == "TRACER_INJECT_Z" || p == "TRACER_INJECT_RADIUS") {
+  } else warn = true;
+  if (warn) g_opt.warn_par = true;
+  if (g_opt.warn_par) fail("unknown libcfd2lcs parameter: " + p);
+}
+
+void cfd2lcs_finalize_c(void) {
+  g_eng.reset();
+  g_grid.reset();
+  g_opt = Options();
+  g_error = 0;
+  g_engine_prec = 0;
+}
+
+int cfd2lcs_error_c(void) { return g_error; }
+
+int cfd2lcs_get_flowmap_c(int lcs_handle, lcsdata_t* fx, lcsdata_t* fy, lcsdata_t* fz) {
+  if (!g_eng) return 1;
+  return g_eng->get_flowmap(lcs_handle, fx, fy, fz);
+}
+int cfd2lcs_get_composed_c(int lcs_handle, lcsdata_t* fx, lcsdata_t* fy, lcsdata_t* fz) {
+  if (!g_eng) return 1;
+  return g_eng->get_composed(lcs_handle, fx, fy, fz);
+}
+int cfd2lcs_get_ftle_c(int lcs_handle, lcsdata_t* ftle) {
+  if (!g_eng) return 1;
+  return g_eng->get_ftle(lcs_handle, ftle);
+}
+const double* cfd2lcs_get_ftle_device_c(int lcs_handle) { return g_eng ? g_eng->ftle_device(lcs_handle) : nullptr; }
+int cfd2lcs_get_noutput_c(int lcs_handle) { return g_eng ? g_eng->noutput(lcs_handle) : -1; }
+void cfd2lcs_get_stats_c(double out[8]) {
+  if (!g_eng) {
+    for (int a = 0; a < 8; a++) out[a] = 0;
+    return;
+  }
+  g_eng->stats(out);
+}
+
+// host-side grid description (for tests): lines[N0+N1+N2], flag[nt],
+// bn[3*np], ps[81], bb[6]; any pointer may be NULL
+int cfd2lcs_debug_grid_c(double* lines, int* flag, double* bn, double* ps, double* bb) {
+  if (!g_eng) return 1;
+  return g_eng->debug_grid(lines, flag, bn, ps, bb);
+}
+
+// ---- Fortran-callable entry points (the reference's module procedures as
+// seen from C, cfd2lcs_c.c:16-58); hidden string lengths come last ----
+void cfd2lcs_init_(int* comm, int* n, int* offset, lcsdata_t* x, lcsdata_t* y, lcsdata_t* z, int* flag) {
+  cfd2lcs_init_c((MPI_Comm)(comm ? *comm : 0), n, offset, x, y, z, flag);
+}
+void cfd2lcs_diagnostic_init_(int* handle, int* type, int* res, lcsdata_t* t, lcsdata_t* h, char* label,
+                              size_t namelen) {
+  std::string l = trim_label(label, namelen);
+  *handle = cfd2lcs_diagnostic_init_c(*type, *res, *t, *h, &l[0]);
+}
+void cfd2lcs_update_(int* n, lcsdata_t* u, lcsdata_t* v, lcsdata_t* w, lcsdata_t* time) {
+  cfd2lcs_update_c(n, u, v, w, *time);
+}
+void cfd2lcs_diagnostic_destroy_(int* handle) { cfd2lcs_diagnostic_destroy_c(*handle); }
+void cfd2lcs_set_option_(char* option, int* val, size_t namelen) {
+  std::string o = trim_label(option, namelen);
+  cfd2lcs_set_option_c(&o[0], *val);
+}
+void cfd2lcs_set_param_(char* param, lcsdata_t* val, size_t namelen) {
+  std::string p = trim_label(param, namelen);
+  cfd2lcs_set_param_c(&p[0], *val);
+}
+void cfd2lcs_finalize_(void) { cfd2lcs_finalize_c(); }
+
+}  // extern "C"

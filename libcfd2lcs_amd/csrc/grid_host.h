@@ -1,0 +1,46 @@
+// grid_host.h -- one-time host set-up of the structured grid (the reference's
+// init_sgrid on one rank) and of the B-spline collocation factors.  Runs once
+// at cfd2lcs_init / on the first spline use; the results are uploaded to HBM.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace lcs {
+
+struct HostGrid {
+  int n[3] = {0, 0, 0};
+  int N[3] = {0, 0, 0};
+  long long nt = 0, np = 0;
+  std::vector<double> g[3];      // ghosted coordinates with ghost/fake values
+  std::vector<int8_t> flag;      // ghosted bcflag
+  std::vector<double> bn[3];     // bcnorm at interior nodes
+  std::vector<double> dl[3];     // 1/delta at interior nodes
+  int per[3] = {0, 0, 0};
+  double ps[27][3] = {};
+  double bb[6] = {};
+  bool rect = false;
+  bool separable = false;        // g[d] depends on the d-th index only
+  int flat = -1;                 // axis with n == 1
+  std::vector<double> line[3];   // ghosted coordinate lines
+  std::vector<double> xl[3];     // interpolation lines (2-D override)
+
+  long long id(int i, int j, int k) const { return i + (long long)N[0] * (j + (long long)N[1] * k); }
+  long long pid(int i, int j, int k) const {
+    return (i - 1) + (long long)n[0] * ((j - 1) + (long long)n[1] * (k - 1));
+  }
+  // Builds the grid from the user's interior arrays (Fortran order).
+  // Returns an empty string on success, else the reason.
+  std::string build(const int nn[3], const double* x, const double* y, const double* z, const int* fl);
+};
+
+// Knots (dbknot rule) and the banded LU factors of the collocation matrix
+// (dbintk + dbnfac) of an order-k interpolating spline on abscissae x.
+struct SplineAxis {
+  int k = 0;
+  std::vector<double> knots;  // n + k
+  std::vector<double> band;   // (2k-1) * n, a(r,c) at (r - c + k - 1) + (2k-1)*c
+};
+std::string spline_axis(const std::vector<double>& x, int k, SplineAxis& out);
+
+}  // namespace lcs

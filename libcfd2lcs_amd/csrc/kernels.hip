@@ -1,0 +1,1120 @@
+// kernels.hip -- HIP kernels of the MI355X FTLE path (gfx950).
+//
+// Stage (a)  semi-Lagrangian backward flowmap step  k_sl_step
+//            forward particle step                  k_fwd_step
+// Stage (b)  composition of the T/h sub-intervals   k_compose
+// Stage (c)  gradient -> Cauchy-Green -> FTLE       k_ftle
+// plus velocity ingest / ghost shell / CFL reduction and the tensor B-spline
+// fit (three banded axis solves).  Reference semantics are cited per kernel.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "lcs_device.h"
+
+namespace lcs {
+
+// 27-neighbour order: self, faces, edges/corners (data_m.f90:123-152)
+__constant__ int8_t c_nbr[27][3] = {
+    {0, 0, 0},   {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},  {1, 0, 0},   {0, 1, 0},  {0, 0, 1},
+    {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {-1, 0, -1}, {1, 0, -1}, {-1, 1, -1}, {0, 1, -1},
+    {1, 1, -1},  {-1, -1, 0}, {1, -1, 0},  {-1, 1, 0},  {1, 1, 0},   {-1, -1, 1}, {0, -1, 1},
+    {1, -1, 1},  {-1, 0, 1},  {1, 0, 1},   {-1, 1, 1},  {0, 1, 1},   {1, 1, 1}};
+
+constexpr int kBlock = 256;
+
+template <typename T>
+__device__ __forceinline__ T dmin(T a, T b) { return fmin(a, b); }
+template <typename T>
+__device__ __forceinline__ T dmax(T a, T b) { return fmax(a, b); }
+// Fortran max(min(r,1),0); a NaN ratio (first update) gives 1 as with minNum
+template <typename T>
+__device__ __forceinline__ T clamp01(T r) { return dmax(dmin(r, T(1)), T(0)); }
+__device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <typename T>
+__device__ __forceinline__ int gid(const GridDev<T>& g, int i, int j, int k) {
+  return i + g.N[0] * (j + g.N[1] * k);
+}
+template <typename T>
+__device__ __forceinline__ void decode_interior(const GridDev<T>& g, int p, int& i, int& j, int& k) {
+  i = p % g.n[0] + 1;
+  int r = p / g.n[0];
+  j = r % g.n[1] + 1;
+  k = r / g.n[1] + 1;
+}
+template <typename T>
+__device__ __forceinline__ int pid(const GridDev<T>& g, int i, int j, int k) {
+  return (i - 1) + g.n[0] * ((j - 1) + g.n[1] * (k - 1));
+}
+template <typename T>
+__device__ __forceinline__ bool self_comm(const GridDev<T>& g, int ei, int ej, int ek) {
+  if (ei == 0 && ej == 0 && ek == 0) return false;
+  return (ei == 0 || g.per[0]) && (ej == 0 || g.per[1]) && (ek == 0 || g.per[2]);
+}
+// ghost shell enumeration: thread t -> ghost cell (i,j,k)
+template <typename T>
+__device__ __forceinline__ bool decode_ghost(const GridDev<T>& g, long long t, int& i, int& j, int& k) {
+  const long long plane = (long long)g.N[0] * g.N[1];
+  if (t < plane) { k = 0; i = (int)(t % g.N[0]); j = (int)(t / g.N[0]); return true; }
+  t -= plane;
+  if (t < plane) { k = g.N[2] - 1; i = (int)(t % g.N[0]); j = (int)(t / g.N[0]); return true; }
+  t -= plane;
+  const long long per_k = 2LL * g.N[0] + 2LL * g.n[1];
+  if (t >= per_k * g.n[2]) return false;
+  k = 1 + (int)(t / per_k);
+  int r = (int)(t % per_k);
+  if (r < g.N[0]) { j = 0; i = r; return true; }
+  r -= g.N[0];
+  if (r < g.N[0]) { j = g.N[1] - 1; i = r; return true; }
+  r -= g.N[0];
+  j = 1 + r / 2;
+  i = (r & 1) ? g.N[0] - 1 : 0;
+  return true;
+}
+template <typename T>
+__device__ __forceinline__ long long ghost_count(const GridDev<T>& g) {
+  return 2LL * g.N[0] * g.N[1] + (2LL * g.N[0] + 2LL * g.n[1]) * g.n[2];
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation (interp_s2u_r1, lp_tracking_m.f90:279-662)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LinW {
+  int base;
+  T t[3], mt[3];
+};
+
+// LINEAR bracket relative to the tracked node, local coordinate clamped to
+// [0,1] (lp_tracking_m.f90:381-396,438-448).  A bracket that would leave the
+// ghosted range is shifted inside.
+template <typename T>
+__device__ __forceinline__ void lin_setup(const GridDev<T>& g, const T q[3], const int nd[3], LinW<T>& w) {
+  int i0[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const T* xl = g.xl[d];
+    int n0 = iclamp(nd[d], 0, g.n[d] + 1);
+    int a = (q[d] > xl[n0]) ? n0 : n0 - 1;
+    a = iclamp(a, 0, g.n[d]);
+    T x0 = xl[a], x1 = xl[a + 1];
+    T tt = clamp01((q[d] - x0) / (x1 - x0));
+    w.t[d] = tt;
+    w.mt[d] = T(1) - tt;
+    i0[d] = a;
+  }
+  w.base = i0[0] + g.N[0] * (i0[1] + g.N[1] * i0[2]);
+}
+
+// trilinear sum in the reference's term order (lp_tracking_m.f90:449-457)
+template <typename T>
+__device__ __forceinline__ T trilin(const LinW<T>& w, T f0, T f1, T f2, T f3, T f4, T f5, T f6, T f7) {
+  return w.mt[0] * w.mt[1] * w.mt[2] * f0 + w.t[0] * w.mt[1] * w.mt[2] * f1 +
+         w.mt[0] * w.t[1] * w.mt[2] * f2 + w.t[0] * w.t[1] * w.mt[2] * f3 +
+         w.mt[0] * w.mt[1] * w.t[2] * f4 + w.t[0] * w.mt[1] * w.t[2] * f5 +
+         w.mt[0] * w.t[1] * w.t[2] * f6 + w.t[0] * w.t[1] * w.t[2] * f7;
+}
+
+// dintrv: largest 1-based i with t(i) <= x over t(1..L)
+template <typename T>
+__device__ __forceinline__ int knot_interval(const T* t, int L, T x) {
+  if (x >= t[L - 1]) return L;
+  int lo = 1, hi = L;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (x >= t[mid - 1]) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// B-spline basis of order K at x (de Boor's BSPLVB recurrence, as dbspvn,
+// bspline_sub_module.f90:2288-2370).  Returns 0 = ok, 1 = outside the knot
+// range (db3val returns 0 and reports), 2 = y/z at the last knot (db3val
+// returns 0 silently, bspline_sub_module.f90:634-636).
+template <typename T, int K>
+__device__ __forceinline__ int bs_setup(const T* t, int n, T x, bool xaxis, int& base, T v[K]) {
+  const int L = n + K;
+  if (!(x >= t[0]) || x > t[L - 1]) return 1;
+  int left = knot_interval(t, L, x);
+  if (left >= L) {
+    if (!xaxis) return 2;
+  }
+  if (left > n) left = n;  // dbvalu's left limit at x = t(n+1)
+  T dp[K], dm[K];
+  v[0] = T(1);
+#pragma unroll
+  for (int j = 1; j < K; j++) {
+    dp[j - 1] = t[left + j - 1] - x;
+    dm[j - 1] = x - t[left - j];
+    T vmprev = T(0);
+#pragma unroll
+    for (int l = 1; l <= j; l++) {
+      T vm = v[l - 1] / (dp[l - 1] + dm[j - l]);
+      v[l - 1] = vm * dp[l - 1] + vmprev;
+      vmprev = vm * dm[j - l];
+    }
+    v[j] = vmprev;
+  }
+  base = left - K;
+  return 0;
+}
+
+template <typename T, int KX, int KZ>
+struct SplineW {
+  int base;
+  T wx[KX], wy[KX], wz[KZ];
+};
+
+template <typename T, int KX, int KZ>
+__device__ __forceinline__ int spline_setup(const GridDev<T>& g, const T q[3], SplineW<T, KX, KZ>& s) {
+  int bx, by, bz;
+  int rx = bs_setup<T, KX>(g.kn[0], g.N[0], q[0], true, bx, s.wx);
+  int ry = bs_setup<T, KX>(g.kn[1], g.N[1], q[1], false, by, s.wy);
+  int rz = bs_setup<T, KZ>(g.kn[2], g.N[2], q[2], false, bz, s.wz);
+  if (rx == 1 || ry == 1 || rz == 1) return 1;
+  if (ry || rz) return 2;
+  s.base = bx + g.N[0] * (by + g.N[1] * bz);
+  return 0;
+}
+
+template <typename T, int KX, int KZ>
+__device__ __forceinline__ T spline_sum(const GridDev<T>& g, const SplineW<T, KX, KZ>& s, const T* __restrict__ c) {
+  const int sy = g.N[0], sz = g.N[0] * g.N[1];
+  T acc = T(0);
+#pragma unroll
+  for (int cz = 0; cz < KZ; cz++) {
+    T accy = T(0);
+#pragma unroll
+    for (int by = 0; by < KX; by++) {
+      const T* row = c + s.base + by * sy + cz * sz;
+      T accx = T(0);
+#pragma unroll
+      for (int a = 0; a < KX; a++) accx += s.wx[a] * row[a];
+      accy += s.wy[by] * accx;
+    }
+    acc += s.wz[cz] * accy;
+  }
+  return acc;
+}
+
+// ---- velocity interpolants: blend of u_n and u_np1 at weight ct ----
+template <typename T>
+struct VelNearest {
+  CF3<T> un, unp1;
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T ct, T out[3],
+                             unsigned long long&) const {
+    int l = gid(g, iclamp(nd[0], 0, g.n[0] + 1), iclamp(nd[1], 0, g.n[1] + 1), iclamp(nd[2], 0, g.n[2] + 1));
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c] = (T(1) - ct) * un.c[c][l] + ct * unp1.c[c][l];
+  }
+};
+
+template <typename T>
+struct VelLinear {
+  CF3<T> un, unp1;
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T ct, T out[3],
+                             unsigned long long&) const {
+    LinW<T> w;
+    lin_setup(g, q, nd, w);
+    const int b = w.base, sy = g.N[0], sz = g.N[0] * g.N[1];
+    const T a = T(1) - ct;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const T* A = un.c[c];
+      const T* B = unp1.c[c];
+#define BL(o) (a * A[b + (o)] + ct * B[b + (o)])
+      out[c] = trilin(w, BL(0), BL(1), BL(sy), BL(sy + 1), BL(sz), BL(sz + 1), BL(sz + sy), BL(sz + sy + 1));
+#undef BL
+    }
+  }
+};
+
+template <typename T, int KX, int KZ>
+struct VelSpline {
+  CF3<T> un, unp1;  // spline coefficients of u_n and u_np1
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T ct, T out[3],
+                             unsigned long long& oob) const {
+    SplineW<T, KX, KZ> s;
+    int r = spline_setup(g, q, s);
+    if (r) {
+      if (r == 1) oob++;
+      out[0] = out[1] = out[2] = T(0);
+      return;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      T vn = spline_sum(g, s, un.c[c]);
+      T vp = spline_sum(g, s, unp1.c[c]);
+      out[c] = (T(1) - ct) * vn + ct * vp;
+    }
+  }
+};
+
+// ---- single-field interpolants (flowmap) ----
+template <typename T>
+struct FieldNearest {
+  CF3<T> f;
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T out[3],
+                             unsigned long long&) const {
+    int l = gid(g, iclamp(nd[0], 0, g.n[0] + 1), iclamp(nd[1], 0, g.n[1] + 1), iclamp(nd[2], 0, g.n[2] + 1));
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c] = f.c[c][l];
+  }
+};
+
+template <typename T>
+struct FieldLinear {
+  CF3<T> f;
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T out[3],
+                             unsigned long long&) const {
+    LinW<T> w;
+    lin_setup(g, q, nd, w);
+    const int b = w.base, sy = g.N[0], sz = g.N[0] * g.N[1];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const T* A = f.c[c];
+      out[c] = trilin(w, A[b], A[b + 1], A[b + sy], A[b + sy + 1], A[b + sz], A[b + sz + 1], A[b + sz + sy],
+                      A[b + sz + sy + 1]);
+    }
+  }
+};
+
+template <typename T, int KX, int KZ>
+struct FieldSpline {
+  CF3<T> f;  // spline coefficients
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T out[3],
+                             unsigned long long& oob) const {
+    SplineW<T, KX, KZ> s;
+    int r = spline_setup(g, q, s);
+    if (r) {
+      if (r == 1) oob++;
+      out[0] = out[1] = out[2] = T(0);
+      return;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c] = spline_sum(g, s, f.c[c]);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Particle operations
+// ---------------------------------------------------------------------------
+
+// integrate_lp stages (lp_motion_m.f90:129-328); RK3 evaluates k3 at
+// xn + 2 k2 as the reference does (lp_motion_m.f90:239-247).
+template <typename T, class V>
+__device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams<T>& sp, const V& vel,
+                                          const int nd[3], const T xn[3], T xp[3], unsigned long long& oob) {
+  const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
+  const T dt = sp.dt;
+  T up[3], k[3], x1[3];
+  switch (sp.integ) {
+    case kEuler:
+    case kTrapezoidal:
+      vel(g, xp, nd, sp.ct[0], up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = xp[d] + dt * up[d];
+      break;
+    case kRK2:
+      vel(g, xp, nd, sp.ct[0], up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = xp[d] + HALF * dt * up[d];
+      vel(g, xp, nd, sp.ct[1], up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = xn[d] + dt * up[d];
+      break;
+    case kRK3:
+#pragma unroll
+      for (int d = 0; d < 3; d++) x1[d] = xn[d];
+      vel(g, xp, nd, sp.ct[0], up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        x1[d] = x1[d] + SIXTH * k[d];
+        xp[d] = xn[d] + HALF * k[d];
+      }
+      vel(g, xp, nd, sp.ct[1], up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        xp[d] = xn[d] + TWO * k[d];
+        x1[d] = x1[d] + TWOTHIRD * k[d];
+      }
+      vel(g, xp, nd, sp.ct[2], up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        xp[d] = x1[d] + SIXTH * k[d];
+      }
+      break;
+    case kRK4:
+    default:
+#pragma unroll
+      for (int d = 0; d < 3; d++) x1[d] = xn[d];
+      vel(g, xp, nd, sp.ct[0], up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        x1[d] = x1[d] + SIXTH * k[d];
+        xp[d] = xn[d] + HALF * k[d];
+      }
+      vel(g, xp, nd, sp.ct[1], up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        x1[d] = x1[d] + THIRD * k[d];
+        xp[d] = xn[d] + HALF * k[d];
+      }
+      vel(g, xp, nd, sp.ct[2], up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        x1[d] = x1[d] + THIRD * k[d];
+        xp[d] = xn[d] + k[d];
+      }
+      vel(g, xp, nd, sp.ct[3], up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        xp[d] = x1[d] + SIXTH * k[d];
+      }
+      break;
+  }
+}
+
+// project_lp2plane on a flat separable grid (lp_m.f90:418-538): the plane
+// normal is the collapsed axis, the node index on it becomes 1 and the
+// particle's coordinate loses its offset from the node plane.
+template <typename T>
+__device__ __forceinline__ void project_flat(const GridDev<T>& g, int nd[3], T q[3]) {
+  if (g.flat < 0) return;
+  const int f = g.flat;
+  nd[f] = 1;
+  q[f] = q[f] - (q[f] - g.gl[f][1]);
+}
+
+// set_lp_bc (lp_tracking_m.f90:185-277); returns true for LP_STICK
+template <typename T>
+__device__ __forceinline__ bool set_lp_bc(const GridDev<T>& g, int nd[3], T q[3], int& err) {
+  const int i = iclamp(nd[0], 1, g.n[0]), j = iclamp(nd[1], 1, g.n[1]), k = iclamp(nd[2], 1, g.n[2]);
+  nd[0] = i; nd[1] = j; nd[2] = k;
+  const int f = g.flag[gid(g, i, j, k)];
+  const bool inner = (i > 1 && j > 1 && k > 1 && i < g.n[0] && j < g.n[1] && k < g.n[2]);
+  switch (f) {
+    case kInternal:
+    case k2D:
+      return false;
+    case kSlip:
+    case kWall:
+      if (inner) return false;
+      // fall through
+    case kMask: {
+      const int p = pid(g, i, j, k);
+      T nrm[3] = {g.bn[0][p], g.bn[1][p], g.bn[2][p]};
+      T p2no[3] = {q[0] - g.gl[0][i], q[1] - g.gl[1][j], q[2] - g.gl[2][k]};
+      T dist = p2no[0] * nrm[0] + p2no[1] * nrm[1] + p2no[2] * nrm[2];
+      if (dist > T(0)) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) q[d] = q[d] - nrm[d] * dist;
+      }
+      return false;
+    }
+    case kInflow:
+    case kOutflow:
+      if (inner) return false;
+      q[0] = g.gl[0][i]; q[1] = g.gl[1][j]; q[2] = g.gl[2][k];
+      return true;
+    default:
+      err |= kErrBcFlag;
+      return false;
+  }
+}
+
+// One pass of the 27-neighbour nearest-node search with the 0.99
+// hysteresis (lp_tracking_m.f90:72-109).  Contraction is off so the squared
+// distances round exactly as the CPU does.
+template <typename T>
+__device__ __noinline__ bool track_pass(const GridDev<T>& g, const T q[3], int nd[3]) {
+#pragma clang fp contract(off)
+  const T huge = sizeof(T) == 8 ? T(DBL_MAX) : T(FLT_MAX);
+  T rmin = huge;
+  int oi = 0, oj = 0, ok = 0;
+  for (int nb = 0; nb < 27; nb++) {
+    const int di = c_nbr[nb][0], dj = c_nbr[nb][1], dk = c_nbr[nb][2];
+    if ((g.n[0] == 1 && di) || (g.n[1] == 1 && dj) || (g.n[2] == 1 && dk)) continue;
+    const int i = iclamp(nd[0] + di, 0, g.n[0] + 1);
+    const int j = iclamp(nd[1] + dj, 0, g.n[1] + 1);
+    const int k = iclamp(nd[2] + dk, 0, g.n[2] + 1);
+    const T a = g.gl[0][i] - q[0], b = g.gl[1][j] - q[1], c = g.gl[2][k] - q[2];
+    const T aa = a * a, bb2 = b * b, cc = c * c;
+    const T rsq = aa + bb2 + cc;
+    if (rsq < T(0.99) * rmin) {
+      rmin = rsq;
+      oi = di; oj = dj; ok = dk;
+    }
+  }
+  nd[0] += oi; nd[1] += oj; nd[2] += ok;
+  return oi != 0 || oj != 0 || ok != 0;
+}
+
+// exchange_lpdata on one rank (comms_m.f90:872-1189): periodic wrap of a
+// particle whose tracked node left [1,n].
+template <typename T>
+__device__ __forceinline__ void wrap(const GridDev<T>& g, int nd[3], T q[3]) {
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) c[d] = nd[d] < 1 ? -1 : (nd[d] > g.n[d] ? 1 : 0);
+  if (!self_comm(g, c[0], c[1], c[2])) return;
+  const int di = (c[0] + 1) + 3 * (c[1] + 1) + 9 * (c[2] + 1);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    q[d] = q[d] + g.ps[di][d];
+    int v = nd[d];
+    if (v > g.n[d]) v -= g.n[d];
+    if (v < 1) v += g.n[d];
+    nd[d] = v;
+  }
+}
+
+__device__ __forceinline__ void flush_status(DevStatus* st, int err, unsigned long long oob) {
+  if (err) atomicOr(&st->err, err);
+  if (oob) atomicAdd(&st->oob, oob);
+}
+
+// ---------------------------------------------------------------------------
+// Velocity ingest, ghost shell, CFL (cfd2lcs.f90:112-117, sgrid_m.f90:810-866,
+// lp_motion_m.f90:86-101)
+// ---------------------------------------------------------------------------
+template <typename U, typename T>
+__global__ void __launch_bounds__(kBlock) k_ingest(GridDev<T> g, F3<T> dst, const U* __restrict__ u,
+                                                   const U* __restrict__ v, const U* __restrict__ w) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  const int l = gid(g, i, j, k);
+  dst.c[0][l] = (T)u[p];
+  dst.c[1][l] = (T)v[p];
+  dst.c[2][l] = (T)w[p];
+}
+
+// ghost cells of u_np1: periodic copies (exchange_sdata) then
+// set_velocity_bc; an INTERNAL ghost without a periodic source keeps the
+// value it had (the previous u_np1, now held by uold).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_vel_ghost(GridDev<T> g, F3<T> u, CF3<T> uold, DevStatus* st) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (t >= ghost_count(g) || !decode_ghost(g, t, i, j, k)) return;
+  const int l = gid(g, i, j, k);
+  const int f = g.flag[l];
+  const int ei = i < 1 ? -1 : (i > g.n[0] ? 1 : 0), ej = j < 1 ? -1 : (j > g.n[1] ? 1 : 0),
+            ek = k < 1 ? -1 : (k > g.n[2] ? 1 : 0);
+  switch (f) {
+    case kInternal:
+      if (self_comm(g, ei, ej, ek)) {
+        const int s = gid(g, ei < 0 ? g.n[0] : (ei > 0 ? 1 : i), ej < 0 ? g.n[1] : (ej > 0 ? 1 : j),
+                          ek < 0 ? g.n[2] : (ek > 0 ? 1 : k));
+        for (int c = 0; c < 3; c++) u.c[c][l] = u.c[c][s];
+      } else {
+        for (int c = 0; c < 3; c++) u.c[c][l] = uold.c[c][l];
+      }
+      break;
+    case kMask:
+    case kWall:
+      for (int c = 0; c < 3; c++) u.c[c][l] = T(0);
+      break;
+    case kSlip:
+    case kInflow:
+    case kOutflow:
+    case k2D: {
+      const int s = gid(g, iclamp(i, 1, g.n[0]), iclamp(j, 1, g.n[1]), iclamp(k, 1, g.n[2]));
+      for (int c = 0; c < 3; c++) u.c[c][l] = u.c[c][s];
+      break;
+    }
+    default:
+      atomicOr(&st->err, (int)kErrBcFlag);
+  }
+}
+
+// MASK nodes get zero velocity; cfl = sum_d |u_d / delta_d| over interior
+// nodes, block max folded into st->cfl_bits (cfl >= 0, so the IEEE bit
+// pattern orders as an unsigned integer).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cfl(GridDev<T> g, F3<T> u, DevStatus* st) {
+  __shared__ double smax[kBlock / 64];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double c = 0.0;
+  if (p < g.np) {
+    int i, j, k;
+    decode_interior(g, p, i, j, k);
+    const int l = gid(g, i, j, k);
+    if (g.flag[l] == kMask) {
+      u.c[0][l] = T(0); u.c[1][l] = T(0); u.c[2][l] = T(0);
+    }
+    T s = T(0);
+    s = s + fabs(u.c[0][l] * g.dl[0][p]);
+    s = s + fabs(u.c[1][l] * g.dl[1][p]);
+    s = s + fabs(u.c[2][l] * g.dl[2][p]);
+    c = (double)s;
+  }
+  for (int o = 32; o > 0; o >>= 1) c = fmax(c, __shfl_down(c, o, 64));
+  if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = smax[0];
+    for (int w = 1; w < kBlock / 64; w++) m = fmax(m, smax[w]);
+    atomicMax(&st->cfl_bits, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ghost cells of a flowmap: periodic copies then set_flowmap_bc's ghost
+// pass (flowmap_m.f90:362-390).  Interior zeroing of MASK/INFLOW/OUTFLOW
+// nodes is done by the producer.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fm_ghost(GridDev<T> g, F3<T> fm, CF3<T> fmold, DevStatus* st) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (t >= ghost_count(g) || !decode_ghost(g, t, i, j, k)) return;
+  const int l = gid(g, i, j, k);
+  const int f = g.flag[l];
+  const int ei = i < 1 ? -1 : (i > g.n[0] ? 1 : 0), ej = j < 1 ? -1 : (j > g.n[1] ? 1 : 0),
+            ek = k < 1 ? -1 : (k > g.n[2] ? 1 : 0);
+  switch (f) {
+    case kInternal:
+      if (self_comm(g, ei, ej, ek)) {
+        const int s = gid(g, ei < 0 ? g.n[0] : (ei > 0 ? 1 : i), ej < 0 ? g.n[1] : (ej > 0 ? 1 : j),
+                          ek < 0 ? g.n[2] : (ek > 0 ? 1 : k));
+        for (int c = 0; c < 3; c++) fm.c[c][l] = fm.c[c][s];
+      } else {
+        for (int c = 0; c < 3; c++) fm.c[c][l] = fmold.c[c][l];
+      }
+      break;
+    case kWall:
+    case kMask:
+    case kInflow:
+    case kOutflow:
+      for (int c = 0; c < 3; c++) fm.c[c][l] = T(0);
+      break;
+    case kSlip:
+    case k2D: {
+      const int s = gid(g, iclamp(i, 1, g.n[0]), iclamp(j, 1, g.n[1]), iclamp(k, 1, g.n[2]));
+      for (int c = 0; c < 3; c++) fm.c[c][l] = fm.c[c][s];
+      break;
+    }
+    default:
+      atomicOr(&st->err, (int)kErrBcFlag);
+  }
+}
+
+// Ghosted flowmap from per-origin values: exchange_lpmap (fm(origin) = dx,
+// comms_m.f90:1340-1359), then exchange_sdata + set_flowmap_bc.  Periodic
+// ghosts copy the raw value, interior MASK/INFLOW/OUTFLOW nodes become 0,
+// INTERNAL ghosts without a periodic source hold the reset value 0.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_field_from_origin(GridDev<T> g, CF3<T> src, F3<T> dst) {
+  const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= g.nt) return;
+  const int i = (int)(l % g.N[0]);
+  const int r = (int)(l / g.N[0]);
+  const int j = r % g.N[1], k = r / g.N[1];
+  const int f = g.flag[l];
+  const bool ib = i >= 1 && j >= 1 && k >= 1 && i <= g.n[0] && j <= g.n[1] && k <= g.n[2];
+  if (ib) {
+    const int p = pid(g, i, j, k);
+    const bool z = (f == kMask || f == kInflow || f == kOutflow);
+    for (int c = 0; c < 3; c++) dst.c[c][l] = z ? T(0) : src.c[c][p];
+    return;
+  }
+  const int ei = i < 1 ? -1 : (i > g.n[0] ? 1 : 0), ej = j < 1 ? -1 : (j > g.n[1] ? 1 : 0),
+            ek = k < 1 ? -1 : (k > g.n[2] ? 1 : 0);
+  int sp = -1;
+  if (f == kInternal) {
+    if (self_comm(g, ei, ej, ek))
+      sp = pid(g, ei < 0 ? g.n[0] : (ei > 0 ? 1 : i), ej < 0 ? g.n[1] : (ej > 0 ? 1 : j),
+               ek < 0 ? g.n[2] : (ek > 0 ? 1 : k));
+  } else if (f == kSlip || f == k2D) {
+    sp = pid(g, iclamp(i, 1, g.n[0]), iclamp(j, 1, g.n[1]), iclamp(k, 1, g.n[2]));
+  }
+  for (int c = 0; c < 3; c++) dst.c[c][l] = sp >= 0 ? src.c[c][sp] : T(0);
+}
+
+// ---------------------------------------------------------------------------
+// Stage (a): backward semi-Lagrangian sub-cycle (flowmap_m.f90:17-104).
+// One thread per grid node: back-trace from the node with the blended
+// velocity (bracket of the origin node, lp_motion_m.f90:171-327), particle
+// BC, old flowmap at the departure point, displacement form write.
+// ---------------------------------------------------------------------------
+template <typename T, class V, class FM>
+__global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T> sp, V vel, FM fmi, F3<T> fmnew,
+                                                    DevStatus* st) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  int nd[3] = {i, j, k};
+  const T x0[3] = {g.gl[0][i], g.gl[1][j], g.gl[2][k]};
+  T xp[3] = {x0[0], x0[1], x0[2]};
+  unsigned long long oob = 0;
+  int err = 0;
+  integrate(g, sp, vel, nd, x0, xp, oob);
+  project_flat(g, nd, xp);
+  if (set_lp_bc(g, nd, xp, err)) {
+    xp[0] = x0[0]; xp[1] = x0[1]; xp[2] = x0[2];
+  }
+  int no[3] = {i, j, k};
+  T fmp[3];
+  fmi(g, xp, no, fmp, oob);
+  const int l = gid(g, i, j, k);
+  const int f = g.flag[l];
+  const T mask = (f != kMask) ? T(1) : T(0);
+  const bool zero = (f == kMask || f == kInflow || f == kOutflow);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    T v = (fmp[d] + (xp[d] - x0[d])) * mask;
+    fmnew.c[d][l] = zero ? T(0) : v;
+  }
+  flush_status(st, err, oob);
+}
+
+// ---------------------------------------------------------------------------
+// Stage (a'): forward particle sub-cycle (update_lp, lp_motion_m.f90:15-54):
+// integrate, BC, stick, dx accumulation, one tracking pass, periodic wrap.
+// ---------------------------------------------------------------------------
+template <typename T, class V>
+__global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T> sp, V vel, F3<T> xp, F3<T> dx,
+                                                     int16_t* __restrict__ ns0, int16_t* __restrict__ ns1,
+                                                     int16_t* __restrict__ ns2, DevStatus* st) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  T q[3] = {xp.c[0][p], xp.c[1][p], xp.c[2][p]};
+  int nd[3] = {ns0[p], ns1[p], ns2[p]};
+  const T xn[3] = {q[0], q[1], q[2]};
+  unsigned long long oob = 0;
+  int err = 0;
+  integrate(g, sp, vel, nd, xn, q, oob);
+  project_flat(g, nd, q);
+  if (set_lp_bc(g, nd, q, err)) {
+    q[0] = xn[0]; q[1] = xn[1]; q[2] = xn[2];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) dx.c[d][p] = dx.c[d][p] + (q[d] - xn[d]);
+  track_pass(g, q, nd);
+  wrap(g, nd, q);
+#pragma unroll
+  for (int d = 0; d < 3; d++) xp.c[d][p] = q[d];
+  ns0[p] = (int16_t)nd[0]; ns1[p] = (int16_t)nd[1]; ns2[p] = (int16_t)nd[2];
+  flush_status(st, err, oob);
+}
+
+// reset_lp for forward particles (lp_m.f90:228-294, resolution 0)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fwd_reset(GridDev<T> g, F3<T> xp, F3<T> dx, int16_t* ns0, int16_t* ns1,
+                                                      int16_t* ns2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  xp.c[0][p] = g.gl[0][i]; xp.c[1][p] = g.gl[1][j]; xp.c[2][p] = g.gl[2][k];
+  dx.c[0][p] = T(0); dx.c[1][p] = T(0); dx.c[2][p] = T(0);
+  ns0[p] = (int16_t)i; ns1[p] = (int16_t)j; ns2[p] = (int16_t)k;
+}
+
+// ---------------------------------------------------------------------------
+// Stage (b): composition of the sub-interval flowmaps
+// (reconstruct_flowmap, flowmap_m.f90:158-323).  One thread per origin node
+// follows its trajectory through the slots in order: BC, interpolate the slot
+// displacement, relocate (exchange_lp_alltoall, comms_m.f90:1378-1709: box /
+// periodic image / node guess) and track recursively with a periodic wrap
+// after each pass (the direction is FWD during reconstruction).
+// ---------------------------------------------------------------------------
+template <typename T, class FM>
+__global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, FM fmi, const T* __restrict__ slots,
+                                                    long long stride, const int* __restrict__ order, int nslot,
+                                                    F3<T> dxout, DevStatus* st) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  int nd[3] = {i, j, k};
+  T q[3] = {g.gl[0][i], g.gl[1][j], g.gl[2][k]};
+  T dxx[3] = {T(0), T(0), T(0)};
+  unsigned long long oob = 0;
+  int err = 0;
+  for (int s = 0; s < nslot; s++) {
+    const T* base = slots + (long long)order[s] * 3 * stride;
+    FM f = fmi;
+    f.f.c[0] = base;
+    f.f.c[1] = base + stride;
+    f.f.c[2] = base + 2 * stride;
+    project_flat(g, nd, q);
+    set_lp_bc(g, nd, q, err);
+    T up[3];
+    f(g, q, nd, up, oob);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      q[d] = q[d] + up[d];
+      dxx[d] = dxx[d] + up[d];
+    }
+    // relocation to the bounding box or its first periodic image
+    const T* bb = g.bb;
+    bool in = q[0] >= bb[0] && q[0] <= bb[3] && q[1] >= bb[1] && q[1] <= bb[4] && q[2] >= bb[2] && q[2] <= bb[5];
+    if (!in) {
+      for (int c = 0; c < 27 && !in; c++) {
+        const T y0 = q[0] - g.ps[c][0], y1 = q[1] - g.ps[c][1], y2 = q[2] - g.ps[c][2];
+        if (y0 >= bb[0] && y0 <= bb[3] && y1 >= bb[1] && y1 <= bb[4] && y2 >= bb[2] && y2 <= bb[5]) {
+          q[0] = y0; q[1] = y1; q[2] = y2;
+          in = true;
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      T sv = clamp01((q[d] - bb[d]) / (bb[3 + d] - bb[d]));
+      T v = (T)(float)g.n[d] * sv;
+      nd[d] = iclamp((int)round(v), 1, g.n[d]);
+    }
+    for (int pass = 1;; pass++) {
+      bool moved = track_pass(g, q, nd);
+      if (pass > g.max_track) {
+        err |= kErrTrack;
+        break;
+      }
+      wrap(g, nd, q);
+      if (!moved) break;
+    }
+  }
+  dxout.c[0][p] = dxx[0];
+  dxout.c[1][p] = dxx[1];
+  dxout.c[2][p] = dxx[2];
+  flush_status(st, err, oob);
+}
+
+// ---------------------------------------------------------------------------
+// Stage (c): central gradient of the displacement flowmap (grad_sr1,
+// gradient_m.f90:11-66, with the periodic face refresh of lcs_m.f90:128),
+// C = G^T G (no identity term, lcs_m.f90:137-145), largest eigenvalue by
+// cyclic Jacobi in FP64, ftle = log(max(sqrt(lmax), tiny))/T
+// (lcs_m.f90:168-185).  Gradient, tensor and eigen-solve run in FP64 for
+// both storage precisions.
+// ---------------------------------------------------------------------------
+__device__ double eig_sym3_max(double a00, double a01, double a02, double a11, double a12, double a22) {
+  double a[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
+  for (int sweep = 0; sweep < 60; sweep++) {
+    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    const double dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+    if (off == 0.0 || off <= 1e-36 * dia) break;
+#pragma unroll
+    for (int pq = 0; pq < 3; pq++) {
+      const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+      const double apq = a[p][q];
+      if (apq == 0.0) continue;
+      const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+      double t;
+      if (fabs(theta) > 1e150)
+        t = 0.5 / theta;
+      else
+        t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        const double arp = a[r][p], arq = a[r][q];
+        a[r][p] = c * arp - s * arq;
+        a[r][q] = s * arp + c * arq;
+      }
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        const double apr = a[p][r], aqr = a[q][r];
+        a[p][r] = c * apr - s * aqr;
+        a[q][r] = s * apr + c * aqr;
+      }
+      a[p][q] = 0.0;
+      a[q][p] = 0.0;
+    }
+  }
+  return fmax(a[0][0], fmax(a[1][1], a[2][2]));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_ftle(GridDev<T> g, CF3<T> fm, double Tint, int incomp, double tiny,
+                                                 double* __restrict__ ftle, F3<T> comp) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  const int l = gid(g, i, j, k);
+  const int ip = (i == g.n[0] && g.per[0]) ? gid(g, 1, j, k) : gid(g, i + 1, j, k);
+  const int im = (i == 1 && g.per[0]) ? gid(g, g.n[0], j, k) : gid(g, i - 1, j, k);
+  const int jp = (j == g.n[1] && g.per[1]) ? gid(g, i, 1, k) : gid(g, i, j + 1, k);
+  const int jm = (j == 1 && g.per[1]) ? gid(g, i, g.n[1], k) : gid(g, i, j - 1, k);
+  const int kp = (k == g.n[2] && g.per[2]) ? gid(g, i, j, 1) : gid(g, i, j, k + 1);
+  const int km = (k == 1 && g.per[2]) ? gid(g, i, j, g.n[2]) : gid(g, i, j, k - 1);
+  const double hx = (double)g.gl[0][i + 1] - (double)g.gl[0][i - 1];
+  const double hy = (double)g.gl[1][j + 1] - (double)g.gl[1][j - 1];
+  const double hz = (double)g.gl[2][k + 1] - (double)g.gl[2][k - 1];
+  double G[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const T* F = fm.c[c];
+    G[c][0] = ((double)F[ip] - (double)F[im]) / hx;
+    G[c][1] = ((double)F[jp] - (double)F[jm]) / hy;
+    G[c][2] = ((double)F[kp] - (double)F[km]) / hz;
+    comp.c[c][p] = F[l];
+  }
+  if (g.n[0] == 1) { G[0][0] = 0; G[1][0] = 0; G[2][0] = 0; }
+  if (g.n[1] == 1) { G[0][1] = 0; G[1][1] = 0; G[2][1] = 0; }
+  if (g.n[2] == 1) { G[0][2] = 0; G[1][2] = 0; G[2][2] = 0; }
+  const double cxx = G[0][0] * G[0][0] + G[1][0] * G[1][0] + G[2][0] * G[2][0];
+  const double cxy = G[0][0] * G[0][1] + G[1][0] * G[1][1] + G[2][0] * G[2][1];
+  const double cxz = G[0][0] * G[0][2] + G[1][0] * G[1][2] + G[2][0] * G[2][2];
+  const double cyy = G[0][1] * G[0][1] + G[1][1] * G[1][1] + G[2][1] * G[2][1];
+  const double cyz = G[0][1] * G[0][2] + G[1][1] * G[1][2] + G[2][1] * G[2][2];
+  const double czz = G[0][2] * G[0][2] + G[1][2] * G[1][2] + G[2][2] * G[2][2];
+  const double lam = eig_sym3_max(cxx, cxy, cxz, cyy, cyz, czz);
+  double r = log(fmax(sqrt(lam), tiny)) / Tint;
+  if (incomp) r = fmax(r, 0.0);
+  ftle[p] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Tensor B-spline fit (db3ink -> dbtpcf x3, bspline_sub_module.f90:451-533,
+// 1801-1856): for every line of the ghosted block along one axis, solve the
+// banded collocation system with the LU factors computed on the host
+// (dbintk/dbnfac).  Forward and backward substitution in the order of
+// dbnslv (bspline_sub_module.f90:2202-2259), in place.
+// ---------------------------------------------------------------------------
+template <typename T, int K>
+__global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restrict__ W, int n, int axis, int N0,
+                                                     int N1, int nlines) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int comp = blockIdx.y;
+  if (id >= nlines) return;
+  long long start, stride;
+  if (axis == 0) {
+    start = (long long)N0 * id; stride = 1;
+  } else if (axis == 1) {
+    const int i = id % N0, kk = id / N0;
+    start = i + (long long)N0 * N1 * kk; stride = N0;
+  } else {
+    start = id; stride = (long long)N0 * N1;
+  }
+  T* b = a.c[comp] + start;
+  constexpr int M = K - 1, NB = 2 * K - 1;
+  T win[M > 0 ? M : 1];  // win[0] = b(r-1), win[1] = b(r-2), ...
+#pragma unroll
+  for (int m = 0; m < M; m++) win[m] = T(0);
+  for (int r = 0; r < n; r++) {
+    T v = b[r * stride];
+#pragma unroll
+    for (int jj = M; jj >= 1; jj--)
+      if (r - jj >= 0) v = v - win[jj - 1] * W[(jj + M) + NB * (r - jj)];
+#pragma unroll
+    for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
+    win[0] = v;
+    b[r * stride] = v;
+  }
+#pragma unroll
+  for (int m = 0; m < M; m++) win[m] = T(0);  // win[0] = b(r+1), ...
+  for (int r = n - 1; r >= 0; r--) {
+    T v = b[r * stride];
+#pragma unroll
+    for (int jj = M; jj >= 1; jj--)
+      if (r + jj <= n - 1) v = v - win[jj - 1] * W[(M - jj) + NB * (r + jj)];
+    v = v / W[M + NB * r];
+#pragma unroll
+    for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
+    win[0] = v;
+    b[r * stride] = v;
+  }
+}
+
+// interior (or per-origin) values to a user-typed array, Fortran order
+template <typename T, typename U>
+__global__ void __launch_bounds__(kBlock) k_to_user(GridDev<T> g, CF3<T> s, int ghosted, U* a, U* b, U* c) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int l = p;
+  if (ghosted) {
+    int i, j, k;
+    decode_interior(g, p, i, j, k);
+    l = gid(g, i, j, k);
+  }
+  a[p] = (U)s.c[0][l];
+  b[p] = (U)s.c[1][l];
+  c[p] = (U)s.c[2][l];
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+static inline dim3 grid_for(long long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+template <typename U, typename T>
+void launch_ingest(const GridDev<T>& g, F3<T> dst, const U* u, const U* v, const U* w, hipStream_t s) {
+  hipLaunchKernelGGL((k_ingest<U, T>), grid_for(g.np), dim3(kBlock), 0, s, g, dst, u, v, w);
+}
+
+template <typename T>
+void launch_vel_ghost_impl(const GridDev<T>& g, F3<T> u, CF3<T> uold, DevStatus* st, hipStream_t s) {
+  long long ng = 2LL * g.N[0] * g.N[1] + (2LL * g.N[0] + 2LL * g.n[1]) * g.n[2];
+  hipLaunchKernelGGL((k_vel_ghost<T>), grid_for(ng), dim3(kBlock), 0, s, g, u, uold, st);
+}
+
+template <typename T>
+void launch_cfl(const GridDev<T>& g, F3<T> u, DevStatus* st, hipStream_t s) {
+  hipLaunchKernelGGL((k_cfl<T>), grid_for(g.np), dim3(kBlock), 0, s, g, u, st);
+}
+
+template <typename T>
+void launch_fm_ghost_impl(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st, hipStream_t s) {
+  long long ng = 2LL * g.N[0] * g.N[1] + (2LL * g.N[0] + 2LL * g.n[1]) * g.n[2];
+  hipLaunchKernelGGL((k_fm_ghost<T>), grid_for(ng), dim3(kBlock), 0, s, g, fm, fmold, st);
+}
+
+template <typename T>
+void launch_field_from_origin(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStream_t s) {
+  hipLaunchKernelGGL((k_field_from_origin<T>), grid_for(g.nt), dim3(kBlock), 0, s, g, src, dst);
+}
+
+// fits in place: coef holds the data on entry and the coefficients on exit
+template <typename T>
+void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t s) {
+  for (int axis = 0; axis < 3; axis++) {
+    const int n = g.N[axis];
+    const int nlines = (int)(g.nt / n);
+    dim3 grid((nlines + kBlock - 1) / kBlock, ncomp);
+    switch (g.kord[axis]) {
+      case 2: hipLaunchKernelGGL((k_fit_axis<T, 2>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+      case 3: hipLaunchKernelGGL((k_fit_axis<T, 3>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+      default: hipLaunchKernelGGL((k_fit_axis<T, 4>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+    }
+  }
+}
+
+// dispatch over interpolator and spline orders (KX = orders of x and y, KZ of z)
+template <typename T, class F>
+bool dispatch_spline(const GridDev<T>& g, F&& f) {
+  const int kx = g.kord[0], kz = g.kord[2];
+  if (g.kord[1] != kx) return false;
+  if (kx == 4 && kz == 4) { f(std::integral_constant<int, 4>{}, std::integral_constant<int, 4>{}); return true; }
+  if (kx == 4 && kz == 2) { f(std::integral_constant<int, 4>{}, std::integral_constant<int, 2>{}); return true; }
+  if (kx == 3 && kz == 3) { f(std::integral_constant<int, 3>{}, std::integral_constant<int, 3>{}); return true; }
+  if (kx == 3 && kz == 2) { f(std::integral_constant<int, 3>{}, std::integral_constant<int, 2>{}); return true; }
+  return false;
+}
+
+template <typename T>
+void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> un, CF3<T> unp1,
+                    CF3<T> fmold, F3<T> fmnew, DevStatus* st, hipStream_t s) {
+  const dim3 grid = grid_for(g.np), blk(kBlock);
+  if (interp == kLinear) {
+    hipLaunchKernelGGL((k_sl_step<T, VelLinear<T>, FieldLinear<T>>), grid, blk, 0, s, g, sp,
+                       VelLinear<T>{un, unp1}, FieldLinear<T>{fmold}, fmnew, st);
+  } else if (interp == kNearest) {
+    hipLaunchKernelGGL((k_sl_step<T, VelNearest<T>, FieldNearest<T>>), grid, blk, 0, s, g, sp,
+                       VelNearest<T>{un, unp1}, FieldNearest<T>{fmold}, fmnew, st);
+  } else {
+    dispatch_spline(g, [&](auto KX, auto KZ) {
+      constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
+      hipLaunchKernelGGL((k_sl_step<T, VelSpline<T, kx, kz>, FieldSpline<T, kx, kz>>), grid, blk, 0, s, g, sp,
+                         VelSpline<T, kx, kz>{un, unp1}, FieldSpline<T, kx, kz>{fmold}, fmnew, st);
+    });
+  }
+}
+
+template <typename T>
+void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> un, CF3<T> unp1, F3<T> xp,
+                     F3<T> dx, int16_t* ns0, int16_t* ns1, int16_t* ns2, DevStatus* st, hipStream_t s) {
+  const dim3 grid = grid_for(g.np), blk(kBlock);
+  if (interp == kLinear) {
+    hipLaunchKernelGGL((k_fwd_step<T, VelLinear<T>>), grid, blk, 0, s, g, sp, VelLinear<T>{un, unp1}, xp, dx, ns0,
+                       ns1, ns2, st);
+  } else if (interp == kNearest) {
+    hipLaunchKernelGGL((k_fwd_step<T, VelNearest<T>>), grid, blk, 0, s, g, sp, VelNearest<T>{un, unp1}, xp, dx,
+                       ns0, ns1, ns2, st);
+  } else {
+    dispatch_spline(g, [&](auto KX, auto KZ) {
+      constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
+      hipLaunchKernelGGL((k_fwd_step<T, VelSpline<T, kx, kz>>), grid, blk, 0, s, g, sp,
+                         VelSpline<T, kx, kz>{un, unp1}, xp, dx, ns0, ns1, ns2, st);
+    });
+  }
+}
+
+template <typename T>
+void launch_fwd_reset(const GridDev<T>& g, F3<T> xp, F3<T> dx, int16_t* ns0, int16_t* ns1, int16_t* ns2,
+                      hipStream_t s) {
+  hipLaunchKernelGGL((k_fwd_reset<T>), grid_for(g.np), dim3(kBlock), 0, s, g, xp, dx, ns0, ns1, ns2);
+}
+
+template <typename T>
+void launch_compose(const GridDev<T>& g, int interp, const T* slots, long long stride, const int* order, int nslot,
+                    F3<T> dxout, DevStatus* st, hipStream_t s) {
+  const dim3 grid = grid_for(g.np), blk(kBlock);
+  CF3<T> none{{nullptr, nullptr, nullptr}};
+  if (interp == kLinear) {
+    hipLaunchKernelGGL((k_compose<T, FieldLinear<T>>), grid, blk, 0, s, g, FieldLinear<T>{none}, slots, stride,
+                       order, nslot, dxout, st);
+  } else if (interp == kNearest) {
+    hipLaunchKernelGGL((k_compose<T, FieldNearest<T>>), grid, blk, 0, s, g, FieldNearest<T>{none}, slots, stride,
+                       order, nslot, dxout, st);
+  } else {
+    dispatch_spline(g, [&](auto KX, auto KZ) {
+      constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
+      hipLaunchKernelGGL((k_compose<T, FieldSpline<T, kx, kz>>), grid, blk, 0, s, g, FieldSpline<T, kx, kz>{none},
+                         slots, stride, order, nslot, dxout, st);
+    });
+  }
+}
+
+template <typename T>
+void launch_ftle(const GridDev<T>& g, CF3<T> fm, double T_int, int incompressible, double tiny, double* ftle,
+                 F3<T> comp, hipStream_t s) {
+  hipLaunchKernelGGL((k_ftle<T>), grid_for(g.np), dim3(kBlock), 0, s, g, fm, T_int, incompressible, tiny, ftle, comp);
+}
+
+template <typename T, typename U>
+void launch_interior_to_user(const GridDev<T>& g, CF3<T> src, int ghosted, U* a, U* b, U* c, hipStream_t s) {
+  hipLaunchKernelGGL((k_to_user<T, U>), grid_for(g.np), dim3(kBlock), 0, s, g, src, ghosted, a, b, c);
+}
+
+template <typename T>
+void launch_vel_ghost(const GridDev<T>& g, F3<T> u, CF3<T> uold, DevStatus* st, hipStream_t s) {
+  launch_vel_ghost_impl(g, u, uold, st, s);
+}
+template <typename T>
+void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st, hipStream_t s) {
+  launch_fm_ghost_impl(g, fm, fmold, st, s);
+}
+
+// explicit instantiations
+#define INST_T(T)                                                                                             \
+  template void launch_vel_ghost<T>(const GridDev<T>&, F3<T>, CF3<T>, DevStatus*, hipStream_t);               \
+  template void launch_cfl<T>(const GridDev<T>&, F3<T>, DevStatus*, hipStream_t);                             \
+  template void launch_fm_ghost<T>(const GridDev<T>&, F3<T>, CF3<T>, DevStatus*, hipStream_t);                \
+  template void launch_field_from_origin<T>(const GridDev<T>&, CF3<T>, F3<T>, hipStream_t);                   \
+  template void launch_spline_fit<T>(const GridDev<T>&, F3<T>, int, hipStream_t);                             \
+  template void launch_sl_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, CF3<T>, CF3<T>,     \
+                                  F3<T>, DevStatus*, hipStream_t);                                            \
+  template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, CF3<T>, F3<T>,     \
+                                   F3<T>, int16_t*, int16_t*, int16_t*, DevStatus*, hipStream_t);             \
+  template void launch_fwd_reset<T>(const GridDev<T>&, F3<T>, F3<T>, int16_t*, int16_t*, int16_t*,            \
+                                    hipStream_t);                                                             \
+  template void launch_compose<T>(const GridDev<T>&, int, const T*, long long, const int*, int, F3<T>,       \
+                                  DevStatus*, hipStream_t);                                                   \
+  template void launch_ftle<T>(const GridDev<T>&, CF3<T>, double, int, double, double*, F3<T>, hipStream_t); \
+  template void launch_ingest<float, T>(const GridDev<T>&, F3<T>, const float*, const float*, const float*,  \
+                                        hipStream_t);                                                         \
+  template void launch_ingest<double, T>(const GridDev<T>&, F3<T>, const double*, const double*,             \
+                                         const double*, hipStream_t);                                         \
+  template void launch_interior_to_user<T, float>(const GridDev<T>&, CF3<T>, int, float*, float*, float*,    \
+                                                  hipStream_t);                                               \
+  template void launch_interior_to_user<T, double>(const GridDev<T>&, CF3<T>, int, double*, double*,         \
+                                                   double*, hipStream_t);
+
+INST_T(float)
+INST_T(double)
+
+}  // namespace lcs

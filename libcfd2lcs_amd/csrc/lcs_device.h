@@ -1,0 +1,100 @@
+// lcs_device.h -- device-side data descriptors and launcher interface of the
+// MI355X FTLE path.  The grid is one GPU's whole domain with a one-cell ghost
+// shell (the reference's ng = 1, data_m.f90:106); fields are SoA, i fastest:
+//   ghosted index  l = i + N0*(j + N1*k),  i in [0, n0+1]   (Fortran i)
+//   interior index p = (i-1) + n0*((j-1) + n1*(k-1))        (lcs_ijk2l - 1)
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace lcs {
+
+enum BcFlag { kInternal = 0, kInflow = 1, kOutflow = 2, kWall = 3, kSlip = 4, k2D = 5, kMask = 6 };
+enum Integrator { kEuler = 0, kTrapezoidal = 1, kRK2 = 2, kRK3 = 3, kRK4 = 4 };
+enum Interpolator { kNearest = 0, kLinear = 1, kQuadratic = 2, kCubic = 3 };
+
+// device error bits (folded into the sticky error flag on the host)
+enum DevErr { kErrBcFlag = 1, kErrTrack = 2, kErrSpline = 4 };
+
+// Grid description passed by value to every kernel.
+template <typename T>
+struct GridDev {
+  int n[3];          // interior extents
+  int N[3];          // ghosted extents (n + 2)
+  long long nt, np;  // ghosted / interior sizes
+  const T* gl[3];    // ghosted coordinate lines (grid%x(:,1,1) ...), N[d] each
+  const T* xl[3];    // interpolation lines, 2-D override (lp_tracking_m.f90:362-378)
+  const int8_t* flag;  // ghosted bcflag
+  const T* bn[3];    // bcnorm at interior nodes (np)
+  const T* dl[3];    // 1/delta at interior nodes (np)
+  int per[3];        // periodic axes
+  int flat;          // the axis with n == 1, or -1
+  int max_track;     // 2*max(gn) (lp_tracking_m.f90:127)
+  T ps[27][3];       // periodic_shift(i,j,k,:) at (i+1)+3(j+1)+9(k+1)
+  T bb[6];           // ghost-inclusive bounding box
+  int kord[3];       // spline orders
+  const T* kn[3];    // spline knots, N[d] + kord[d] each
+  const T* band[3];  // banded LU of the collocation matrices, (2k-1)*N[d]
+};
+
+template <typename T>
+struct F3 {
+  T* c[3];
+};
+template <typename T>
+struct CF3 {
+  const T* c[3];
+};
+
+struct DevStatus {
+  unsigned long long cfl_bits;  // max CFL over interior nodes (as ordered bits)
+  unsigned long long oob;       // spline evaluations outside the knot range
+  int err;                      // DevErr bits
+  int pad;
+};
+
+// Per-subcycle integration parameters (lp_motion_m.f90:129-360).
+template <typename T>
+struct StageParams {
+  int integ;   // Integrator
+  T dt;        // signed sub-cycle step
+  T ct[4];     // time-blend weights of the stages, clamped to [0,1]
+};
+
+// ---- launchers (kernels.hip); all asynchronous on `stream` ----
+template <typename U, typename T>
+void launch_ingest(const GridDev<T>& g, F3<T> dst, const U* u, const U* v, const U* w,
+                   hipStream_t stream);
+template <typename T>
+void launch_vel_ghost(const GridDev<T>& g, F3<T> u, CF3<T> uold, DevStatus* st, hipStream_t stream);
+template <typename T>
+void launch_cfl(const GridDev<T>& g, F3<T> u, DevStatus* st, hipStream_t stream);
+template <typename T>
+void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st, hipStream_t stream);
+template <typename T>
+void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t stream);
+template <typename T>
+void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> un,
+                    CF3<T> unp1, CF3<T> fmold, F3<T> fmnew, DevStatus* st, hipStream_t stream);
+template <typename T>
+void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> un,
+                     CF3<T> unp1, F3<T> xp, F3<T> dx, int16_t* ns0, int16_t* ns1, int16_t* ns2,
+                     DevStatus* st, hipStream_t stream);
+template <typename T>
+void launch_fwd_reset(const GridDev<T>& g, F3<T> xp, F3<T> dx, int16_t* ns0, int16_t* ns1,
+                      int16_t* ns2, hipStream_t stream);
+template <typename T>
+void launch_field_from_origin(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStream_t stream);
+template <typename T>
+void launch_compose(const GridDev<T>& g, int interp, const T* slots, long long slot_stride,
+                    const int* order, int nslot, F3<T> dxout, DevStatus* st, hipStream_t stream);
+template <typename T>
+void launch_ftle(const GridDev<T>& g, CF3<T> fm, double T_int, int incompressible,
+                 double tiny, double* ftle, F3<T> comp, hipStream_t stream);
+template <typename T, typename U>
+void launch_interior_to_user(const GridDev<T>& g, CF3<T> src, int ghosted, U* a, U* b, U* c,
+                             hipStream_t stream);
+
+}  // namespace lcs

@@ -1,0 +1,80 @@
+"""Shared scenario driver: run the same sequence of API calls on the oracle
+and on the GPU library and collect what each exposes."""
+import numpy as np
+
+import flows as F
+from oracle_api import Oracle
+
+
+def make_case(name, n=None):
+    if name == "abc":
+        n = n or 16
+        X, Y, Z, fl = F.abc_grid(n)
+        vel = lambda t: F.abc_velocity(X, Y, Z)
+    elif name == "gyre":
+        ni, nj = n or (64, 32)
+        X, Y, Z, fl = F.double_gyre_grid(ni, nj)
+        vel = lambda t: F.double_gyre_velocity(X, Y, t)
+    elif name == "gyre_slip":
+        ni, nj = n or (64, 32)
+        X, Y, Z, fl = F.double_gyre_grid(ni, nj, walls=False)
+        fl[0, :, :] = F.LCS_SLIP
+        fl[-1, :, :] = F.LCS_SLIP
+        fl[:, 0, :] = F.LCS_SLIP
+        fl[:, -1, :] = F.LCS_SLIP
+        vel = lambda t: F.double_gyre_velocity(X, Y, t)
+    else:
+        raise ValueError(name)
+    return X, Y, Z, fl, vel
+
+
+def run_both(lib, case, n=None, integrator=F.RK2, interpolator=F.LINEAR, T=1.0, h=0.25, dt=0.05,
+             nsteps=30, diags=(F.FTLE_FWD, F.FTLE_BKWD), cfl=None, every_step=None):
+    """Returns a list of per-step records {step, fm: {handle: (gpu, oracle)}, ftle ...}."""
+    X, Y, Z, fl, vel = make_case(case, n)
+    o = Oracle(X, Y, Z, fl)
+    lib.finalize()
+    lib.init(X, Y, Z, fl)
+    handles = []
+    for d in diags:
+        label = "fwd" if d == F.FTLE_FWD else "bkwd"
+        ho = o.diagnostic_init(d, 0, T, h, label)
+        hg = lib.diagnostic_init(d, 0, T, h, label)
+        assert ho == hg
+        handles.append(hg)
+    for name, val in (("INTEGRATOR", integrator), ("INTERPOLATOR", interpolator)):
+        o.set_option(name, val)
+        lib.set_option(name, val)
+    if cfl is not None:
+        o.set_param("CFL", cfl)
+        lib.set_param("CFL", cfl)
+    recs = []
+    prev = {hd: 0 for hd in handles}
+    for s in range(nsteps):
+        t = s * dt
+        u, v, w = vel(t)
+        o.update(u, v, w, t)
+        lib.update(u, v, w, t)
+        assert o.error == 0, f"oracle error at step {s}"
+        assert lib.error == 0, f"library error at step {s}"
+        rec = {"step": s, "t": t, "fm": {}, "out": {}}
+        if every_step is None or s % every_step == 0 or s == nsteps - 1:
+            for hd in handles:
+                rec["fm"][hd] = (lib.flowmap(hd), o.flowmap(hd))
+        for hd in handles:
+            if lib.noutput(hd) != o.noutput(hd):
+                raise AssertionError(f"output count differs at step {s}: {lib.noutput(hd)} vs {o.noutput(hd)}")
+        for hd in handles:
+            if o.noutput(hd) > prev[hd]:
+                rec["out"][hd] = (lib.composed(hd), o.composed(hd), lib.ftle(hd), o.ftle(hd))
+                prev[hd] = o.noutput(hd)
+        recs.append(rec)
+    o.close()
+    return recs, handles
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)

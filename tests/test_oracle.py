@@ -1,0 +1,268 @@
+"""Pins the FP64 oracle (oracle/oracle.c) to what the mathematics fixes, since
+the reference ships no tests, golden vectors or fixtures for this path:
+
+* B-spline knots/fit/evaluation against scipy.interpolate with the same knot
+  vector (dbknot rule, bspline_sub_module.f90:1737-1785) and against
+  polynomial reproduction;
+* the 3x3 eigen-solve and C = G^T G against numpy;
+* integrators, the semi-Lagrangian step, forward particles, composition and
+  FTLE against closed forms for uniform and linear-strain flows, including the
+  reference's RK3 (k3 at xn + 2 k2, lp_motion_m.f90:239-247), whose
+  amplification factor differs from classical RK3;
+* grid set-up (ghost/fake coordinates, periodic shifts, normals, 1/delta).
+"""
+import numpy as np
+import pytest
+from scipy.interpolate import BSpline, NdBSpline, make_interp_spline
+
+import flows as F
+import oracle_api as OA
+from oracle_api import Oracle
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    OA.lib()
+
+
+# ---------------------------------------------------------------- B-splines
+def dbknot_rule(x, k):
+    """The knot rule stated in bspline_sub_module.f90:1737-1785."""
+    n = len(x)
+    t = np.empty(n + k)
+    t[:k] = x[0]
+    t[n:] = x[-1] + 0.1 * (x[-1] - x[-2])
+    if k % 2:
+        for j in range(k + 1, n + 1):
+            m = j + (k - 1) // 2 - k
+            t[j - 1] = 0.5 * (x[m - 1] + x[m])
+    else:
+        for j in range(k + 1, n + 1):
+            t[j - 1] = x[j + k // 2 - k - 1]
+    return t
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_knots_match_rule(k):
+    x = np.cumsum(np.random.default_rng(1).uniform(0.5, 1.5, 11))
+    np.testing.assert_array_equal(OA.bspline_knots(x, k), dbknot_rule(x, k))
+
+
+@pytest.mark.parametrize("k", [3, 4])
+def test_fit3_matches_scipy(k):
+    rng = np.random.default_rng(7)
+    x = np.cumsum(rng.uniform(0.5, 1.5, 9))
+    y = np.cumsum(rng.uniform(0.5, 1.5, 8))
+    z = np.cumsum(rng.uniform(0.5, 1.5, 7))
+    f = rng.normal(size=(9, 8, 7))
+    rc, tx, ty, tz, coef = OA.bspline_fit3(x, y, z, f, k, k, k)
+    assert rc == 0
+    # scipy: interpolate along each axis with the same knots
+    c = f.copy()
+    for ax, (pts, t) in enumerate(((x, tx), (y, ty), (z, tz))):
+        c = np.moveaxis(make_interp_spline(pts, c, k=k - 1, t=t, axis=ax).c, 0, ax)
+    np.testing.assert_allclose(coef, c, rtol=0, atol=1e-11 * np.abs(c).max())
+    nd = NdBSpline((tx, ty, tz), coef, k - 1)
+    pts = np.stack([rng.uniform(x[0], x[-1], 200), rng.uniform(y[0], y[-1], 200),
+                    rng.uniform(z[0], z[-1], 200)], axis=1)
+    ref = nd(pts)
+    got = np.array([OA.bspline_eval3(p, tx, ty, tz, coef, k, k, k)[0] for p in pts])
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+    # the data are reproduced at the abscissae
+    for (i, j, l) in [(0, 0, 0), (8, 7, 6), (3, 2, 5)]:
+        v, fl = OA.bspline_eval3((x[i], y[j], z[l]), tx, ty, tz, coef, k, k, k)
+        assert fl == 0
+        assert abs(v - f[i, j, l]) < 1e-11
+
+
+def test_cubic_reproduces_cubics_and_out_of_range_is_zero():
+    rng = np.random.default_rng(3)
+    x = np.linspace(-1.0, 1.2, 12)
+    y = np.linspace(0.0, 2.0, 10)
+    z = np.linspace(0.5, 1.0, 9)
+    P = lambda a, b, c: 1.0 + a - 2 * a ** 3 + 0.5 * b ** 2 * c + c ** 3 - a * b * c
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    rc, tx, ty, tz, coef = OA.bspline_fit3(x, y, z, P(X, Y, Z), 4, 4, 4)
+    assert rc == 0
+    for _ in range(100):
+        p = (rng.uniform(x[0], x[-1]), rng.uniform(y[0], y[-1]), rng.uniform(z[0], z[-1]))
+        v, fl = OA.bspline_eval3(p, tx, ty, tz, coef, 4, 4, 4)
+        assert fl == 0 and abs(v - P(*p)) < 1e-11
+    # db3val: 0 outside [t(1), t(n+k)] (bspline_sub_module.f90:612-629)
+    v, fl = OA.bspline_eval3((x[0] - 1e-3, y[3], z[3]), tx, ty, tz, coef, 4, 4, 4)
+    assert v == 0.0 and fl == 1
+    # ... and 0 at y equal to the last knot (mflag /= 0, 631-636)
+    v, fl = OA.bspline_eval3((x[3], ty[-1], z[3]), tx, ty, tz, coef, 4, 4, 4)
+    assert v == 0.0
+
+
+# ---------------------------------------------------------------- eigen / CG
+def test_eig_max_against_numpy():
+    rng = np.random.default_rng(11)
+    for scale in (1e-8, 1.0, 1e6, 1e12):
+        for _ in range(200):
+            g = rng.normal(size=(3, 3)) * scale
+            c = g.T @ g
+            ref = np.linalg.eigvalsh(c).max()
+            assert abs(OA.eig_sym3_max(c) - ref) <= 1e-13 * ref
+    # degenerate and diagonal cases
+    assert OA.eig_sym3_max(np.diag([3.0, 3.0, 3.0])) == 3.0
+    assert OA.eig_sym3_max(np.zeros((3, 3))) == 0.0
+    assert OA.eig_sym3_max(np.diag([1.0, 7.0, 2.0])) == 7.0
+
+
+def test_cauchy_green_is_gtg():
+    g = np.random.default_rng(2).normal(size=(3, 3))
+    np.testing.assert_allclose(OA.cauchy_green(g), g.T @ g, rtol=1e-15, atol=1e-15)
+
+
+# ---------------------------------------------------------------- grid
+def test_periodic_grid_ghosts_shifts_and_delta():
+    n = 8
+    X, Y, Z, fl = F.abc_grid(n)
+    o = Oracle(X, Y, Z, fl)
+    (gx, gy, gz), gflag, bn = o.ghost_grid()
+    h = 2 * np.pi / n
+    L = 2 * np.pi
+    xs = gx[:, 1, 1]
+    np.testing.assert_allclose(xs[1:-1], X[:, 0, 0], rtol=0, atol=0)
+    assert abs(xs[0] - (X[0, 0, 0] - h)) < 1e-13 and abs(xs[-1] - (X[-1, 0, 0] + h)) < 1e-13
+    ps, bb = o.periodic_shift()
+    ps = ps.reshape(3, 3, 3, 3)  # [k][j][i][d]
+    assert abs(ps[1, 1, 0, 0] - L) < 1e-13 and abs(ps[1, 1, 2, 0] + L) < 1e-13
+    assert abs(ps[2, 2, 2, 2] + L) < 1e-13 and ps[1, 1, 1].tolist() == [0, 0, 0]
+    assert abs(bb[0] - (X.min() - h)) < 1e-13 and abs(bb[3] - (X.max() + h)) < 1e-13
+    assert np.all(gflag == F.LCS_INTERNAL)
+    for d in range(3):
+        assert np.all(bn[d] == 0.0)
+    o.close()
+
+
+def test_wall_fakes_flags_and_normals_2d():
+    X, Y, Z, fl = F.double_gyre_grid(16, 8)
+    o = Oracle(X, Y, Z, fl)
+    (gx, gy, gz), gflag, bn = o.ghost_grid()
+    dx = 2.0 / 16
+    # mirror fakes in x and y, 2-D offsets in z by the largest extent / max(n)
+    assert abs(gx[0, 3, 1] - (X[0, 2, 0] - dx)) < 1e-14
+    assert abs(gy[4, 0, 1] - (Y[3, 0, 0] - 1.0 / 8)) < 1e-14
+    scaling = 2.0 / 16 * 15 / 16  # range of x (2 - dx) over max(n)
+    assert abs(gz[4, 4, 0] - (Z[3, 3, 0] - scaling)) < 1e-14
+    assert abs(gz[4, 4, 2] - (Z[3, 3, 0] + scaling)) < 1e-14
+    # flags: WALL ghosts next to WALL nodes, LCS_2D above/below INTERNAL nodes
+    assert gflag[0, 4, 1] == F.LCS_WALL and gflag[5, 4, 0] == F.LCS_2D and gflag[1, 4, 0] == F.LCS_WALL
+    # outward unit normal on a straight wall, diagonal at a corner
+    # (bcnorm is returned on the ghosted layout: node (i,j,k) is at [i+1, j+1, k+1])
+    np.testing.assert_allclose([bn[0][1, 5, 1], bn[1][1, 5, 1], bn[2][1, 5, 1]], [-1, 0, 0], atol=1e-12)
+    np.testing.assert_allclose([bn[0][9, 8, 1], bn[1][9, 8, 1]], [0, 1], atol=1e-12)
+    c = np.array([bn[0][1, 1, 1], bn[1][1, 1, 1]])
+    assert c[0] < 0 and c[1] < 0 and abs(np.hypot(*c) - 1) < 1e-12
+    assert np.all([bn[d][6, 5, 1] == 0 for d in range(3)])
+    o.close()
+
+
+# ---------------------------------------------------------------- closed forms
+@pytest.mark.parametrize("integ", [F.EULER, F.TRAPEZOIDAL, F.RK2, F.RK3, F.RK4])
+@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC])
+def test_uniform_flow_displacements(integ, interp):
+    """u = c: every scheme moves by c dt per step; T-flowmaps are c T (FWD) and
+    -c T (BKWD); forward particles wrap the periodic box."""
+    X, Y, Z, fl = F.abc_grid(8)
+    o = Oracle(X, Y, Z, fl)
+    T, h, dt = 0.6, 0.3, 0.1
+    hf = o.diagnostic_init(F.FTLE_FWD, 0, T, h, "f")
+    hb = o.diagnostic_init(F.FTLE_BKWD, 0, T, h, "b")
+    o.set_option("INTEGRATOR", integ)
+    o.set_option("INTERPOLATOR", interp)
+    c = (2.9, -1.7, 0.8)  # about half the box over the run: particles wrap
+    one = np.ones(X.shape)
+    for s in range(14):
+        o.update(c[0] * one, c[1] * one, c[2] * one, s * dt)
+        assert o.error == 0
+    assert o.noutput(hf) == 4 and o.noutput(hb) == 4
+    for d in range(3):
+        assert np.abs(o.composed(hf)[d] - c[d] * T).max() < 1e-11
+        assert np.abs(o.composed(hb)[d] + c[d] * T).max() < 1e-11
+        # 14 updates = 13 CFD steps, 4 outputs every 3 steps: 1 step in the interval
+        assert np.abs(o.flowmap(hf)[d] - c[d] * dt).max() < 1e-12
+        assert np.abs(o.flowmap(hb)[d] + c[d] * dt).max() < 1e-12
+    # CFL sub-cycles (lp_motion_m.f90:90-101): 1/delta = 1/dx, CFL 0.5
+    dxg = 2 * np.pi / 8
+    assert o.nsub == int(np.ceil(sum(abs(v) for v in c) / dxg * dt / 0.5))
+    o.close()
+
+
+def amplification(integ, z):
+    """Per-subcycle growth factor of x' = lam x for each integrator as the
+    reference codes them (lp_motion_m.f90:168-327)."""
+    if integ in (F.EULER, F.TRAPEZOIDAL):
+        return 1 + z
+    if integ == F.RK2:
+        return 1 + z + z * z / 2
+    if integ == F.RK3:  # k3 = z (x + 2 k2): not classical RK3
+        return 1 + z + 2 * z * z / 3 + z ** 3 / 6
+    return 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+
+
+@pytest.mark.parametrize("integ", [F.EULER, F.RK2, F.RK3, F.RK4])
+@pytest.mark.parametrize("direction", [F.FTLE_FWD, F.FTLE_BKWD])
+def test_linear_strain_flowmap_and_ftle(integ, direction):
+    """u = (lam x, -lam y, 0): LINEAR interpolation is exact, so every node
+    whose trajectory stays inside the domain carries fm = (R^N - 1) x with R
+    the scheme's amplification factor, and the displacement-gradient FTLE is
+    log(max|R^N - 1|)/T (lcs_m.f90:137-176)."""
+    lam, T, h, dt = 0.8, 0.5, 0.25, 0.05
+    X, Y, Z = F.cell_grid((60, 60, 1), (-1, -1, 0), (1, 1, 1))
+    fl = np.full(X.shape, F.LCS_INTERNAL, dtype=np.int32)
+    fl[0], fl[-1], fl[:, 0], fl[:, -1] = F.LCS_SLIP, F.LCS_SLIP, F.LCS_SLIP, F.LCS_SLIP
+    o = Oracle(X, Y, Z, fl)
+    hd = o.diagnostic_init(direction, 0, T, h, "s")
+    o.set_option("INTEGRATOR", integ)
+    u, v, w = lam * X, -lam * Y, 0 * X
+    nsteps = 16
+    for s in range(nsteps):
+        o.update(u, v, w, s * dt)
+        assert o.error == 0
+    nsub = o.nsub
+    sgn = 1 if direction == F.FTLE_FWD else -1
+    z = sgn * lam * dt / nsub
+    Rx, Ry = amplification(integ, z), amplification(integ, -z)
+    Ntot = round(T / dt) * nsub
+    cx, cy = Rx ** Ntot - 1, Ry ** Ntot - 1
+    comp = o.composed(hd)
+    ftle = o.ftle(hd)
+    # the SLIP ghosts are zero-gradient copies, which is not linear: keep away
+    # from the boundary layer that the repeated interpolation smears inward
+    m = (np.abs(X) < 0.25) & (np.abs(Y) < 0.25)
+    assert np.abs(comp[0][m] - cx * X[m]).max() < 1e-12
+    assert np.abs(comp[1][m] - cy * Y[m]).max() < 1e-12
+    assert np.abs(comp[2]).max() == 0.0
+    ref = np.log(max(abs(cx), abs(cy))) / T
+    assert np.abs(ftle[m] - ref).max() < 1e-10
+    o.close()
+
+
+def test_error_paths():
+    X, Y, Z, fl = F.abc_grid(4)
+    o = Oracle(X, Y, Z, fl)
+    o.set_option("NOT_AN_OPTION", 1)
+    assert o.error == 1
+    o.close()
+    o = Oracle(X, Y, Z, fl)
+    o.set_param("CFL", 0.4)
+    assert o.error == 0
+    o.set_param("BOGUS", 1.0)
+    assert o.error == 1
+    o.close()
+    # periodicity mismatch: INTERNAL on one face only
+    fl2 = fl.copy()
+    fl2[-1] = F.LCS_WALL
+    o = Oracle(X, Y, Z, fl2)
+    assert o.error == 1
+    o.close()
+    # wrong size on update
+    o = Oracle(X, Y, Z, fl)
+    o.n = np.array([4, 4, 5], dtype=np.int32)
+    o.update(np.zeros((4, 4, 5)), np.zeros((4, 4, 5)), np.zeros((4, 4, 5)), 0.0)
+    assert o.error == 1
+    o.close()
