@@ -1,0 +1,258 @@
+"""Benchmark of the MI355X FTLE path on the headline workload.
+
+Workload (ABC flow at 256^3, the configuration the metric is quoted on):
+steady ABC velocity u = (A sin z + C cos y, B sin x + A cos z, C sin y + B cos x),
+A = sqrt(3), B = sqrt(2), C = 1, periodic [0, 2 pi]^3 on a 256^3 cell-centred
+grid, dt = 0.02, forward and backward FTLE with T = 5 and h = 0.5 (10 flowmap
+sub-intervals), RK4 with cubic B-spline interpolation, CFL 0.5 (10 sub-cycles
+per step).  A step is one cfd2lcs_update over the whole grid; the velocity
+is resident in HBM before the timed region (cfd2lcs_update_device_c).
+
+Prints ONE JSON line: grid-point updates/s (one grid point advanced through one
+CFL sub-cycle, summed over both directions) in FP32, with the FP64 run, FTLE
+fields/s, the roofline of the dominant kernel (HIP events on the library's
+stream) and the CPU FP64 oracle timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "flowmap grid-pt updates/s + FTLE fields/s at 256³ (fp32 vs fp64); bwd/fwd ratio"
+HBM_PEAK_GBS = 8000.0
+TIMER_NAMES = {8: "k_fwd_step", 9: "k_sl_step", 10: "k_fit_axis", 11: "k_compose", 12: "k_ftle"}
+
+
+def abc_grid(n):
+    h = 2 * np.pi / n
+    a = 0.5 * h + np.arange(n) * h
+    X, Y, Z = np.meshgrid(a, a, a, indexing="ij")
+    return X, Y, Z
+
+
+def run_gpu(variant, n, steps, warmup, dt, T, h):
+    import torch
+
+    import libcfd2lcs_amd as L
+
+    lib = L.Cfd2lcs(variant)
+    lib.finalize()
+    X, Y, Z = abc_grid(n)
+    flag = np.zeros(X.shape, dtype=np.int32)
+    lib.init(X, Y, Z, flag)
+    if lib.error:
+        raise RuntimeError("cfd2lcs_init failed")
+    hf = lib.diagnostic_init(L.FTLE_FWD, 0, T, h, "fwdFTLE")
+    hb = lib.diagnostic_init(L.FTLE_BKWD, 0, T, h, "bkwdFTLE")
+    lib.set_option("INTEGRATOR", L.RK4)
+    lib.set_option("INTERPOLATOR", L.CUBIC)
+    lib.set_option("KERNEL_TIMING", L.LCS_TRUE)
+    lib.set_param("CFL", 0.5)
+    dtype = torch.float64 if variant == "dp" else torch.float32
+    dev = torch.device("cuda:0")
+    # velocity on the device, Fortran order (i fastest) = C order of (k, j, i)
+    ax = torch.tensor(X[:, 0, 0], dtype=torch.float64, device=dev)
+    zz, yy, xx = torch.meshgrid(ax, ax, ax, indexing="ij")
+    A, B, Cc = np.sqrt(3.0), np.sqrt(2.0), 1.0
+    u = (A * torch.sin(zz) + Cc * torch.cos(yy)).to(dtype).contiguous()
+    v = (B * torch.sin(xx) + A * torch.cos(zz)).to(dtype).contiguous()
+    w = (Cc * torch.sin(yy) + B * torch.cos(xx)).to(dtype).contiguous()
+    del xx, yy, zz
+    torch.cuda.synchronize()
+    ptrs = (u.data_ptr(), v.data_ptr(), w.data_ptr())
+    step = 0
+    for _ in range(warmup):
+        lib.update_device(*ptrs, step * dt)
+        step += 1
+    if lib.error:
+        raise RuntimeError("update failed during warm-up")
+    t0_sec, _ = timers(lib)
+    st0 = lib.stats()
+    nout0 = lib.noutput(hf) + lib.noutput(hb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lib.update_device(*ptrs, step * dt)
+        step += 1
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if lib.error:
+        raise RuntimeError("update failed in the timed region")
+    st1 = lib.stats()
+    t1_sec, t1_cnt = timers(lib)
+    dsec = t1_sec - t0_sec
+    _, c0 = t0_sec, None
+    nout = lib.noutput(hf) + lib.noutput(hb) - nout0
+    upd = (st1["integrations_fwd"] - st0["integrations_fwd"]) + (st1["integrations_bkwd"] - st0["integrations_bkwd"])
+    res = {
+        "elapsed": elapsed,
+        "updates": upd,
+        "nsub": int(st1["nsub"]),
+        "ftle_outputs": nout,
+        "t_compose_ftle": float(dsec[2] + dsec[3]),
+        "t_fwd": float(dsec[0]),
+        "t_bkwd": float(dsec[1]),
+        "kernel_sec": {TIMER_NAMES[k]: float(t1_sec[k]) for k in TIMER_NAMES},
+        "kernel_cnt": {TIMER_NAMES[k]: int(t1_cnt[k]) for k in TIMER_NAMES},
+        "np": n ** 3,
+        "oob": st1["spline_out_of_range"],
+    }
+    lib.finalize()
+    del u, v, w
+    torch.cuda.empty_cache()
+    return res
+
+
+def timers(lib):
+    import ctypes as C
+    sec = (C.c_double * 16)()
+    cnt = (C.c_longlong * 16)()
+    lib.L.cfd2lcs_get_timers_c.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
+    lib.L.cfd2lcs_get_timers_c.restype = C.c_int
+    lib.L.cfd2lcs_get_timers_c(sec, cnt, 16)
+    return np.array(list(sec)), np.array(list(cnt))
+
+
+def kernel_bytes_per_point(name, s):
+    """Algorithmic HBM bytes per grid point per launch of each hot kernel, for
+    storage size s (4 or 8): the semi-Lagrangian step reads both velocity
+    snapshots and the old flowmap and writes the new one (6 s + 6 s); the
+    forward step reads both snapshots and reads+writes position,
+    displacement and the tracked node (6 s + 2 (6 s + 6))."""
+    if name == "k_sl_step":
+        return 12 * s
+    if name == "k_fwd_step":
+        return 6 * s + 2 * (6 * s + 6)
+    return None
+
+
+def cpu_baseline(n_cpu, budget_s):
+    """FP64 oracle (CPU restatement) on ABC n_cpu^3, same scheme, FWD+BKWD,
+    steps until the budget is spent; OpenMP threads = OMP_NUM_THREADS."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_api import Oracle
+
+    X, Y, Z = abc_grid(n_cpu)
+    flag = np.zeros(X.shape, dtype=np.int32)
+    o = Oracle(X, Y, Z, flag)
+    o.diagnostic_init(0, 0, 5.0, 0.5, "f")
+    o.diagnostic_init(1, 0, 5.0, 0.5, "b")
+    o.set_option("INTEGRATOR", 4)
+    o.set_option("INTERPOLATOR", 3)
+    A, B, Cc = np.sqrt(3.0), np.sqrt(2.0), 1.0
+    u = A * np.sin(Z) + Cc * np.cos(Y)
+    v = B * np.sin(X) + A * np.cos(Z)
+    w = Cc * np.sin(Y) + B * np.cos(X)
+    o.update(u, v, w, 0.0)  # first call: n_sub = 1, dt = 0
+    i0f, i0b = o.integrations(1), o.integrations(-1)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        steps += 1
+        o.update(u, v, w, steps * 0.02)
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    upd = (o.integrations(1) - i0f) + (o.integrations(-1) - i0b)
+    o.close()
+    return {
+        "value": upd / el,
+        "unit": "grid-pt updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP64 oracle (C/OpenMP restatement), ABC {n_cpu}^3, RK4 + cubic B-spline, "
+                  f"FWD+BKWD, {steps} steps (n_sub {o_nsub_guess(n_cpu)}), {el:.1f} s",
+    }
+
+
+def o_nsub_guess(n):
+    # max CFL of the ABC field: (|u|/dx + |v|/dy + |w|/dz) dt with dx = 2 pi / n
+    return int(np.ceil((np.sqrt(3) + np.sqrt(2) + 1) * 1.2 * 0.02 / (2 * np.pi / n) / 0.5))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=25)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU (the whole domain on one MI355X); "
+              "multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+    dt, T, h = 0.02, 5.0, 0.5
+    r32 = run_gpu("sp", args.n, args.steps, args.warmup, dt, T, h)
+    print(f"[bench] fp32: {r32['updates'] / r32['elapsed']:.4e} updates/s, "
+          f"{1e3 * r32['elapsed'] / args.steps:.1f} ms/step", file=sys.stderr, flush=True)
+    r64 = None
+    if not args.no_fp64:
+        r64 = run_gpu("dp", args.n, args.steps, args.warmup, dt, T, h)
+        print(f"[bench] fp64: {r64['updates'] / r64['elapsed']:.4e} updates/s", file=sys.stderr, flush=True)
+    # dominant kernel of the FP32 run
+    ks = r32["kernel_sec"]
+    dom = max(("k_sl_step", "k_fwd_step"), key=lambda k: ks[k])
+    cnt = max(r32["kernel_cnt"][dom], 1)
+    avg = ks[dom] / cnt
+    bpp = kernel_bytes_per_point(dom, 4)
+    achieved = bpp * r32["np"] / avg / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_ms": round(avg * 1e3, 4), "bytes_per_point": bpp}
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        try:
+            tr = json.load(open(traffic_file))
+            if dom in tr and tr[dom].get("n") == args.n:
+                roof["traffic"] = tr[dom]["bytes_per_launch"]
+        except Exception:
+            pass
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_n, args.cpu_budget)
+    value = r32["updates"] / r32["elapsed"]
+    ftle32 = r32["ftle_outputs"] / r32["t_compose_ftle"] if r32["t_compose_ftle"] > 0 else None
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-pt updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r32["elapsed"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: analytic steady ABC velocity, resident in HBM",
+        "config": {"workload": f"abc{args.n}_rk4_cubic_fwd_bkwd", "grid": [args.n] * 3, "dt": dt, "T": T,
+                   "h": h, "integrator": "RK4", "interpolator": "CUBIC", "cfl": 0.5, "n_sub": r32["nsub"],
+                   "directions": ["FTLE_FWD", "FTLE_BKWD"]},
+        "ftle_fields_per_s": ftle32,
+        "ftle_outputs_timed": r32["ftle_outputs"],
+        "fp64": None if r64 is None else {
+            "value": r64["updates"] / r64["elapsed"], "ms_per_step": 1e3 * r64["elapsed"] / args.steps,
+            "ftle_fields_per_s": (r64["ftle_outputs"] / r64["t_compose_ftle"]) if r64["t_compose_ftle"] > 0 else None,
+            "fp32_over_fp64": value / (r64["updates"] / r64["elapsed"])},
+        "bwd_fwd_ratio": None,
+        "kernel_ms": {k: round(1e3 * s / max(r32["kernel_cnt"][k], 1), 4) for k, s in ks.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -144,6 +144,22 @@ int cfd2lcs_get_noutput_c(int lcs_handle);
  *   [7] spline evaluations outside the knot range (db3val returns 0) */
 void cfd2lcs_get_stats_c(double out[8]);
 
+/* GPU time (seconds) and launch counts accumulated per category, measured
+ * with HIP events on the library's stream; returns the number of categories
+ * filled (at most n, 16 in this build):
+ *   0 forward advection   1 backward advection   2 composition   3 FTLE
+ * and, when the option "KERNEL_TIMING" is LCS_TRUE, per kernel:
+ *   8 forward particle step   9 semi-Lagrangian step   10 spline fit
+ *   11 composition kernel    12 FTLE kernel           13 ghost/ingest/CFL */
+int cfd2lcs_get_timers_c(double *seconds, long long *counts, int n);
+
+/* Host-side grid set-up, for inspection: lines[N0+N1+N2] (ghosted
+ * coordinate lines), flag[(n0+2)(n1+2)(n2+2)], bn[3*n0*n1*n2] (boundary
+ * normals), ps[81] (periodic shifts), bb[6] (bounding box).  Any pointer may
+ * be NULL.  Returns 0 on success. */
+int cfd2lcs_debug_grid_c(double *lines, int *flag, double *bn, double *ps,
+                         double *bb);
+
 #ifdef __cplusplus
 }
 #endif

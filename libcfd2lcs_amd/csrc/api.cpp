@@ -47,6 +47,7 @@ struct Options {
   int sync_timer = 0;
   U cfl_max = (U)0.5;
   int precision = sizeof(U) == 8 ? LCS_FP64 : LCS_FP32;
+  int kernel_timing = 0;  // time every hot kernel launch with HIP events
   bool warn_opt = false, warn_par = false;  // SAVEd warn flags (cfd2lcs.f90:444,632)
 };
 
@@ -110,6 +111,7 @@ struct EngineBase {
   virtual void stats(double out[8]) = 0;
   virtual int debug_grid(double* lines, int* flag, double* bn, double* ps, double* bb) = 0;
   virtual bool fresh() const = 0;
+  virtual int timers(double* sec, long long* cnt, int n) = 0;
   virtual int ndiag() const = 0;
 };
 
@@ -137,7 +139,13 @@ struct Engine : EngineBase {
   U t_n = 0, t_np1 = 0;
   int nsub_last = 0;
   long long integ_fwd = 0, integ_bkwd = 0;
-  double time_fwd = 0, time_bkwd = 0, time_comp = 0, time_ftle = 0;
+  // timers: 0 FWD advect, 1 BKWD advect, 2 composition, 3 FTLE (phases);
+  // with KERNEL_TIMING: 8 k_fwd_step, 9 k_sl_step, 10 spline fit,
+  // 11 k_compose, 12 k_ftle, 13 ghost/ingest/CFL kernels
+  enum { kTimers = 16 };
+  double tsum[kTimers] = {};
+  long long tcount[kTimers] = {};
+  std::vector<hipEvent_t> evpool;
   unsigned long long oob_total = 0;
 
   struct Timed {
@@ -173,6 +181,7 @@ struct Engine : EngineBase {
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
       }
+      for (auto e : evpool) (void)hipEventDestroy(e);
       (void)hipStreamDestroy(stream);
     }
     if (st_host) (void)hipHostFree(st_host);
@@ -180,6 +189,15 @@ struct Engine : EngineBase {
 
   bool fresh() const override { return diags.empty() && first_call; }
   int ndiag() const override { return (int)diags.size(); }
+  int timers(double* sec, long long* cnt, int n) override {
+    sync_status();
+    const int m = std::min(n, (int)kTimers);
+    for (int a = 0; a < m; a++) {
+      if (sec) sec[a] = tsum[a];
+      if (cnt) cnt[a] = tcount[a];
+    }
+    return m;
+  }
 
   size_t fbytes() const { return sizeof(T) * (size_t)g.nt; }
   F3<T> f3(DevBuf& b) {
@@ -318,28 +336,47 @@ struct Engine : EngineBase {
   }
   bool is_spline(int interp) const { return interp == kQuadratic || interp == kCubic; }
 
+  hipEvent_t new_event() {
+    hipEvent_t e = nullptr;
+    if (!evpool.empty()) {
+      e = evpool.back();
+      evpool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+    }
+    return e;
+  }
   void timed_begin(Timed& t, int cat) {
     t.cat = cat;
-    (void)hipEventCreate(&t.a);
-    (void)hipEventCreate(&t.b);
-    (void)hipEventRecord(t.a, stream);
+    t.a = new_event();
+    t.b = new_event();
+    if (t.a) (void)hipEventRecord(t.a, stream);
   }
   void timed_end(Timed& t) {
-    (void)hipEventRecord(t.b, stream);
-    pending.push_back(t);
+    if (t.b) (void)hipEventRecord(t.b, stream);
+    if (t.a && t.b) pending.push_back(t);
+  }
+  // kernel-level timing, on request only (it adds two events per launch)
+  template <class F>
+  void ktimed(int cat, F&& launch) {
+    if (!g_opt.kernel_timing) {
+      launch();
+      return;
+    }
+    Timed t;
+    timed_begin(t, cat);
+    launch();
+    timed_end(t);
   }
   void harvest_timers() {
     for (auto& t : pending) {
       float ms = 0;
-      if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
-        const double s = ms * 1e-3;
-        if (t.cat == 0) time_fwd += s;
-        if (t.cat == 1) time_bkwd += s;
-        if (t.cat == 2) time_comp += s;
-        if (t.cat == 3) time_ftle += s;
+      if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess && t.cat >= 0 && t.cat < kTimers) {
+        tsum[t.cat] += ms * 1e-3;
+        tcount[t.cat] += 1;
       }
-      (void)hipEventDestroy(t.a);
-      (void)hipEventDestroy(t.b);
+      evpool.push_back(t.a);
+      evpool.push_back(t.b);
     }
     pending.clear();
   }
@@ -473,10 +510,12 @@ struct Engine : EngineBase {
     }
     F3<T> unp1 = f3(vel[cur]);
     CF3<T> unm = cf3(vel[old]);
-    launch_ingest<U, T>(g, unp1, su, sv, sw, stream);
-    launch_vel_ghost(g, unp1, unm, st, stream);
     HIPCHK(hipMemsetAsync(&st->cfl_bits, 0, sizeof(st->cfl_bits), stream));
-    launch_cfl(g, unp1, st, stream);
+    ktimed(13, [&] {
+      launch_ingest<U, T>(g, unp1, su, sv, sw, stream);
+      launch_vel_ghost(g, unp1, unm, st, stream);
+      launch_cfl(g, unp1, st, stream);
+    });
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
@@ -523,8 +562,11 @@ struct Engine : EngineBase {
         U t = t_n;
         int16_t* ns = (int16_t*)dg.ns.p;
         for (int s = 0; s < nsub; s++) {
-          launch_fwd_step(g, interp, stage_params(t, dt), cu_n, cu_np1, p3(dg.xp), p3(dg.dx), ns, ns + g.np,
-                          ns + 2 * g.np, st, stream);
+          const StageParams<T> sp = stage_params(t, dt);
+          ktimed(8, [&] {
+            launch_fwd_step(g, interp, sp, cu_n, cu_np1, p3(dg.xp), p3(dg.dx), ns, ns + g.np, ns + 2 * g.np, st,
+                            stream);
+          });
           t = t + dt;
         }
       } else {
@@ -539,11 +581,12 @@ struct Engine : EngineBase {
           CF3<T> src = cf3(a);
           if (spl) {
             HIPCHK(hipMemcpyAsync(dg.fmcoef.p, a.p, 3 * fbytes(), hipMemcpyDeviceToDevice, stream));
-            launch_spline_fit(g, f3(dg.fmcoef), 3, stream);
+            ktimed(10, [&] { launch_spline_fit(g, f3(dg.fmcoef), 3, stream); });
             src = cf3(dg.fmcoef);
           }
-          launch_sl_step(g, interp, stage_params(t, dt), cu_n, cu_np1, src, f3(b), st, stream);
-          launch_fm_ghost(g, f3(b), cf3(a), st, stream);
+          const StageParams<T> sp = stage_params(t, dt);
+          ktimed(9, [&] { launch_sl_step(g, interp, sp, cu_n, cu_np1, src, f3(b), st, stream); });
+          ktimed(13, [&] { launch_fm_ghost(g, f3(b), cf3(a), st, stream); });
           dg.fcur = 1 - dg.fcur;
           t = t + dt;
         }
@@ -569,7 +612,7 @@ struct Engine : EngineBase {
       fail("copy failed");
       return false;
     }
-    launch_spline_fit(g, f3(vcoef[b]), 3, stream);
+    ktimed(10, [&] { launch_spline_fit(g, f3(vcoef[b]), 3, stream); });
     vcoef_ok[b] = true;
     return true;
   }
@@ -604,7 +647,7 @@ struct Engine : EngineBase {
         if (!dg.slot_ok[s] || dg.slot_coef_ok[s]) continue;
         T* cp = slot_ptr(dg.slotcoef, s);
         HIPCHK(hipMemcpyAsync(cp, slot_ptr(dg.slots, s), 3 * fb, hipMemcpyDeviceToDevice, stream));
-        launch_spline_fit(g, F3<T>{{cp, cp + g.nt, cp + 2 * g.nt}}, 3, stream);
+        ktimed(10, [&] { launch_spline_fit(g, F3<T>{{cp, cp + g.nt, cp + 2 * g.nt}}, 3, stream); });
         dg.slot_coef_ok[s] = 1;
       }
     }
@@ -622,15 +665,19 @@ struct Engine : EngineBase {
     HIPCHK(hipMemcpyAsync(dg.order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream));
     Timed tc;
     timed_begin(tc, 2);
-    launch_compose(g, interp, (const T*)(spl ? dg.slotcoef.p : dg.slots.p), g.nt, (const int*)dg.order.p,
-                   (int)order.size(), p3(dg.dxout), st, stream);
-    launch_field_from_origin(g, cp3(dg.dxout), f3(dg.compfm), stream);
+    ktimed(11, [&] {
+      launch_compose(g, interp, (const T*)(spl ? dg.slotcoef.p : dg.slots.p), g.nt, (const int*)dg.order.p,
+                     (int)order.size(), p3(dg.dxout), st, stream);
+    });
+    ktimed(13, [&] { launch_field_from_origin(g, cp3(dg.dxout), f3(dg.compfm), stream); });
     timed_end(tc);
     Timed tf;
     timed_begin(tf, 3);
     const double tiny = sizeof(U) == 8 ? DBL_MIN : (double)FLT_MIN;
-    launch_ftle(g, cf3(dg.compfm), (double)dg.Tint, g_opt.incompressible, tiny, (double*)dg.ftle.p, p3(dg.comp),
-                stream);
+    ktimed(12, [&] {
+      launch_ftle(g, cf3(dg.compfm), (double)dg.Tint, g_opt.incompressible, tiny, (double*)dg.ftle.p, p3(dg.comp),
+                  stream);
+    });
     timed_end(tf);
     reset_diag(dg);
     dg.noutput++;
@@ -705,10 +752,10 @@ struct Engine : EngineBase {
     out[0] = nsub_last;
     out[1] = (double)integ_fwd;
     out[2] = (double)integ_bkwd;
-    out[3] = time_fwd;
-    out[4] = time_bkwd;
-    out[5] = time_comp;
-    out[6] = time_ftle;
+    out[3] = tsum[0];
+    out[4] = tsum[1];
+    out[5] = tsum[2];
+    out[6] = tsum[3];
     out[7] = (double)oob_total;
   }
   int debug_grid(double* lines, int* flag, double* bn, double* ps, double* bb) override {
@@ -869,6 +916,8 @@ void cfd2lcs_set_option_c(char option[], int val) {
       fail(o + " = LCS_TRUE is not supported by this build");
       return;
     }
+  } else if (o == "KERNEL_TIMING") {
+    boolopt(g_opt.kernel_timing);
   } else if (o == "UPDATE_FREQ") {
   } else if (o == "INTERPOLATOR") {
     if (val != NEAREST_NBR && val != LINEAR && val != QUADRATIC && val != CUBIC && val != TSE && val != TSE_LIMIT)
@@ -941,6 +990,11 @@ void cfd2lcs_get_stats_c(double out[8]) {
     return;
   }
   g_eng->stats(out);
+}
+
+int cfd2lcs_get_timers_c(double* seconds, long long* counts, int n) {
+  if (!g_eng) return 0;
+  return g_eng->timers(seconds, counts, n);
 }
 
 // host-side grid description (for tests): lines[N0+N1+N2], flag[nt],
