@@ -141,7 +141,7 @@ struct Engine : EngineBase {
   long long integ_fwd = 0, integ_bkwd = 0;
   // timers: 0 FWD advect, 1 BKWD advect, 2 composition, 3 FTLE (phases);
   // with KERNEL_TIMING: 8 k_fwd_step, 9 k_sl_step, 10 spline fit,
-  // 11 k_compose, 12 k_ftle, 13 ghost/ingest/CFL kernels
+  // 11 k_compose, 12 k_ftle, 13 ghost/ingest/CFL kernels, 14 time blend
   enum { kTimers = 16 };
   double tsum[kTimers] = {};
   long long tcount[kTimers] = {};
@@ -230,6 +230,7 @@ struct Engine : EngineBase {
       g.kord[d] = 0;
       g.kn[d] = nullptr;
       g.band[d] = nullptr;
+      g.bpoly[d] = nullptr;
     }
     g.nt = hg.nt;
     g.np = hg.np;
@@ -312,10 +313,11 @@ struct Engine : EngineBase {
         fail(e);
         return false;
       }
-      const size_t nr = ax.knots.size() + ax.band.size();
+      const size_t nr = ax.knots.size() + ax.band.size() + ax.poly.size();
       std::vector<T> hv(nr);
       for (size_t a = 0; a < ax.knots.size(); a++) hv[a] = (T)ax.knots[a];
       for (size_t a = 0; a < ax.band.size(); a++) hv[ax.knots.size() + a] = (T)ax.band[a];
+      for (size_t a = 0; a < ax.poly.size(); a++) hv[ax.knots.size() + ax.band.size() + a] = (T)ax.poly[a];
       if (!spline_mem[d].alloc(nr * sizeof(T)) ||
           hipMemcpy(spline_mem[d].p, hv.data(), nr * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
         fail("upload of the spline factors failed");
@@ -324,6 +326,7 @@ struct Engine : EngineBase {
       g.kord[d] = kk;
       g.kn[d] = (const T*)spline_mem[d].p;
       g.band[d] = (const T*)spline_mem[d].p + ax.knots.size();
+      g.bpoly[d] = (const T*)spline_mem[d].p + ax.knots.size() + ax.band.size();
     }
     if (!(g.kord[0] == g.kord[1] && (g.kord[2] == g.kord[0] || g.kord[2] == 2))) {
       fail("spline orders per axis not supported by this build (need equal x/y orders)");
@@ -439,35 +442,79 @@ struct Engine : EngineBase {
     }
   }
 
+  // Time-blended stage velocities u = (1-ct) u_n + ct u_np1 (values, or
+  // spline coefficients: the fit is linear) are formed once per distinct ct
+  // of the update and shared by all stages and directions that use it.
+  struct Blend {
+    DevBuf buf;
+    U ct = 0;
+    bool valid = false;
+    unsigned long long stamp = 0;
+  };
+  Blend blends[6];
+  unsigned long long blend_clock = 0;
+  int vold = 0;  // index of u_n in vel[] during an update
+
+  CF3<T> stage_field(U ct, bool spl) {
+    DevBuf& A = spl ? vcoef[vold] : vel[vold];
+    DevBuf& B = spl ? vcoef[cur] : vel[cur];
+    if (ct == (U)0) return cf3(A);
+    if (ct == (U)1) return cf3(B);
+    Blend* victim = &blends[0];
+    for (auto& b : blends) {
+      if (b.valid && b.ct == ct) {
+        b.stamp = ++blend_clock;
+        return cf3(b.buf);
+      }
+      if (!b.valid || b.stamp < victim->stamp || !victim->valid) {
+        if (!b.valid && victim->valid) victim = &b;
+        else if (b.valid == victim->valid && b.stamp < victim->stamp) victim = &b;
+      }
+    }
+    if (!victim->buf.p && !victim->buf.alloc(3 * fbytes())) {
+      fail("hipMalloc failed (blended velocity)");
+      return cf3(A);
+    }
+    ktimed(14, [&] { launch_blend(g, f3(victim->buf), cf3(A), cf3(B), (T)ct, stream); });
+    victim->ct = ct;
+    victim->valid = true;
+    victim->stamp = ++blend_clock;
+    return cf3(victim->buf);
+  }
+
   // ct of a stage at time tt (lp_motion_m.f90:170 etc.); NaN (first update)
   // clamps to 1 as with IEEE minNum/maxNum
   static U ctf(U tt, U t0, U t1) { return std::fmax(std::fmin((tt - t0) / (t1 - t0), (U)1), (U)0); }
 
-  StageParams<T> stage_params(U t, U dt) const {
+  StageParams<T> stage_params(U t, U dt, bool spl) {
     StageParams<T> sp{};
     sp.integ = g_opt.integrator;
     sp.dt = (T)dt;
     const U half = (U)1 / (U)2;
     U c[4] = {0, 0, 0, 0};
+    int ns = 1;
     switch (g_opt.integrator) {
       case kEuler: c[0] = ctf(t, t_n, t_np1); break;
       case kTrapezoidal: c[0] = ctf(t + half * dt, t_n, t_np1); break;
       case kRK2:
         c[0] = ctf(t, t_n, t_np1);
         c[1] = ctf(t + half * dt, t_n, t_np1);
+        ns = 2;
         break;
       case kRK3:
         c[0] = ctf(t, t_n, t_np1);
         c[1] = ctf(t + half * dt, t_n, t_np1);
         c[2] = ctf(t + dt, t_n, t_np1);
+        ns = 3;
         break;
       default:
         c[0] = ctf(t, t_n, t_np1);
         c[1] = ctf(t + half * dt, t_n, t_np1);
         c[2] = c[1];
         c[3] = ctf(t + dt, t_n, t_np1);
+        ns = 4;
     }
-    for (int a = 0; a < 4; a++) sp.ct[a] = (T)c[a];
+    for (int a = 0; a < 4; a++) sp.v[a] = stage_field(c[a < ns ? a : 0], spl);
     return sp;
   }
 
@@ -494,7 +541,9 @@ struct Engine : EngineBase {
     }
     const int old = cur;
     cur = 1 - cur;
+    vold = old;
     vcoef_ok[cur] = false;
+    for (auto& b : blends) b.valid = false;
     // vel[cur] becomes u_np1; vel[old] (previous u_np1) is u_n; vcoef[b]
     // holds the spline coefficients of vel[b]
     const U *su = u, *sv = v, *sw = w;
@@ -556,16 +605,13 @@ struct Engine : EngineBase {
       }
       Timed tm;
       timed_begin(tm, dg.dir > 0 ? 0 : 1);
-      CF3<T> cu_n = spl ? cf3(vcoef[old]) : cf3(vel[old]);
-      CF3<T> cu_np1 = spl ? cf3(vcoef[cur]) : cf3(vel[cur]);
       if (dg.dir > 0) {
         U t = t_n;
         int16_t* ns = (int16_t*)dg.ns.p;
         for (int s = 0; s < nsub; s++) {
-          const StageParams<T> sp = stage_params(t, dt);
+          const StageParams<T> sp = stage_params(t, dt, spl);
           ktimed(8, [&] {
-            launch_fwd_step(g, interp, sp, cu_n, cu_np1, p3(dg.xp), p3(dg.dx), ns, ns + g.np, ns + 2 * g.np, st,
-                            stream);
+            launch_fwd_step(g, interp, sp, p3(dg.xp), p3(dg.dx), ns, ns + g.np, ns + 2 * g.np, st, stream);
           });
           t = t + dt;
         }
@@ -584,8 +630,8 @@ struct Engine : EngineBase {
             ktimed(10, [&] { launch_spline_fit(g, f3(dg.fmcoef), 3, stream); });
             src = cf3(dg.fmcoef);
           }
-          const StageParams<T> sp = stage_params(t, dt);
-          ktimed(9, [&] { launch_sl_step(g, interp, sp, cu_n, cu_np1, src, f3(b), st, stream); });
+          const StageParams<T> sp = stage_params(t, dt, spl);
+          ktimed(9, [&] { launch_sl_step(g, interp, sp, src, f3(b), st, stream); });
           ktimed(13, [&] { launch_fm_ghost(g, f3(b), cf3(a), st, stream); });
           dg.fcur = 1 - dg.fcur;
           t = t + dt;

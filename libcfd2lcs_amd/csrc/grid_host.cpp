@@ -410,6 +410,55 @@ std::string spline_axis(const std::vector<double>& x, int k, SplineAxis& out) {
     }
   }
   if (A(n - 1, n - 1) == 0.0) return "spline collocation matrix has a zero pivot";
+  // piecewise-polynomial form of the basis on every knot interval, by the
+  // Cox-de Boor recurrence carried out on polynomials in s
+  const int rowlen = k * k + 1;
+  out.poly.assign((size_t)(n + 1) * rowlen, 0.0);
+  for (int L = k; L <= n; L++) {
+    const double tl = t[L - 1], hL = t[L] - t[L - 1];
+    if (!(hL > 0)) return "empty knot interval";
+    // B[i] for i = L-k+1..L (index i - (L-k+1)), each a polynomial of degree < k
+    std::vector<std::vector<double>> B(k, std::vector<double>(k, 0.0));
+    B[k - 1][0] = 1.0;  // order 1: B_L = 1 on [t(L), t(L+1))
+    for (int ord = 2; ord <= k; ord++) {
+      std::vector<std::vector<double>> Nb(k, std::vector<double>(k, 0.0));
+      for (int i = L - ord + 1; i <= L; i++) {
+        const int ii = i - (L - k + 1);
+        // omega_i(x) = (x - t_i)/(t_{i+ord-1} - t_i), x = tl + hL s
+        auto lin = [&](int i0, int span, double& c0, double& c1) {
+          const double den = t[i0 + span - 1] - t[i0 - 1];
+          if (den == 0.0) { c0 = 0.0; c1 = 0.0; return; }
+          c0 = (tl - t[i0 - 1]) / den;
+          c1 = hL / den;
+        };
+        double a0, a1, b0, b1;
+        lin(i, ord - 1, a0, a1);      // omega_{i,ord}
+        lin(i + 1, ord - 1, b0, b1);  // omega_{i+1,ord}
+        std::vector<double>& r = Nb[ii];
+        if (i >= L - ord + 2) {  // B_{i,ord-1} nonzero on this interval
+          const std::vector<double>& P = B[ii];
+          for (int m = 0; m < k; m++) {
+            if (P[m] == 0.0) continue;
+            r[m] += a0 * P[m];
+            if (m + 1 < k) r[m + 1] += a1 * P[m];
+          }
+        }
+        if (i + 1 <= L) {  // (1 - omega_{i+1}) B_{i+1,ord-1}
+          const std::vector<double>& P = B[ii + 1];
+          for (int m = 0; m < k; m++) {
+            if (P[m] == 0.0) continue;
+            r[m] += (1.0 - b0) * P[m];
+            if (m + 1 < k) r[m + 1] += -b1 * P[m];
+          }
+        }
+      }
+      B.swap(Nb);
+    }
+    double* row = &out.poly[(size_t)L * rowlen];
+    for (int a = 0; a < k; a++)
+      for (int m = 0; m < k; m++) row[a * k + m] = B[a][m];
+    row[k * k] = 1.0 / hL;
+  }
   return "";
 }
 

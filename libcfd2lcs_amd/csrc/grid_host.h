@@ -40,6 +40,10 @@ struct SplineAxis {
   int k = 0;
   std::vector<double> knots;  // n + k
   std::vector<double> band;   // (2k-1) * n, a(r,c) at (r - c + k - 1) + (2k-1)*c
+  // per knot interval L = k..n (1-based, t(L) <= x < t(L+1)): the k nonzero
+  // B-splines B_{L-k+1..L} as polynomials in s = (x - t(L)) / (t(L+1) - t(L)),
+  // row L at poly[L*(k*k+1)]: coefficients [a*k + m] of s^m, then 1/(t(L+1)-t(L))
+  std::vector<double> poly;
 };
 std::string spline_axis(const std::vector<double>& x, int k, SplineAxis& out);
 

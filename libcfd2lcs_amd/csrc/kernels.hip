@@ -8,6 +8,7 @@
 // fit (three banded axis solves).  Reference semantics are cited per kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdint>
 
@@ -78,6 +79,17 @@ __device__ __forceinline__ long long ghost_count(const GridDev<T>& g) {
   return 2LL * g.N[0] * g.N[1] + (2LL * g.N[0] + 2LL * g.n[1]) * g.n[2];
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; this bijection gives
+// each XCD a contiguous range of work items so that neighbouring items share
+// that XCD's L2.
+__device__ __forceinline__ int xcd_brick(int w, int nb) {
+  const int xcd = w & 7, idx = w >> 3, q = nb >> 3, r = nb & 7;
+  return xcd * q + min(xcd, r) + idx;
+}
+__device__ __forceinline__ int xcd_thread_index() {
+  return xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+}
+
 // ---------------------------------------------------------------------------
 // Interpolation (interp_s2u_r1, lp_tracking_m.f90:279-662)
 // ---------------------------------------------------------------------------
@@ -117,45 +129,37 @@ __device__ __forceinline__ T trilin(const LinW<T>& w, T f0, T f1, T f2, T f3, T 
          w.mt[0] * w.t[1] * w.t[2] * f6 + w.t[0] * w.t[1] * w.t[2] * f7;
 }
 
-// dintrv: largest 1-based i with t(i) <= x over t(1..L)
-template <typename T>
-__device__ __forceinline__ int knot_interval(const T* t, int L, T x) {
-  if (x >= t[L - 1]) return L;
-  int lo = 1, hi = L;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (x >= t[mid - 1]) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
-// B-spline basis of order K at x (de Boor's BSPLVB recurrence, as dbspvn,
-// bspline_sub_module.f90:2288-2370).  Returns 0 = ok, 1 = outside the knot
-// range (db3val returns 0 and reports), 2 = y/z at the last knot (db3val
-// returns 0 silently, bspline_sub_module.f90:634-636).
+// B-spline basis of order K at x (bspline_sub_module.f90: dintrv 2558-2634,
+// dbspvn 2288-2370).  The interval index is the largest L in [K, n] with
+// t(L) <= x (dintrv; dbvalu's left limit at x = t(n+1)), found by probing from
+// a hint next to the tracked node; the K nonzero basis values come from the
+// interval's polynomial table (exact piecewise form of the Cox-de Boor
+// recurrence, evaluated by Horner in the normalised interval coordinate).
+// Returns 0 = ok, 1 = outside the knot range (db3val returns 0 and reports),
+// 2 = y/z at the last knot (db3val returns 0 silently,
+// bspline_sub_module.f90:634-636).
 template <typename T, int K>
-__device__ __forceinline__ int bs_setup(const T* t, int n, T x, bool xaxis, int& base, T v[K]) {
+__device__ __forceinline__ int bs_setup(const T* __restrict__ t, const T* __restrict__ poly, int n, T x, bool xaxis,
+                                        int hint, int& base, T v[K]) {
   const int L = n + K;
   if (!(x >= t[0]) || x > t[L - 1]) return 1;
-  int left = knot_interval(t, L, x);
-  if (left >= L) {
+  int left;
+  if (x >= t[L - 1]) {
     if (!xaxis) return 2;
+    left = n;
+  } else {
+    left = iclamp(hint, K, n);
+    while (left < n && x >= t[left]) left++;
+    while (left > K && x < t[left - 1]) left--;
   }
-  if (left > n) left = n;  // dbvalu's left limit at x = t(n+1)
-  T dp[K], dm[K];
-  v[0] = T(1);
+  const T* P = poly + left * (K * K + 1);
+  const T s = (x - t[left - 1]) * P[K * K];
 #pragma unroll
-  for (int j = 1; j < K; j++) {
-    dp[j - 1] = t[left + j - 1] - x;
-    dm[j - 1] = x - t[left - j];
-    T vmprev = T(0);
+  for (int a = 0; a < K; a++) {
+    T w = P[a * K + K - 1];
 #pragma unroll
-    for (int l = 1; l <= j; l++) {
-      T vm = v[l - 1] / (dp[l - 1] + dm[j - l]);
-      v[l - 1] = vm * dp[l - 1] + vmprev;
-      vmprev = vm * dm[j - l];
-    }
-    v[j] = vmprev;
+    for (int m = K - 2; m >= 0; m--) w = w * s + P[a * K + m];
+    v[a] = w;
   }
   base = left - K;
   return 0;
@@ -163,19 +167,23 @@ __device__ __forceinline__ int bs_setup(const T* t, int n, T x, bool xaxis, int&
 
 template <typename T, int KX, int KZ>
 struct SplineW {
-  int base;
+  int base;       // flattened ghosted index of the first coefficient
+  int b3[3];      // first coefficient index per axis
   T wx[KX], wy[KX], wz[KZ];
 };
 
+// nd: node next to q (tracked node), used as the interval hint
 template <typename T, int KX, int KZ>
-__device__ __forceinline__ int spline_setup(const GridDev<T>& g, const T q[3], SplineW<T, KX, KZ>& s) {
+__device__ __forceinline__ int spline_setup(const GridDev<T>& g, const T q[3], const int nd[3],
+                                            SplineW<T, KX, KZ>& s) {
   int bx, by, bz;
-  int rx = bs_setup<T, KX>(g.kn[0], g.N[0], q[0], true, bx, s.wx);
-  int ry = bs_setup<T, KX>(g.kn[1], g.N[1], q[1], false, by, s.wy);
-  int rz = bs_setup<T, KZ>(g.kn[2], g.N[2], q[2], false, bz, s.wz);
+  int rx = bs_setup<T, KX>(g.kn[0], g.bpoly[0], g.N[0], q[0], true, nd[0] + (KX + 2) / 2, bx, s.wx);
+  int ry = bs_setup<T, KX>(g.kn[1], g.bpoly[1], g.N[1], q[1], false, nd[1] + (KX + 2) / 2, by, s.wy);
+  int rz = bs_setup<T, KZ>(g.kn[2], g.bpoly[2], g.N[2], q[2], false, nd[2] + (KZ + 2) / 2, bz, s.wz);
   if (rx == 1 || ry == 1 || rz == 1) return 1;
   if (ry || rz) return 2;
   s.base = bx + g.N[0] * (by + g.N[1] * bz);
+  s.b3[0] = bx; s.b3[1] = by; s.b3[2] = bz;
   return 0;
 }
 
@@ -199,64 +207,13 @@ __device__ __forceinline__ T spline_sum(const GridDev<T>& g, const SplineW<T, KX
   return acc;
 }
 
-// ---- velocity interpolants: blend of u_n and u_np1 at weight ct ----
+// ---- interpolants of a vector field f at q, bracket/hint from node nd.  For
+// the velocity, f is the stage's time-blended field (values for NEAREST and
+// LINEAR, spline coefficients for QUADRATIC/CUBIC); for the flowmap, the
+// (fitted) flowmap. ----
 template <typename T>
-struct VelNearest {
-  CF3<T> un, unp1;
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T ct, T out[3],
-                             unsigned long long&) const {
-    int l = gid(g, iclamp(nd[0], 0, g.n[0] + 1), iclamp(nd[1], 0, g.n[1] + 1), iclamp(nd[2], 0, g.n[2] + 1));
-#pragma unroll
-    for (int c = 0; c < 3; c++) out[c] = (T(1) - ct) * un.c[c][l] + ct * unp1.c[c][l];
-  }
-};
-
-template <typename T>
-struct VelLinear {
-  CF3<T> un, unp1;
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T ct, T out[3],
-                             unsigned long long&) const {
-    LinW<T> w;
-    lin_setup(g, q, nd, w);
-    const int b = w.base, sy = g.N[0], sz = g.N[0] * g.N[1];
-    const T a = T(1) - ct;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const T* A = un.c[c];
-      const T* B = unp1.c[c];
-#define BL(o) (a * A[b + (o)] + ct * B[b + (o)])
-      out[c] = trilin(w, BL(0), BL(1), BL(sy), BL(sy + 1), BL(sz), BL(sz + 1), BL(sz + sy), BL(sz + sy + 1));
-#undef BL
-    }
-  }
-};
-
-template <typename T, int KX, int KZ>
-struct VelSpline {
-  CF3<T> un, unp1;  // spline coefficients of u_n and u_np1
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T ct, T out[3],
-                             unsigned long long& oob) const {
-    SplineW<T, KX, KZ> s;
-    int r = spline_setup(g, q, s);
-    if (r) {
-      if (r == 1) oob++;
-      out[0] = out[1] = out[2] = T(0);
-      return;
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      T vn = spline_sum(g, s, un.c[c]);
-      T vp = spline_sum(g, s, unp1.c[c]);
-      out[c] = (T(1) - ct) * vn + ct * vp;
-    }
-  }
-};
-
-// ---- single-field interpolants (flowmap) ----
-template <typename T>
-struct FieldNearest {
-  CF3<T> f;
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T out[3],
+struct IntNearest {
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
                              unsigned long long&) const {
     int l = gid(g, iclamp(nd[0], 0, g.n[0] + 1), iclamp(nd[1], 0, g.n[1] + 1), iclamp(nd[2], 0, g.n[2] + 1));
 #pragma unroll
@@ -265,9 +222,8 @@ struct FieldNearest {
 };
 
 template <typename T>
-struct FieldLinear {
-  CF3<T> f;
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T out[3],
+struct IntLinear {
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
                              unsigned long long&) const {
     LinW<T> w;
     lin_setup(g, q, nd, w);
@@ -282,12 +238,11 @@ struct FieldLinear {
 };
 
 template <typename T, int KX, int KZ>
-struct FieldSpline {
-  CF3<T> f;  // spline coefficients
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], T out[3],
+struct IntSpline {
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
                              unsigned long long& oob) const {
     SplineW<T, KX, KZ> s;
-    int r = spline_setup(g, q, s);
+    int r = spline_setup(g, q, nd, s);
     if (r) {
       if (r == 1) oob++;
       out[0] = out[1] = out[2] = T(0);
@@ -313,36 +268,36 @@ __device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams
   switch (sp.integ) {
     case kEuler:
     case kTrapezoidal:
-      vel(g, xp, nd, sp.ct[0], up, oob);
+      vel(g, xp, nd, sp.v[0], up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) xp[d] = xp[d] + dt * up[d];
       break;
     case kRK2:
-      vel(g, xp, nd, sp.ct[0], up, oob);
+      vel(g, xp, nd, sp.v[0], up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) xp[d] = xp[d] + HALF * dt * up[d];
-      vel(g, xp, nd, sp.ct[1], up, oob);
+      vel(g, xp, nd, sp.v[1], up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) xp[d] = xn[d] + dt * up[d];
       break;
     case kRK3:
 #pragma unroll
       for (int d = 0; d < 3; d++) x1[d] = xn[d];
-      vel(g, xp, nd, sp.ct[0], up, oob);
+      vel(g, xp, nd, sp.v[0], up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         x1[d] = x1[d] + SIXTH * k[d];
         xp[d] = xn[d] + HALF * k[d];
       }
-      vel(g, xp, nd, sp.ct[1], up, oob);
+      vel(g, xp, nd, sp.v[1], up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         xp[d] = xn[d] + TWO * k[d];
         x1[d] = x1[d] + TWOTHIRD * k[d];
       }
-      vel(g, xp, nd, sp.ct[2], up, oob);
+      vel(g, xp, nd, sp.v[2], up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
@@ -353,28 +308,28 @@ __device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams
     default:
 #pragma unroll
       for (int d = 0; d < 3; d++) x1[d] = xn[d];
-      vel(g, xp, nd, sp.ct[0], up, oob);
+      vel(g, xp, nd, sp.v[0], up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         x1[d] = x1[d] + SIXTH * k[d];
         xp[d] = xn[d] + HALF * k[d];
       }
-      vel(g, xp, nd, sp.ct[1], up, oob);
+      vel(g, xp, nd, sp.v[1], up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         x1[d] = x1[d] + THIRD * k[d];
         xp[d] = xn[d] + HALF * k[d];
       }
-      vel(g, xp, nd, sp.ct[2], up, oob);
+      vel(g, xp, nd, sp.v[2], up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         x1[d] = x1[d] + THIRD * k[d];
         xp[d] = xn[d] + k[d];
       }
-      vel(g, xp, nd, sp.ct[3], up, oob);
+      vel(g, xp, nd, sp.v[3], up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
@@ -433,23 +388,35 @@ __device__ __forceinline__ bool set_lp_bc(const GridDev<T>& g, int nd[3], T q[3]
 }
 
 // One pass of the 27-neighbour nearest-node search with the 0.99
-// hysteresis (lp_tracking_m.f90:72-109).  Contraction is off so the squared
-// distances round exactly as the CPU does.
+// hysteresis (lp_tracking_m.f90:72-109), in the reference's neighbour order.
+// The three candidate coordinates per axis are loaded once; contraction is
+// off so the squared distances round exactly as the CPU does.
 template <typename T>
-__device__ __noinline__ bool track_pass(const GridDev<T>& g, const T q[3], int nd[3]) {
+__device__ __forceinline__ bool track_pass(const GridDev<T>& g, const T q[3], int nd[3]) {
 #pragma clang fp contract(off)
+  T sq[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+#pragma unroll
+    for (int o = 0; o < 3; o++) {
+      const T c = g.gl[d][iclamp(nd[d] + o - 1, 0, g.n[d] + 1)] - q[d];
+      sq[d][o] = c * c;
+    }
+  }
+  const bool fx = g.n[0] == 1, fy = g.n[1] == 1, fz = g.n[2] == 1;
   const T huge = sizeof(T) == 8 ? T(DBL_MAX) : T(FLT_MAX);
   T rmin = huge;
   int oi = 0, oj = 0, ok = 0;
-  for (int nb = 0; nb < 27; nb++) {
-    const int di = c_nbr[nb][0], dj = c_nbr[nb][1], dk = c_nbr[nb][2];
-    if ((g.n[0] == 1 && di) || (g.n[1] == 1 && dj) || (g.n[2] == 1 && dk)) continue;
-    const int i = iclamp(nd[0] + di, 0, g.n[0] + 1);
-    const int j = iclamp(nd[1] + dj, 0, g.n[1] + 1);
-    const int k = iclamp(nd[2] + dk, 0, g.n[2] + 1);
-    const T a = g.gl[0][i] - q[0], b = g.gl[1][j] - q[1], c = g.gl[2][k] - q[2];
-    const T aa = a * a, bb2 = b * b, cc = c * c;
-    const T rsq = aa + bb2 + cc;
+  constexpr int8_t nb[27][3] = {
+      {0, 0, 0},   {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},  {1, 0, 0},   {0, 1, 0},  {0, 0, 1},
+      {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {-1, 0, -1}, {1, 0, -1}, {-1, 1, -1}, {0, 1, -1},
+      {1, 1, -1},  {-1, -1, 0}, {1, -1, 0},  {-1, 1, 0},  {1, 1, 0},   {-1, -1, 1}, {0, -1, 1},
+      {1, -1, 1},  {-1, 0, 1},  {1, 0, 1},   {-1, 1, 1},  {0, 1, 1},   {1, 1, 1}};
+#pragma unroll
+  for (int b = 0; b < 27; b++) {
+    const int di = nb[b][0], dj = nb[b][1], dk = nb[b][2];
+    if ((fx && di) || (fy && dj) || (fz && dk)) continue;
+    const T rsq = (sq[0][di + 1] + sq[1][dj + 1]) + sq[2][dk + 1];
     if (rsq < T(0.99) * rmin) {
       rmin = rsq;
       oi = di; oj = dj; ok = dk;
@@ -540,25 +507,28 @@ __global__ void __launch_bounds__(kBlock) k_vel_ghost(GridDev<T> g, F3<T> u, CF3
 }
 
 // MASK nodes get zero velocity; cfl = sum_d |u_d / delta_d| over interior
-// nodes, block max folded into st->cfl_bits (cfl >= 0, so the IEEE bit
-// pattern orders as an unsigned integer).
+// nodes (lp_motion_m.f90:90-98).  Grid-stride loop, block max folded into
+// st->cfl_bits (cfl >= 0, so the IEEE bit pattern orders as an unsigned
+// integer).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cfl(GridDev<T> g, F3<T> u, DevStatus* st) {
   __shared__ double smax[kBlock / 64];
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
   double c = 0.0;
-  if (p < g.np) {
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < g.np;
+       p += (long long)gridDim.x * blockDim.x) {
     int i, j, k;
-    decode_interior(g, p, i, j, k);
+    decode_interior(g, (int)p, i, j, k);
     const int l = gid(g, i, j, k);
+    T a = u.c[0][l], b = u.c[1][l], d = u.c[2][l];
     if (g.flag[l] == kMask) {
       u.c[0][l] = T(0); u.c[1][l] = T(0); u.c[2][l] = T(0);
+      a = b = d = T(0);
     }
     T s = T(0);
-    s = s + fabs(u.c[0][l] * g.dl[0][p]);
-    s = s + fabs(u.c[1][l] * g.dl[1][p]);
-    s = s + fabs(u.c[2][l] * g.dl[2][p]);
-    c = (double)s;
+    s = s + fabs(a * g.dl[0][p]);
+    s = s + fabs(b * g.dl[1][p]);
+    s = s + fabs(d * g.dl[2][p]);
+    c = fmax(c, (double)s);
   }
   for (int o = 32; o > 0; o >>= 1) c = fmax(c, __shfl_down(c, o, 64));
   if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = c;
@@ -647,10 +617,11 @@ __global__ void __launch_bounds__(kBlock) k_field_from_origin(GridDev<T> g, CF3<
 // velocity (bracket of the origin node, lp_motion_m.f90:171-327), particle
 // BC, old flowmap at the departure point, displacement form write.
 // ---------------------------------------------------------------------------
-template <typename T, class V, class FM>
-__global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T> sp, V vel, FM fmi, F3<T> fmnew,
+template <typename T, class I>
+__global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
                                                     DevStatus* st) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const I interp{};
+  const int p = xcd_thread_index();
   if (p >= g.np) return;
   int i, j, k;
   decode_interior(g, p, i, j, k);
@@ -659,14 +630,14 @@ __global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T>
   T xp[3] = {x0[0], x0[1], x0[2]};
   unsigned long long oob = 0;
   int err = 0;
-  integrate(g, sp, vel, nd, x0, xp, oob);
+  integrate(g, sp, interp, nd, x0, xp, oob);
   project_flat(g, nd, xp);
   if (set_lp_bc(g, nd, xp, err)) {
     xp[0] = x0[0]; xp[1] = x0[1]; xp[2] = x0[2];
   }
   int no[3] = {i, j, k};
   T fmp[3];
-  fmi(g, xp, no, fmp, oob);
+  interp(g, xp, no, fmold, fmp, oob);
   const int l = gid(g, i, j, k);
   const int f = g.flag[l];
   const T mask = (f != kMask) ? T(1) : T(0);
@@ -679,22 +650,35 @@ __global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T>
   flush_status(st, err, oob);
 }
 
+// time blend u = (1-ct) u_n + ct u_np1 over the ghosted grid, in the
+// reference's expression order (lp_motion_m.f90:171-173)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_blend(F3<T> out, CF3<T> a, CF3<T> b, T ct, long long nt) {
+#pragma clang fp contract(off)
+  const T w0 = T(1) - ct;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < nt; l += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) out.c[c][l] = w0 * a.c[c][l] + ct * b.c[c][l];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Stage (a'): forward particle sub-cycle (update_lp, lp_motion_m.f90:15-54):
 // integrate, BC, stick, dx accumulation, one tracking pass, periodic wrap.
 // ---------------------------------------------------------------------------
-template <typename T, class V>
-__global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T> sp, V vel, F3<T> xp, F3<T> dx,
+template <typename T, class I>
+__global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T> sp, F3<T> xp, F3<T> dx,
                                                      int16_t* __restrict__ ns0, int16_t* __restrict__ ns1,
                                                      int16_t* __restrict__ ns2, DevStatus* st) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_thread_index();
   if (p >= g.np) return;
   T q[3] = {xp.c[0][p], xp.c[1][p], xp.c[2][p]};
   int nd[3] = {ns0[p], ns1[p], ns2[p]};
   const T xn[3] = {q[0], q[1], q[2]};
   unsigned long long oob = 0;
   int err = 0;
-  integrate(g, sp, vel, nd, xn, q, oob);
+  const I interp{};
+  integrate(g, sp, interp, nd, xn, q, oob);
   project_flat(g, nd, q);
   if (set_lp_bc(g, nd, q, err)) {
     q[0] = xn[0]; q[1] = xn[1]; q[2] = xn[2];
@@ -730,11 +714,11 @@ __global__ void __launch_bounds__(kBlock) k_fwd_reset(GridDev<T> g, F3<T> xp, F3
 // periodic image / node guess) and track recursively with a periodic wrap
 // after each pass (the direction is FWD during reconstruction).
 // ---------------------------------------------------------------------------
-template <typename T, class FM>
-__global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, FM fmi, const T* __restrict__ slots,
+template <typename T, class I>
+__global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __restrict__ slots,
                                                     long long stride, const int* __restrict__ order, int nslot,
                                                     F3<T> dxout, DevStatus* st) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_thread_index();
   if (p >= g.np) return;
   int i, j, k;
   decode_interior(g, p, i, j, k);
@@ -745,14 +729,11 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, FM fmi, const 
   int err = 0;
   for (int s = 0; s < nslot; s++) {
     const T* base = slots + (long long)order[s] * 3 * stride;
-    FM f = fmi;
-    f.f.c[0] = base;
-    f.f.c[1] = base + stride;
-    f.f.c[2] = base + 2 * stride;
+    const CF3<T> f{{base, base + stride, base + 2 * stride}};
     project_flat(g, nd, q);
     set_lp_bc(g, nd, q, err);
     T up[3];
-    f(g, q, nd, up, oob);
+    I{}(g, q, nd, f, up, oob);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       q[d] = q[d] + up[d];
@@ -930,6 +911,63 @@ __global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restric
   }
 }
 
+// x-axis pass of the fit: the lines are contiguous rows, so a block stages
+// kFitLines consecutive rows in LDS with coalesced loads/stores and each lane
+// solves one row (padded stride n+1 avoids bank conflicts).  L <= 64 rows
+// per block of 64 threads, sized so that the tile fits in 64 KiB of LDS.
+constexpr int kFitLines = 64;
+constexpr int kFitThreads = 256;
+template <typename T, int K>
+__global__ void __launch_bounds__(kFitThreads) k_fit_x(F3<T> a, const T* __restrict__ W, int n, int nlines, int L) {
+  extern __shared__ unsigned char fit_smem[];
+  T* sm = reinterpret_cast<T*>(fit_smem);
+  const int comp = blockIdx.y;
+  const int line0 = blockIdx.x * L;
+  const int nl = min(L, nlines - line0);
+  T* base = a.c[comp] + (long long)line0 * n;
+  const int tot = nl * n;
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int l = e / n, r = e - l * n;
+    sm[l * (n + 1) + r] = base[e];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nl) {
+    T* b = sm + threadIdx.x * (n + 1);
+    constexpr int M = K - 1, NB = 2 * K - 1;
+    T win[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) win[m] = T(0);
+    for (int r = 0; r < n; r++) {
+      T v = b[r];
+#pragma unroll
+      for (int jj = M; jj >= 1; jj--)
+        if (r - jj >= 0) v = v - win[jj - 1] * W[(jj + M) + NB * (r - jj)];
+#pragma unroll
+      for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
+      win[0] = v;
+      b[r] = v;
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) win[m] = T(0);
+    for (int r = n - 1; r >= 0; r--) {
+      T v = b[r];
+#pragma unroll
+      for (int jj = M; jj >= 1; jj--)
+        if (r + jj <= n - 1) v = v - win[jj - 1] * W[(M - jj) + NB * (r + jj)];
+      v = v / W[M + NB * r];
+#pragma unroll
+      for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
+      win[0] = v;
+      b[r] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int l = e / n, r = e - l * n;
+    base[e] = sm[l * (n + 1) + r];
+  }
+}
+
 // interior (or per-origin) values to a user-typed array, Fortran order
 template <typename T, typename U>
 __global__ void __launch_bounds__(kBlock) k_to_user(GridDev<T> g, CF3<T> s, int ghosted, U* a, U* b, U* c) {
@@ -964,7 +1002,8 @@ void launch_vel_ghost_impl(const GridDev<T>& g, F3<T> u, CF3<T> uold, DevStatus*
 
 template <typename T>
 void launch_cfl(const GridDev<T>& g, F3<T> u, DevStatus* st, hipStream_t s) {
-  hipLaunchKernelGGL((k_cfl<T>), grid_for(g.np), dim3(kBlock), 0, s, g, u, st);
+  const long long nb = std::min<long long>((g.np + kBlock - 1) / kBlock, 2048);
+  hipLaunchKernelGGL((k_cfl<T>), dim3((unsigned)nb), dim3(kBlock), 0, s, g, u, st);
 }
 
 template <typename T>
@@ -984,6 +1023,17 @@ void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t s
   for (int axis = 0; axis < 3; axis++) {
     const int n = g.N[axis];
     const int nlines = (int)(g.nt / n);
+    const int L = (int)std::min<size_t>(kFitLines, (64 * 1024) / (sizeof(T) * (size_t)(n + 1)));
+    if (axis == 0 && L >= 8) {
+      const size_t lds = sizeof(T) * (size_t)L * (n + 1);
+      dim3 gx((nlines + L - 1) / L, ncomp);
+      switch (g.kord[0]) {
+        case 2: hipLaunchKernelGGL((k_fit_x<T, 2>), gx, dim3(kFitThreads), lds, s, coef, g.band[0], n, nlines, L); break;
+        case 3: hipLaunchKernelGGL((k_fit_x<T, 3>), gx, dim3(kFitThreads), lds, s, coef, g.band[0], n, nlines, L); break;
+        default: hipLaunchKernelGGL((k_fit_x<T, 4>), gx, dim3(kFitThreads), lds, s, coef, g.band[0], n, nlines, L); break;
+      }
+      continue;
+    }
     dim3 grid((nlines + kBlock - 1) / kBlock, ncomp);
     switch (g.kord[axis]) {
       case 2: hipLaunchKernelGGL((k_fit_axis<T, 2>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
@@ -1006,39 +1056,40 @@ bool dispatch_spline(const GridDev<T>& g, F&& f) {
 }
 
 template <typename T>
-void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> un, CF3<T> unp1,
-                    CF3<T> fmold, F3<T> fmnew, DevStatus* st, hipStream_t s) {
+void launch_blend(const GridDev<T>& g, F3<T> out, CF3<T> a, CF3<T> b, T ct, hipStream_t s) {
+  const long long nb = std::min<long long>((g.nt + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL((k_blend<T>), dim3((unsigned)nb), dim3(kBlock), 0, s, out, a, b, ct, g.nt);
+}
+
+template <typename T>
+void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, F3<T> fmnew,
+                    DevStatus* st, hipStream_t s) {
   const dim3 grid = grid_for(g.np), blk(kBlock);
   if (interp == kLinear) {
-    hipLaunchKernelGGL((k_sl_step<T, VelLinear<T>, FieldLinear<T>>), grid, blk, 0, s, g, sp,
-                       VelLinear<T>{un, unp1}, FieldLinear<T>{fmold}, fmnew, st);
+    hipLaunchKernelGGL((k_sl_step<T, IntLinear<T>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
   } else if (interp == kNearest) {
-    hipLaunchKernelGGL((k_sl_step<T, VelNearest<T>, FieldNearest<T>>), grid, blk, 0, s, g, sp,
-                       VelNearest<T>{un, unp1}, FieldNearest<T>{fmold}, fmnew, st);
+    hipLaunchKernelGGL((k_sl_step<T, IntNearest<T>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
   } else {
     dispatch_spline(g, [&](auto KX, auto KZ) {
       constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
-      hipLaunchKernelGGL((k_sl_step<T, VelSpline<T, kx, kz>, FieldSpline<T, kx, kz>>), grid, blk, 0, s, g, sp,
-                         VelSpline<T, kx, kz>{un, unp1}, FieldSpline<T, kx, kz>{fmold}, fmnew, st);
+      hipLaunchKernelGGL((k_sl_step<T, IntSpline<T, kx, kz>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
     });
   }
 }
 
 template <typename T>
-void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> un, CF3<T> unp1, F3<T> xp,
-                     F3<T> dx, int16_t* ns0, int16_t* ns1, int16_t* ns2, DevStatus* st, hipStream_t s) {
+void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, F3<T> xp, F3<T> dx, int16_t* ns0,
+                     int16_t* ns1, int16_t* ns2, DevStatus* st, hipStream_t s) {
   const dim3 grid = grid_for(g.np), blk(kBlock);
   if (interp == kLinear) {
-    hipLaunchKernelGGL((k_fwd_step<T, VelLinear<T>>), grid, blk, 0, s, g, sp, VelLinear<T>{un, unp1}, xp, dx, ns0,
-                       ns1, ns2, st);
+    hipLaunchKernelGGL((k_fwd_step<T, IntLinear<T>>), grid, blk, 0, s, g, sp, xp, dx, ns0, ns1, ns2, st);
   } else if (interp == kNearest) {
-    hipLaunchKernelGGL((k_fwd_step<T, VelNearest<T>>), grid, blk, 0, s, g, sp, VelNearest<T>{un, unp1}, xp, dx,
-                       ns0, ns1, ns2, st);
+    hipLaunchKernelGGL((k_fwd_step<T, IntNearest<T>>), grid, blk, 0, s, g, sp, xp, dx, ns0, ns1, ns2, st);
   } else {
     dispatch_spline(g, [&](auto KX, auto KZ) {
       constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
-      hipLaunchKernelGGL((k_fwd_step<T, VelSpline<T, kx, kz>>), grid, blk, 0, s, g, sp,
-                         VelSpline<T, kx, kz>{un, unp1}, xp, dx, ns0, ns1, ns2, st);
+      hipLaunchKernelGGL((k_fwd_step<T, IntSpline<T, kx, kz>>), grid, blk, 0, s, g, sp, xp, dx, ns0, ns1, ns2,
+                         st);
     });
   }
 }
@@ -1053,18 +1104,15 @@ template <typename T>
 void launch_compose(const GridDev<T>& g, int interp, const T* slots, long long stride, const int* order, int nslot,
                     F3<T> dxout, DevStatus* st, hipStream_t s) {
   const dim3 grid = grid_for(g.np), blk(kBlock);
-  CF3<T> none{{nullptr, nullptr, nullptr}};
   if (interp == kLinear) {
-    hipLaunchKernelGGL((k_compose<T, FieldLinear<T>>), grid, blk, 0, s, g, FieldLinear<T>{none}, slots, stride,
-                       order, nslot, dxout, st);
+    hipLaunchKernelGGL((k_compose<T, IntLinear<T>>), grid, blk, 0, s, g, slots, stride, order, nslot, dxout, st);
   } else if (interp == kNearest) {
-    hipLaunchKernelGGL((k_compose<T, FieldNearest<T>>), grid, blk, 0, s, g, FieldNearest<T>{none}, slots, stride,
-                       order, nslot, dxout, st);
+    hipLaunchKernelGGL((k_compose<T, IntNearest<T>>), grid, blk, 0, s, g, slots, stride, order, nslot, dxout, st);
   } else {
     dispatch_spline(g, [&](auto KX, auto KZ) {
       constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
-      hipLaunchKernelGGL((k_compose<T, FieldSpline<T, kx, kz>>), grid, blk, 0, s, g, FieldSpline<T, kx, kz>{none},
-                         slots, stride, order, nslot, dxout, st);
+      hipLaunchKernelGGL((k_compose<T, IntSpline<T, kx, kz>>), grid, blk, 0, s, g, slots, stride, order, nslot,
+                         dxout, st);
     });
   }
 }
@@ -1096,10 +1144,11 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
   template void launch_fm_ghost<T>(const GridDev<T>&, F3<T>, CF3<T>, DevStatus*, hipStream_t);                \
   template void launch_field_from_origin<T>(const GridDev<T>&, CF3<T>, F3<T>, hipStream_t);                   \
   template void launch_spline_fit<T>(const GridDev<T>&, F3<T>, int, hipStream_t);                             \
-  template void launch_sl_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, CF3<T>, CF3<T>,     \
-                                  F3<T>, DevStatus*, hipStream_t);                                            \
-  template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, CF3<T>, F3<T>,     \
-                                   F3<T>, int16_t*, int16_t*, int16_t*, DevStatus*, hipStream_t);             \
+  template void launch_blend<T>(const GridDev<T>&, F3<T>, CF3<T>, CF3<T>, T, hipStream_t);                    \
+  template void launch_sl_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, F3<T>, DevStatus*,  \
+                                  hipStream_t);                                                               \
+  template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, F3<T>, F3<T>, int16_t*,    \
+                                   int16_t*, int16_t*, DevStatus*, hipStream_t);                              \
   template void launch_fwd_reset<T>(const GridDev<T>&, F3<T>, F3<T>, int16_t*, int16_t*, int16_t*,            \
                                     hipStream_t);                                                             \
   template void launch_compose<T>(const GridDev<T>&, int, const T*, long long, const int*, int, F3<T>,       \

@@ -37,6 +37,8 @@ struct GridDev {
   int kord[3];       // spline orders
   const T* kn[3];    // spline knots, N[d] + kord[d] each
   const T* band[3];  // banded LU of the collocation matrices, (2k-1)*N[d]
+  const T* bpoly[3]; // per knot interval: basis polynomials in the normalised
+                     // interval coordinate and 1/length, (N[d]+1)*(k*k+1)
 };
 
 template <typename T>
@@ -60,7 +62,10 @@ template <typename T>
 struct StageParams {
   int integ;   // Integrator
   T dt;        // signed sub-cycle step
-  T ct[4];     // time-blend weights of the stages, clamped to [0,1]
+  // velocity of each stage, u = (1-ct) u_n + ct u_np1 at the stage's ct
+  // (lp_motion_m.f90:170-316): grid values for NEAREST/LINEAR, spline
+  // coefficients for QUADRATIC/CUBIC
+  CF3<T> v[4];
 };
 
 // ---- launchers (kernels.hip); all asynchronous on `stream` ----
@@ -76,12 +81,13 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
 template <typename T>
 void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t stream);
 template <typename T>
-void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> un,
-                    CF3<T> unp1, CF3<T> fmold, F3<T> fmnew, DevStatus* st, hipStream_t stream);
+void launch_blend(const GridDev<T>& g, F3<T> out, CF3<T> a, CF3<T> b, T ct, hipStream_t stream);
 template <typename T>
-void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> un,
-                     CF3<T> unp1, F3<T> xp, F3<T> dx, int16_t* ns0, int16_t* ns1, int16_t* ns2,
-                     DevStatus* st, hipStream_t stream);
+void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold,
+                    F3<T> fmnew, DevStatus* st, hipStream_t stream);
+template <typename T>
+void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, F3<T> xp, F3<T> dx,
+                     int16_t* ns0, int16_t* ns1, int16_t* ns2, DevStatus* st, hipStream_t stream);
 template <typename T>
 void launch_fwd_reset(const GridDev<T>& g, F3<T> xp, F3<T> dx, int16_t* ns0, int16_t* ns1,
                       int16_t* ns2, hipStream_t stream);

@@ -67,3 +67,28 @@ def test_gyre_slip_rk3_cubic_fp64(dp):
     recs, hs = run_both(dp, "gyre_slip", n=(32, 16), integrator=F.RK3, interpolator=F.CUBIC, T=0.6, h=0.2,
                         dt=0.05, nsteps=16)
     _check(recs, hs, 1e-10, 1e-9)
+
+
+def test_abc_rk4_cubic_fp64_multibrick(dp):
+    """36^3 spans several 8x4x4 / 16x4x4 bricks, including partial ones, so
+    the tiled semi-Lagrangian kernel's halo and fallback paths are covered."""
+    recs, hs = run_both(dp, "abc", n=36, integrator=F.RK4, interpolator=F.CUBIC, T=0.2, h=0.1,
+                        dt=0.05, nsteps=5, every_step=2)
+    assert _check(recs, hs, 1e-10, 1e-9) >= 2
+
+
+@pytest.fixture(scope="module")
+def sp():
+    import libcfd2lcs_amd as L
+    lib = L.Cfd2lcs("sp")
+    yield lib
+    lib.finalize()
+
+
+@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC])
+def test_abc_fp32_against_fp64_oracle(sp, interp):
+    """FP32 library (lcsdata_t = float, FP32 compute) against the FP64
+    oracle: flowmaps within 2e-5 relative, FTLE within 1e-3 absolute."""
+    recs, hs = run_both(sp, "abc", n=20, integrator=F.RK4, interpolator=interp, T=0.4, h=0.2,
+                        dt=0.05, nsteps=10, every_step=3)
+    assert _check(recs, hs, 2e-5, 1e-3) >= 2
