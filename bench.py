@@ -36,7 +36,7 @@ def abc_grid(n):
     return X, Y, Z
 
 
-def run_gpu(variant, n, steps, warmup, dt, T, h):
+def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True):
     import torch
 
     import libcfd2lcs_amd as L
@@ -72,9 +72,8 @@ def run_gpu(variant, n, steps, warmup, dt, T, h):
         step += 1
     if lib.error:
         raise RuntimeError("update failed during warm-up")
-    t0_sec, _ = timers(lib)
+    t0_sec, t0_cnt = timers(lib)
     st0 = lib.stats()
-    nout0 = lib.noutput(hf) + lib.noutput(hb)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -86,23 +85,49 @@ def run_gpu(variant, n, steps, warmup, dt, T, h):
         raise RuntimeError("update failed in the timed region")
     st1 = lib.stats()
     t1_sec, t1_cnt = timers(lib)
-    dsec = t1_sec - t0_sec
-    _, c0 = t0_sec, None
-    nout = lib.noutput(hf) + lib.noutput(hb) - nout0
     upd = (st1["integrations_fwd"] - st0["integrations_fwd"]) + (st1["integrations_bkwd"] - st0["integrations_bkwd"])
     res = {
         "elapsed": elapsed,
         "updates": upd,
         "nsub": int(st1["nsub"]),
-        "ftle_outputs": nout,
-        "t_compose_ftle": float(dsec[2] + dsec[3]),
-        "t_fwd": float(dsec[0]),
-        "t_bkwd": float(dsec[1]),
-        "kernel_sec": {TIMER_NAMES[k]: float(t1_sec[k]) for k in TIMER_NAMES},
-        "kernel_cnt": {TIMER_NAMES[k]: int(t1_cnt[k]) for k in TIMER_NAMES},
+        "kernel_sec": {TIMER_NAMES[k]: float(t1_sec[k] - t0_sec[k]) for k in TIMER_NAMES},
+        "kernel_cnt": {TIMER_NAMES[k]: int(t1_cnt[k] - t0_cnt[k]) for k in TIMER_NAMES},
         "np": n ** 3,
-        "oob": st1["spline_out_of_range"],
     }
+    # FTLE fields: continue the same run until both diagnostics have composed
+    # outputs over a full ring of T/h sub-intervals, and time those outputs
+    # (composition + gradient/Cauchy-Green/FTLE, GPU time) and the wall time
+    # of the h interval that produced them.
+    if u2:
+        nstep = int(round(T / h))
+        t_u2, n_u2, wall_u2, nint_u2 = 0.0, 0, 0.0, 0
+        last_out_wall = None
+        while n_u2 < 2:
+            nb = (lib.noutput(hf), lib.noutput(hb))
+            sb, _ = timers(lib)
+            lib.update_device(*ptrs, step * dt)
+            step += 1
+            na = (lib.noutput(hf), lib.noutput(hb))
+            if lib.error:
+                raise RuntimeError("update failed while filling the flowmap ring")
+            if na != nb:
+                torch.cuda.synchronize()
+                now = time.perf_counter()
+                sa, _ = timers(lib)
+                full = [a > b and b >= nstep for a, b in zip(na, nb)]
+                if all(full):
+                    t_u2 += float((sa[2] - sb[2]) + (sa[3] - sb[3]))
+                    n_u2 += 2
+                    if last_out_wall is not None:
+                        wall_u2 += now - last_out_wall
+                        nint_u2 += 1
+                last_out_wall = now
+                print(f"[bench] {variant}: step {step}, outputs {na}", file=sys.stderr, flush=True)
+        res["u2_fields"] = n_u2
+        res["u2_sec"] = t_u2
+        res["u2_pipeline"] = (2 * nint_u2 / wall_u2) if nint_u2 else None
+        res["u2_wall_interval"] = (wall_u2 / nint_u2) if nint_u2 else None
+        res["steps_total"] = step
     lib.finalize()
     del u, v, w
     torch.cuda.empty_cache()
@@ -186,6 +211,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ftle", action="store_true", help="skip the full-ring FTLE fields/s measurement")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -194,12 +220,12 @@ def main():
               "multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
     dt, T, h = 0.02, 5.0, 0.5
-    r32 = run_gpu("sp", args.n, args.steps, args.warmup, dt, T, h)
+    r32 = run_gpu("sp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle)
     print(f"[bench] fp32: {r32['updates'] / r32['elapsed']:.4e} updates/s, "
           f"{1e3 * r32['elapsed'] / args.steps:.1f} ms/step", file=sys.stderr, flush=True)
     r64 = None
     if not args.no_fp64:
-        r64 = run_gpu("dp", args.n, args.steps, args.warmup, dt, T, h)
+        r64 = run_gpu("dp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle)
         print(f"[bench] fp64: {r64['updates'] / r64['elapsed']:.4e} updates/s", file=sys.stderr, flush=True)
     # dominant kernel of the FP32 run
     ks = r32["kernel_sec"]
@@ -223,7 +249,8 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_n, args.cpu_budget)
     value = r32["updates"] / r32["elapsed"]
-    ftle32 = r32["ftle_outputs"] / r32["t_compose_ftle"] if r32["t_compose_ftle"] > 0 else None
+    def u2(r):
+        return (r["u2_fields"] / r["u2_sec"]) if r.get("u2_sec") else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -240,11 +267,14 @@ def main():
         "config": {"workload": f"abc{args.n}_rk4_cubic_fwd_bkwd", "grid": [args.n] * 3, "dt": dt, "T": T,
                    "h": h, "integrator": "RK4", "interpolator": "CUBIC", "cfl": 0.5, "n_sub": r32["nsub"],
                    "directions": ["FTLE_FWD", "FTLE_BKWD"]},
-        "ftle_fields_per_s": ftle32,
-        "ftle_outputs_timed": r32["ftle_outputs"],
+        "ftle_fields_per_s": u2(r32),
+        "ftle_fields_per_s_pipeline": r32.get("u2_pipeline"),
+        "ftle_note": "fields/s = FTLE fields (composition of T/h = %d sub-intervals + gradient, Cauchy-Green, "
+                     "FP64 eigen-solve over 256^3) per second of GPU time, at a full flowmap ring; pipeline = "
+                     "fields per wall second sustained with both directions advected" % int(round(T / h)),
         "fp64": None if r64 is None else {
             "value": r64["updates"] / r64["elapsed"], "ms_per_step": 1e3 * r64["elapsed"] / args.steps,
-            "ftle_fields_per_s": (r64["ftle_outputs"] / r64["t_compose_ftle"]) if r64["t_compose_ftle"] > 0 else None,
+            "ftle_fields_per_s": u2(r64), "ftle_fields_per_s_pipeline": r64.get("u2_pipeline"),
             "fp32_over_fp64": value / (r64["updates"] / r64["elapsed"])},
         "bwd_fwd_ratio": None,
         "kernel_ms": {k: round(1e3 * s / max(r32["kernel_cnt"][k], 1), 4) for k, s in ks.items()},

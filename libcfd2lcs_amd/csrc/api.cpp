@@ -48,6 +48,7 @@ struct Options {
   U cfl_max = (U)0.5;
   int precision = sizeof(U) == 8 ? LCS_FP64 : LCS_FP32;
   int kernel_timing = 0;  // time every hot kernel launch with HIP events
+  int particle_sort = 1;  // re-order forward particles by node every n updates (0: never)
   bool warn_opt = false, warn_par = false;  // SAVEd warn flags (cfd2lcs.f90:444,632)
 };
 
@@ -141,7 +142,8 @@ struct Engine : EngineBase {
   long long integ_fwd = 0, integ_bkwd = 0;
   // timers: 0 FWD advect, 1 BKWD advect, 2 composition, 3 FTLE (phases);
   // with KERNEL_TIMING: 8 k_fwd_step, 9 k_sl_step, 10 spline fit,
-  // 11 k_compose, 12 k_ftle, 13 ghost/ingest/CFL kernels, 14 time blend
+  // 11 k_compose, 12 k_ftle, 13 ghost/ingest/CFL kernels, 14 time blend,
+  // 15 forward particle sort
   enum { kTimers = 16 };
   double tsum[kTimers] = {};
   long long tcount[kTimers] = {};
@@ -160,7 +162,12 @@ struct Engine : EngineBase {
     std::string label;
     int nstep, nslot;
     std::vector<char> slot_ok, slot_coef_ok;
-    DevBuf xp, dx, ns;     // FWD particles (origin order)
+    DevBuf part[2];        // FWD particle sets (current = part[pcur])
+    int pcur = 0;
+    bool sorted = false;   // part[pcur] is not in origin order
+    int since_sort = 0;    // updates advected since the last reset or sort
+    DevBuf skeys, svals, stemp;  // sort scratch
+    size_t stemp_bytes = 0;
     DevBuf fm[2];          // BKWD semi-Lagrangian flowmap ping-pong (ghosted)
     int fcur = 0;
     DevBuf fmcoef;         // spline coefficients of fm[fcur]
@@ -207,6 +214,56 @@ struct Engine : EngineBase {
   CF3<T> cf3(DevBuf& b) {
     const T* p = (const T*)b.p;
     return CF3<T>{{p, p + g.nt, p + 2 * g.nt}};
+  }
+  // particle-set layout inside one allocation: x[3], d[3] (T), n[3]
+  // (int16), org (int), each array aligned to 256 bytes
+  size_t pset_bytes() const {
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    return 6 * al(sizeof(T) * (size_t)g.np) + 3 * al(sizeof(int16_t) * (size_t)g.np) + al(sizeof(int) * (size_t)g.np);
+  }
+  ParticleSet<T> pset(DevBuf& b) {
+    char* base = (char*)b.p;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      char* q = base + off;
+      off += (bytes + 255) & ~size_t(255);
+      return q;
+    };
+    ParticleSet<T> ps;
+    for (int d = 0; d < 3; d++) ps.x[d] = (T*)take(sizeof(T) * (size_t)g.np);
+    for (int d = 0; d < 3; d++) ps.d[d] = (T*)take(sizeof(T) * (size_t)g.np);
+    for (int d = 0; d < 3; d++) ps.n[d] = (int16_t*)take(sizeof(int16_t) * (size_t)g.np);
+    ps.org = (int*)take(sizeof(int) * (size_t)g.np);
+    return ps;
+  }
+  // forward displacement by origin node: the current set's dx, or (when the
+  // set is sorted) its scatter to origin order in the spare set's dx arrays
+  CF3<T> fwd_dx_by_origin(Diag& dg) {
+    ParticleSet<T> a = pset(dg.part[dg.pcur]);
+    if (!dg.sorted) return CF3<T>{{a.d[0], a.d[1], a.d[2]}};
+    ParticleSet<T> b = pset(dg.part[1 - dg.pcur]);
+    launch_unpermute(g, a.org, CF3<T>{{a.d[0], a.d[1], a.d[2]}}, F3<T>{{b.d[0], b.d[1], b.d[2]}}, stream);
+    return CF3<T>{{b.d[0], b.d[1], b.d[2]}};
+  }
+  bool sort_particles(Diag& dg) {
+    const size_t np = (size_t)g.np;
+    if (!dg.part[1].p) {
+      dg.stemp_bytes = particle_sort_temp_bytes(g.np);
+      if (!dg.stemp_bytes || !dg.part[1].alloc(pset_bytes()) || !dg.skeys.alloc(2 * sizeof(unsigned) * np) ||
+          !dg.svals.alloc(2 * sizeof(unsigned) * np) || !dg.stemp.alloc(dg.stemp_bytes)) {
+        fail("hipMalloc failed (particle sort buffers)");
+        return false;
+      }
+    }
+    if (launch_particle_sort(g, pset(dg.part[dg.pcur]), pset(dg.part[1 - dg.pcur]), (unsigned*)dg.skeys.p,
+                             (unsigned*)dg.svals.p, dg.stemp.p, dg.stemp_bytes, stream)) {
+      fail("particle sort failed");
+      return false;
+    }
+    dg.pcur = 1 - dg.pcur;
+    dg.sorted = true;
+    dg.since_sort = 0;
+    return true;
   }
   F3<T> p3(DevBuf& b) {
     T* p = (T*)b.p;
@@ -405,7 +462,7 @@ struct Engine : EngineBase {
     const size_t fb = fbytes(), pb = sizeof(T) * (size_t)g.np;
     bool ok = true;
     if (dg->dir > 0) {
-      ok = ok && dg->xp.alloc(3 * pb) && dg->dx.alloc(3 * pb) && dg->ns.alloc(3 * sizeof(int16_t) * (size_t)g.np);
+      ok = ok && dg->part[0].alloc(pset_bytes());
     } else {
       ok = ok && dg->fm[0].alloc(3 * fb) && dg->fm[1].alloc(3 * fb);
     }
@@ -434,8 +491,9 @@ struct Engine : EngineBase {
   // reset_lp (lp_m.f90:228-294): particles back on their nodes, fm = 0
   void reset_diag(Diag& dg) {
     if (dg.dir > 0) {
-      int16_t* ns = (int16_t*)dg.ns.p;
-      launch_fwd_reset(g, p3(dg.xp), p3(dg.dx), ns, ns + g.np, ns + 2 * g.np, stream);
+      launch_fwd_reset(g, pset(dg.part[dg.pcur]), stream);
+      dg.sorted = false;
+      dg.since_sort = 0;
     } else {
       (void)hipMemsetAsync(dg.fm[0].p, 0, 3 * fbytes(), stream);
       (void)hipMemsetAsync(dg.fm[1].p, 0, 3 * fbytes(), stream);
@@ -607,14 +665,19 @@ struct Engine : EngineBase {
       timed_begin(tm, dg.dir > 0 ? 0 : 1);
       if (dg.dir > 0) {
         U t = t_n;
-        int16_t* ns = (int16_t*)dg.ns.p;
+        if (g_opt.particle_sort > 0 && dg.since_sort >= g_opt.particle_sort) {
+          bool ok = true;
+          ktimed(15, [&] { ok = sort_particles(dg); });
+          if (!ok) return;
+        }
+        const ParticleSet<T> ps = pset(dg.part[dg.pcur]);
+        const F3<T> xp{{ps.x[0], ps.x[1], ps.x[2]}}, dx{{ps.d[0], ps.d[1], ps.d[2]}};
         for (int s = 0; s < nsub; s++) {
           const StageParams<T> sp = stage_params(t, dt, spl);
-          ktimed(8, [&] {
-            launch_fwd_step(g, interp, sp, p3(dg.xp), p3(dg.dx), ns, ns + g.np, ns + 2 * g.np, st, stream);
-          });
+          ktimed(8, [&] { launch_fwd_step(g, interp, sp, xp, dx, ps.n[0], ps.n[1], ps.n[2], st, stream); });
           t = t + dt;
         }
+        if (std::fabs(dt) > 0) dg.since_sort++;
       } else {
         U t = t_np1;
         if (spl && !dg.fmcoef.p && !dg.fmcoef.alloc(3 * fbytes())) {
@@ -677,7 +740,7 @@ struct Engine : EngineBase {
     T* sp = slot_ptr(dg.slots, step);
     F3<T> slot{{sp, sp + g.nt, sp + 2 * g.nt}};
     if (dg.dir > 0) {
-      launch_field_from_origin(g, cp3(dg.dx), slot, stream);
+      launch_field_from_origin(g, fwd_dx_by_origin(dg), slot, stream);
     } else {
       HIPCHK(hipMemcpyAsync(sp, dg.fm[dg.fcur].p, 3 * fb, hipMemcpyDeviceToDevice, stream));
     }
@@ -766,7 +829,7 @@ struct Engine : EngineBase {
   int get_flowmap(int h, U* a, U* b, U* c) override {
     Diag* dg = diag(h);
     if (!dg) return 1;
-    if (dg->dir > 0) return copy_out(cp3(dg->dx), 0, a, b, c);
+    if (dg->dir > 0) return copy_out(fwd_dx_by_origin(*dg), 0, a, b, c);
     return copy_out(cf3(dg->fm[dg->fcur]), 1, a, b, c);
   }
   int get_composed(int h, U* a, U* b, U* c) override {
@@ -964,6 +1027,9 @@ void cfd2lcs_set_option_c(char option[], int val) {
     }
   } else if (o == "KERNEL_TIMING") {
     boolopt(g_opt.kernel_timing);
+  } else if (o == "PARTICLE_SORT") {
+    if (val < 0) warn = true;
+    else g_opt.particle_sort = val;
   } else if (o == "UPDATE_FREQ") {
   } else if (o == "INTERPOLATOR") {
     if (val != NEAREST_NBR && val != LINEAR && val != QUADRATIC && val != CUBIC && val != TSE && val != TSE_LIMIT)

@@ -695,15 +695,15 @@ __global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T
 
 // reset_lp for forward particles (lp_m.f90:228-294, resolution 0)
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_fwd_reset(GridDev<T> g, F3<T> xp, F3<T> dx, int16_t* ns0, int16_t* ns1,
-                                                      int16_t* ns2) {
+__global__ void __launch_bounds__(kBlock) k_fwd_reset(GridDev<T> g, ParticleSet<T> ps) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= g.np) return;
   int i, j, k;
   decode_interior(g, p, i, j, k);
-  xp.c[0][p] = g.gl[0][i]; xp.c[1][p] = g.gl[1][j]; xp.c[2][p] = g.gl[2][k];
-  dx.c[0][p] = T(0); dx.c[1][p] = T(0); dx.c[2][p] = T(0);
-  ns0[p] = (int16_t)i; ns1[p] = (int16_t)j; ns2[p] = (int16_t)k;
+  ps.x[0][p] = g.gl[0][i]; ps.x[1][p] = g.gl[1][j]; ps.x[2][p] = g.gl[2][k];
+  ps.d[0][p] = T(0); ps.d[1][p] = T(0); ps.d[2][p] = T(0);
+  ps.n[0][p] = (int16_t)i; ps.n[1][p] = (int16_t)j; ps.n[2][p] = (int16_t)k;
+  ps.org[p] = p;
 }
 
 // ---------------------------------------------------------------------------
@@ -866,6 +866,68 @@ __global__ void __launch_bounds__(kBlock) k_ftle(GridDev<T> g, CF3<T> fm, double
 // (dbintk/dbnfac).  Forward and backward substitution in the order of
 // dbnslv (bspline_sub_module.f90:2202-2259), in place.
 // ---------------------------------------------------------------------------
+// One line of the banded solve through the accessor at(r) (a reference to
+// row r).  The rows a recurrence step needs are independent of the recurrence
+// itself, so they are loaded kFitPf rows ahead (double-buffered) and the
+// dependent chain runs on registers only; the arithmetic and its order are
+// those of the unpipelined loop.
+constexpr int kFitPf = 8;
+template <typename T, int K, class At>
+__device__ __forceinline__ void band_solve(At at, const T* __restrict__ W, int n) {
+  constexpr int M = K - 1, NB = 2 * K - 1, PF = kFitPf;
+  T win[M > 0 ? M : 1];  // forward: win[0] = b(r-1), ...; backward: win[0] = b(r+1), ...
+#pragma unroll
+  for (int m = 0; m < M; m++) win[m] = T(0);
+  T cur[PF], nxt[PF];
+#pragma unroll
+  for (int u = 0; u < PF; u++) cur[u] = (u < n) ? at(u) : T(0);
+  for (int r0 = 0; r0 < n; r0 += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; u++) nxt[u] = (r0 + PF + u < n) ? at(r0 + PF + u) : T(0);
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+      const int r = r0 + u;
+      if (r < n) {
+        T v = cur[u];
+#pragma unroll
+        for (int jj = M; jj >= 1; jj--)
+          if (r - jj >= 0) v = v - win[jj - 1] * W[(jj + M) + NB * (r - jj)];
+#pragma unroll
+        for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
+        win[0] = v;
+        at(r) = v;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PF; u++) cur[u] = nxt[u];
+  }
+#pragma unroll
+  for (int m = 0; m < M; m++) win[m] = T(0);
+#pragma unroll
+  for (int u = 0; u < PF; u++) cur[u] = (n - 1 - u >= 0) ? at(n - 1 - u) : T(0);
+  for (int r0 = n - 1; r0 >= 0; r0 -= PF) {
+#pragma unroll
+    for (int u = 0; u < PF; u++) nxt[u] = (r0 - PF - u >= 0) ? at(r0 - PF - u) : T(0);
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+      const int r = r0 - u;
+      if (r >= 0) {
+        T v = cur[u];
+#pragma unroll
+        for (int jj = M; jj >= 1; jj--)
+          if (r + jj <= n - 1) v = v - win[jj - 1] * W[(M - jj) + NB * (r + jj)];
+        v = v / W[M + NB * r];
+#pragma unroll
+        for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
+        win[0] = v;
+        at(r) = v;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PF; u++) cur[u] = nxt[u];
+  }
+}
+
 template <typename T, int K>
 __global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restrict__ W, int n, int axis, int N0,
                                                      int N1, int nlines) {
@@ -882,33 +944,7 @@ __global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restric
     start = id; stride = (long long)N0 * N1;
   }
   T* b = a.c[comp] + start;
-  constexpr int M = K - 1, NB = 2 * K - 1;
-  T win[M > 0 ? M : 1];  // win[0] = b(r-1), win[1] = b(r-2), ...
-#pragma unroll
-  for (int m = 0; m < M; m++) win[m] = T(0);
-  for (int r = 0; r < n; r++) {
-    T v = b[r * stride];
-#pragma unroll
-    for (int jj = M; jj >= 1; jj--)
-      if (r - jj >= 0) v = v - win[jj - 1] * W[(jj + M) + NB * (r - jj)];
-#pragma unroll
-    for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
-    win[0] = v;
-    b[r * stride] = v;
-  }
-#pragma unroll
-  for (int m = 0; m < M; m++) win[m] = T(0);  // win[0] = b(r+1), ...
-  for (int r = n - 1; r >= 0; r--) {
-    T v = b[r * stride];
-#pragma unroll
-    for (int jj = M; jj >= 1; jj--)
-      if (r + jj <= n - 1) v = v - win[jj - 1] * W[(M - jj) + NB * (r + jj)];
-    v = v / W[M + NB * r];
-#pragma unroll
-    for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
-    win[0] = v;
-    b[r * stride] = v;
-  }
+  band_solve<T, K>([&](int r) -> T& { return b[r * stride]; }, W, n);
 }
 
 // x-axis pass of the fit: the lines are contiguous rows, so a block stages
@@ -933,33 +969,7 @@ __global__ void __launch_bounds__(kFitThreads) k_fit_x(F3<T> a, const T* __restr
   __syncthreads();
   if ((int)threadIdx.x < nl) {
     T* b = sm + threadIdx.x * (n + 1);
-    constexpr int M = K - 1, NB = 2 * K - 1;
-    T win[M];
-#pragma unroll
-    for (int m = 0; m < M; m++) win[m] = T(0);
-    for (int r = 0; r < n; r++) {
-      T v = b[r];
-#pragma unroll
-      for (int jj = M; jj >= 1; jj--)
-        if (r - jj >= 0) v = v - win[jj - 1] * W[(jj + M) + NB * (r - jj)];
-#pragma unroll
-      for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
-      win[0] = v;
-      b[r] = v;
-    }
-#pragma unroll
-    for (int m = 0; m < M; m++) win[m] = T(0);
-    for (int r = n - 1; r >= 0; r--) {
-      T v = b[r];
-#pragma unroll
-      for (int jj = M; jj >= 1; jj--)
-        if (r + jj <= n - 1) v = v - win[jj - 1] * W[(M - jj) + NB * (r + jj)];
-      v = v / W[M + NB * r];
-#pragma unroll
-      for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
-      win[0] = v;
-      b[r] = v;
-    }
+    band_solve<T, K>([&](int r) -> T& { return b[r]; }, W, n);
   }
   __syncthreads();
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
@@ -1095,9 +1105,8 @@ void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, 
 }
 
 template <typename T>
-void launch_fwd_reset(const GridDev<T>& g, F3<T> xp, F3<T> dx, int16_t* ns0, int16_t* ns1, int16_t* ns2,
-                      hipStream_t s) {
-  hipLaunchKernelGGL((k_fwd_reset<T>), grid_for(g.np), dim3(kBlock), 0, s, g, xp, dx, ns0, ns1, ns2);
+void launch_fwd_reset(const GridDev<T>& g, ParticleSet<T> ps, hipStream_t s) {
+  hipLaunchKernelGGL((k_fwd_reset<T>), grid_for(g.np), dim3(kBlock), 0, s, g, ps);
 }
 
 template <typename T>
@@ -1149,8 +1158,7 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
                                   hipStream_t);                                                               \
   template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, F3<T>, F3<T>, int16_t*,    \
                                    int16_t*, int16_t*, DevStatus*, hipStream_t);                              \
-  template void launch_fwd_reset<T>(const GridDev<T>&, F3<T>, F3<T>, int16_t*, int16_t*, int16_t*,            \
-                                    hipStream_t);                                                             \
+  template void launch_fwd_reset<T>(const GridDev<T>&, ParticleSet<T>, hipStream_t);                          \
   template void launch_compose<T>(const GridDev<T>&, int, const T*, long long, const int*, int, F3<T>,       \
                                   DevStatus*, hipStream_t);                                                   \
   template void launch_ftle<T>(const GridDev<T>&, CF3<T>, double, int, double, double*, F3<T>, hipStream_t); \

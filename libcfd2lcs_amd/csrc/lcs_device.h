@@ -68,7 +68,18 @@ struct StageParams {
   CF3<T> v[4];
 };
 
-// ---- launchers (kernels.hip); all asynchronous on `stream` ----
+// The forward particle set in one storage order: position, displacement since
+// the last reset, tracked node (lp%xp, lp%dx, lp%no, data_m.f90:317-337) and
+// the interior index of the node the particle started from (lp%no0).
+template <typename T>
+struct ParticleSet {
+  T* x[3];
+  T* d[3];
+  int16_t* n[3];
+  int* org;
+};
+
+// ---- launchers (kernels.hip, particles.hip); all asynchronous on `stream` ----
 template <typename U, typename T>
 void launch_ingest(const GridDev<T>& g, F3<T> dst, const U* u, const U* v, const U* w,
                    hipStream_t stream);
@@ -89,8 +100,16 @@ template <typename T>
 void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, F3<T> xp, F3<T> dx,
                      int16_t* ns0, int16_t* ns1, int16_t* ns2, DevStatus* st, hipStream_t stream);
 template <typename T>
-void launch_fwd_reset(const GridDev<T>& g, F3<T> xp, F3<T> dx, int16_t* ns0, int16_t* ns1,
-                      int16_t* ns2, hipStream_t stream);
+void launch_fwd_reset(const GridDev<T>& g, ParticleSet<T> ps, hipStream_t stream);
+// device temporary bytes of the particle sort for np particles (0 on error)
+size_t particle_sort_temp_bytes(long long np);
+// sorts set a by tracked node into set b; keys and vals hold 2*np entries
+template <typename T>
+int launch_particle_sort(const GridDev<T>& g, ParticleSet<T> a, ParticleSet<T> b, unsigned* keys,
+                         unsigned* vals, void* temp, size_t temp_bytes, hipStream_t stream);
+// dst[org[q]] = src[q]: per-particle values back to origin order
+template <typename T>
+void launch_unpermute(const GridDev<T>& g, const int* org, CF3<T> src, F3<T> dst, hipStream_t stream);
 template <typename T>
 void launch_field_from_origin(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStream_t stream);
 template <typename T>

@@ -1,0 +1,96 @@
+// particles.hip -- spatial ordering of the forward particle set.
+//
+// A forward particle (lp_motion_m.f90:15-54) is advected from its origin node
+// for a whole h interval; as the set disperses, a wavefront of particles that
+// started on neighbouring nodes ends up spread over the domain and its
+// interpolation gathers stop sharing cache lines.  Every particle's update
+// depends on its own state only, so the set may be stored in any order: it is
+// radix-sorted by the interior index of its tracked node, which makes the 64
+// particles of a wavefront gather from a few neighbouring grid rows again.
+// The origin index (lp%no0, data_m.f90:392-396) travels with each particle,
+// and the displacement is scattered back to origin order wherever the
+// reference reads it by origin (exchange_lpmap, comms_m.f90:1191-1376).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "lcs_device.h"
+
+namespace lcs {
+
+constexpr int kPBlock = 256;
+
+__global__ void __launch_bounds__(kPBlock) k_sort_keys(int n0, int n1, int np, const int16_t* __restrict__ ns0,
+                                                       const int16_t* __restrict__ ns1,
+                                                       const int16_t* __restrict__ ns2, unsigned* __restrict__ key,
+                                                       unsigned* __restrict__ val) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  key[q] = (unsigned)((ns0[q] - 1) + n0 * ((ns1[q] - 1) + n1 * (ns2[q] - 1)));
+  val[q] = (unsigned)q;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kPBlock) k_permute(int np, const unsigned* __restrict__ perm, ParticleSet<T> a,
+                                                     ParticleSet<T> b) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const unsigned s = perm[q];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    b.x[d][q] = a.x[d][s];
+    b.d[d][q] = a.d[d][s];
+    b.n[d][q] = a.n[d][s];
+  }
+  b.org[q] = a.org[s];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kPBlock) k_unpermute(int np, const int* __restrict__ org, CF3<T> src, F3<T> dst) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const int o = org[q];
+#pragma unroll
+  for (int d = 0; d < 3; d++) dst.c[d][o] = src.c[d][q];
+}
+
+static int key_bits(long long np) {
+  int b = 1;
+  while (b < 31 && (1LL << b) < np) b++;
+  return b;
+}
+
+size_t particle_sort_temp_bytes(long long np) {
+  size_t bytes = 0;
+  hipcub::DoubleBuffer<unsigned> k(nullptr, nullptr), v(nullptr, nullptr);
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)np, 0, key_bits(np)) != hipSuccess) return 0;
+  return bytes;
+}
+
+template <typename T>
+int launch_particle_sort(const GridDev<T>& g, ParticleSet<T> a, ParticleSet<T> b, unsigned* keys, unsigned* vals,
+                         void* temp, size_t temp_bytes, hipStream_t s) {
+  const int np = (int)g.np;
+  const dim3 grid((np + kPBlock - 1) / kPBlock);
+  hipLaunchKernelGGL(k_sort_keys, grid, dim3(kPBlock), 0, s, g.n[0], g.n[1], np, a.n[0], a.n[1], a.n[2], keys,
+                     vals);
+  hipcub::DoubleBuffer<unsigned> kb(keys, keys + np), vb(vals, vals + np);
+  size_t tb = temp_bytes;
+  if (hipcub::DeviceRadixSort::SortPairs(temp, tb, kb, vb, np, 0, key_bits(np), s) != hipSuccess) return 1;
+  hipLaunchKernelGGL((k_permute<T>), grid, dim3(kPBlock), 0, s, np, (const unsigned*)vb.Current(), a, b);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+template <typename T>
+void launch_unpermute(const GridDev<T>& g, const int* org, CF3<T> src, F3<T> dst, hipStream_t s) {
+  const int np = (int)g.np;
+  hipLaunchKernelGGL((k_unpermute<T>), dim3((np + kPBlock - 1) / kPBlock), dim3(kPBlock), 0, s, np, org, src, dst);
+}
+
+template int launch_particle_sort<float>(const GridDev<float>&, ParticleSet<float>, ParticleSet<float>, unsigned*,
+                                         unsigned*, void*, size_t, hipStream_t);
+template int launch_particle_sort<double>(const GridDev<double>&, ParticleSet<double>, ParticleSet<double>,
+                                          unsigned*, unsigned*, void*, size_t, hipStream_t);
+template void launch_unpermute<float>(const GridDev<float>&, const int*, CF3<float>, F3<float>, hipStream_t);
+template void launch_unpermute<double>(const GridDev<double>&, const int*, CF3<double>, F3<double>, hipStream_t);
+
+}  // namespace lcs

@@ -70,8 +70,7 @@ def test_gyre_slip_rk3_cubic_fp64(dp):
 
 
 def test_abc_rk4_cubic_fp64_multibrick(dp):
-    """36^3 spans several 8x4x4 / 16x4x4 bricks, including partial ones, so
-    the tiled semi-Lagrangian kernel's halo and fallback paths are covered."""
+    """36^3: more than one workgroup per grid row and a partial last block."""
     recs, hs = run_both(dp, "abc", n=36, integrator=F.RK4, interpolator=F.CUBIC, T=0.2, h=0.1,
                         dt=0.05, nsteps=5, every_step=2)
     assert _check(recs, hs, 1e-10, 1e-9) >= 2
@@ -92,3 +91,40 @@ def test_abc_fp32_against_fp64_oracle(sp, interp):
     recs, hs = run_both(sp, "abc", n=20, integrator=F.RK4, interpolator=interp, T=0.4, h=0.2,
                         dt=0.05, nsteps=10, every_step=3)
     assert _check(recs, hs, 2e-5, 1e-3) >= 2
+
+
+def _fwd_run(lib, sort_every, n=24, nsteps=14):
+    import libcfd2lcs_amd as L
+    X, Y, Z, fl = F.abc_grid(n)
+    lib.finalize()
+    lib.init(X, Y, Z, fl)
+    hd = lib.diagnostic_init(L.FTLE_FWD, 0, 0.4, 0.2, "fwd")
+    lib.set_option("INTEGRATOR", L.RK4)
+    lib.set_option("INTERPOLATOR", L.CUBIC)
+    lib.set_option("PARTICLE_SORT", sort_every)
+    u, v, w = F.abc_velocity(X, Y, Z)
+    fms, outs = [], []
+    for s in range(nsteps):
+        lib.update(u, v, w, 0.05 * s)
+        assert lib.error == 0
+        fms.append(lib.flowmap(hd))
+        if lib.noutput(hd) > len(outs):
+            outs.append((lib.composed(hd), lib.ftle(hd)))
+    return fms, outs
+
+
+@pytest.mark.parametrize("every", [1, 3])
+def test_particle_sort_is_bit_exact(dp, every):
+    """Re-ordering the forward particle set by tracked node changes only the
+    storage order: flowmaps, composed flowmaps and FTLE are bit-identical to
+    the run that keeps origin order."""
+    a_fm, a_out = _fwd_run(dp, 0)
+    b_fm, b_out = _fwd_run(dp, every)
+    assert len(a_out) == len(b_out) >= 2
+    for x, y in zip(a_fm, b_fm):
+        for c in range(3):
+            assert np.array_equal(x[c], y[c])
+    for (xc, xf), (yc, yf) in zip(a_out, b_out):
+        for c in range(3):
+            assert np.array_equal(xc[c], yc[c])
+        assert np.array_equal(xf, yf, equal_nan=True)
