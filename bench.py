@@ -212,6 +212,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ftle", action="store_true", help="skip the full-ring FTLE fields/s measurement")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a fill (write 206.1 MB) and a plain copy b = 1.0 * a (read 206.1 MB, "
+                         "write 206.1 MB) of 3*258^3 floats, the reference dispatches for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -220,6 +224,14 @@ def main():
               "multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
     dt, T, h = 0.02, 5.0, 0.5
+    if args.calibrate:
+        import torch
+        a = torch.empty(3 * 258 ** 3, dtype=torch.float32, device="cuda:0")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     r32 = run_gpu("sp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle)
     print(f"[bench] fp32: {r32['updates'] / r32['elapsed']:.4e} updates/s, "
           f"{1e3 * r32['elapsed'] / args.steps:.1f} ms/step", file=sys.stderr, flush=True)

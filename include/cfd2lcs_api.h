@@ -161,6 +161,34 @@ int cfd2lcs_get_timers_c(double *seconds, long long *counts, int n);
 int cfd2lcs_debug_grid_c(double *lines, int *flag, double *bn, double *ps,
                          double *bb);
 
+/* ---------------------------------------------------------------------
+ * Part 3: Fortran-callable entry points.  A Fortran host calls the
+ * reference's external subroutines cfd2lcs_init, cfd2lcs_diagnostic_init,
+ * ... (cfd2lcs.f90), which a Fortran compiler links as the lower-case,
+ * underscore-suffixed names below with every argument by reference and the
+ * hidden character lengths last (the same names the reference's C shim
+ * binds, cfd2lcs_c.c:16-58).
+ * --------------------------------------------------------------------- */
+#include <stddef.h>
+/* replaces cfd2lcs_init (cfd2lcs.f90:2-53) */
+void cfd2lcs_init_(int *usercomm, int *n, int *offset, lcsdata_t *x,
+                   lcsdata_t *y, lcsdata_t *z, int *flag);
+/* replaces cfd2lcs_diagnostic_init (cfd2lcs.f90:252-358) */
+void cfd2lcs_diagnostic_init_(int *lcs_handle, int *lcs_type, int *resolution,
+                              lcsdata_t *t, lcsdata_t *h, char *label,
+                              size_t namelen);
+/* replaces cfd2lcs_update (cfd2lcs.f90:55-250) */
+void cfd2lcs_update_(int *n, lcsdata_t *u, lcsdata_t *v, lcsdata_t *w,
+                     lcsdata_t *time);
+/* replaces cfd2lcs_diagnostic_destroy (cfd2lcs.f90:360-386) */
+void cfd2lcs_diagnostic_destroy_(int *lcs_handle);
+/* replaces cfd2lcs_set_option (cfd2lcs.f90:436-622) */
+void cfd2lcs_set_option_(char *option, int *val, size_t namelen);
+/* replaces cfd2lcs_set_param (cfd2lcs.f90:624-682) */
+void cfd2lcs_set_param_(char *param, lcsdata_t *val, size_t namelen);
+/* replaces cfd2lcs_finalize (cfd2lcs.f90:388-434) */
+void cfd2lcs_finalize_(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,7 @@ EXPORTED = (
     "cfd2lcs_finalize_c", "cfd2lcs_error_c", "cfd2lcs_update_device_c",
     "cfd2lcs_get_flowmap_c", "cfd2lcs_get_composed_c", "cfd2lcs_get_ftle_c",
     "cfd2lcs_get_ftle_device_c", "cfd2lcs_get_noutput_c", "cfd2lcs_get_stats_c",
+    "cfd2lcs_get_timers_c", "cfd2lcs_debug_grid_c",
     "cfd2lcs_init_", "cfd2lcs_diagnostic_init_", "cfd2lcs_update_",
     "cfd2lcs_diagnostic_destroy_", "cfd2lcs_set_option_", "cfd2lcs_set_param_",
     "cfd2lcs_finalize_",

@@ -573,6 +573,9 @@ struct Engine : EngineBase {
         ns = 4;
     }
     for (int a = 0; a < 4; a++) sp.v[a] = stage_field(c[a < ns ? a : 0], spl);
+    sp.un = cf3(vel[vold]);
+    sp.unp1 = cf3(vel[cur]);
+    sp.ct0 = (T)c[0];
     return sp;
   }
 

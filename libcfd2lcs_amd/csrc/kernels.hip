@@ -258,22 +258,25 @@ struct IntSpline {
 // ---------------------------------------------------------------------------
 
 // integrate_lp stages (lp_motion_m.f90:129-328); RK3 evaluates k3 at
-// xn + 2 k2 as the reference does (lp_motion_m.f90:239-247).
-template <typename T, class V>
+// xn + 2 k2 as the reference does (lp_motion_m.f90:239-247).  The first
+// stage of every scheme samples the velocity at the start position xn, which
+// first(up) supplies (an interpolation, or the nodal value when xn is a node).
+template <typename T, class V, class V0>
 __device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams<T>& sp, const V& vel,
-                                          const int nd[3], const T xn[3], T xp[3], unsigned long long& oob) {
+                                          const V0& first, const int nd[3], const T xn[3], T xp[3],
+                                          unsigned long long& oob) {
   const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
   const T dt = sp.dt;
   T up[3], k[3], x1[3];
   switch (sp.integ) {
     case kEuler:
     case kTrapezoidal:
-      vel(g, xp, nd, sp.v[0], up, oob);
+      first(up);
 #pragma unroll
       for (int d = 0; d < 3; d++) xp[d] = xp[d] + dt * up[d];
       break;
     case kRK2:
-      vel(g, xp, nd, sp.v[0], up, oob);
+      first(up);
 #pragma unroll
       for (int d = 0; d < 3; d++) xp[d] = xp[d] + HALF * dt * up[d];
       vel(g, xp, nd, sp.v[1], up, oob);
@@ -283,7 +286,7 @@ __device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams
     case kRK3:
 #pragma unroll
       for (int d = 0; d < 3; d++) x1[d] = xn[d];
-      vel(g, xp, nd, sp.v[0], up, oob);
+      first(up);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
@@ -308,7 +311,7 @@ __device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams
     default:
 #pragma unroll
       for (int d = 0; d < 3; d++) x1[d] = xn[d];
-      vel(g, xp, nd, sp.v[0], up, oob);
+      first(up);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
@@ -630,7 +633,17 @@ __global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T>
   T xp[3] = {x0[0], x0[1], x0[2]};
   unsigned long long oob = 0;
   int err = 0;
-  integrate(g, sp, interp, nd, x0, xp, oob);
+  const int l = gid(g, i, j, k);
+  // first stage at the node itself: the interpolant there is the node's
+  // time-blended value (exactly for NEAREST/LINEAR, to rounding for the
+  // interpolating splines), so it is read instead of gathered
+  auto first = [&](T up[3]) {
+#pragma clang fp contract(off)
+    const T w0 = T(1) - sp.ct0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) up[c] = w0 * sp.un.c[c][l] + sp.ct0 * sp.unp1.c[c][l];
+  };
+  integrate(g, sp, interp, first, nd, x0, xp, oob);
   project_flat(g, nd, xp);
   if (set_lp_bc(g, nd, xp, err)) {
     xp[0] = x0[0]; xp[1] = x0[1]; xp[2] = x0[2];
@@ -638,7 +651,6 @@ __global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T>
   int no[3] = {i, j, k};
   T fmp[3];
   interp(g, xp, no, fmold, fmp, oob);
-  const int l = gid(g, i, j, k);
   const int f = g.flag[l];
   const T mask = (f != kMask) ? T(1) : T(0);
   const bool zero = (f == kMask || f == kInflow || f == kOutflow);
@@ -678,7 +690,8 @@ __global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T
   unsigned long long oob = 0;
   int err = 0;
   const I interp{};
-  integrate(g, sp, interp, nd, xn, q, oob);
+  auto first = [&](T up[3]) { interp(g, xn, nd, sp.v[0], up, oob); };
+  integrate(g, sp, interp, first, nd, xn, q, oob);
   project_flat(g, nd, q);
   if (set_lp_bc(g, nd, q, err)) {
     q[0] = xn[0]; q[1] = xn[1]; q[2] = xn[2];
@@ -947,34 +960,100 @@ __global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restric
   band_solve<T, K>([&](int r) -> T& { return b[r * stride]; }, W, n);
 }
 
-// x-axis pass of the fit: the lines are contiguous rows, so a block stages
-// kFitLines consecutive rows in LDS with coalesced loads/stores and each lane
-// solves one row (padded stride n+1 avoids bank conflicts).  L <= 64 rows
-// per block of 64 threads, sized so that the tile fits in 64 KiB of LDS.
-constexpr int kFitLines = 64;
-constexpr int kFitThreads = 256;
+// x-axis pass of the fit: the lines are contiguous rows.  One wavefront
+// (block of 64) owns 64 rows and streams them in chunks of kXC columns: each
+// chunk is loaded with row-contiguous accesses, transposed through a small
+// LDS tile so that lane r runs the recurrence of row r, and stored back the
+// same way; the next chunk's loads are in flight during the recurrence.  The
+// forward sweep leaves its result in place and the backward sweep reads it
+// back, as the strided passes do.  Same arithmetic, in the same order, as
+// band_solve.
+constexpr int kXC = 16;
 template <typename T, int K>
-__global__ void __launch_bounds__(kFitThreads) k_fit_x(F3<T> a, const T* __restrict__ W, int n, int nlines, int L) {
-  extern __shared__ unsigned char fit_smem[];
-  T* sm = reinterpret_cast<T*>(fit_smem);
-  const int comp = blockIdx.y;
-  const int line0 = blockIdx.x * L;
-  const int nl = min(L, nlines - line0);
-  T* base = a.c[comp] + (long long)line0 * n;
-  const int tot = nl * n;
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int l = e / n, r = e - l * n;
-    sm[l * (n + 1) + r] = base[e];
+__global__ void __launch_bounds__(64) k_fit_x(F3<T> a, const T* __restrict__ W, int n, int nlines) {
+  __shared__ T tile[64][kXC + 1];
+  constexpr int M = K - 1, NB = 2 * K - 1;
+  const int lane = threadIdx.x;
+  const long long line0 = (long long)blockIdx.x * 64;
+  const int nl = (int)min((long long)64, nlines - line0);
+  T* base = a.c[blockIdx.y] + line0 * n;
+  const int nch = (n + kXC - 1) / kXC;
+  T buf[kXC];
+  // element it*64+lane of a chunk: row (it*64+lane)/kXC, column (it*64+lane)%kXC
+  auto load = [&](int ch) {
+#pragma unroll
+    for (int it = 0; it < kXC; it++) {
+      const int e = it * 64 + lane, row = e / kXC, col = ch * kXC + e % kXC;
+      buf[it] = (row < nl && col < n) ? base[(long long)row * n + col] : T(0);
+    }
+  };
+  auto to_lds = [&]() {
+#pragma unroll
+    for (int it = 0; it < kXC; it++) {
+      const int e = it * 64 + lane;
+      tile[e / kXC][e % kXC] = buf[it];
+    }
+  };
+  auto store = [&](int ch) {
+#pragma unroll
+    for (int it = 0; it < kXC; it++) {
+      const int e = it * 64 + lane, row = e / kXC, col = ch * kXC + e % kXC;
+      if (row < nl && col < n) base[(long long)row * n + col] = tile[e / kXC][e % kXC];
+    }
+  };
+  T win[M > 0 ? M : 1];
+  // forward substitution (unit lower factor)
+#pragma unroll
+  for (int m = 0; m < M; m++) win[m] = T(0);
+  load(0);
+  for (int ch = 0; ch < nch; ch++) {
+    to_lds();
+    if (ch + 1 < nch) load(ch + 1);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kXC; u++) {
+      const int r = ch * kXC + u;
+      if (r < n) {
+        T v = tile[lane][u];
+#pragma unroll
+        for (int jj = M; jj >= 1; jj--)
+          if (r - jj >= 0) v = v - win[jj - 1] * W[(jj + M) + NB * (r - jj)];
+#pragma unroll
+        for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
+        win[0] = v;
+        tile[lane][u] = v;
+      }
+    }
+    __syncthreads();
+    store(ch);
+    __syncthreads();
   }
-  __syncthreads();
-  if ((int)threadIdx.x < nl) {
-    T* b = sm + threadIdx.x * (n + 1);
-    band_solve<T, K>([&](int r) -> T& { return b[r]; }, W, n);
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int l = e / n, r = e - l * n;
-    base[e] = sm[l * (n + 1) + r];
+  // backward substitution (upper factor), last chunk first
+#pragma unroll
+  for (int m = 0; m < M; m++) win[m] = T(0);
+  load(nch - 1);
+  for (int ch = nch - 1; ch >= 0; ch--) {
+    to_lds();
+    if (ch > 0) load(ch - 1);
+    __syncthreads();
+#pragma unroll
+    for (int u = kXC - 1; u >= 0; u--) {
+      const int r = ch * kXC + u;
+      if (r < n) {
+        T v = tile[lane][u];
+#pragma unroll
+        for (int jj = M; jj >= 1; jj--)
+          if (r + jj <= n - 1) v = v - win[jj - 1] * W[(M - jj) + NB * (r + jj)];
+        v = v / W[M + NB * r];
+#pragma unroll
+        for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
+        win[0] = v;
+        tile[lane][u] = v;
+      }
+    }
+    __syncthreads();
+    store(ch);
+    __syncthreads();
   }
 }
 
@@ -1033,14 +1112,12 @@ void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t s
   for (int axis = 0; axis < 3; axis++) {
     const int n = g.N[axis];
     const int nlines = (int)(g.nt / n);
-    const int L = (int)std::min<size_t>(kFitLines, (64 * 1024) / (sizeof(T) * (size_t)(n + 1)));
-    if (axis == 0 && L >= 8) {
-      const size_t lds = sizeof(T) * (size_t)L * (n + 1);
-      dim3 gx((nlines + L - 1) / L, ncomp);
+    if (axis == 0) {
+      dim3 gx((nlines + 63) / 64, ncomp);
       switch (g.kord[0]) {
-        case 2: hipLaunchKernelGGL((k_fit_x<T, 2>), gx, dim3(kFitThreads), lds, s, coef, g.band[0], n, nlines, L); break;
-        case 3: hipLaunchKernelGGL((k_fit_x<T, 3>), gx, dim3(kFitThreads), lds, s, coef, g.band[0], n, nlines, L); break;
-        default: hipLaunchKernelGGL((k_fit_x<T, 4>), gx, dim3(kFitThreads), lds, s, coef, g.band[0], n, nlines, L); break;
+        case 2: hipLaunchKernelGGL((k_fit_x<T, 2>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
+        case 3: hipLaunchKernelGGL((k_fit_x<T, 3>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
+        default: hipLaunchKernelGGL((k_fit_x<T, 4>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
       }
       continue;
     }

@@ -66,6 +66,10 @@ struct StageParams {
   // (lp_motion_m.f90:170-316): grid values for NEAREST/LINEAR, spline
   // coefficients for QUADRATIC/CUBIC
   CF3<T> v[4];
+  // grid values of u_n and u_np1 and the first stage's ct, for the
+  // semi-Lagrangian step whose first stage starts on a node
+  CF3<T> un, unp1;
+  T ct0;
 };
 
 // The forward particle set in one storage order: position, displacement since
