@@ -36,7 +36,7 @@ def abc_grid(n):
     return X, Y, Z
 
 
-def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True):
+def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None):
     import torch
 
     import libcfd2lcs_amd as L
@@ -54,6 +54,8 @@ def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True):
     lib.set_option("INTERPOLATOR", L.CUBIC)
     lib.set_option("KERNEL_TIMING", L.LCS_TRUE)
     lib.set_param("CFL", 0.5)
+    for k, v in (opts or {}).items():
+        lib.set_option(k, v)
     dtype = torch.float64 if variant == "dp" else torch.float32
     dev = torch.device("cuda:0")
     # velocity on the device, Fortran order (i fastest) = C order of (k, j, i)
@@ -128,6 +130,10 @@ def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True):
         res["u2_pipeline"] = (2 * nint_u2 / wall_u2) if nint_u2 else None
         res["u2_wall_interval"] = (wall_u2 / nint_u2) if nint_u2 else None
         res["steps_total"] = step
+        # FTLE of the last (full-ring) output of each direction, for the
+        # error against the FP64 run
+        res["ftle"] = {"fwd": np.asarray(lib.ftle(hf), dtype=np.float64),
+                       "bkwd": np.asarray(lib.ftle(hb), dtype=np.float64)}
     lib.finalize()
     del u, v, w
     torch.cuda.empty_cache()
@@ -212,6 +218,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ftle", action="store_true", help="skip the full-ring FTLE fields/s measurement")
+    ap.add_argument("--mixed", choices=["fp16", "bf16", "none"], default="fp16",
+                    help="also run FP32 advection with 16-bit stage velocities and compensated positions")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a fill (write 206.1 MB) and a plain copy b = 1.0 * a (read 206.1 MB, "
                          "write 206.1 MB) of 3*258^3 floats, the reference dispatches for scaling the "
@@ -239,6 +247,30 @@ def main():
     if not args.no_fp64:
         r64 = run_gpu("dp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle)
         print(f"[bench] fp64: {r64['updates'] / r64['elapsed']:.4e} updates/s", file=sys.stderr, flush=True)
+    rmix = None
+    if args.mixed != "none":
+        import libcfd2lcs_amd as L
+        stor = {"fp16": L.LCS_FP16, "bf16": L.LCS_BF16}[args.mixed]
+        rmix = run_gpu("sp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle,
+                       {"VELOCITY_STORAGE": stor, "COMPENSATED": L.LCS_TRUE})
+        print(f"[bench] mixed ({args.mixed} velocity, compensated FP32 positions): "
+              f"{rmix['updates'] / rmix['elapsed']:.4e} updates/s", file=sys.stderr, flush=True)
+
+    def ftle_err(r):
+        """FTLE of a run against the FP64 run at the same output (both
+        directions): max and RMS absolute error and max error relative to
+        the largest FP64 FTLE magnitude."""
+        if r is None or r64 is None or "ftle" not in r or "ftle" not in r64:
+            return None
+        mx, ss, cnt, scale = 0.0, 0.0, 0, 0.0
+        for d in ("fwd", "bkwd"):
+            a, b = r["ftle"][d], r64["ftle"][d]
+            e = np.abs(a - b)
+            mx = max(mx, float(e.max()))
+            ss += float((e * e).sum())
+            cnt += e.size
+            scale = max(scale, float(np.abs(b).max()))
+        return {"max_abs": mx, "rms": (ss / cnt) ** 0.5, "max_rel": mx / scale, "ftle_scale": scale}
     # dominant kernel of the FP32 run
     ks = r32["kernel_sec"]
     dom = max(("k_sl_step", "k_fwd_step"), key=lambda k: ks[k])
@@ -292,6 +324,14 @@ def main():
             "value": r64["updates"] / r64["elapsed"], "ms_per_step": 1e3 * r64["elapsed"] / args.steps,
             "ftle_fields_per_s": u2(r64), "ftle_fields_per_s_pipeline": r64.get("u2_pipeline"),
             "fp32_over_fp64": value / (r64["updates"] / r64["elapsed"])},
+        "ftle_error_vs_fp64": ftle_err(r32),
+        "mixed": None if rmix is None else {
+            "velocity_storage": args.mixed, "compensated_positions": True,
+            "value": rmix["updates"] / rmix["elapsed"], "ms_per_step": 1e3 * rmix["elapsed"] / args.steps,
+            "over_fp64": (rmix["updates"] / rmix["elapsed"]) / (r64["updates"] / r64["elapsed"]) if r64 else None,
+            "ftle_fields_per_s": u2(rmix), "ftle_error_vs_fp64": ftle_err(rmix),
+            "kernel_ms": {k: round(1e3 * s_ / max(rmix["kernel_cnt"][k], 1), 4)
+                          for k, s_ in rmix["kernel_sec"].items()}},
         "bwd_fwd_ratio": None,
         "kernel_ms": {k: round(1e3 * s / max(r32["kernel_cnt"][k], 1), 4) for k, s in ks.items()},
         "roofline": roof,

@@ -49,6 +49,8 @@ struct Options {
   int precision = sizeof(U) == 8 ? LCS_FP64 : LCS_FP32;
   int kernel_timing = 0;  // time every hot kernel launch with HIP events
   int particle_sort = 1;  // re-order forward particles by node every n updates (0: never)
+  int vel_storage = kStoreNative;  // storage of the gathered stage velocities
+  int compensated = 0;    // two-sum position/displacement updates of forward particles
   bool warn_opt = false, warn_par = false;  // SAVEd warn flags (cfd2lcs.f90:444,632)
 };
 
@@ -166,6 +168,7 @@ struct Engine : EngineBase {
     int pcur = 0;
     bool sorted = false;   // part[pcur] is not in origin order
     int since_sort = 0;    // updates advected since the last reset or sort
+    int cmode = 0;         // compensated updates in the current h interval
     DevBuf skeys, svals, stemp;  // sort scratch
     size_t stemp_bytes = 0;
     DevBuf fm[2];          // BKWD semi-Lagrangian flowmap ping-pong (ghosted)
@@ -219,7 +222,7 @@ struct Engine : EngineBase {
   // (int16), org (int), each array aligned to 256 bytes
   size_t pset_bytes() const {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    return 6 * al(sizeof(T) * (size_t)g.np) + 3 * al(sizeof(int16_t) * (size_t)g.np) + al(sizeof(int) * (size_t)g.np);
+    return 12 * al(sizeof(T) * (size_t)g.np) + 3 * al(sizeof(int16_t) * (size_t)g.np) + al(sizeof(int) * (size_t)g.np);
   }
   ParticleSet<T> pset(DevBuf& b) {
     char* base = (char*)b.p;
@@ -232,24 +235,34 @@ struct Engine : EngineBase {
     ParticleSet<T> ps;
     for (int d = 0; d < 3; d++) ps.x[d] = (T*)take(sizeof(T) * (size_t)g.np);
     for (int d = 0; d < 3; d++) ps.d[d] = (T*)take(sizeof(T) * (size_t)g.np);
+    for (int d = 0; d < 3; d++) ps.xl[d] = (T*)take(sizeof(T) * (size_t)g.np);
+    for (int d = 0; d < 3; d++) ps.dl[d] = (T*)take(sizeof(T) * (size_t)g.np);
     for (int d = 0; d < 3; d++) ps.n[d] = (int16_t*)take(sizeof(int16_t) * (size_t)g.np);
     ps.org = (int*)take(sizeof(int) * (size_t)g.np);
     return ps;
   }
   // forward displacement by origin node: the current set's dx, or (when the
-  // set is sorted) its scatter to origin order in the spare set's dx arrays
+  // set is sorted, or dx is a compensated sum) its scatter to origin order in
+  // the spare set's dx arrays
   CF3<T> fwd_dx_by_origin(Diag& dg) {
     ParticleSet<T> a = pset(dg.part[dg.pcur]);
-    if (!dg.sorted) return CF3<T>{{a.d[0], a.d[1], a.d[2]}};
+    if (!dg.sorted && !dg.cmode) return CF3<T>{{a.d[0], a.d[1], a.d[2]}};
+    if (!dg.part[1].p && !dg.part[1].alloc(pset_bytes())) {
+      fail("hipMalloc failed (particle set)");
+      return CF3<T>{{a.d[0], a.d[1], a.d[2]}};
+    }
     ParticleSet<T> b = pset(dg.part[1 - dg.pcur]);
-    launch_unpermute(g, a.org, CF3<T>{{a.d[0], a.d[1], a.d[2]}}, F3<T>{{b.d[0], b.d[1], b.d[2]}}, stream);
+    const T* lo[3] = {a.dl[0], a.dl[1], a.dl[2]};
+    launch_unpermute(g, a.org, CF3<T>{{a.d[0], a.d[1], a.d[2]}}, dg.cmode ? lo : nullptr,
+                     F3<T>{{b.d[0], b.d[1], b.d[2]}}, stream);
     return CF3<T>{{b.d[0], b.d[1], b.d[2]}};
   }
   bool sort_particles(Diag& dg) {
     const size_t np = (size_t)g.np;
-    if (!dg.part[1].p) {
+    if (!dg.stemp.p) {
       dg.stemp_bytes = particle_sort_temp_bytes(g.np);
-      if (!dg.stemp_bytes || !dg.part[1].alloc(pset_bytes()) || !dg.skeys.alloc(2 * sizeof(unsigned) * np) ||
+      if (!dg.stemp_bytes || (!dg.part[1].p && !dg.part[1].alloc(pset_bytes())) ||
+          !dg.skeys.alloc(2 * sizeof(unsigned) * np) ||
           !dg.svals.alloc(2 * sizeof(unsigned) * np) || !dg.stemp.alloc(dg.stemp_bytes)) {
         fail("hipMalloc failed (particle sort buffers)");
         return false;
@@ -494,6 +507,7 @@ struct Engine : EngineBase {
       launch_fwd_reset(g, pset(dg.part[dg.pcur]), stream);
       dg.sorted = false;
       dg.since_sort = 0;
+      dg.cmode = g_opt.compensated;
     } else {
       (void)hipMemsetAsync(dg.fm[0].p, 0, 3 * fbytes(), stream);
       (void)hipMemsetAsync(dg.fm[1].p, 0, 3 * fbytes(), stream);
@@ -506,6 +520,8 @@ struct Engine : EngineBase {
   struct Blend {
     DevBuf buf;
     U ct = 0;
+    int storage = kStoreNative;
+    bool spl = false;
     bool valid = false;
     unsigned long long stamp = 0;
   };
@@ -513,31 +529,48 @@ struct Engine : EngineBase {
   unsigned long long blend_clock = 0;
   int vold = 0;  // index of u_n in vel[] during an update
 
-  CF3<T> stage_field(U ct, bool spl) {
+  struct StageField {
+    const void* c[3];
+  };
+  // stage velocity at blend weight ct, in the storage the gathers read
+  StageField stage_field(U ct, bool spl) {
     DevBuf& A = spl ? vcoef[vold] : vel[vold];
     DevBuf& B = spl ? vcoef[cur] : vel[cur];
-    if (ct == (U)0) return cf3(A);
-    if (ct == (U)1) return cf3(B);
+    const int stor = g_opt.vel_storage;
+    auto whole = [&](DevBuf& b) {
+      CF3<T> f = cf3(b);
+      return StageField{{f.c[0], f.c[1], f.c[2]}};
+    };
+    if (stor == kStoreNative) {
+      if (ct == (U)0) return whole(A);
+      if (ct == (U)1) return whole(B);
+    }
+    const size_t eb = storage_bytes(stor, sizeof(T));
     Blend* victim = &blends[0];
     for (auto& b : blends) {
-      if (b.valid && b.ct == ct) {
+      if (b.valid && b.ct == ct && b.storage == stor && b.spl == spl) {
         b.stamp = ++blend_clock;
-        return cf3(b.buf);
+        char* p = (char*)b.buf.p;
+        return StageField{{p, p + eb * g.nt, p + 2 * eb * g.nt}};
       }
       if (!b.valid || b.stamp < victim->stamp || !victim->valid) {
         if (!b.valid && victim->valid) victim = &b;
         else if (b.valid == victim->valid && b.stamp < victim->stamp) victim = &b;
       }
     }
-    if (!victim->buf.p && !victim->buf.alloc(3 * fbytes())) {
+    if ((!victim->buf.p || victim->buf.bytes < 3 * eb * (size_t)g.nt) && !victim->buf.alloc(3 * fbytes())) {
       fail("hipMalloc failed (blended velocity)");
-      return cf3(A);
+      return whole(A);
     }
-    ktimed(14, [&] { launch_blend(g, f3(victim->buf), cf3(A), cf3(B), (T)ct, stream); });
+    char* p = (char*)victim->buf.p;
+    void* out[3] = {p, p + eb * g.nt, p + 2 * eb * g.nt};
+    ktimed(14, [&] { launch_blend(g, out, stor, cf3(A), cf3(B), (T)ct, stream); });
     victim->ct = ct;
+    victim->storage = stor;
+    victim->spl = spl;
     victim->valid = true;
     victim->stamp = ++blend_clock;
-    return cf3(victim->buf);
+    return StageField{{out[0], out[1], out[2]}};
   }
 
   // ct of a stage at time tt (lp_motion_m.f90:170 etc.); NaN (first update)
@@ -572,7 +605,11 @@ struct Engine : EngineBase {
         c[3] = ctf(t + dt, t_n, t_np1);
         ns = 4;
     }
-    for (int a = 0; a < 4; a++) sp.v[a] = stage_field(c[a < ns ? a : 0], spl);
+    sp.storage = g_opt.vel_storage;
+    for (int a = 0; a < 4; a++) {
+      const StageField f = stage_field(c[a < ns ? a : 0], spl);
+      for (int d = 0; d < 3; d++) sp.v[a][d] = f.c[d];
+    }
     sp.un = cf3(vel[vold]);
     sp.unp1 = cf3(vel[cur]);
     sp.ct0 = (T)c[0];
@@ -581,6 +618,10 @@ struct Engine : EngineBase {
 
   void update(const U* u, const U* v, const U* w, bool device, U time) override {
     const int interp = g_opt.interpolator;
+    if (sizeof(T) == 8 && g_opt.vel_storage != kStoreNative) {
+      fail("VELOCITY_STORAGE = LCS_FP16 / LCS_BF16 needs PRECISION = LCS_FP32");
+      return;
+    }
     if (interp != kNearest && interp != kLinear && interp != kQuadratic && interp != kCubic) {
       fail("INTERPOLATOR TSE/TSE_LIMIT is not supported by this build");
       return;
@@ -674,10 +715,9 @@ struct Engine : EngineBase {
           if (!ok) return;
         }
         const ParticleSet<T> ps = pset(dg.part[dg.pcur]);
-        const F3<T> xp{{ps.x[0], ps.x[1], ps.x[2]}}, dx{{ps.d[0], ps.d[1], ps.d[2]}};
         for (int s = 0; s < nsub; s++) {
           const StageParams<T> sp = stage_params(t, dt, spl);
-          ktimed(8, [&] { launch_fwd_step(g, interp, sp, xp, dx, ps.n[0], ps.n[1], ps.n[2], st, stream); });
+          ktimed(8, [&] { launch_fwd_step(g, interp, sp, ps, dg.cmode, st, stream); });
           t = t + dt;
         }
         if (std::fabs(dt) > 0) dg.since_sort++;
@@ -1030,6 +1070,15 @@ void cfd2lcs_set_option_c(char option[], int val) {
     }
   } else if (o == "KERNEL_TIMING") {
     boolopt(g_opt.kernel_timing);
+  } else if (o == "VELOCITY_STORAGE") {
+    // storage of the gathered stage velocities: the compute precision, or
+    // 16-bit (FP32 compute only)
+    if (val == LCS_FP16) g_opt.vel_storage = kStoreF16;
+    else if (val == LCS_BF16) g_opt.vel_storage = kStoreBF16;
+    else if (val == LCS_FP32 || val == LCS_FP64) g_opt.vel_storage = kStoreNative;
+    else warn = true;
+  } else if (o == "COMPENSATED") {
+    boolopt(g_opt.compensated);
   } else if (o == "PARTICLE_SORT") {
     if (val < 0) warn = true;
     else g_opt.particle_sort = val;

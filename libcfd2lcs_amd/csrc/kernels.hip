@@ -6,6 +6,7 @@
 // Stage (c)  gradient -> Cauchy-Green -> FTLE       k_ftle
 // plus velocity ingest / ghost shell / CFL reduction and the tensor B-spline
 // fit (three banded axis solves).  Reference semantics are cited per kernel.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -165,6 +166,38 @@ __device__ __forceinline__ int bs_setup(const T* __restrict__ t, const T* __rest
   return 0;
 }
 
+// ---- stored element -> compute type ----
+template <typename T>
+__device__ __forceinline__ T widen(T x) { return x; }
+__device__ __forceinline__ float widen(half_t h) { return __half2float(__ushort_as_half(h.u)); }
+__device__ __forceinline__ float widen(bf16_t h) { return __uint_as_float((unsigned)h.u << 16); }
+// compute type -> stored element, round to nearest even
+template <typename S, typename T>
+__device__ __forceinline__ S narrow(T x) { return S(x); }
+template <>
+__device__ __forceinline__ half_t narrow<half_t, float>(float x) { return half_t{__half_as_ushort(__float2half_rn(x))}; }
+template <>
+__device__ __forceinline__ bf16_t narrow<bf16_t, float>(float x) {
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return bf16_t{(uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u))};
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return bf16_t{(uint16_t)(u >> 16)};
+}
+// K consecutive stored elements (one x row of a spline stencil); 16-bit rows
+// are fetched as one unaligned 2K-byte load
+template <typename T, int K, typename S>
+__device__ __forceinline__ void load_row(const S* __restrict__ row, T v[K]) {
+  if constexpr (sizeof(S) >= 4) {
+#pragma unroll
+    for (int a = 0; a < K; a++) v[a] = T(row[a]);
+  } else {
+    struct __attribute__((packed, aligned(2))) R { S e[K]; } r;
+    __builtin_memcpy(&r, row, sizeof(R));
+#pragma unroll
+    for (int a = 0; a < K; a++) v[a] = T(widen(r.e[a]));
+  }
+}
+
 template <typename T, int KX, int KZ>
 struct SplineW {
   int base;       // flattened ghosted index of the first coefficient
@@ -187,8 +220,8 @@ __device__ __forceinline__ int spline_setup(const GridDev<T>& g, const T q[3], c
   return 0;
 }
 
-template <typename T, int KX, int KZ>
-__device__ __forceinline__ T spline_sum(const GridDev<T>& g, const SplineW<T, KX, KZ>& s, const T* __restrict__ c) {
+template <typename T, int KX, int KZ, typename S>
+__device__ __forceinline__ T spline_sum(const GridDev<T>& g, const SplineW<T, KX, KZ>& s, const S* __restrict__ c) {
   const int sy = g.N[0], sz = g.N[0] * g.N[1];
   T acc = T(0);
 #pragma unroll
@@ -196,7 +229,8 @@ __device__ __forceinline__ T spline_sum(const GridDev<T>& g, const SplineW<T, KX
     T accy = T(0);
 #pragma unroll
     for (int by = 0; by < KX; by++) {
-      const T* row = c + s.base + by * sy + cz * sz;
+      T row[KX];
+      load_row<T, KX>(c + s.base + by * sy + cz * sz, row);
       T accx = T(0);
 #pragma unroll
       for (int a = 0; a < KX; a++) accx += s.wx[a] * row[a];
@@ -213,33 +247,37 @@ __device__ __forceinline__ T spline_sum(const GridDev<T>& g, const SplineW<T, KX
 // (fitted) flowmap. ----
 template <typename T>
 struct IntNearest {
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+  template <typename S>
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
                              unsigned long long&) const {
     int l = gid(g, iclamp(nd[0], 0, g.n[0] + 1), iclamp(nd[1], 0, g.n[1] + 1), iclamp(nd[2], 0, g.n[2] + 1));
 #pragma unroll
-    for (int c = 0; c < 3; c++) out[c] = f.c[c][l];
+    for (int c = 0; c < 3; c++) out[c] = T(widen(f.c[c][l]));
   }
 };
 
 template <typename T>
 struct IntLinear {
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+  template <typename S>
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
                              unsigned long long&) const {
     LinW<T> w;
     lin_setup(g, q, nd, w);
     const int b = w.base, sy = g.N[0], sz = g.N[0] * g.N[1];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const T* A = f.c[c];
-      out[c] = trilin(w, A[b], A[b + 1], A[b + sy], A[b + sy + 1], A[b + sz], A[b + sz + 1], A[b + sz + sy],
-                      A[b + sz + sy + 1]);
+      const S* A = f.c[c];
+      out[c] = trilin(w, T(widen(A[b])), T(widen(A[b + 1])), T(widen(A[b + sy])), T(widen(A[b + sy + 1])),
+                      T(widen(A[b + sz])), T(widen(A[b + sz + 1])), T(widen(A[b + sz + sy])),
+                      T(widen(A[b + sz + sy + 1])));
     }
   }
 };
 
 template <typename T, int KX, int KZ>
 struct IntSpline {
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+  template <typename S>
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
                              unsigned long long& oob) const {
     SplineW<T, KX, KZ> s;
     int r = spline_setup(g, q, nd, s);
@@ -261,7 +299,12 @@ struct IntSpline {
 // xn + 2 k2 as the reference does (lp_motion_m.f90:239-247).  The first
 // stage of every scheme samples the velocity at the start position xn, which
 // first(up) supplies (an interpolation, or the nodal value when xn is a node).
-template <typename T, class V, class V0>
+template <typename S, typename T>
+__device__ __forceinline__ CF3<S> stage(const StageParams<T>& sp, int i) {
+  return CF3<S>{{(const S*)sp.v[i][0], (const S*)sp.v[i][1], (const S*)sp.v[i][2]}};
+}
+
+template <typename S, typename T, class V, class V0>
 __device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams<T>& sp, const V& vel,
                                           const V0& first, const int nd[3], const T xn[3], T xp[3],
                                           unsigned long long& oob) {
@@ -279,7 +322,7 @@ __device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams
       first(up);
 #pragma unroll
       for (int d = 0; d < 3; d++) xp[d] = xp[d] + HALF * dt * up[d];
-      vel(g, xp, nd, sp.v[1], up, oob);
+      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) xp[d] = xn[d] + dt * up[d];
       break;
@@ -293,14 +336,14 @@ __device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams
         x1[d] = x1[d] + SIXTH * k[d];
         xp[d] = xn[d] + HALF * k[d];
       }
-      vel(g, xp, nd, sp.v[1], up, oob);
+      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         xp[d] = xn[d] + TWO * k[d];
         x1[d] = x1[d] + TWOTHIRD * k[d];
       }
-      vel(g, xp, nd, sp.v[2], up, oob);
+      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
@@ -318,21 +361,21 @@ __device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams
         x1[d] = x1[d] + SIXTH * k[d];
         xp[d] = xn[d] + HALF * k[d];
       }
-      vel(g, xp, nd, sp.v[1], up, oob);
+      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         x1[d] = x1[d] + THIRD * k[d];
         xp[d] = xn[d] + HALF * k[d];
       }
-      vel(g, xp, nd, sp.v[2], up, oob);
+      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         x1[d] = x1[d] + THIRD * k[d];
         xp[d] = xn[d] + k[d];
       }
-      vel(g, xp, nd, sp.v[3], up, oob);
+      vel(g, xp, nd, stage<S>(sp, 3), up, oob);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
@@ -345,6 +388,90 @@ __device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams
 // project_lp2plane on a flat separable grid (lp_m.f90:418-538): the plane
 // normal is the collapsed axis, the node index on it becomes 1 and the
 // particle's coordinate loses its offset from the node plane.
+
+// The same stages in increment form for compensated updates: inc = x_new - xn
+// summed from the stage terms alone, so that no term is rounded against |xn|
+// (same stage positions and weights as integrate()).
+template <typename S, typename T, class V, class V0>
+__device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StageParams<T>& sp, const V& vel,
+                                              const V0& first, const int nd[3], const T xn[3], T inc[3],
+                                              unsigned long long& oob) {
+  const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
+  const T dt = sp.dt;
+  T up[3], k[3], xp[3];
+  switch (sp.integ) {
+    case kEuler:
+    case kTrapezoidal:
+      first(up);
+#pragma unroll
+      for (int d = 0; d < 3; d++) inc[d] = dt * up[d];
+      break;
+    case kRK2:
+      first(up);
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = xn[d] + HALF * dt * up[d];
+      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) inc[d] = dt * up[d];
+      break;
+    case kRK3:
+      first(up);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        inc[d] = SIXTH * k[d];
+        xp[d] = xn[d] + HALF * k[d];
+      }
+      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        xp[d] = xn[d] + TWO * k[d];
+        inc[d] = inc[d] + TWOTHIRD * k[d];
+      }
+      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) inc[d] = inc[d] + SIXTH * (dt * up[d]);
+      break;
+    case kRK4:
+    default:
+      first(up);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        inc[d] = SIXTH * k[d];
+        xp[d] = xn[d] + HALF * k[d];
+      }
+      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        inc[d] = inc[d] + THIRD * k[d];
+        xp[d] = xn[d] + HALF * k[d];
+      }
+      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        k[d] = dt * up[d];
+        inc[d] = inc[d] + THIRD * k[d];
+        xp[d] = xn[d] + k[d];
+      }
+      vel(g, xp, nd, stage<S>(sp, 3), up, oob);
+#pragma unroll
+      for (int d = 0; d < 3; d++) inc[d] = inc[d] + SIXTH * (dt * up[d]);
+      break;
+  }
+}
+
+// error-free transformation a + b = s + e (Knuth two-sum)
+template <typename T>
+__device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const T bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
 template <typename T>
 __device__ __forceinline__ void project_flat(const GridDev<T>& g, int nd[3], T q[3]) {
   if (g.flat < 0) return;
@@ -432,14 +559,16 @@ __device__ __forceinline__ bool track_pass(const GridDev<T>& g, const T q[3], in
 // exchange_lpdata on one rank (comms_m.f90:872-1189): periodic wrap of a
 // particle whose tracked node left [1,n].
 template <typename T>
-__device__ __forceinline__ void wrap(const GridDev<T>& g, int nd[3], T q[3]) {
+__device__ __forceinline__ void wrap(const GridDev<T>& g, int nd[3], T q[3], T sh[3] = nullptr) {
   int c[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) c[d] = nd[d] < 1 ? -1 : (nd[d] > g.n[d] ? 1 : 0);
+  if (sh) sh[0] = sh[1] = sh[2] = T(0);
   if (!self_comm(g, c[0], c[1], c[2])) return;
   const int di = (c[0] + 1) + 3 * (c[1] + 1) + 9 * (c[2] + 1);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
+    if (sh) sh[d] = g.ps[di][d];
     q[d] = q[d] + g.ps[di][d];
     int v = nd[d];
     if (v > g.n[d]) v -= g.n[d];
@@ -620,7 +749,7 @@ __global__ void __launch_bounds__(kBlock) k_field_from_origin(GridDev<T> g, CF3<
 // velocity (bracket of the origin node, lp_motion_m.f90:171-327), particle
 // BC, old flowmap at the departure point, displacement form write.
 // ---------------------------------------------------------------------------
-template <typename T, class I>
+template <typename T, typename S, class I>
 __global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
                                                     DevStatus* st) {
   const I interp{};
@@ -643,7 +772,7 @@ __global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T>
 #pragma unroll
     for (int c = 0; c < 3; c++) up[c] = w0 * sp.un.c[c][l] + sp.ct0 * sp.unp1.c[c][l];
   };
-  integrate(g, sp, interp, first, nd, x0, xp, oob);
+  integrate<S>(g, sp, interp, first, nd, x0, xp, oob);
   project_flat(g, nd, xp);
   if (set_lp_bc(g, nd, xp, err)) {
     xp[0] = x0[0]; xp[1] = x0[1]; xp[2] = x0[2];
@@ -664,13 +793,13 @@ __global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T>
 
 // time blend u = (1-ct) u_n + ct u_np1 over the ghosted grid, in the
 // reference's expression order (lp_motion_m.f90:171-173)
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_blend(F3<T> out, CF3<T> a, CF3<T> b, T ct, long long nt) {
+template <typename T, typename S>
+__global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b, T ct, long long nt) {
 #pragma clang fp contract(off)
   const T w0 = T(1) - ct;
   for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < nt; l += (long long)gridDim.x * blockDim.x) {
 #pragma unroll
-    for (int c = 0; c < 3; c++) out.c[c][l] = w0 * a.c[c][l] + ct * b.c[c][l];
+    for (int c = 0; c < 3; c++) out.c[c][l] = narrow<S>(w0 * a.c[c][l] + ct * b.c[c][l]);
   }
 }
 
@@ -678,31 +807,75 @@ __global__ void __launch_bounds__(kBlock) k_blend(F3<T> out, CF3<T> a, CF3<T> b,
 // Stage (a'): forward particle sub-cycle (update_lp, lp_motion_m.f90:15-54):
 // integrate, BC, stick, dx accumulation, one tracking pass, periodic wrap.
 // ---------------------------------------------------------------------------
-template <typename T, class I>
-__global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T> sp, F3<T> xp, F3<T> dx,
-                                                     int16_t* __restrict__ ns0, int16_t* __restrict__ ns1,
-                                                     int16_t* __restrict__ ns2, DevStatus* st) {
+template <typename T, typename S, class I, bool COMP>
+__global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T> sp, ParticleSet<T> ps,
+                                                     DevStatus* st) {
   const int p = xcd_thread_index();
   if (p >= g.np) return;
-  T q[3] = {xp.c[0][p], xp.c[1][p], xp.c[2][p]};
-  int nd[3] = {ns0[p], ns1[p], ns2[p]};
+  T q[3] = {ps.x[0][p], ps.x[1][p], ps.x[2][p]};
+  int nd[3] = {ps.n[0][p], ps.n[1][p], ps.n[2][p]};
   const T xn[3] = {q[0], q[1], q[2]};
   unsigned long long oob = 0;
   int err = 0;
   const I interp{};
-  auto first = [&](T up[3]) { interp(g, xn, nd, sp.v[0], up, oob); };
-  integrate(g, sp, interp, first, nd, xn, q, oob);
-  project_flat(g, nd, q);
-  if (set_lp_bc(g, nd, q, err)) {
-    q[0] = xn[0]; q[1] = xn[1]; q[2] = xn[2];
+  auto first = [&](T up[3]) { interp(g, xn, nd, stage<S>(sp, 0), up, oob); };
+  if constexpr (!COMP) {
+    integrate<S>(g, sp, interp, first, nd, xn, q, oob);
+    project_flat(g, nd, q);
+    if (set_lp_bc(g, nd, q, err)) {
+      q[0] = xn[0]; q[1] = xn[1]; q[2] = xn[2];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) ps.d[d][p] = ps.d[d][p] + (q[d] - xn[d]);
+    track_pass(g, q, nd);
+    wrap(g, nd, q);
+  } else {
+    // position and displacement as unevaluated sums hi + lo, advanced with
+    // two-sum; BC moves and stick reset the low part of what they change
+    const T xnl[3] = {ps.xl[0][p], ps.xl[1][p], ps.xl[2][p]};
+    T ql[3], inc[3];
+    integrate_inc<S>(g, sp, interp, first, nd, xn, inc, oob);
+#pragma unroll
+    for (int d = 0; d < 3; d++) two_sum(xn[d], inc[d] + xnl[d], q[d], ql[d]);
+    const T qb[3] = {q[0], q[1], q[2]};
+    project_flat(g, nd, q);
+    bool stick = set_lp_bc(g, nd, q, err);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if (stick) {
+        q[d] = xn[d];
+        ql[d] = xnl[d];
+      } else if (q[d] != qb[d]) {
+        ql[d] = T(0);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      T step;
+      {
+#pragma clang fp contract(off)
+        step = (q[d] - xn[d]) + (ql[d] - xnl[d]);
+      }
+      T s, e;
+      two_sum(ps.d[d][p], step + ps.dl[d][p], s, e);
+      ps.d[d][p] = s;
+      ps.dl[d][p] = e;
+    }
+    track_pass(g, q, nd);
+    const T qw[3] = {q[0], q[1], q[2]};
+    T sh[3];
+    wrap(g, nd, q, sh);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      // the periodic shift's rounding goes into the low part
+      T s, e;
+      two_sum(qw[d], sh[d], s, e);
+      ps.xl[d][p] = ql[d] + e;
+    }
   }
 #pragma unroll
-  for (int d = 0; d < 3; d++) dx.c[d][p] = dx.c[d][p] + (q[d] - xn[d]);
-  track_pass(g, q, nd);
-  wrap(g, nd, q);
-#pragma unroll
-  for (int d = 0; d < 3; d++) xp.c[d][p] = q[d];
-  ns0[p] = (int16_t)nd[0]; ns1[p] = (int16_t)nd[1]; ns2[p] = (int16_t)nd[2];
+  for (int d = 0; d < 3; d++) ps.x[d][p] = q[d];
+  ps.n[0][p] = (int16_t)nd[0]; ps.n[1][p] = (int16_t)nd[1]; ps.n[2][p] = (int16_t)nd[2];
   flush_status(st, err, oob);
 }
 
@@ -715,6 +888,8 @@ __global__ void __launch_bounds__(kBlock) k_fwd_reset(GridDev<T> g, ParticleSet<
   decode_interior(g, p, i, j, k);
   ps.x[0][p] = g.gl[0][i]; ps.x[1][p] = g.gl[1][j]; ps.x[2][p] = g.gl[2][k];
   ps.d[0][p] = T(0); ps.d[1][p] = T(0); ps.d[2][p] = T(0);
+  ps.xl[0][p] = T(0); ps.xl[1][p] = T(0); ps.xl[2][p] = T(0);
+  ps.dl[0][p] = T(0); ps.dl[1][p] = T(0); ps.dl[2][p] = T(0);
   ps.n[0][p] = (int16_t)i; ps.n[1][p] = (int16_t)j; ps.n[2][p] = (int16_t)k;
   ps.org[p] = p;
 }
@@ -1143,42 +1318,71 @@ bool dispatch_spline(const GridDev<T>& g, F&& f) {
 }
 
 template <typename T>
-void launch_blend(const GridDev<T>& g, F3<T> out, CF3<T> a, CF3<T> b, T ct, hipStream_t s) {
+void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a, CF3<T> b, T ct, hipStream_t s) {
   const long long nb = std::min<long long>((g.nt + kBlock - 1) / kBlock, 8192);
-  hipLaunchKernelGGL((k_blend<T>), dim3((unsigned)nb), dim3(kBlock), 0, s, out, a, b, ct, g.nt);
+  auto go = [&](auto tag) {
+    using S = decltype(tag);
+    F3<S> o{{(S*)out[0], (S*)out[1], (S*)out[2]}};
+    hipLaunchKernelGGL((k_blend<T, S>), dim3((unsigned)nb), dim3(kBlock), 0, s, o, a, b, ct, g.nt);
+  };
+  if constexpr (std::is_same<T, float>::value) {
+    if (storage == kStoreF16) return go(half_t{});
+    if (storage == kStoreBF16) return go(bf16_t{});
+  }
+  go(T{});
+}
+
+// storage dispatch: the element type of the stage velocity fields
+template <typename T, class F>
+void with_storage(int storage, F&& f) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (storage == kStoreF16) return f(half_t{});
+    if (storage == kStoreBF16) return f(bf16_t{});
+  }
+  f(T{});
 }
 
 template <typename T>
 void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, F3<T> fmnew,
                     DevStatus* st, hipStream_t s) {
   const dim3 grid = grid_for(g.np), blk(kBlock);
-  if (interp == kLinear) {
-    hipLaunchKernelGGL((k_sl_step<T, IntLinear<T>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
-  } else if (interp == kNearest) {
-    hipLaunchKernelGGL((k_sl_step<T, IntNearest<T>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
-  } else {
-    dispatch_spline(g, [&](auto KX, auto KZ) {
-      constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
-      hipLaunchKernelGGL((k_sl_step<T, IntSpline<T, kx, kz>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
-    });
-  }
+  with_storage<T>(sp.storage, [&](auto tag) {
+    using S = decltype(tag);
+    if (interp == kLinear) {
+      hipLaunchKernelGGL((k_sl_step<T, S, IntLinear<T>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+    } else if (interp == kNearest) {
+      hipLaunchKernelGGL((k_sl_step<T, S, IntNearest<T>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+    } else {
+      dispatch_spline(g, [&](auto KX, auto KZ) {
+        constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
+        hipLaunchKernelGGL((k_sl_step<T, S, IntSpline<T, kx, kz>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+      });
+    }
+  });
 }
 
 template <typename T>
-void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, F3<T> xp, F3<T> dx, int16_t* ns0,
-                     int16_t* ns1, int16_t* ns2, DevStatus* st, hipStream_t s) {
+void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, ParticleSet<T> ps, int comp,
+                     DevStatus* st, hipStream_t s) {
   const dim3 grid = grid_for(g.np), blk(kBlock);
-  if (interp == kLinear) {
-    hipLaunchKernelGGL((k_fwd_step<T, IntLinear<T>>), grid, blk, 0, s, g, sp, xp, dx, ns0, ns1, ns2, st);
-  } else if (interp == kNearest) {
-    hipLaunchKernelGGL((k_fwd_step<T, IntNearest<T>>), grid, blk, 0, s, g, sp, xp, dx, ns0, ns1, ns2, st);
-  } else {
-    dispatch_spline(g, [&](auto KX, auto KZ) {
-      constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
-      hipLaunchKernelGGL((k_fwd_step<T, IntSpline<T, kx, kz>>), grid, blk, 0, s, g, sp, xp, dx, ns0, ns1, ns2,
-                         st);
-    });
-  }
+  with_storage<T>(sp.storage, [&](auto tag) {
+    using S = decltype(tag);
+    auto go = [&](auto C) {
+      constexpr bool cm = decltype(C)::value;
+      if (interp == kLinear) {
+        hipLaunchKernelGGL((k_fwd_step<T, S, IntLinear<T>, cm>), grid, blk, 0, s, g, sp, ps, st);
+      } else if (interp == kNearest) {
+        hipLaunchKernelGGL((k_fwd_step<T, S, IntNearest<T>, cm>), grid, blk, 0, s, g, sp, ps, st);
+      } else {
+        dispatch_spline(g, [&](auto KX, auto KZ) {
+          constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
+          hipLaunchKernelGGL((k_fwd_step<T, S, IntSpline<T, kx, kz>, cm>), grid, blk, 0, s, g, sp, ps, st);
+        });
+      }
+    };
+    if (comp) go(std::true_type{});
+    else go(std::false_type{});
+  });
 }
 
 template <typename T>
@@ -1230,11 +1434,11 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
   template void launch_fm_ghost<T>(const GridDev<T>&, F3<T>, CF3<T>, DevStatus*, hipStream_t);                \
   template void launch_field_from_origin<T>(const GridDev<T>&, CF3<T>, F3<T>, hipStream_t);                   \
   template void launch_spline_fit<T>(const GridDev<T>&, F3<T>, int, hipStream_t);                             \
-  template void launch_blend<T>(const GridDev<T>&, F3<T>, CF3<T>, CF3<T>, T, hipStream_t);                    \
+  template void launch_blend<T>(const GridDev<T>&, void* const*, int, CF3<T>, CF3<T>, T, hipStream_t);     \
   template void launch_sl_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, F3<T>, DevStatus*,  \
                                   hipStream_t);                                                               \
-  template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, F3<T>, F3<T>, int16_t*,    \
-                                   int16_t*, int16_t*, DevStatus*, hipStream_t);                              \
+  template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, ParticleSet<T>, int,     \
+                                   DevStatus*, hipStream_t);                                                   \
   template void launch_fwd_reset<T>(const GridDev<T>&, ParticleSet<T>, hipStream_t);                          \
   template void launch_compose<T>(const GridDev<T>&, int, const T*, long long, const int*, int, F3<T>,       \
                                   DevStatus*, hipStream_t);                                                   \

@@ -57,6 +57,14 @@ struct DevStatus {
   int pad;
 };
 
+// Storage of the gathered stage velocities.  kStoreNative keeps the compute
+// type; the 16-bit formats (FP32 compute only) halve the bytes of every
+// velocity gather and are widened to FP32 in registers.
+enum Storage { kStoreNative = 0, kStoreF16 = 1, kStoreBF16 = 2 };
+struct half_t { uint16_t u; };  // IEEE binary16
+struct bf16_t { uint16_t u; };  // bfloat16 (upper half of a binary32)
+inline size_t storage_bytes(int storage, size_t native) { return storage == kStoreNative ? native : 2; }
+
 // Per-subcycle integration parameters (lp_motion_m.f90:129-360).
 template <typename T>
 struct StageParams {
@@ -64,8 +72,9 @@ struct StageParams {
   T dt;        // signed sub-cycle step
   // velocity of each stage, u = (1-ct) u_n + ct u_np1 at the stage's ct
   // (lp_motion_m.f90:170-316): grid values for NEAREST/LINEAR, spline
-  // coefficients for QUADRATIC/CUBIC
-  CF3<T> v[4];
+  // coefficients for QUADRATIC/CUBIC; element type given by `storage`
+  const void* v[4][3];
+  int storage;
   // grid values of u_n and u_np1 and the first stage's ct, for the
   // semi-Lagrangian step whose first stage starts on a node
   CF3<T> un, unp1;
@@ -75,10 +84,14 @@ struct StageParams {
 // The forward particle set in one storage order: position, displacement since
 // the last reset, tracked node (lp%xp, lp%dx, lp%no, data_m.f90:317-337) and
 // the interior index of the node the particle started from (lp%no0).
+// With compensated updates, xl and dl carry the rounding errors of x and d
+// (x + xl and d + dl are the unevaluated sums).
 template <typename T>
 struct ParticleSet {
   T* x[3];
   T* d[3];
+  T* xl[3];
+  T* dl[3];
   int16_t* n[3];
   int* org;
 };
@@ -95,14 +108,17 @@ template <typename T>
 void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st, hipStream_t stream);
 template <typename T>
 void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t stream);
+// out (element type per `storage`) = (1-ct) a + ct b
 template <typename T>
-void launch_blend(const GridDev<T>& g, F3<T> out, CF3<T> a, CF3<T> b, T ct, hipStream_t stream);
+void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a, CF3<T> b, T ct,
+                  hipStream_t stream);
 template <typename T>
 void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold,
                     F3<T> fmnew, DevStatus* st, hipStream_t stream);
+// compensated: two-sum updates of position and displacement (uses xl, dl)
 template <typename T>
-void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, F3<T> xp, F3<T> dx,
-                     int16_t* ns0, int16_t* ns1, int16_t* ns2, DevStatus* st, hipStream_t stream);
+void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, ParticleSet<T> ps,
+                     int compensated, DevStatus* st, hipStream_t stream);
 template <typename T>
 void launch_fwd_reset(const GridDev<T>& g, ParticleSet<T> ps, hipStream_t stream);
 // device temporary bytes of the particle sort for np particles (0 on error)
@@ -111,9 +127,11 @@ size_t particle_sort_temp_bytes(long long np);
 template <typename T>
 int launch_particle_sort(const GridDev<T>& g, ParticleSet<T> a, ParticleSet<T> b, unsigned* keys,
                          unsigned* vals, void* temp, size_t temp_bytes, hipStream_t stream);
-// dst[org[q]] = src[q]: per-particle values back to origin order
+// dst[org[q]] = src[q] (+ lo[q] when lo is given): per-particle values back
+// to origin order
 template <typename T>
-void launch_unpermute(const GridDev<T>& g, const int* org, CF3<T> src, F3<T> dst, hipStream_t stream);
+void launch_unpermute(const GridDev<T>& g, const int* org, CF3<T> src, const T* const lo[3], F3<T> dst,
+                      hipStream_t stream);
 template <typename T>
 void launch_field_from_origin(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStream_t stream);
 template <typename T>

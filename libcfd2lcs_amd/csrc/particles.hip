@@ -39,18 +39,21 @@ __global__ void __launch_bounds__(kPBlock) k_permute(int np, const unsigned* __r
   for (int d = 0; d < 3; d++) {
     b.x[d][q] = a.x[d][s];
     b.d[d][q] = a.d[d][s];
+    b.xl[d][q] = a.xl[d][s];
+    b.dl[d][q] = a.dl[d][s];
     b.n[d][q] = a.n[d][s];
   }
   b.org[q] = a.org[s];
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kPBlock) k_unpermute(int np, const int* __restrict__ org, CF3<T> src, F3<T> dst) {
+__global__ void __launch_bounds__(kPBlock) k_unpermute(int np, const int* __restrict__ org, CF3<T> src, CF3<T> lo,
+                                                       F3<T> dst) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
   const int o = org[q];
 #pragma unroll
-  for (int d = 0; d < 3; d++) dst.c[d][o] = src.c[d][q];
+  for (int d = 0; d < 3; d++) dst.c[d][o] = lo.c[d] ? src.c[d][q] + lo.c[d][q] : src.c[d][q];
 }
 
 static int key_bits(long long np) {
@@ -81,16 +84,21 @@ int launch_particle_sort(const GridDev<T>& g, ParticleSet<T> a, ParticleSet<T> b
 }
 
 template <typename T>
-void launch_unpermute(const GridDev<T>& g, const int* org, CF3<T> src, F3<T> dst, hipStream_t s) {
+void launch_unpermute(const GridDev<T>& g, const int* org, CF3<T> src, const T* const lo[3], F3<T> dst,
+                      hipStream_t s) {
   const int np = (int)g.np;
-  hipLaunchKernelGGL((k_unpermute<T>), dim3((np + kPBlock - 1) / kPBlock), dim3(kPBlock), 0, s, np, org, src, dst);
+  CF3<T> l{{lo ? lo[0] : nullptr, lo ? lo[1] : nullptr, lo ? lo[2] : nullptr}};
+  hipLaunchKernelGGL((k_unpermute<T>), dim3((np + kPBlock - 1) / kPBlock), dim3(kPBlock), 0, s, np, org, src, l,
+                     dst);
 }
 
 template int launch_particle_sort<float>(const GridDev<float>&, ParticleSet<float>, ParticleSet<float>, unsigned*,
                                          unsigned*, void*, size_t, hipStream_t);
 template int launch_particle_sort<double>(const GridDev<double>&, ParticleSet<double>, ParticleSet<double>,
                                           unsigned*, unsigned*, void*, size_t, hipStream_t);
-template void launch_unpermute<float>(const GridDev<float>&, const int*, CF3<float>, F3<float>, hipStream_t);
-template void launch_unpermute<double>(const GridDev<double>&, const int*, CF3<double>, F3<double>, hipStream_t);
+template void launch_unpermute<float>(const GridDev<float>&, const int*, CF3<float>, const float* const*,
+                                      F3<float>, hipStream_t);
+template void launch_unpermute<double>(const GridDev<double>&, const int*, CF3<double>, const double* const*,
+                                       F3<double>, hipStream_t);
 
 }  // namespace lcs

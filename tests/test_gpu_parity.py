@@ -128,3 +128,80 @@ def test_particle_sort_is_bit_exact(dp, every):
         for c in range(3):
             assert np.array_equal(xc[c], yc[c])
         assert np.array_equal(xf, yf, equal_nan=True)
+
+
+def _vs_oracle(lib, opts, interp, n=20, T=0.4, h=0.2, nsteps=10):
+    """Max relative flowmap error (composed and per-step) and max absolute
+    FTLE error of the library against the FP64 oracle."""
+    from oracle_api import Oracle
+    X, Y, Z, fl = F.abc_grid(n)
+    lib.finalize()
+    lib.init(X, Y, Z, fl)
+    o = Oracle(X, Y, Z, fl)
+    hs = [lib.diagnostic_init(d, 0, T, h, "x") for d in (F.FTLE_FWD, F.FTLE_BKWD)]
+    for d in (F.FTLE_FWD, F.FTLE_BKWD):
+        o.diagnostic_init(d, 0, T, h, "x")
+    for k, v in (("INTEGRATOR", F.RK4), ("INTERPOLATOR", interp)):
+        lib.set_option(k, v)
+        o.set_option(k, v)
+    for k, v in opts.items():
+        lib.set_option(k, v)
+    u, v, w = F.abc_velocity(X, Y, Z)
+    efm, eftle = 0.0, 0.0
+    for s in range(nsteps):
+        lib.update(u, v, w, 0.05 * s)
+        o.update(u, v, w, 0.05 * s)
+        assert lib.error == 0 and o.error == 0
+        for hd in hs:
+            for c in range(3):
+                efm = max(efm, rel_err(lib.flowmap(hd)[c], o.flowmap(hd)[c]))
+    for hd in hs:
+        assert lib.noutput(hd) == o.noutput(hd) >= 2
+        for c in range(3):
+            efm = max(efm, rel_err(lib.composed(hd)[c], o.composed(hd)[c]))
+        eftle = max(eftle, float(np.abs(lib.ftle(hd) - o.ftle(hd)).max()))
+    o.close()
+    return efm, eftle
+
+
+@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC])
+@pytest.mark.parametrize("storage,tol_fm,tol_ftle", [("FP16", 1e-3, 1e-2), ("BF16", 8e-3, 8e-2)])
+def test_16bit_velocity_storage_against_fp64_oracle(sp, interp, storage, tol_fm, tol_ftle):
+    """FP32 advection with the gathered stage velocities stored in FP16 or
+    BF16 (unit roundoff 2^-11 and 2^-8): flowmaps and FTLE stay within a few
+    storage roundoffs of the FP64 oracle, and the 16-bit run is measurably
+    different from the FP32-storage run (the 16-bit path is the one that
+    ran)."""
+    import libcfd2lcs_amd as L
+    val = L.LCS_FP16 if storage == "FP16" else L.LCS_BF16
+    e16, f16 = _vs_oracle(sp, {"VELOCITY_STORAGE": val}, interp)
+    e32, f32 = _vs_oracle(sp, {}, interp)
+    print(f"{storage} interp {interp}: flowmap {e16:.3e} (fp32 {e32:.3e}), ftle {f16:.3e} (fp32 {f32:.3e})")
+    assert e16 <= tol_fm and f16 <= tol_ftle
+    assert e16 > 2 * e32
+
+
+@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC])
+def test_compensated_positions_against_fp64_oracle(sp, interp):
+    """Two-sum position/displacement updates of the forward particles keep
+    the FP32 run at least as close to the FP64 oracle as plain FP32."""
+    import libcfd2lcs_amd as L
+    ec, fc = _vs_oracle(sp, {"COMPENSATED": L.LCS_TRUE}, interp)
+    ep, fp = _vs_oracle(sp, {}, interp)
+    print(f"compensated interp {interp}: flowmap {ec:.3e} (plain {ep:.3e}), ftle {fc:.3e} (plain {fp:.3e})")
+    assert ec <= 2e-5 and fc <= 1e-3
+    assert ec <= 1.5 * ep
+
+
+def test_16bit_storage_needs_fp32_compute(dp):
+    import libcfd2lcs_amd as L
+    X, Y, Z, fl = F.abc_grid(8)
+    dp.finalize()
+    dp.init(X, Y, Z, fl)
+    dp.diagnostic_init(L.FTLE_BKWD, 0, 0.2, 0.1, "b")
+    dp.set_option("VELOCITY_STORAGE", L.LCS_BF16)
+    u, v, w = F.abc_velocity(X, Y, Z)
+    dp.update(u, v, w, 0.0)
+    assert dp.error != 0
+    dp.finalize()
+    assert dp.error == 0
