@@ -161,6 +161,32 @@ int cfd2lcs_get_timers_c(double *seconds, long long *counts, int n);
 int cfd2lcs_debug_grid_c(double *lines, int *flag, double *bn, double *ps,
                          double *bb);
 
+/* Adjoint of the backward (semi-Lagrangian) FTLE over one window.  No
+ * counterpart in the reference; the forward/backward pair plays the role of a
+ * torch.autograd.Function for a scalar loss on the FTLE field.
+ *
+ * forward: the BKWD diagnostic's flowmap with h = T over nsnap velocity
+ *   snapshots (device arrays of nsnap * n0*n1*n2 lcsdata_t each, snapshot s
+ *   at offset s*n0*n1*n2, Fortran order) at increasing times[nsnap] (host),
+ *   T = times[nsnap-1] - times[0], with the grid, INTEGRATOR, INTERPOLATOR,
+ *   CFL and INCOMPRESSIBLE of the library; writes the FTLE (device double,
+ *   n0*n1*n2) to d_ftle and keeps a flowmap checkpoint every
+ *   checkpoint_every steps.  The snapshot arrays must stay valid until the
+ *   backward call.  Returns 0 on success.
+ * backward: given dL/dFTLE (device double, n0*n1*n2), writes dL/du, dL/dv,
+ *   dL/dw for every snapshot (device, same layout as the snapshots),
+ *   recomputing each segment from its checkpoint.  Returns 0 on success.
+ * release: frees the tape.  stats: out[0] forward seconds, out[1] backward
+ *   seconds, out[2] tape bytes allocated, out[3] total sub-cycles. */
+int cfd2lcs_adjoint_forward_c(int n[3], int nsnap, const lcsdata_t *d_u,
+                              const lcsdata_t *d_v, const lcsdata_t *d_w,
+                              const lcsdata_t *times, int checkpoint_every,
+                              double *d_ftle);
+int cfd2lcs_adjoint_backward_c(const double *d_dftle, lcsdata_t *d_gu,
+                               lcsdata_t *d_gv, lcsdata_t *d_gw);
+void cfd2lcs_adjoint_release_c(void);
+void cfd2lcs_adjoint_stats_c(double out[4]);
+
 /* ---------------------------------------------------------------------
  * Part 3: Fortran-callable entry points.  A Fortran host calls the
  * reference's external subroutines cfd2lcs_init, cfd2lcs_diagnostic_init,

@@ -34,6 +34,8 @@ EXPORTED = (
     "cfd2lcs_get_flowmap_c", "cfd2lcs_get_composed_c", "cfd2lcs_get_ftle_c",
     "cfd2lcs_get_ftle_device_c", "cfd2lcs_get_noutput_c", "cfd2lcs_get_stats_c",
     "cfd2lcs_get_timers_c", "cfd2lcs_debug_grid_c",
+    "cfd2lcs_adjoint_forward_c", "cfd2lcs_adjoint_backward_c", "cfd2lcs_adjoint_release_c",
+    "cfd2lcs_adjoint_stats_c",
     "cfd2lcs_init_", "cfd2lcs_diagnostic_init_", "cfd2lcs_update_",
     "cfd2lcs_diagnostic_destroy_", "cfd2lcs_set_option_", "cfd2lcs_set_param_",
     "cfd2lcs_finalize_",
@@ -82,6 +84,13 @@ def load(variant="dp"):
     L.cfd2lcs_get_stats_c.argtypes = [C.POINTER(C.c_double)]
     L.cfd2lcs_debug_grid_c.argtypes = [C.c_void_p] * 5
     L.cfd2lcs_debug_grid_c.restype = C.c_int
+    L.cfd2lcs_adjoint_forward_c.argtypes = [ip, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, rp, C.c_int,
+                                            C.c_void_p]
+    L.cfd2lcs_adjoint_forward_c.restype = C.c_int
+    L.cfd2lcs_adjoint_backward_c.argtypes = [C.c_void_p] * 4
+    L.cfd2lcs_adjoint_backward_c.restype = C.c_int
+    L.cfd2lcs_adjoint_release_c.argtypes = []
+    L.cfd2lcs_adjoint_stats_c.argtypes = [C.POINTER(C.c_double)]
     _libs[variant] = L
     return L
 
@@ -164,6 +173,27 @@ class Cfd2lcs:
         if self.L.cfd2lcs_get_ftle_c(handle, self._rp(out)) != 0:
             raise RuntimeError(f"ftle getter failed for handle {handle}")
         return out.reshape(self.shape, order="F")
+
+    # ---- adjoint of the backward FTLE (device pointers) ----
+    def adjoint_forward(self, du, dv, dw, nsnap, times, checkpoint, dftle):
+        """du, dv, dw: device pointers to nsnap snapshots each; times: host
+        sequence; dftle: device pointer to n0*n1*n2 doubles (output)."""
+        n = (C.c_int * 3)(*self.shape)
+        ts = np.ascontiguousarray(np.asarray(times, dtype=self.dtype))
+        return self.L.cfd2lcs_adjoint_forward_c(n, int(nsnap), C.c_void_p(du), C.c_void_p(dv), C.c_void_p(dw),
+                                                self._rp(ts), int(checkpoint), C.c_void_p(dftle))
+
+    def adjoint_backward(self, ddftle, gu, gv, gw):
+        return self.L.cfd2lcs_adjoint_backward_c(C.c_void_p(ddftle), C.c_void_p(gu), C.c_void_p(gv),
+                                                 C.c_void_p(gw))
+
+    def adjoint_release(self):
+        self.L.cfd2lcs_adjoint_release_c()
+
+    def adjoint_stats(self):
+        out = (C.c_double * 4)()
+        self.L.cfd2lcs_adjoint_stats_c(out)
+        return {"forward_s": out[0], "backward_s": out[1], "tape_bytes": out[2], "subcycles": out[3]}
 
     def ftle_device_ptr(self, handle):
         return self.L.cfd2lcs_get_ftle_device_c(handle)

@@ -45,6 +45,7 @@ def build(verbose=False):
     objs = {
         "kernels": os.path.join(BUILD, "kernels.o"),
         "particles": os.path.join(BUILD, "particles.o"),
+        "adjoint": os.path.join(BUILD, "adjoint.o"),
         "grid": os.path.join(BUILD, "grid_host.o"),
         "api_dp": os.path.join(BUILD, "api_dp.o"),
         "api_sp": os.path.join(BUILD, "api_sp.o"),
@@ -52,6 +53,9 @@ def build(verbose=False):
     cmds = [
         [HIPCC, *FLAGS, *inc, "-c", os.path.join(CSRC, "kernels.hip"), "-o", objs["kernels"]],
         [HIPCC, *FLAGS, *inc, "-c", os.path.join(CSRC, "particles.hip"), "-o", objs["particles"]],
+        # hardware floating-point atomics for the adjoint scatters
+        [HIPCC, *FLAGS, "-munsafe-fp-atomics", *inc, "-c", os.path.join(CSRC, "adjoint.hip"), "-o",
+         objs["adjoint"]],
         [HIPCC, *FLAGS, *inc, "-c", os.path.join(CSRC, "grid_host.cpp"), "-o", objs["grid"]],
         [HIPCC, *FLAGS, *inc, "-c", os.path.join(CSRC, "api.cpp"), "-o", objs["api_dp"]],
         [HIPCC, *FLAGS, *inc, "-DLCS_SINGLE", "-c", os.path.join(CSRC, "api.cpp"), "-o", objs["api_sp"]],
@@ -65,7 +69,8 @@ def build(verbose=False):
         out = os.path.join(LIB, f"libcfd2lcs_{tag}.so")
         tmp = out + ".tmp"
         links.append([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
-                      objs[f"api_{tag}"], objs["kernels"], objs["particles"], objs["grid"]])
+                      objs[f"api_{tag}"], objs["kernels"], objs["particles"], objs["adjoint"],
+                      objs["grid"]])
     _run(links)
     for tag in ("dp", "sp"):
         out = os.path.join(LIB, f"libcfd2lcs_{tag}.so")

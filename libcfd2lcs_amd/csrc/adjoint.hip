@@ -1,0 +1,711 @@
+// adjoint.hip -- reverse-mode derivative of the backward semi-Lagrangian FTLE
+// pipeline (no counterpart in the reference).
+//
+// Forward map over one window (the BKWD diagnostic with h = T):
+//   fm_0 = 0;  per sub-cycle  fm_new(x) = mask (I[fm_old](x_dep(x)) + x_dep - x)
+//   with x_dep from the integrator's stages through the time-blended stage
+//   velocities (flowmap_m.f90:17-104, lp_motion_m.f90:129-360), ghosts by the
+//   flowmap BC, and FTLE = log(max(sqrt(lmax(G^T G)), tiny)) / T with G the
+//   central-difference gradient of fm (lcs_m.f90:66-193).
+// The kernels below run one sub-cycle backwards: they recompute the stage
+// positions from the velocity, propagate the adjoint of x_dep back through
+// the stages with the interpolants' Jacobians, and scatter the transposed
+// interpolation weights into the adjoints of the old flowmap and of the stage
+// velocity fields (global atomics).  Transposed banded solves carry
+// coefficient-space adjoints back to grid values, and ghost folds carry ghost
+// adjoints back to the interior nodes they were copied from.
+#include "device_common.h"
+
+namespace lcs {
+
+// ---- B-spline basis values and first derivatives (same interval search and
+// polynomial tables as bs_setup) ----
+template <typename T, int K>
+__device__ __forceinline__ int bs_setup_d(const T* __restrict__ t, const T* __restrict__ poly, int n, T x,
+                                          bool xaxis, int hint, int& base, T v[K], T dv[K]) {
+  const int L = n + K;
+  if (!(x >= t[0]) || x > t[L - 1]) return 1;
+  int left;
+  if (x >= t[L - 1]) {
+    if (!xaxis) return 2;
+    left = n;
+  } else {
+    left = iclamp(hint, K, n);
+    while (left < n && x >= t[left]) left++;
+    while (left > K && x < t[left - 1]) left--;
+  }
+  const T* P = poly + left * (K * K + 1);
+  const T ih = P[K * K];
+  const T s = (x - t[left - 1]) * ih;
+#pragma unroll
+  for (int a = 0; a < K; a++) {
+    T w = P[a * K + K - 1];
+    T d = T(K - 1) * P[a * K + K - 1];
+#pragma unroll
+    for (int m = K - 2; m >= 0; m--) {
+      w = w * s + P[a * K + m];
+      if (m >= 1) d = d * s + T(m) * P[a * K + m];
+    }
+    v[a] = w;
+    dv[a] = d * ih;
+  }
+  base = left - K;
+  return 0;
+}
+
+// lin_setup plus d t_d / d q_d on the chosen bracket (zero where the local
+// coordinate is clamped)
+template <typename T>
+__device__ __forceinline__ void lin_setup_d(const GridDev<T>& g, const T q[3], const int nd[3], LinW<T>& w,
+                                            T dtq[3]) {
+  int i0[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const T* xl = g.xl[d];
+    int n0 = iclamp(nd[d], 0, g.n[d] + 1);
+    int a = (q[d] > xl[n0]) ? n0 : n0 - 1;
+    a = iclamp(a, 0, g.n[d]);
+    const T x0 = xl[a], x1 = xl[a + 1];
+    const T raw = (q[d] - x0) / (x1 - x0);
+    const T tt = clamp01(raw);
+    w.t[d] = tt;
+    w.mt[d] = T(1) - tt;
+    dtq[d] = (raw >= T(0) && raw <= T(1)) ? T(1) / (x1 - x0) : T(0);
+    i0[d] = a;
+  }
+  w.base = i0[0] + g.N[0] * (i0[1] + g.N[1] * i0[2]);
+}
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T* p, T v) {
+  atomicAdd(p, v);
+}
+
+// ---- interpolants with value, Jacobian J[c][d] = d f_c / d q_d, and the
+// transposed weights (adj[idx] += w_c * weight) ----
+template <typename T>
+struct AdjLinear {
+  __device__ void eval(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+                       T J[3][3]) const {
+    LinW<T> w;
+    T dt[3];
+    lin_setup_d(g, q, nd, w, dt);
+    const int b = w.base, sy = g.N[0], sz = g.N[0] * g.N[1];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const T* A = f.c[c];
+      const T f0 = A[b], f1 = A[b + 1], f2 = A[b + sy], f3 = A[b + sy + 1], f4 = A[b + sz], f5 = A[b + sz + 1],
+              f6 = A[b + sz + sy], f7 = A[b + sz + sy + 1];
+      out[c] = trilin(w, f0, f1, f2, f3, f4, f5, f6, f7);
+      // d/dt_x, d/dt_y, d/dt_z of the trilinear form
+      const T ax = w.mt[1] * w.mt[2] * (f1 - f0) + w.t[1] * w.mt[2] * (f3 - f2) + w.mt[1] * w.t[2] * (f5 - f4) +
+                   w.t[1] * w.t[2] * (f7 - f6);
+      const T ay = w.mt[0] * w.mt[2] * (f2 - f0) + w.t[0] * w.mt[2] * (f3 - f1) + w.mt[0] * w.t[2] * (f6 - f4) +
+                   w.t[0] * w.t[2] * (f7 - f5);
+      const T az = w.mt[0] * w.mt[1] * (f4 - f0) + w.t[0] * w.mt[1] * (f5 - f1) + w.mt[0] * w.t[1] * (f6 - f2) +
+                   w.t[0] * w.t[1] * (f7 - f3);
+      J[c][0] = ax * dt[0];
+      J[c][1] = ay * dt[1];
+      J[c][2] = az * dt[2];
+    }
+  }
+  __device__ void scatter(const GridDev<T>& g, const T q[3], const int nd[3], const T wv[3], F3<T> adj) const {
+    LinW<T> w;
+    T dt[3];
+    lin_setup_d(g, q, nd, w, dt);
+    const int b = w.base, sy = g.N[0], sz = g.N[0] * g.N[1];
+    const int idx[8] = {b, b + 1, b + sy, b + sy + 1, b + sz, b + sz + 1, b + sz + sy, b + sz + sy + 1};
+    const T ww[8] = {w.mt[0] * w.mt[1] * w.mt[2], w.t[0] * w.mt[1] * w.mt[2], w.mt[0] * w.t[1] * w.mt[2],
+                     w.t[0] * w.t[1] * w.mt[2],  w.mt[0] * w.mt[1] * w.t[2], w.t[0] * w.mt[1] * w.t[2],
+                     w.mt[0] * w.t[1] * w.t[2],  w.t[0] * w.t[1] * w.t[2]};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if (wv[c] == T(0)) continue;
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (ww[e] != T(0)) atomic_add(adj.c[c] + idx[e], wv[c] * ww[e]);
+    }
+  }
+};
+
+template <typename T, int KX, int KZ>
+struct AdjSpline {
+  struct W {
+    int base;
+    T wx[KX], wy[KX], wz[KZ], dx[KX], dy[KX], dz[KZ];
+  };
+  __device__ int setup(const GridDev<T>& g, const T q[3], const int nd[3], W& s) const {
+    int bx, by, bz;
+    const int rx = bs_setup_d<T, KX>(g.kn[0], g.bpoly[0], g.N[0], q[0], true, nd[0] + (KX + 2) / 2, bx, s.wx, s.dx);
+    const int ry = bs_setup_d<T, KX>(g.kn[1], g.bpoly[1], g.N[1], q[1], false, nd[1] + (KX + 2) / 2, by, s.wy, s.dy);
+    const int rz = bs_setup_d<T, KZ>(g.kn[2], g.bpoly[2], g.N[2], q[2], false, nd[2] + (KZ + 2) / 2, bz, s.wz, s.dz);
+    if (rx || ry || rz) return 1;
+    s.base = bx + g.N[0] * (by + g.N[1] * bz);
+    return 0;
+  }
+  __device__ void eval(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+                       T J[3][3]) const {
+    W s;
+    if (setup(g, q, nd, s)) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        out[c] = T(0);
+        J[c][0] = J[c][1] = J[c][2] = T(0);
+      }
+      return;
+    }
+    const int sy = g.N[0], sz = g.N[0] * g.N[1];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      T v = 0, gx = 0, gy = 0, gz = 0;
+#pragma unroll
+      for (int cz = 0; cz < KZ; cz++) {
+        T vy = 0, gxy = 0, gyy = 0;
+#pragma unroll
+        for (int by = 0; by < KX; by++) {
+          T row[KX];
+          load_row<T, KX>(f.c[c] + s.base + by * sy + cz * sz, row);
+          T sx = 0, sdx = 0;
+#pragma unroll
+          for (int a = 0; a < KX; a++) {
+            sx += s.wx[a] * row[a];
+            sdx += s.dx[a] * row[a];
+          }
+          vy += s.wy[by] * sx;
+          gxy += s.wy[by] * sdx;
+          gyy += s.dy[by] * sx;
+        }
+        v += s.wz[cz] * vy;
+        gx += s.wz[cz] * gxy;
+        gy += s.wz[cz] * gyy;
+        gz += s.dz[cz] * vy;
+      }
+      out[c] = v;
+      J[c][0] = gx;
+      J[c][1] = gy;
+      J[c][2] = gz;
+    }
+  }
+  __device__ void scatter(const GridDev<T>& g, const T q[3], const int nd[3], const T wv[3], F3<T> adj) const {
+    W s;
+    if (setup(g, q, nd, s)) return;
+    const int sy = g.N[0], sz = g.N[0] * g.N[1];
+#pragma unroll
+    for (int cz = 0; cz < KZ; cz++)
+#pragma unroll
+      for (int by = 0; by < KX; by++) {
+        const T wyz = s.wy[by] * s.wz[cz];
+#pragma unroll
+        for (int a = 0; a < KX; a++) {
+          const T w = s.wx[a] * wyz;
+          const int idx = s.base + a + by * sy + cz * sz;
+#pragma unroll
+          for (int c = 0; c < 3; c++)
+            if (wv[c] != T(0) && w != T(0)) atomic_add(adj.c[c] + idx, wv[c] * w);
+        }
+      }
+  }
+};
+
+// dx_new(wrt q) of set_lp_bc: identity, a projection I - n n^T (MASK /
+// boundary WALL / SLIP node with the particle outside), or zero (stick).
+// Mirrors set_lp_bc's branches on the same node and position.
+template <typename T>
+__device__ __forceinline__ int lp_bc_kind(const GridDev<T>& g, const int nd0[3], const T q[3], T nrm[3]) {
+  const int i = iclamp(nd0[0], 1, g.n[0]), j = iclamp(nd0[1], 1, g.n[1]), k = iclamp(nd0[2], 1, g.n[2]);
+  const int f = g.flag[gid(g, i, j, k)];
+  const bool inner = (i > 1 && j > 1 && k > 1 && i < g.n[0] && j < g.n[1] && k < g.n[2]);
+  switch (f) {
+    case kSlip:
+    case kWall:
+      if (inner) return 0;
+      // fall through
+    case kMask: {
+      const int p = pid(g, i, j, k);
+      nrm[0] = g.bn[0][p]; nrm[1] = g.bn[1][p]; nrm[2] = g.bn[2][p];
+      const T dist = (q[0] - g.gl[0][i]) * nrm[0] + (q[1] - g.gl[1][j]) * nrm[1] + (q[2] - g.gl[2][k]) * nrm[2];
+      return dist > T(0) ? 1 : 0;
+    }
+    case kInflow:
+    case kOutflow:
+      return inner ? 0 : 2;
+    default:
+      return 0;
+  }
+}
+
+// One backward semi-Lagrangian sub-cycle in reverse.  lam: adjoint of the new
+// flowmap at the interior nodes (ghosted indexing).  Accumulates:
+//   adj_fm      transposed interpolation of the old flowmap (field the
+//               forward kernel read: values or spline coefficients)
+//   adj_v[b]    transposed interpolation of stage velocity buffer b; stage
+//               s >= 2 goes to buffer sbuf[s]
+//   dun, dunp1  first-stage (nodal value) adjoint, split by ct0
+template <typename T, class A>
+__global__ void __launch_bounds__(kBlock) k_sl_adjoint(GridDev<T> g, StageParams<T> sp, CF3<T> fmold, CF3<T> lam,
+                                                       F3<T> adj_fm, F3<T> adj_v0, F3<T> adj_v1, F3<T> adj_v2,
+                                                       int4 sbuf, F3<T> dun, F3<T> dunp1) {
+  const A ad{};
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  const int l = gid(g, i, j, k);
+  const int f = g.flag[l];
+  if (f == kMask || f == kInflow || f == kOutflow) return;
+  T gl3[3] = {lam.c[0][l], lam.c[1][l], lam.c[2][l]};
+  if (gl3[0] == T(0) && gl3[1] == T(0) && gl3[2] == T(0)) return;
+  const int nd[3] = {i, j, k};
+  const T x0[3] = {g.gl[0][i], g.gl[1][j], g.gl[2][k]};
+  const T dt = sp.dt;
+  const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
+  // ---- recompute the stages (same expressions as integrate()) ----
+  T xs[4][3], Js[4][3][3], up[3], kk[3], x1[3], xp[3];
+  CF3<T> vf[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) vf[a] = CF3<T>{{(const T*)sp.v[a][0], (const T*)sp.v[a][1], (const T*)sp.v[a][2]}};
+  // first stage: nodal blended value
+  {
+    const T w0 = T(1) - sp.ct0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) up[c] = w0 * sp.un.c[c][l] + sp.ct0 * sp.unp1.c[c][l];
+  }
+  int ns;
+  T a_[4] = {0, 0, 0, 0}, b_[4] = {0, 0, 0, 0};
+  switch (sp.integ) {
+    case kEuler:
+    case kTrapezoidal:
+      ns = 1; b_[0] = T(1);
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = x0[d] + dt * up[d];
+      break;
+    case kRK2:
+      ns = 2; a_[1] = HALF; b_[1] = T(1);
+#pragma unroll
+      for (int d = 0; d < 3; d++) xs[1][d] = x0[d] + HALF * dt * up[d];
+      ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = x0[d] + dt * up[d];
+      break;
+    case kRK3:
+      ns = 3; a_[1] = HALF; a_[2] = TWO; b_[0] = SIXTH; b_[1] = TWOTHIRD; b_[2] = SIXTH;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        kk[d] = dt * up[d];
+        x1[d] = x0[d] + SIXTH * kk[d];
+        xs[1][d] = x0[d] + HALF * kk[d];
+      }
+      ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        kk[d] = dt * up[d];
+        xs[2][d] = x0[d] + TWO * kk[d];
+        x1[d] = x1[d] + TWOTHIRD * kk[d];
+      }
+      ad.eval(g, xs[2], nd, vf[2], up, Js[2]);
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = x1[d] + SIXTH * (dt * up[d]);
+      break;
+    case kRK4:
+    default:
+      ns = 4; a_[1] = HALF; a_[2] = HALF; a_[3] = T(1);
+      b_[0] = SIXTH; b_[1] = THIRD; b_[2] = THIRD; b_[3] = SIXTH;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        kk[d] = dt * up[d];
+        x1[d] = x0[d] + SIXTH * kk[d];
+        xs[1][d] = x0[d] + HALF * kk[d];
+      }
+      ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        kk[d] = dt * up[d];
+        x1[d] = x1[d] + THIRD * kk[d];
+        xs[2][d] = x0[d] + HALF * kk[d];
+      }
+      ad.eval(g, xs[2], nd, vf[2], up, Js[2]);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        kk[d] = dt * up[d];
+        x1[d] = x1[d] + THIRD * kk[d];
+        xs[3][d] = x0[d] + kk[d];
+      }
+      ad.eval(g, xs[3], nd, vf[3], up, Js[3]);
+#pragma unroll
+      for (int d = 0; d < 3; d++) xp[d] = x1[d] + SIXTH * (dt * up[d]);
+      break;
+  }
+  // ---- BC of the departure point ----
+  int ndb[3] = {i, j, k};
+  T xb[3] = {xp[0], xp[1], xp[2]};
+  project_flat(g, ndb, xb);
+  T nrm[3] = {0, 0, 0};
+  const int kind = lp_bc_kind(g, ndb, xb, nrm);
+  int err = 0;
+  if (set_lp_bc(g, ndb, xb, err)) {
+    xb[0] = x0[0]; xb[1] = x0[1]; xb[2] = x0[2];
+  }
+  // ---- reverse ----
+  // fm_new = F(xb) + xb - x0
+  T Fv[3], JF[3][3];
+  const int no[3] = {i, j, k};
+  ad.eval(g, xb, no, fmold, Fv, JF);
+  ad.scatter(g, xb, no, gl3, adj_fm);
+  T xbar[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) xbar[d] = gl3[d] + JF[0][d] * gl3[0] + JF[1][d] * gl3[1] + JF[2][d] * gl3[2];
+  if (kind == 2) return;  // stuck: x_dep does not depend on the velocity
+  if (kind == 1) {
+    const T dn = xbar[0] * nrm[0] + xbar[1] * nrm[1] + xbar[2] * nrm[2];
+#pragma unroll
+    for (int d = 0; d < 3; d++) xbar[d] -= nrm[d] * dn;
+  }
+  if (g.flat >= 0) xbar[g.flat] = T(0);
+  // k_bar_s = b_s * xbar; x_s = x0 + a_s k_{s-1}
+  T kb[4][3];
+#pragma unroll
+  for (int s = 0; s < 4; s++)
+#pragma unroll
+    for (int d = 0; d < 3; d++) kb[s][d] = b_[s] * xbar[d];
+  F3<T> av[3] = {adj_v0, adj_v1, adj_v2};
+  const int sb[4] = {sbuf.x, sbuf.y, sbuf.z, sbuf.w};
+  for (int s = ns - 1; s >= 1; s--) {
+    // k_s = dt * v_s(x_s)
+    T w[3] = {dt * kb[s][0], dt * kb[s][1], dt * kb[s][2]};
+    ad.scatter(g, xs[s], nd, w, av[sb[s]]);
+    T xsb[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) xsb[d] = Js[s][0][d] * w[0] + Js[s][1][d] * w[1] + Js[s][2][d] * w[2];
+#pragma unroll
+    for (int d = 0; d < 3; d++) kb[s - 1][d] += a_[s] * xsb[d];
+  }
+  // first stage: k_1 = dt * ((1-ct0) u_n + ct0 u_np1) at the node
+  const T w0 = T(1) - sp.ct0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const T kbar = dt * kb[0][c];
+    dun.c[c][l] += w0 * kbar;
+    dunp1.c[c][l] += sp.ct0 * kbar;
+  }
+}
+
+// adj_n += (1-ct) b, adj_np1 += ct b, b = 0 (transpose of the stage blend)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_adj_unblend(F3<T> b, T ct, F3<T> an, F3<T> anp1, long long nt) {
+  const T w0 = T(1) - ct;
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < nt; l += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const T v = b.c[c][l];
+      if (v != T(0)) {
+        an.c[c][l] += w0 * v;
+        anp1.c[c][l] += ct * v;
+        b.c[c][l] = T(0);
+      }
+    }
+  }
+}
+
+// Transposed banded solve of one line, A^T y = b with A = L U (dbnfac):
+// U^T z = b forward, then L^T y = z backward; band element (r, c) at
+// (r - c + K - 1) + (2K - 1) c.
+template <typename T, int K>
+__global__ void __launch_bounds__(kBlock) k_fit_T_axis(F3<T> a, const T* __restrict__ W, int n, int axis, int N0,
+                                                       int N1, int nlines) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nlines) return;
+  long long start, stride;
+  if (axis == 0) {
+    start = (long long)N0 * id; stride = 1;
+  } else if (axis == 1) {
+    const int i = id % N0, kk = id / N0;
+    start = i + (long long)N0 * N1 * kk; stride = N0;
+  } else {
+    start = id; stride = (long long)N0 * N1;
+  }
+  T* b = a.c[blockIdx.y] + start;
+  constexpr int M = K - 1, NB = 2 * K - 1;
+  T win[M > 0 ? M : 1];
+#pragma unroll
+  for (int m = 0; m < M; m++) win[m] = T(0);
+  for (int r = 0; r < n; r++) {
+    T v = b[r * stride];
+#pragma unroll
+    for (int jj = M; jj >= 1; jj--)
+      if (r - jj >= 0) v = v - win[jj - 1] * W[(M - jj) + NB * r];
+    v = v / W[M + NB * r];
+#pragma unroll
+    for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
+    win[0] = v;
+    b[r * stride] = v;
+  }
+#pragma unroll
+  for (int m = 0; m < M; m++) win[m] = T(0);
+  for (int r = n - 1; r >= 0; r--) {
+    T v = b[r * stride];
+#pragma unroll
+    for (int jj = M; jj >= 1; jj--)
+      if (r + jj <= n - 1) v = v - win[jj - 1] * W[(jj + M) + NB * r];
+#pragma unroll
+    for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
+    win[0] = v;
+    b[r * stride] = v;
+  }
+}
+
+// Ghost folds: every ghost whose forward value was copied from an interior
+// node hands its adjoint back to that node; ghosts that were set to zero (or
+// held an earlier value) drop theirs.  mode 0: velocity rules (k_vel_ghost),
+// 1: flowmap rules (k_fm_ghost / k_field_from_origin).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fold_ghost(GridDev<T> g, F3<T> adj, int mode) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (t >= ghost_count(g) || !decode_ghost(g, t, i, j, k)) return;
+  const int l = gid(g, i, j, k);
+  const int f = g.flag[l];
+  const int ei = i < 1 ? -1 : (i > g.n[0] ? 1 : 0), ej = j < 1 ? -1 : (j > g.n[1] ? 1 : 0),
+            ek = k < 1 ? -1 : (k > g.n[2] ? 1 : 0);
+  int s = -1;
+  if (f == kInternal) {
+    if (self_comm(g, ei, ej, ek))
+      s = gid(g, ei < 0 ? g.n[0] : (ei > 0 ? 1 : i), ej < 0 ? g.n[1] : (ej > 0 ? 1 : j),
+              ek < 0 ? g.n[2] : (ek > 0 ? 1 : k));
+  } else if (f == kSlip || f == k2D || (mode == 0 && (f == kInflow || f == kOutflow))) {
+    s = gid(g, iclamp(i, 1, g.n[0]), iclamp(j, 1, g.n[1]), iclamp(k, 1, g.n[2]));
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const T v = adj.c[c][l];
+    adj.c[c][l] = T(0);
+    if (s >= 0 && v != T(0)) atomic_add(adj.c[c] + s, v);
+  }
+}
+
+// interior nodes whose forward value is forced to zero: MASK for the velocity
+// (k_cfl), MASK / INFLOW / OUTFLOW for a flowmap built from origins
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_zero_forced(GridDev<T> g, F3<T> adj, int mode) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  const int l = gid(g, i, j, k);
+  const int f = g.flag[l];
+  const bool z = mode == 0 ? (f == kMask) : (f == kMask || f == kInflow || f == kOutflow);
+  if (z) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) adj.c[c][l] = T(0);
+  }
+}
+
+// eigenvector of the largest eigenvalue lam of symmetric C (FP64): the
+// largest cross product of two rows of C - lam I
+__device__ inline void eigvec3(const double C[3][3], double lam, double v[3]) {
+  const double r[3][3] = {{C[0][0] - lam, C[0][1], C[0][2]}, {C[1][0], C[1][1] - lam, C[1][2]},
+                          {C[2][0], C[2][1], C[2][2] - lam}};
+  double best = -1.0;
+  v[0] = 1; v[1] = 0; v[2] = 0;
+  for (int a = 0; a < 3; a++)
+    for (int b = a + 1; b < 3; b++) {
+      const double c0 = r[a][1] * r[b][2] - r[a][2] * r[b][1];
+      const double c1 = r[a][2] * r[b][0] - r[a][0] * r[b][2];
+      const double c2 = r[a][0] * r[b][1] - r[a][1] * r[b][0];
+      const double nn = c0 * c0 + c1 * c1 + c2 * c2;
+      if (nn > best) {
+        best = nn;
+        v[0] = c0; v[1] = c1; v[2] = c2;
+      }
+    }
+  const double nn = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (nn > 0) {
+    v[0] /= nn; v[1] /= nn; v[2] /= nn;
+  } else {
+    v[0] = 1; v[1] = 0; v[2] = 0;
+  }
+}
+
+// Adjoint of k_ftle: dL/dftle -> dL/dfm (ghosted, atomic scatter to the six
+// stencil neighbours).  ftle = log(max(sqrt(lam), tiny))/T, lam = lmax(G^T G):
+// d ftle / d lam = 1/(2 T lam), d lam / d G = 2 G v v^T.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_ftle_adjoint(GridDev<T> g, CF3<T> fm, double Tint, int incomp,
+                                                         double tiny, const double* __restrict__ dftle,
+                                                         F3<T> adj) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  const double fb = dftle[p];
+  if (fb == 0.0) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  const int ip = (i == g.n[0] && g.per[0]) ? gid(g, 1, j, k) : gid(g, i + 1, j, k);
+  const int im = (i == 1 && g.per[0]) ? gid(g, g.n[0], j, k) : gid(g, i - 1, j, k);
+  const int jp = (j == g.n[1] && g.per[1]) ? gid(g, i, 1, k) : gid(g, i, j + 1, k);
+  const int jm = (j == 1 && g.per[1]) ? gid(g, i, g.n[1], k) : gid(g, i, j - 1, k);
+  const int kp = (k == g.n[2] && g.per[2]) ? gid(g, i, j, 1) : gid(g, i, j, k + 1);
+  const int km = (k == 1 && g.per[2]) ? gid(g, i, j, g.n[2]) : gid(g, i, j, k - 1);
+  const double h[3] = {(double)g.gl[0][i + 1] - (double)g.gl[0][i - 1],
+                       (double)g.gl[1][j + 1] - (double)g.gl[1][j - 1],
+                       (double)g.gl[2][k + 1] - (double)g.gl[2][k - 1]};
+  double G[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const T* F = fm.c[c];
+    G[c][0] = ((double)F[ip] - (double)F[im]) / h[0];
+    G[c][1] = ((double)F[jp] - (double)F[jm]) / h[1];
+    G[c][2] = ((double)F[kp] - (double)F[km]) / h[2];
+  }
+  bool live[3] = {g.n[0] != 1, g.n[1] != 1, g.n[2] != 1};
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    if (!live[d]) G[0][d] = G[1][d] = G[2][d] = 0;
+  double C[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) C[a][b] = G[0][a] * G[0][b] + G[1][a] * G[1][b] + G[2][a] * G[2][b];
+  const double lam = eig_sym3_max(C[0][0], C[0][1], C[0][2], C[1][1], C[1][2], C[2][2]);
+  if (!(sqrt(lam) > tiny)) return;
+  const double r = log(sqrt(lam)) / Tint;
+  if (incomp && r < 0.0) return;
+  double v[3];
+  eigvec3(C, lam, v);
+  const double lb = fb / (2.0 * Tint * lam);
+  const int pl[3] = {ip, jp, kp}, mi[3] = {im, jm, km};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double gv = G[c][0] * v[0] + G[c][1] * v[1] + G[c][2] * v[2];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if (!live[d]) continue;
+      const double gb = 2.0 * lb * gv * v[d] / h[d];
+      atomic_add(adj.c[c] + pl[d], (T)gb);
+      atomic_add(adj.c[c] + mi[d], (T)(-gb));
+    }
+  }
+}
+
+// out[s*np + p] (+)= adj interior (ghosted index), user layout
+template <typename T, typename U>
+__global__ void __launch_bounds__(kBlock) k_adj_to_user(GridDev<T> g, CF3<T> adj, U* a, U* b, U* c, int accumulate) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  const int l = gid(g, i, j, k);
+  if (accumulate) {
+    a[p] += (U)adj.c[0][l];
+    b[p] += (U)adj.c[1][l];
+    c[p] += (U)adj.c[2][l];
+  } else {
+    a[p] = (U)adj.c[0][l];
+    b[p] = (U)adj.c[1][l];
+    c[p] = (U)adj.c[2][l];
+  }
+}
+
+// interior values of a field (ghosted index) to origin order (np)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_interior(GridDev<T> g, CF3<T> src, F3<T> dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  const int l = gid(g, i, j, k);
+#pragma unroll
+  for (int c = 0; c < 3; c++) dst.c[c][p] = src.c[c][l];
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+static inline dim3 agrid(long long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+template <typename T>
+void launch_sl_adjoint(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
+                       F3<T> adj_fm, F3<T> const adj_v[3], const int sbuf[4], F3<T> dun, F3<T> dunp1,
+                       hipStream_t s) {
+  const int4 sb = make_int4(sbuf[0], sbuf[1], sbuf[2], sbuf[3]);
+  if (interp == kLinear || interp == kNearest) {
+    hipLaunchKernelGGL((k_sl_adjoint<T, AdjLinear<T>>), agrid(g.np), dim3(kBlock), 0, s, g, sp, fmold, lam, adj_fm,
+                       adj_v[0], adj_v[1], adj_v[2], sb, dun, dunp1);
+    return;
+  }
+  const int kx = g.kord[0], kz = g.kord[2];
+  if (kx == 4 && kz == 4)
+    hipLaunchKernelGGL((k_sl_adjoint<T, AdjSpline<T, 4, 4>>), agrid(g.np), dim3(kBlock), 0, s, g, sp, fmold, lam,
+                       adj_fm, adj_v[0], adj_v[1], adj_v[2], sb, dun, dunp1);
+  else if (kx == 4 && kz == 2)
+    hipLaunchKernelGGL((k_sl_adjoint<T, AdjSpline<T, 4, 2>>), agrid(g.np), dim3(kBlock), 0, s, g, sp, fmold, lam,
+                       adj_fm, adj_v[0], adj_v[1], adj_v[2], sb, dun, dunp1);
+  else if (kx == 3 && kz == 3)
+    hipLaunchKernelGGL((k_sl_adjoint<T, AdjSpline<T, 3, 3>>), agrid(g.np), dim3(kBlock), 0, s, g, sp, fmold, lam,
+                       adj_fm, adj_v[0], adj_v[1], adj_v[2], sb, dun, dunp1);
+  else
+    hipLaunchKernelGGL((k_sl_adjoint<T, AdjSpline<T, 3, 2>>), agrid(g.np), dim3(kBlock), 0, s, g, sp, fmold, lam,
+                       adj_fm, adj_v[0], adj_v[1], adj_v[2], sb, dun, dunp1);
+}
+
+template <typename T>
+void launch_adj_unblend(const GridDev<T>& g, F3<T> b, T ct, F3<T> an, F3<T> anp1, hipStream_t s) {
+  const long long nb = std::min<long long>((g.nt + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL((k_adj_unblend<T>), dim3((unsigned)nb), dim3(kBlock), 0, s, b, ct, an, anp1, g.nt);
+}
+
+// transpose of launch_spline_fit: z, y, then x pass of A^T solves
+template <typename T>
+void launch_spline_fit_T(const GridDev<T>& g, F3<T> a, hipStream_t s) {
+  for (int axis = 2; axis >= 0; axis--) {
+    const int n = g.N[axis];
+    const int nlines = (int)(g.nt / n);
+    dim3 grid((nlines + kBlock - 1) / kBlock, 3);
+    switch (g.kord[axis]) {
+      case 2: hipLaunchKernelGGL((k_fit_T_axis<T, 2>), grid, dim3(kBlock), 0, s, a, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+      case 3: hipLaunchKernelGGL((k_fit_T_axis<T, 3>), grid, dim3(kBlock), 0, s, a, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+      default: hipLaunchKernelGGL((k_fit_T_axis<T, 4>), grid, dim3(kBlock), 0, s, a, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+    }
+  }
+}
+
+template <typename T>
+void launch_fold_ghost(const GridDev<T>& g, F3<T> adj, int mode, hipStream_t s) {
+  hipLaunchKernelGGL((k_zero_forced<T>), agrid(g.np), dim3(kBlock), 0, s, g, adj, mode);
+  const long long ng = 2LL * g.N[0] * g.N[1] + (2LL * g.N[0] + 2LL * g.n[1]) * g.n[2];
+  hipLaunchKernelGGL((k_fold_ghost<T>), agrid(ng), dim3(kBlock), 0, s, g, adj, mode);
+}
+
+template <typename T>
+void launch_ftle_adjoint(const GridDev<T>& g, CF3<T> fm, double Tint, int incomp, double tiny, const double* dftle,
+                         F3<T> adj, hipStream_t s) {
+  hipLaunchKernelGGL((k_ftle_adjoint<T>), agrid(g.np), dim3(kBlock), 0, s, g, fm, Tint, incomp, tiny, dftle, adj);
+}
+
+template <typename T, typename U>
+void launch_adj_to_user(const GridDev<T>& g, CF3<T> adj, U* a, U* b, U* c, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL((k_adj_to_user<T, U>), agrid(g.np), dim3(kBlock), 0, s, g, adj, a, b, c, accumulate);
+}
+
+template <typename T>
+void launch_interior(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStream_t s) {
+  hipLaunchKernelGGL((k_interior<T>), agrid(g.np), dim3(kBlock), 0, s, g, src, dst);
+}
+
+#define INST_ADJ(T, U)                                                                                          \
+  template void launch_sl_adjoint<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, CF3<T>, F3<T>,      \
+                                     F3<T> const*, const int*, F3<T>, F3<T>, hipStream_t);                      \
+  template void launch_adj_unblend<T>(const GridDev<T>&, F3<T>, T, F3<T>, F3<T>, hipStream_t);                  \
+  template void launch_spline_fit_T<T>(const GridDev<T>&, F3<T>, hipStream_t);                                 \
+  template void launch_fold_ghost<T>(const GridDev<T>&, F3<T>, int, hipStream_t);                               \
+  template void launch_ftle_adjoint<T>(const GridDev<T>&, CF3<T>, double, int, double, const double*, F3<T>,    \
+                                       hipStream_t);                                                            \
+  template void launch_adj_to_user<T, U>(const GridDev<T>&, CF3<T>, U*, U*, U*, int, hipStream_t);              \
+  template void launch_interior<T>(const GridDev<T>&, CF3<T>, F3<T>, hipStream_t);
+
+INST_ADJ(float, float)
+INST_ADJ(double, double)
+template void launch_adj_to_user<float, double>(const GridDev<float>&, CF3<float>, double*, double*, double*, int,
+                                                hipStream_t);
+template void launch_adj_to_user<double, float>(const GridDev<double>&, CF3<double>, float*, float*, float*, int,
+                                                hipStream_t);
+
+}  // namespace lcs

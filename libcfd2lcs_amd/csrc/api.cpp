@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -84,6 +85,22 @@ static void fail(const std::string& msg) {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p; bytes = o.bytes;
+      o.p = nullptr; o.bytes = 0;
+    }
+    return *this;
+  }
+  void swap(DevBuf& o) noexcept {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
+  }
   ~DevBuf() { release(); }
   void release() {
     if (p) (void)hipFree(p);
@@ -116,6 +133,11 @@ struct EngineBase {
   virtual bool fresh() const = 0;
   virtual int timers(double* sec, long long* cnt, int n) = 0;
   virtual int ndiag() const = 0;
+  virtual int adjoint_forward(int nsnap, const U* du, const U* dv, const U* dw, const U* times, int ckpt,
+                              double* d_ftle) = 0;
+  virtual int adjoint_backward(const double* d_dftle, U* gu, U* gv, U* gw) = 0;
+  virtual void adjoint_release() = 0;
+  virtual void adjoint_stats(double out[4]) = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -858,6 +880,376 @@ struct Engine : EngineBase {
 
   Diag* diag(int h) { return (h >= 1 && h <= (int)diags.size()) ? diags[h - 1].get() : nullptr; }
 
+  // =======================================================================
+  // Adjoint of the backward semi-Lagrangian FTLE over one window of
+  // snapshots (the BKWD diagnostic with h = T): forward pass with flowmap
+  // checkpoints every `ckpt` steps, backward pass that recomputes each
+  // segment from its checkpoint (keeping every sub-cycle's flowmap) and runs
+  // the sub-cycles in reverse.  The tape owns its own velocity ring, which is
+  // swapped with the streaming one for the duration of each call.
+  // =======================================================================
+  struct Tape {
+    bool have = false, fwd_done = false;
+    int nsnap = 0, ckpt = 1, interp = 0, integ = 0;
+    bool spl = false;
+    const U *du = nullptr, *dv = nullptr, *dw = nullptr;
+    std::vector<U> times;
+    std::vector<int> nsub;       // per step s = 1..nsnap-1 (index s)
+    U Tint = 0;
+    // swapped with the streaming state while the tape runs
+    DevBuf vel[2], vcoef[2];
+    bool vcoef_ok[2] = {false, false};
+    int cur = 1, vold = 0;
+    U t_n = 0, t_np1 = 0;
+    Blend blends[6];
+    // state
+    DevBuf fm[2], fmcoef, compfm, dxout, ftle, lam, adjfm, adjv[3], dcn, dcnp1, dun, dunp1;
+    int fcur = 0;
+    std::vector<DevBuf> ckpts;   // fm (ghosted values) after steps 0, ckpt, 2 ckpt, ...
+    std::vector<DevBuf> segf;    // per sub-cycle of a segment: the field k_sl_step read
+    double t_fwd = 0, t_bwd = 0;
+    size_t bytes = 0;
+  };
+  Tape tape;
+
+  void swap_vel_state() {
+    for (int b = 0; b < 2; b++) {
+      vel[b].swap(tape.vel[b]);
+      vcoef[b].swap(tape.vcoef[b]);
+      std::swap(vcoef_ok[b], tape.vcoef_ok[b]);
+    }
+    std::swap(cur, tape.cur);
+    std::swap(vold, tape.vold);
+    std::swap(t_n, tape.t_n);
+    std::swap(t_np1, tape.t_np1);
+    for (int b = 0; b < 6; b++) {
+      blends[b].buf.swap(tape.blends[b].buf);
+      std::swap(blends[b].ct, tape.blends[b].ct);
+      std::swap(blends[b].storage, tape.blends[b].storage);
+      std::swap(blends[b].spl, tape.blends[b].spl);
+      std::swap(blends[b].valid, tape.blends[b].valid);
+      std::swap(blends[b].stamp, tape.blends[b].stamp);
+    }
+  }
+
+  bool ensure(DevBuf& b, size_t bytes, bool zero = false) {
+    if (!b.p || b.bytes < bytes) {
+      if (!b.alloc(bytes)) {
+        fail("hipMalloc failed (adjoint buffers; out of device memory?)");
+        return false;
+      }
+      tape.bytes += bytes;
+    }
+    if (zero && hipMemsetAsync(b.p, 0, bytes, stream) != hipSuccess) {
+      fail("hipMemset failed");
+      return false;
+    }
+    return true;
+  }
+
+  // snapshots s-1 and s into the ring as u_n and u_np1 (ghosts, MASK zero,
+  // spline coefficients); returns the CFL rate of u_np1
+  bool tape_load_step(int s, double& cfl_rate) {
+    const int old = cur;
+    cur = 1 - cur;
+    vold = old;
+    vcoef_ok[cur] = false;
+    vcoef_ok[old] = false;
+    for (auto& b : blends) b.valid = false;
+    t_n = tape.times[s - 1];
+    t_np1 = tape.times[s];
+    const size_t np = (size_t)g.np;
+    for (int pass = 0; pass < 2; pass++) {
+      const int b = pass == 0 ? old : cur;
+      const int snap = pass == 0 ? s - 1 : s;
+      HIPCHK_R(hipMemsetAsync(&st->cfl_bits, 0, sizeof(st->cfl_bits), stream), false);
+      launch_ingest<U, T>(g, f3(vel[b]), tape.du + snap * np, tape.dv + snap * np, tape.dw + snap * np, stream);
+      launch_vel_ghost(g, f3(vel[b]), cf3(vel[1 - b]), st, stream);
+      launch_cfl(g, f3(vel[b]), st, stream);
+    }
+    HIPCHK_R(hipMemcpyAsync(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream), false);
+    HIPCHK_R(hipStreamSynchronize(stream), false);
+    if (st_host->err) {
+      fail("device error while loading snapshots");
+      return false;
+    }
+    unsigned long long bits = st_host->cfl_bits;
+    std::memcpy(&cfl_rate, &bits, sizeof(double));
+    if (tape.spl) {
+      if (!fit_vel(cur)) return false;
+      if (!fit_vel(old)) return false;
+    }
+    return true;
+  }
+
+  // the sub-cycle times of step s, as the forward loop accumulates them
+  void step_times(int s, std::vector<U>& ts, U& dt) {
+    const int ns = tape.nsub[s];
+    dt = (t_np1 - t_n) / (U)ns;
+    dt = (U)-1 * dt;
+    ts.resize(ns);
+    U t = t_np1;
+    for (int k = 0; k < ns; k++) {
+      ts[k] = t;
+      t = t + dt;
+    }
+  }
+
+  // one forward sub-cycle; keeps the field k_sl_step read in `keep` if given
+  bool tape_subcycle(U t, U dt, DevBuf* keep) {
+    DevBuf& a = tape.fm[tape.fcur];
+    DevBuf& b = tape.fm[1 - tape.fcur];
+    CF3<T> src = cf3(a);
+    const size_t fb = 3 * fbytes();
+    if (tape.spl) {
+      if (!ensure(tape.fmcoef, fb)) return false;
+      HIPCHK_R(hipMemcpyAsync(tape.fmcoef.p, a.p, fb, hipMemcpyDeviceToDevice, stream), false);
+      launch_spline_fit(g, f3(tape.fmcoef), 3, stream);
+      src = cf3(tape.fmcoef);
+    }
+    if (keep) {
+      if (!ensure(*keep, fb)) return false;
+      HIPCHK_R(hipMemcpyAsync(keep->p, tape.spl ? tape.fmcoef.p : a.p, fb, hipMemcpyDeviceToDevice, stream), false);
+    }
+    const StageParams<T> sp = stage_params(t, dt, tape.spl);
+    launch_sl_step(g, tape.interp, sp, src, f3(b), st, stream);
+    launch_fm_ghost(g, f3(b), cf3(a), st, stream);
+    tape.fcur = 1 - tape.fcur;
+    return true;
+  }
+
+  int adjoint_forward(int nsnap, const U* du, const U* dv, const U* dw, const U* times, int ckpt,
+                      double* d_ftle) override {
+    const int interp = g_opt.interpolator;
+    if (nsnap < 2 || !du || !dv || !dw || !times || ckpt < 1) {
+      fail("cfd2lcs_adjoint_forward_c: need at least 2 snapshots, device arrays, times and checkpoint >= 1");
+      return 1;
+    }
+    if (g_opt.vel_storage != kStoreNative) {
+      fail("the adjoint runs with VELOCITY_STORAGE equal to the compute precision");
+      return 1;
+    }
+    if (interp != kNearest && interp != kLinear && interp != kQuadratic && interp != kCubic) {
+      fail("INTERPOLATOR TSE/TSE_LIMIT is not supported by this build");
+      return 1;
+    }
+    for (int s = 1; s < nsnap; s++)
+      if (!(times[s] > times[s - 1])) {
+        fail("cfd2lcs_adjoint_forward_c: snapshot times must increase");
+        return 1;
+      }
+    auto t0 = std::chrono::steady_clock::now();
+    tape.have = false;
+    tape.fwd_done = false;
+    tape.nsnap = nsnap;
+    tape.ckpt = ckpt;
+    tape.interp = interp;
+    tape.integ = g_opt.integrator;
+    tape.spl = is_spline(interp);
+    tape.du = du; tape.dv = dv; tape.dw = dw;
+    tape.times.assign(times, times + nsnap);
+    tape.nsub.assign(nsnap, 0);
+    tape.Tint = times[nsnap - 1] - times[0];
+    if (tape.spl && !ensure_spline(interp)) return 1;
+    const size_t fb = 3 * fbytes();
+    for (int b = 0; b < 2; b++)
+      if (!ensure(tape.vel[b], fb, true) || !ensure(tape.fm[b], fb, true)) return 1;
+    tape.fcur = 0;
+    const int nseg = (nsnap - 2) / ckpt + 1;
+    tape.ckpts.resize(nseg);
+    swap_vel_state();
+    bool ok = true;
+    for (int s = 1; s < nsnap && ok; s++) {
+      if ((s - 1) % ckpt == 0) {
+        DevBuf& c = tape.ckpts[(s - 1) / ckpt];
+        ok = ensure(c, fb);
+        if (ok) ok = hipMemcpyAsync(c.p, tape.fm[tape.fcur].p, fb, hipMemcpyDeviceToDevice, stream) == hipSuccess;
+        if (!ok) break;
+      }
+      double cfl_rate = 0;
+      if (!(ok = tape_load_step(s, cfl_rate))) break;
+      const U dt_f = t_np1 - t_n;
+      const U cmax = (U)((T)cfl_rate) * dt_f;
+      const double r = std::ceil((double)(cmax / g_opt.cfl_max));
+      if (r > 1e6) {
+        fail("CFL sub-cycle count above 1e6");
+        ok = false;
+        break;
+      }
+      tape.nsub[s] = r >= 1 ? (int)r : 1;
+      std::vector<U> ts;
+      U dt;
+      step_times(s, ts, dt);
+      for (int k = 0; k < tape.nsub[s] && ok; k++) ok = tape_subcycle(ts[k], dt, nullptr);
+    }
+    swap_vel_state();
+    if (!ok) return 1;
+    // FTLE of the window (the composition of a single slot is the slot itself)
+    if (!ensure(tape.dxout, 3 * sizeof(T) * (size_t)g.np) || !ensure(tape.compfm, fb, true) ||
+        !ensure(tape.ftle, sizeof(double) * (size_t)g.np))
+      return 1;
+    launch_interior(g, cf3(tape.fm[tape.fcur]), p3(tape.dxout), stream);
+    launch_field_from_origin(g, cp3(tape.dxout), f3(tape.compfm), stream);
+    const double tiny = sizeof(U) == 8 ? DBL_MIN : (double)FLT_MIN;
+    launch_ftle(g, cf3(tape.compfm), (double)tape.Tint, g_opt.incompressible, tiny, (double*)tape.ftle.p,
+                p3(tape.dxout), stream);
+    if (d_ftle)
+      HIPCHK_R(hipMemcpyAsync(d_ftle, tape.ftle.p, sizeof(double) * (size_t)g.np, hipMemcpyDeviceToDevice, stream),
+               1);
+    HIPCHK_R(hipGetLastError(), 1);
+    HIPCHK_R(hipMemcpyAsync(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream), 1);
+    HIPCHK_R(hipStreamSynchronize(stream), 1);
+    if (st_host->err) {
+      fail("device error in the adjoint forward pass");
+      return 1;
+    }
+    tape.have = true;
+    tape.fwd_done = true;
+    tape.t_fwd = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+  }
+
+  int adjoint_backward(const double* d_dftle, U* gu, U* gv, U* gw) override {
+    if (!tape.have) {
+      fail("cfd2lcs_adjoint_backward_c called without a successful cfd2lcs_adjoint_forward_c");
+      return 1;
+    }
+    if (!d_dftle || !gu || !gv || !gw) {
+      fail("cfd2lcs_adjoint_backward_c: null device pointer");
+      return 1;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    const size_t fb = 3 * fbytes(), np = (size_t)g.np;
+    const int nsnap = tape.nsnap;
+    for (DevBuf* b : {&tape.lam, &tape.adjfm, &tape.adjv[0], &tape.adjv[1], &tape.adjv[2], &tape.dcn, &tape.dcnp1,
+                      &tape.dun, &tape.dunp1})
+      if (!ensure(*b, fb, true)) return 1;
+    for (U* o : {gu, gv, gw}) HIPCHK_R(hipMemsetAsync(o, 0, sizeof(U) * np * nsnap, stream), 1);
+    const double tiny = sizeof(U) == 8 ? DBL_MIN : (double)FLT_MIN;
+    // FTLE -> flowmap of the window's end
+    launch_ftle_adjoint(g, cf3(tape.compfm), (double)tape.Tint, g_opt.incompressible, tiny, d_dftle, f3(tape.lam),
+                        stream);
+    launch_fold_ghost(g, f3(tape.lam), 1, stream);
+    int maxsub = 0;
+    for (int s = 1; s < nsnap; s++) maxsub = std::max(maxsub, tape.nsub[s]);
+    swap_vel_state();
+    bool ok = true;
+    const int nseg = (int)tape.ckpts.size();
+    for (int seg = nseg - 1; seg >= 0 && ok; seg--) {
+      const int s0 = seg * tape.ckpt + 1, s1 = std::min(s0 + tape.ckpt - 1, nsnap - 1);
+      // recompute the segment from its checkpoint, keeping each sub-cycle's
+      // flowmap as k_sl_step read it
+      ok = hipMemcpyAsync(tape.fm[tape.fcur].p, tape.ckpts[seg].p, fb, hipMemcpyDeviceToDevice, stream) ==
+           hipSuccess;
+      const size_t need = (size_t)(s1 - s0 + 1) * maxsub;
+      if (tape.segf.size() < need) tape.segf.resize(need);
+      for (int s = s0; s <= s1 && ok; s++) {
+        double cfl_rate;
+        if (!(ok = tape_load_step(s, cfl_rate))) break;
+        std::vector<U> ts;
+        U dt;
+        step_times(s, ts, dt);
+        for (int k = 0; k < tape.nsub[s] && ok; k++)
+          ok = tape_subcycle(ts[k], dt, &tape.segf[(size_t)(s - s0) * maxsub + k]);
+      }
+      // reverse sweep over the segment
+      for (int s = s1; s >= s0 && ok; s--) {
+        double cfl_rate;
+        if (!(ok = tape_load_step(s, cfl_rate))) break;
+        std::vector<U> ts;
+        U dt;
+        step_times(s, ts, dt);
+        for (DevBuf* b : {&tape.dcn, &tape.dcnp1, &tape.dun, &tape.dunp1})
+          HIPCHK_R(hipMemsetAsync(b->p, 0, fb, stream), 1);
+        for (int k = tape.nsub[s] - 1; k >= 0 && ok; k--) {
+          const StageParams<T> sp = stage_params(ts[k], dt, tape.spl);
+          // stage ct values and their adjoint buffers (stages >= 2)
+          const U half = (U)1 / (U)2;
+          U c[4] = {0, 0, 0, 0};
+          int nst = 1;
+          switch (tape.integ) {
+            case kRK2: c[1] = ctf(ts[k] + half * dt, t_n, t_np1); nst = 2; break;
+            case kRK3:
+              c[1] = ctf(ts[k] + half * dt, t_n, t_np1);
+              c[2] = ctf(ts[k] + dt, t_n, t_np1);
+              nst = 3;
+              break;
+            case kRK4:
+              c[1] = ctf(ts[k] + half * dt, t_n, t_np1);
+              c[2] = c[1];
+              c[3] = ctf(ts[k] + dt, t_n, t_np1);
+              nst = 4;
+              break;
+            default: break;
+          }
+          int sbuf[4] = {0, 0, 0, 0};
+          U bct[3] = {0, 0, 0};
+          int nb = 0;
+          for (int a = 1; a < nst; a++) {
+            int f = -1;
+            for (int b = 0; b < nb; b++)
+              if (bct[b] == c[a]) f = b;
+            if (f < 0) {
+              f = nb++;
+              bct[f] = c[a];
+            }
+            sbuf[a] = f;
+          }
+          HIPCHK_R(hipMemsetAsync(tape.adjfm.p, 0, fb, stream), 1);
+          const DevBuf& fread = tape.segf[(size_t)(s - s0) * maxsub + k];
+          F3<T> av[3] = {f3(tape.adjv[0]), f3(tape.adjv[1]), f3(tape.adjv[2])};
+          launch_sl_adjoint(g, tape.interp, sp, cf3(const_cast<DevBuf&>(fread)), cf3(tape.lam), f3(tape.adjfm), av,
+                            sbuf, f3(tape.dun), f3(tape.dunp1), stream);
+          for (int b = 0; b < nb; b++)
+            launch_adj_unblend(g, f3(tape.adjv[b]), (T)bct[b], f3(tape.dcn), f3(tape.dcnp1), stream);
+          if (tape.spl) launch_spline_fit_T(g, f3(tape.adjfm), stream);
+          launch_fold_ghost(g, f3(tape.adjfm), 1, stream);
+          tape.lam.swap(tape.adjfm);
+        }
+        // velocity adjoints of this step, back to the snapshots
+        if (tape.spl) {
+          launch_spline_fit_T(g, f3(tape.dcn), stream);
+          launch_spline_fit_T(g, f3(tape.dcnp1), stream);
+        }
+        launch_adj_unblend(g, f3(tape.dun), (T)0, f3(tape.dcn), f3(tape.dcnp1), stream);
+        launch_adj_unblend(g, f3(tape.dunp1), (T)1, f3(tape.dcn), f3(tape.dcnp1), stream);
+        launch_fold_ghost(g, f3(tape.dcn), 0, stream);
+        launch_fold_ghost(g, f3(tape.dcnp1), 0, stream);
+        launch_adj_to_user<T, U>(g, cf3(tape.dcn), gu + (s - 1) * np, gv + (s - 1) * np, gw + (s - 1) * np, 1,
+                                 stream);
+        launch_adj_to_user<T, U>(g, cf3(tape.dcnp1), gu + s * np, gv + s * np, gw + s * np, 1, stream);
+        ok = hipGetLastError() == hipSuccess;
+      }
+    }
+    swap_vel_state();
+    if (!ok) {
+      if (!g_error) fail("adjoint backward pass failed");
+      return 1;
+    }
+    HIPCHK_R(hipMemcpyAsync(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream), 1);
+    HIPCHK_R(hipStreamSynchronize(stream), 1);
+    if (st_host->err) {
+      fail("device error in the adjoint backward pass");
+      return 1;
+    }
+    tape.t_bwd = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+  }
+
+  void adjoint_release() override {
+    (void)hipStreamSynchronize(stream);
+    tape = Tape();
+  }
+
+  void adjoint_stats(double out[4]) override {
+    out[0] = tape.t_fwd;
+    out[1] = tape.t_bwd;
+    out[2] = (double)tape.bytes;
+    long long tot = 0;
+    for (int v : tape.nsub) tot += v;
+    out[3] = (double)tot;
+  }
+
   int copy_out(CF3<T> src, int ghosted, U* a, U* b, U* c) {
     if (!stage_out.p && !stage_out.alloc(3 * sizeof(U) * (size_t)g.np)) return 1;
     U* o = (U*)stage_out.p;
@@ -1159,6 +1551,36 @@ void cfd2lcs_get_stats_c(double out[8]) {
 int cfd2lcs_get_timers_c(double* seconds, long long* counts, int n) {
   if (!g_eng) return 0;
   return g_eng->timers(seconds, counts, n);
+}
+
+// ---- adjoint of the backward semi-Lagrangian FTLE (no reference counterpart)
+int cfd2lcs_adjoint_forward_c(int n[3], int nsnap, const lcsdata_t* d_u, const lcsdata_t* d_v,
+                              const lcsdata_t* d_w, const lcsdata_t* times, int checkpoint_every, double* d_ftle) {
+  if (g_error) return 1;
+  if (!ensure_engine()) return 1;
+  if (!n || n[0] != g_grid->n[0] || n[1] != g_grid->n[1] || n[2] != g_grid->n[2]) {
+    fail("received velocity array of incorrect dimension");
+    return 1;
+  }
+  return g_eng->adjoint_forward(nsnap, d_u, d_v, d_w, times, checkpoint_every, d_ftle);
+}
+
+int cfd2lcs_adjoint_backward_c(const double* d_dftle, lcsdata_t* d_gu, lcsdata_t* d_gv, lcsdata_t* d_gw) {
+  if (g_error) return 1;
+  if (!g_eng) {
+    fail("cfd2lcs_adjoint_backward_c called before cfd2lcs_init_c");
+    return 1;
+  }
+  return g_eng->adjoint_backward(d_dftle, d_gu, d_gv, d_gw);
+}
+
+void cfd2lcs_adjoint_release_c(void) {
+  if (g_eng) g_eng->adjoint_release();
+}
+
+void cfd2lcs_adjoint_stats_c(double out[4]) {
+  for (int a = 0; a < 4; a++) out[a] = 0;
+  if (g_eng) g_eng->adjoint_stats(out);
 }
 
 // host-side grid description (for tests): lines[N0+N1+N2], flag[nt],

@@ -6,581 +6,9 @@
 // Stage (c)  gradient -> Cauchy-Green -> FTLE       k_ftle
 // plus velocity ingest / ghost shell / CFL reduction and the tensor B-spline
 // fit (three banded axis solves).  Reference semantics are cited per kernel.
-#include <hip/hip_fp16.h>
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cfloat>
-#include <cstdint>
-
-#include "lcs_device.h"
+#include "device_common.h"
 
 namespace lcs {
-
-// 27-neighbour order: self, faces, edges/corners (data_m.f90:123-152)
-__constant__ int8_t c_nbr[27][3] = {
-    {0, 0, 0},   {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},  {1, 0, 0},   {0, 1, 0},  {0, 0, 1},
-    {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {-1, 0, -1}, {1, 0, -1}, {-1, 1, -1}, {0, 1, -1},
-    {1, 1, -1},  {-1, -1, 0}, {1, -1, 0},  {-1, 1, 0},  {1, 1, 0},   {-1, -1, 1}, {0, -1, 1},
-    {1, -1, 1},  {-1, 0, 1},  {1, 0, 1},   {-1, 1, 1},  {0, 1, 1},   {1, 1, 1}};
-
-constexpr int kBlock = 256;
-
-template <typename T>
-__device__ __forceinline__ T dmin(T a, T b) { return fmin(a, b); }
-template <typename T>
-__device__ __forceinline__ T dmax(T a, T b) { return fmax(a, b); }
-// Fortran max(min(r,1),0); a NaN ratio (first update) gives 1 as with minNum
-template <typename T>
-__device__ __forceinline__ T clamp01(T r) { return dmax(dmin(r, T(1)), T(0)); }
-__device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-template <typename T>
-__device__ __forceinline__ int gid(const GridDev<T>& g, int i, int j, int k) {
-  return i + g.N[0] * (j + g.N[1] * k);
-}
-template <typename T>
-__device__ __forceinline__ void decode_interior(const GridDev<T>& g, int p, int& i, int& j, int& k) {
-  i = p % g.n[0] + 1;
-  int r = p / g.n[0];
-  j = r % g.n[1] + 1;
-  k = r / g.n[1] + 1;
-}
-template <typename T>
-__device__ __forceinline__ int pid(const GridDev<T>& g, int i, int j, int k) {
-  return (i - 1) + g.n[0] * ((j - 1) + g.n[1] * (k - 1));
-}
-template <typename T>
-__device__ __forceinline__ bool self_comm(const GridDev<T>& g, int ei, int ej, int ek) {
-  if (ei == 0 && ej == 0 && ek == 0) return false;
-  return (ei == 0 || g.per[0]) && (ej == 0 || g.per[1]) && (ek == 0 || g.per[2]);
-}
-// ghost shell enumeration: thread t -> ghost cell (i,j,k)
-template <typename T>
-__device__ __forceinline__ bool decode_ghost(const GridDev<T>& g, long long t, int& i, int& j, int& k) {
-  const long long plane = (long long)g.N[0] * g.N[1];
-  if (t < plane) { k = 0; i = (int)(t % g.N[0]); j = (int)(t / g.N[0]); return true; }
-  t -= plane;
-  if (t < plane) { k = g.N[2] - 1; i = (int)(t % g.N[0]); j = (int)(t / g.N[0]); return true; }
-  t -= plane;
-  const long long per_k = 2LL * g.N[0] + 2LL * g.n[1];
-  if (t >= per_k * g.n[2]) return false;
-  k = 1 + (int)(t / per_k);
-  int r = (int)(t % per_k);
-  if (r < g.N[0]) { j = 0; i = r; return true; }
-  r -= g.N[0];
-  if (r < g.N[0]) { j = g.N[1] - 1; i = r; return true; }
-  r -= g.N[0];
-  j = 1 + r / 2;
-  i = (r & 1) ? g.N[0] - 1 : 0;
-  return true;
-}
-template <typename T>
-__device__ __forceinline__ long long ghost_count(const GridDev<T>& g) {
-  return 2LL * g.N[0] * g.N[1] + (2LL * g.N[0] + 2LL * g.n[1]) * g.n[2];
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs; this bijection gives
-// each XCD a contiguous range of work items so that neighbouring items share
-// that XCD's L2.
-__device__ __forceinline__ int xcd_brick(int w, int nb) {
-  const int xcd = w & 7, idx = w >> 3, q = nb >> 3, r = nb & 7;
-  return xcd * q + min(xcd, r) + idx;
-}
-__device__ __forceinline__ int xcd_thread_index() {
-  return xcd_brick(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-}
-
-// ---------------------------------------------------------------------------
-// Interpolation (interp_s2u_r1, lp_tracking_m.f90:279-662)
-// ---------------------------------------------------------------------------
-template <typename T>
-struct LinW {
-  int base;
-  T t[3], mt[3];
-};
-
-// LINEAR bracket relative to the tracked node, local coordinate clamped to
-// [0,1] (lp_tracking_m.f90:381-396,438-448).  A bracket that would leave the
-// ghosted range is shifted inside.
-template <typename T>
-__device__ __forceinline__ void lin_setup(const GridDev<T>& g, const T q[3], const int nd[3], LinW<T>& w) {
-  int i0[3];
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const T* xl = g.xl[d];
-    int n0 = iclamp(nd[d], 0, g.n[d] + 1);
-    int a = (q[d] > xl[n0]) ? n0 : n0 - 1;
-    a = iclamp(a, 0, g.n[d]);
-    T x0 = xl[a], x1 = xl[a + 1];
-    T tt = clamp01((q[d] - x0) / (x1 - x0));
-    w.t[d] = tt;
-    w.mt[d] = T(1) - tt;
-    i0[d] = a;
-  }
-  w.base = i0[0] + g.N[0] * (i0[1] + g.N[1] * i0[2]);
-}
-
-// trilinear sum in the reference's term order (lp_tracking_m.f90:449-457)
-template <typename T>
-__device__ __forceinline__ T trilin(const LinW<T>& w, T f0, T f1, T f2, T f3, T f4, T f5, T f6, T f7) {
-  return w.mt[0] * w.mt[1] * w.mt[2] * f0 + w.t[0] * w.mt[1] * w.mt[2] * f1 +
-         w.mt[0] * w.t[1] * w.mt[2] * f2 + w.t[0] * w.t[1] * w.mt[2] * f3 +
-         w.mt[0] * w.mt[1] * w.t[2] * f4 + w.t[0] * w.mt[1] * w.t[2] * f5 +
-         w.mt[0] * w.t[1] * w.t[2] * f6 + w.t[0] * w.t[1] * w.t[2] * f7;
-}
-
-// B-spline basis of order K at x (bspline_sub_module.f90: dintrv 2558-2634,
-// dbspvn 2288-2370).  The interval index is the largest L in [K, n] with
-// t(L) <= x (dintrv; dbvalu's left limit at x = t(n+1)), found by probing from
-// a hint next to the tracked node; the K nonzero basis values come from the
-// interval's polynomial table (exact piecewise form of the Cox-de Boor
-// recurrence, evaluated by Horner in the normalised interval coordinate).
-// Returns 0 = ok, 1 = outside the knot range (db3val returns 0 and reports),
-// 2 = y/z at the last knot (db3val returns 0 silently,
-// bspline_sub_module.f90:634-636).
-template <typename T, int K>
-__device__ __forceinline__ int bs_setup(const T* __restrict__ t, const T* __restrict__ poly, int n, T x, bool xaxis,
-                                        int hint, int& base, T v[K]) {
-  const int L = n + K;
-  if (!(x >= t[0]) || x > t[L - 1]) return 1;
-  int left;
-  if (x >= t[L - 1]) {
-    if (!xaxis) return 2;
-    left = n;
-  } else {
-    left = iclamp(hint, K, n);
-    while (left < n && x >= t[left]) left++;
-    while (left > K && x < t[left - 1]) left--;
-  }
-  const T* P = poly + left * (K * K + 1);
-  const T s = (x - t[left - 1]) * P[K * K];
-#pragma unroll
-  for (int a = 0; a < K; a++) {
-    T w = P[a * K + K - 1];
-#pragma unroll
-    for (int m = K - 2; m >= 0; m--) w = w * s + P[a * K + m];
-    v[a] = w;
-  }
-  base = left - K;
-  return 0;
-}
-
-// ---- stored element -> compute type ----
-template <typename T>
-__device__ __forceinline__ T widen(T x) { return x; }
-__device__ __forceinline__ float widen(half_t h) { return __half2float(__ushort_as_half(h.u)); }
-__device__ __forceinline__ float widen(bf16_t h) { return __uint_as_float((unsigned)h.u << 16); }
-// compute type -> stored element, round to nearest even
-template <typename S, typename T>
-__device__ __forceinline__ S narrow(T x) { return S(x); }
-template <>
-__device__ __forceinline__ half_t narrow<half_t, float>(float x) { return half_t{__half_as_ushort(__float2half_rn(x))}; }
-template <>
-__device__ __forceinline__ bf16_t narrow<bf16_t, float>(float x) {
-  unsigned u = __float_as_uint(x);
-  if ((u & 0x7f800000u) == 0x7f800000u) return bf16_t{(uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u))};
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return bf16_t{(uint16_t)(u >> 16)};
-}
-// K consecutive stored elements (one x row of a spline stencil); 16-bit rows
-// are fetched as one unaligned 2K-byte load
-template <typename T, int K, typename S>
-__device__ __forceinline__ void load_row(const S* __restrict__ row, T v[K]) {
-  if constexpr (sizeof(S) >= 4) {
-#pragma unroll
-    for (int a = 0; a < K; a++) v[a] = T(row[a]);
-  } else {
-    struct __attribute__((packed, aligned(2))) R { S e[K]; } r;
-    __builtin_memcpy(&r, row, sizeof(R));
-#pragma unroll
-    for (int a = 0; a < K; a++) v[a] = T(widen(r.e[a]));
-  }
-}
-
-template <typename T, int KX, int KZ>
-struct SplineW {
-  int base;       // flattened ghosted index of the first coefficient
-  int b3[3];      // first coefficient index per axis
-  T wx[KX], wy[KX], wz[KZ];
-};
-
-// nd: node next to q (tracked node), used as the interval hint
-template <typename T, int KX, int KZ>
-__device__ __forceinline__ int spline_setup(const GridDev<T>& g, const T q[3], const int nd[3],
-                                            SplineW<T, KX, KZ>& s) {
-  int bx, by, bz;
-  int rx = bs_setup<T, KX>(g.kn[0], g.bpoly[0], g.N[0], q[0], true, nd[0] + (KX + 2) / 2, bx, s.wx);
-  int ry = bs_setup<T, KX>(g.kn[1], g.bpoly[1], g.N[1], q[1], false, nd[1] + (KX + 2) / 2, by, s.wy);
-  int rz = bs_setup<T, KZ>(g.kn[2], g.bpoly[2], g.N[2], q[2], false, nd[2] + (KZ + 2) / 2, bz, s.wz);
-  if (rx == 1 || ry == 1 || rz == 1) return 1;
-  if (ry || rz) return 2;
-  s.base = bx + g.N[0] * (by + g.N[1] * bz);
-  s.b3[0] = bx; s.b3[1] = by; s.b3[2] = bz;
-  return 0;
-}
-
-template <typename T, int KX, int KZ, typename S>
-__device__ __forceinline__ T spline_sum(const GridDev<T>& g, const SplineW<T, KX, KZ>& s, const S* __restrict__ c) {
-  const int sy = g.N[0], sz = g.N[0] * g.N[1];
-  T acc = T(0);
-#pragma unroll
-  for (int cz = 0; cz < KZ; cz++) {
-    T accy = T(0);
-#pragma unroll
-    for (int by = 0; by < KX; by++) {
-      T row[KX];
-      load_row<T, KX>(c + s.base + by * sy + cz * sz, row);
-      T accx = T(0);
-#pragma unroll
-      for (int a = 0; a < KX; a++) accx += s.wx[a] * row[a];
-      accy += s.wy[by] * accx;
-    }
-    acc += s.wz[cz] * accy;
-  }
-  return acc;
-}
-
-// ---- interpolants of a vector field f at q, bracket/hint from node nd.  For
-// the velocity, f is the stage's time-blended field (values for NEAREST and
-// LINEAR, spline coefficients for QUADRATIC/CUBIC); for the flowmap, the
-// (fitted) flowmap. ----
-template <typename T>
-struct IntNearest {
-  template <typename S>
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
-                             unsigned long long&) const {
-    int l = gid(g, iclamp(nd[0], 0, g.n[0] + 1), iclamp(nd[1], 0, g.n[1] + 1), iclamp(nd[2], 0, g.n[2] + 1));
-#pragma unroll
-    for (int c = 0; c < 3; c++) out[c] = T(widen(f.c[c][l]));
-  }
-};
-
-template <typename T>
-struct IntLinear {
-  template <typename S>
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
-                             unsigned long long&) const {
-    LinW<T> w;
-    lin_setup(g, q, nd, w);
-    const int b = w.base, sy = g.N[0], sz = g.N[0] * g.N[1];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const S* A = f.c[c];
-      out[c] = trilin(w, T(widen(A[b])), T(widen(A[b + 1])), T(widen(A[b + sy])), T(widen(A[b + sy + 1])),
-                      T(widen(A[b + sz])), T(widen(A[b + sz + 1])), T(widen(A[b + sz + sy])),
-                      T(widen(A[b + sz + sy + 1])));
-    }
-  }
-};
-
-template <typename T, int KX, int KZ>
-struct IntSpline {
-  template <typename S>
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
-                             unsigned long long& oob) const {
-    SplineW<T, KX, KZ> s;
-    int r = spline_setup(g, q, nd, s);
-    if (r) {
-      if (r == 1) oob++;
-      out[0] = out[1] = out[2] = T(0);
-      return;
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) out[c] = spline_sum(g, s, f.c[c]);
-  }
-};
-
-// ---------------------------------------------------------------------------
-// Particle operations
-// ---------------------------------------------------------------------------
-
-// integrate_lp stages (lp_motion_m.f90:129-328); RK3 evaluates k3 at
-// xn + 2 k2 as the reference does (lp_motion_m.f90:239-247).  The first
-// stage of every scheme samples the velocity at the start position xn, which
-// first(up) supplies (an interpolation, or the nodal value when xn is a node).
-template <typename S, typename T>
-__device__ __forceinline__ CF3<S> stage(const StageParams<T>& sp, int i) {
-  return CF3<S>{{(const S*)sp.v[i][0], (const S*)sp.v[i][1], (const S*)sp.v[i][2]}};
-}
-
-template <typename S, typename T, class V, class V0>
-__device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams<T>& sp, const V& vel,
-                                          const V0& first, const int nd[3], const T xn[3], T xp[3],
-                                          unsigned long long& oob) {
-  const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
-  const T dt = sp.dt;
-  T up[3], k[3], x1[3];
-  switch (sp.integ) {
-    case kEuler:
-    case kTrapezoidal:
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = xp[d] + dt * up[d];
-      break;
-    case kRK2:
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = xp[d] + HALF * dt * up[d];
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = xn[d] + dt * up[d];
-      break;
-    case kRK3:
-#pragma unroll
-      for (int d = 0; d < 3; d++) x1[d] = xn[d];
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        x1[d] = x1[d] + SIXTH * k[d];
-        xp[d] = xn[d] + HALF * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        xp[d] = xn[d] + TWO * k[d];
-        x1[d] = x1[d] + TWOTHIRD * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        xp[d] = x1[d] + SIXTH * k[d];
-      }
-      break;
-    case kRK4:
-    default:
-#pragma unroll
-      for (int d = 0; d < 3; d++) x1[d] = xn[d];
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        x1[d] = x1[d] + SIXTH * k[d];
-        xp[d] = xn[d] + HALF * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        x1[d] = x1[d] + THIRD * k[d];
-        xp[d] = xn[d] + HALF * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        x1[d] = x1[d] + THIRD * k[d];
-        xp[d] = xn[d] + k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 3), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        xp[d] = x1[d] + SIXTH * k[d];
-      }
-      break;
-  }
-}
-
-// project_lp2plane on a flat separable grid (lp_m.f90:418-538): the plane
-// normal is the collapsed axis, the node index on it becomes 1 and the
-// particle's coordinate loses its offset from the node plane.
-
-// The same stages in increment form for compensated updates: inc = x_new - xn
-// summed from the stage terms alone, so that no term is rounded against |xn|
-// (same stage positions and weights as integrate()).
-template <typename S, typename T, class V, class V0>
-__device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StageParams<T>& sp, const V& vel,
-                                              const V0& first, const int nd[3], const T xn[3], T inc[3],
-                                              unsigned long long& oob) {
-  const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
-  const T dt = sp.dt;
-  T up[3], k[3], xp[3];
-  switch (sp.integ) {
-    case kEuler:
-    case kTrapezoidal:
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) inc[d] = dt * up[d];
-      break;
-    case kRK2:
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = xn[d] + HALF * dt * up[d];
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) inc[d] = dt * up[d];
-      break;
-    case kRK3:
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        inc[d] = SIXTH * k[d];
-        xp[d] = xn[d] + HALF * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        xp[d] = xn[d] + TWO * k[d];
-        inc[d] = inc[d] + TWOTHIRD * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) inc[d] = inc[d] + SIXTH * (dt * up[d]);
-      break;
-    case kRK4:
-    default:
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        inc[d] = SIXTH * k[d];
-        xp[d] = xn[d] + HALF * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        inc[d] = inc[d] + THIRD * k[d];
-        xp[d] = xn[d] + HALF * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        inc[d] = inc[d] + THIRD * k[d];
-        xp[d] = xn[d] + k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 3), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) inc[d] = inc[d] + SIXTH * (dt * up[d]);
-      break;
-  }
-}
-
-// error-free transformation a + b = s + e (Knuth two-sum)
-template <typename T>
-__device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
-#pragma clang fp contract(off)
-  s = a + b;
-  const T bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-}
-
-template <typename T>
-__device__ __forceinline__ void project_flat(const GridDev<T>& g, int nd[3], T q[3]) {
-  if (g.flat < 0) return;
-  const int f = g.flat;
-  nd[f] = 1;
-  q[f] = q[f] - (q[f] - g.gl[f][1]);
-}
-
-// set_lp_bc (lp_tracking_m.f90:185-277); returns true for LP_STICK
-template <typename T>
-__device__ __forceinline__ bool set_lp_bc(const GridDev<T>& g, int nd[3], T q[3], int& err) {
-  const int i = iclamp(nd[0], 1, g.n[0]), j = iclamp(nd[1], 1, g.n[1]), k = iclamp(nd[2], 1, g.n[2]);
-  nd[0] = i; nd[1] = j; nd[2] = k;
-  const int f = g.flag[gid(g, i, j, k)];
-  const bool inner = (i > 1 && j > 1 && k > 1 && i < g.n[0] && j < g.n[1] && k < g.n[2]);
-  switch (f) {
-    case kInternal:
-    case k2D:
-      return false;
-    case kSlip:
-    case kWall:
-      if (inner) return false;
-      // fall through
-    case kMask: {
-      const int p = pid(g, i, j, k);
-      T nrm[3] = {g.bn[0][p], g.bn[1][p], g.bn[2][p]};
-      T p2no[3] = {q[0] - g.gl[0][i], q[1] - g.gl[1][j], q[2] - g.gl[2][k]};
-      T dist = p2no[0] * nrm[0] + p2no[1] * nrm[1] + p2no[2] * nrm[2];
-      if (dist > T(0)) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) q[d] = q[d] - nrm[d] * dist;
-      }
-      return false;
-    }
-    case kInflow:
-    case kOutflow:
-      if (inner) return false;
-      q[0] = g.gl[0][i]; q[1] = g.gl[1][j]; q[2] = g.gl[2][k];
-      return true;
-    default:
-      err |= kErrBcFlag;
-      return false;
-  }
-}
-
-// One pass of the 27-neighbour nearest-node search with the 0.99
-// hysteresis (lp_tracking_m.f90:72-109), in the reference's neighbour order.
-// The three candidate coordinates per axis are loaded once; contraction is
-// off so the squared distances round exactly as the CPU does.
-template <typename T>
-__device__ __forceinline__ bool track_pass(const GridDev<T>& g, const T q[3], int nd[3]) {
-#pragma clang fp contract(off)
-  T sq[3][3];
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-#pragma unroll
-    for (int o = 0; o < 3; o++) {
-      const T c = g.gl[d][iclamp(nd[d] + o - 1, 0, g.n[d] + 1)] - q[d];
-      sq[d][o] = c * c;
-    }
-  }
-  const bool fx = g.n[0] == 1, fy = g.n[1] == 1, fz = g.n[2] == 1;
-  const T huge = sizeof(T) == 8 ? T(DBL_MAX) : T(FLT_MAX);
-  T rmin = huge;
-  int oi = 0, oj = 0, ok = 0;
-  constexpr int8_t nb[27][3] = {
-      {0, 0, 0},   {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},  {1, 0, 0},   {0, 1, 0},  {0, 0, 1},
-      {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {-1, 0, -1}, {1, 0, -1}, {-1, 1, -1}, {0, 1, -1},
-      {1, 1, -1},  {-1, -1, 0}, {1, -1, 0},  {-1, 1, 0},  {1, 1, 0},   {-1, -1, 1}, {0, -1, 1},
-      {1, -1, 1},  {-1, 0, 1},  {1, 0, 1},   {-1, 1, 1},  {0, 1, 1},   {1, 1, 1}};
-#pragma unroll
-  for (int b = 0; b < 27; b++) {
-    const int di = nb[b][0], dj = nb[b][1], dk = nb[b][2];
-    if ((fx && di) || (fy && dj) || (fz && dk)) continue;
-    const T rsq = (sq[0][di + 1] + sq[1][dj + 1]) + sq[2][dk + 1];
-    if (rsq < T(0.99) * rmin) {
-      rmin = rsq;
-      oi = di; oj = dj; ok = dk;
-    }
-  }
-  nd[0] += oi; nd[1] += oj; nd[2] += ok;
-  return oi != 0 || oj != 0 || ok != 0;
-}
-
-// exchange_lpdata on one rank (comms_m.f90:872-1189): periodic wrap of a
-// particle whose tracked node left [1,n].
-template <typename T>
-__device__ __forceinline__ void wrap(const GridDev<T>& g, int nd[3], T q[3], T sh[3] = nullptr) {
-  int c[3];
-#pragma unroll
-  for (int d = 0; d < 3; d++) c[d] = nd[d] < 1 ? -1 : (nd[d] > g.n[d] ? 1 : 0);
-  if (sh) sh[0] = sh[1] = sh[2] = T(0);
-  if (!self_comm(g, c[0], c[1], c[2])) return;
-  const int di = (c[0] + 1) + 3 * (c[1] + 1) + 9 * (c[2] + 1);
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    if (sh) sh[d] = g.ps[di][d];
-    q[d] = q[d] + g.ps[di][d];
-    int v = nd[d];
-    if (v > g.n[d]) v -= g.n[d];
-    if (v < 1) v += g.n[d];
-    nd[d] = v;
-  }
-}
-
-__device__ __forceinline__ void flush_status(DevStatus* st, int err, unsigned long long oob) {
-  if (err) atomicOr(&st->err, err);
-  if (oob) atomicAdd(&st->oob, oob);
-}
 
 // ---------------------------------------------------------------------------
 // Velocity ingest, ghost shell, CFL (cfd2lcs.f90:112-117, sgrid_m.f90:810-866,
@@ -969,43 +397,6 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __res
 // (lcs_m.f90:168-185).  Gradient, tensor and eigen-solve run in FP64 for
 // both storage precisions.
 // ---------------------------------------------------------------------------
-__device__ double eig_sym3_max(double a00, double a01, double a02, double a11, double a12, double a22) {
-  double a[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
-  for (int sweep = 0; sweep < 60; sweep++) {
-    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
-    const double dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
-    if (off == 0.0 || off <= 1e-36 * dia) break;
-#pragma unroll
-    for (int pq = 0; pq < 3; pq++) {
-      const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
-      const double apq = a[p][q];
-      if (apq == 0.0) continue;
-      const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
-      double t;
-      if (fabs(theta) > 1e150)
-        t = 0.5 / theta;
-      else
-        t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-#pragma unroll
-      for (int r = 0; r < 3; r++) {
-        const double arp = a[r][p], arq = a[r][q];
-        a[r][p] = c * arp - s * arq;
-        a[r][q] = s * arp + c * arq;
-      }
-#pragma unroll
-      for (int r = 0; r < 3; r++) {
-        const double apr = a[p][r], aqr = a[q][r];
-        a[p][r] = c * apr - s * aqr;
-        a[q][r] = s * apr + c * aqr;
-      }
-      a[p][q] = 0.0;
-      a[q][p] = 0.0;
-    }
-  }
-  return fmax(a[0][0], fmax(a[1][1], a[2][2]));
-}
-
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_ftle(GridDev<T> g, CF3<T> fm, double Tint, int incomp, double tiny,
                                                  double* __restrict__ ftle, F3<T> comp) {

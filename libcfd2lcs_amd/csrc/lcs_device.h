@@ -144,4 +144,29 @@ template <typename T, typename U>
 void launch_interior_to_user(const GridDev<T>& g, CF3<T> src, int ghosted, U* a, U* b, U* c,
                              hipStream_t stream);
 
+// ---- adjoint launchers (adjoint.hip) ----
+// one semi-Lagrangian sub-cycle in reverse; stage s >= 2 scatters into
+// adj_v[sbuf[s]]; the first stage's nodal adjoint goes to dun / dunp1
+template <typename T>
+void launch_sl_adjoint(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
+                       F3<T> adj_fm, F3<T> const adj_v[3], const int sbuf[4], F3<T> dun, F3<T> dunp1,
+                       hipStream_t stream);
+// an += (1-ct) b, anp1 += ct b, b = 0
+template <typename T>
+void launch_adj_unblend(const GridDev<T>& g, F3<T> b, T ct, F3<T> an, F3<T> anp1, hipStream_t stream);
+// transpose of launch_spline_fit (three components, in place)
+template <typename T>
+void launch_spline_fit_T(const GridDev<T>& g, F3<T> a, hipStream_t stream);
+// ghost adjoints back to their interior sources; mode 0 velocity rules, 1
+// flowmap rules; interior nodes forced to zero drop their adjoint first
+template <typename T>
+void launch_fold_ghost(const GridDev<T>& g, F3<T> adj, int mode, hipStream_t stream);
+template <typename T>
+void launch_ftle_adjoint(const GridDev<T>& g, CF3<T> fm, double Tint, int incomp, double tiny, const double* dftle,
+                         F3<T> adj, hipStream_t stream);
+template <typename T, typename U>
+void launch_adj_to_user(const GridDev<T>& g, CF3<T> adj, U* a, U* b, U* c, int accumulate, hipStream_t stream);
+template <typename T>
+void launch_interior(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStream_t stream);
+
 }  // namespace lcs

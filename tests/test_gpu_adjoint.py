@@ -1,0 +1,210 @@
+"""GPU tests of the adjoint C-ABI pair (cfd2lcs_adjoint_forward_c /
+cfd2lcs_adjoint_backward_c), which has no counterpart in the reference:
+
+* its forward FTLE equals the library's own BKWD diagnostic with h = T on the
+  same snapshots (bit-identical for linear interpolation, where composing a
+  single slot is exact);
+* the gradient of a scalar loss matches FP64 central finite differences in
+  random directions over all snapshots (smooth cubic B-spline path to 1e-6
+  relative, linear path to 1e-4, whose bracket switches are kinks);
+* checkpointing every k steps changes nothing but memory;
+* the torch.autograd binding carries the gradient to flow parameters
+  (double-gyre A, epsilon, omega), checked against finite differences of the
+  parameters.
+"""
+import numpy as np
+import pytest
+
+import flows as F
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dp():
+    import libcfd2lcs_amd as L
+    lib = L.Cfd2lcs("dp")
+    yield lib
+    lib.finalize()
+
+
+def gyre_snapshots(X, Y, times, A=0.1, eps=0.1, om=2 * np.pi / 10):
+    """(nsnap, n2, n1, n0) arrays of the double gyre on the grid (C order of
+    (k, j, i) = Fortran order of the library's arrays)."""
+    us, vs = [], []
+    for t in times:
+        u, v, _ = F.double_gyre_velocity(X, Y, t, A=A, eps=eps, omega=om)
+        us.append(np.transpose(u, (2, 1, 0)))
+        vs.append(np.transpose(v, (2, 1, 0)))
+    us, vs = np.stack(us), np.stack(vs)
+    return us, vs, np.zeros_like(us)
+
+
+def abc_snapshots(X, Y, Z, times, amp=0.3):
+    """ABC flow with a time-periodic modulation, so that every snapshot
+    differs; (nsnap, n2, n1, n0)."""
+    u0, v0, w0 = F.abc_velocity(X, Y, Z)
+    us, vs, ws = [], [], []
+    for t in times:
+        m = 1.0 + amp * np.sin(2.0 * t)
+        us.append(np.transpose(m * u0, (2, 1, 0)))
+        vs.append(np.transpose(v0 + amp * np.cos(t) * np.sin(Z), (2, 1, 0)))
+        ws.append(np.transpose(w0, (2, 1, 0)))
+    return np.stack(us), np.stack(vs), np.stack(ws)
+
+
+def setup(lib, case, interp, integ, n):
+    lib.finalize()
+    if case == "gyre":
+        X, Y, Z, fl = F.double_gyre_grid(*n)
+    else:
+        X, Y, Z, fl = F.abc_grid(n)
+    lib.init(X, Y, Z, fl)
+    lib.set_option("INTEGRATOR", integ)
+    lib.set_option("INTERPOLATOR", interp)
+    return X, Y, Z, fl
+
+
+def dev(a):
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")
+
+
+def fwd(lib, us, vs, ws, times, ckpt=1):
+    ftle = torch.empty(us.shape[1:], dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    assert lib.adjoint_forward(us.data_ptr(), vs.data_ptr(), ws.data_ptr(), us.shape[0], times, ckpt,
+                               ftle.data_ptr()) == 0
+    assert lib.error == 0
+    return ftle
+
+
+def bwd(lib, wts, us):
+    gu, gv, gw = torch.empty_like(us), torch.empty_like(us), torch.empty_like(us)
+    torch.cuda.synchronize()
+    assert lib.adjoint_backward(wts.data_ptr(), gu.data_ptr(), gv.data_ptr(), gw.data_ptr()) == 0
+    assert lib.error == 0
+    return gu, gv, gw
+
+
+CASES = [
+    ("gyre", F.LINEAR, F.RK2, (32, 16)),
+    ("gyre", F.CUBIC, F.RK4, (24, 12)),
+    ("abc", F.LINEAR, F.RK3, 10),
+    ("abc", F.CUBIC, F.RK4, 10),
+]
+
+
+@pytest.mark.parametrize("case,interp,integ,n", CASES)
+def test_adjoint_forward_matches_bkwd_diagnostic(dp, case, interp, integ, n):
+    import libcfd2lcs_amd as L
+    X, Y, Z, fl = setup(dp, case, interp, integ, n)
+    times = [0.05 * s for s in range(9)]
+    snaps = gyre_snapshots(X, Y, times) if case == "gyre" else abc_snapshots(X, Y, Z, times)
+    us, vs, ws = (dev(a) for a in snaps)
+    f_adj = fwd(dp, us, vs, ws, times).cpu().numpy()
+    T = times[-1] - times[0]
+    setup(dp, case, interp, integ, n)
+    hd = dp.diagnostic_init(L.FTLE_BKWD, 0, T, T, "b")
+    for s, t in enumerate(times):
+        u, v, w = (np.transpose(a[s], (2, 1, 0)) for a in snaps)
+        dp.update(u, v, w, t)
+    assert dp.error == 0 and dp.noutput(hd) == 1
+    f_diag = np.transpose(dp.ftle(hd), (2, 1, 0))
+    if interp == F.LINEAR:
+        assert np.array_equal(f_adj, f_diag)
+    else:
+        assert np.abs(f_adj - f_diag).max() <= 1e-9
+
+
+def _loss_and_grad(lib, us, vs, ws, times, wts, ckpt=1):
+    f = fwd(lib, us, vs, ws, times, ckpt)
+    L = float((wts * f).sum())
+    return L, bwd(lib, wts, us)
+
+
+@pytest.mark.parametrize("case,interp,integ,n", CASES)
+def test_adjoint_gradient_matches_central_differences(dp, case, interp, integ, n):
+    X, Y, Z, fl = setup(dp, case, interp, integ, n)
+    times = [0.05 * s for s in range(7)]
+    snaps = gyre_snapshots(X, Y, times) if case == "gyre" else abc_snapshots(X, Y, Z, times)
+    us, vs, ws = (dev(a) for a in snaps)
+    rng = np.random.default_rng(7)
+    wts = dev(rng.standard_normal(us.shape[1:]))
+    L0, (gu, gv, gw) = _loss_and_grad(dp, us, vs, ws, times, wts)
+    nsub0 = dp.adjoint_stats()["subcycles"]
+    smooth = interp == F.CUBIC
+    h = 1e-6 if smooth else 1e-7
+    tol = 1e-6 if smooth else 1e-4
+    errs = []
+    for trial in range(3):
+        d = [dev(rng.standard_normal(us.shape)) for _ in range(3)]
+        if case == "gyre":
+            d[2].zero_()
+        ad = float(sum((g * e).sum() for g, e in zip((gu, gv, gw), d)))
+        Lp = float((wts * fwd(dp, us + h * d[0], vs + h * d[1], ws + h * d[2], times)).sum())
+        assert dp.adjoint_stats()["subcycles"] == nsub0
+        Lm = float((wts * fwd(dp, us - h * d[0], vs - h * d[1], ws - h * d[2], times)).sum())
+        fd = (Lp - Lm) / (2 * h)
+        errs.append(abs(ad - fd) / max(abs(fd), 1e-30))
+        print(f"{case} interp {interp} integ {integ}: adjoint {ad:.10e} fd {fd:.10e} rel {errs[-1]:.2e}")
+    assert max(errs) <= tol
+
+
+def test_checkpointing_changes_nothing(dp):
+    X, Y, Z, fl = setup(dp, "abc", F.CUBIC, F.RK4, 10)
+    times = [0.05 * s for s in range(8)]
+    us, vs, ws = (dev(a) for a in abc_snapshots(X, Y, Z, times))
+    wts = dev(np.random.default_rng(3).standard_normal(us.shape[1:]))
+    ref = None
+    for ck in (1, 3, 7):
+        L, g = _loss_and_grad(dp, us, vs, ws, times, wts, ck)
+        g = torch.stack(g).cpu().numpy()
+        if ref is None:
+            ref = (L, g)
+        else:
+            assert L == ref[0]
+            assert np.abs(g - ref[1]).max() <= 1e-12 * np.abs(ref[1]).max()
+
+
+def test_autograd_flow_parameters_double_gyre(dp):
+    """dL/d(A, eps, omega) through torch autograd (snapshots built by torch
+    from the parameters) against central differences of the parameters."""
+    from libcfd2lcs_amd.autograd import BkwdFTLE
+    X, Y, Z, fl = setup(dp, "gyre", F.LINEAR, F.RK2, (32, 16))
+    times = [0.1 * s for s in range(6)]
+    xs = dev(np.transpose(X, (2, 1, 0)))
+    ys = dev(np.transpose(Y, (2, 1, 0)))
+
+    def snaps(p):
+        A, eps, om = p
+        us, vs = [], []
+        for t in times:
+            a = eps * torch.sin(om * t)
+            b = 1 - 2 * eps * torch.sin(om * t)
+            f = a * xs ** 2 + b * xs
+            dfdx = 2 * a * xs + b
+            us.append(-np.pi * A * torch.sin(np.pi * f) * torch.cos(np.pi * ys))
+            vs.append(np.pi * A * torch.cos(np.pi * f) * torch.sin(np.pi * ys) * dfdx)
+        u, v = torch.stack(us), torch.stack(vs)
+        return u, v, torch.zeros_like(u)
+
+    target = BkwdFTLE.apply(dp, *snaps(torch.tensor([0.12, 0.15, 2 * np.pi / 10], dtype=torch.float64,
+                                                    device="cuda:0")), times, 2).detach()
+
+    def loss(p):
+        return ((BkwdFTLE.apply(dp, *snaps(p), times, 2) - target) ** 2).sum()
+
+    p = torch.tensor([0.1, 0.1, 2 * np.pi / 10], dtype=torch.float64, device="cuda:0", requires_grad=True)
+    L = loss(p)
+    L.backward()
+    g = p.grad.cpu().numpy()
+    with torch.no_grad():
+        for a in range(3):
+            h = 1e-7
+            e = torch.zeros(3, dtype=torch.float64, device="cuda:0")
+            e[a] = h
+            fd = (float(loss(p + e)) - float(loss(p - e))) / (2 * h)
+            print(f"param {a}: autograd {g[a]:.8e} fd {fd:.8e}")
+            assert abs(g[a] - fd) <= 1e-4 * max(abs(fd), 1e-12) + 1e-10
