@@ -140,6 +140,107 @@ def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None):
     return res
 
 
+def abc_perturbed_snapshots(n, nsnap, sigma=0.05, width=4.0, seed=0, dtype=None):
+    """ABC (A = sqrt 3, B = sqrt 2, C = 1) on the periodic cell-centred n^3
+    grid plus, per snapshot and component, a seeded Gaussian random field
+    smoothed with a Gaussian of `width` cells and scaled to standard
+    deviation `sigma`; (nsnap, n, n, n) device tensors in C order (k, j, i)."""
+    import torch
+    dev = torch.device("cuda:0")
+    h = 2 * np.pi / n
+    ax = torch.tensor(0.5 * h + np.arange(n) * h, dtype=torch.float64, device=dev)
+    zz, yy, xx = torch.meshgrid(ax, ax, ax, indexing="ij")
+    A, B, Cc = np.sqrt(3.0), np.sqrt(2.0), 1.0
+    base = (A * torch.sin(zz) + Cc * torch.cos(yy), B * torch.sin(xx) + A * torch.cos(zz),
+            Cc * torch.sin(yy) + B * torch.cos(xx))
+    del xx, yy, zz
+    k = torch.fft.fftfreq(n, d=1.0, device=dev) * 2 * np.pi
+    kz, ky, kx = torch.meshgrid(k, k, torch.fft.rfftfreq(n, d=1.0, device=dev) * 2 * np.pi, indexing="ij")
+    filt = torch.exp(-0.5 * (kx ** 2 + ky ** 2 + kz ** 2) * width ** 2)
+    del kx, ky, kz
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    out = [torch.empty((nsnap, n, n, n), dtype=dtype, device=dev) for _ in range(3)]
+    for s in range(nsnap):
+        for c in range(3):
+            r = torch.randn((n, n, n), generator=g, dtype=torch.float64, device=dev)
+            r = torch.fft.irfftn(torch.fft.rfftn(r) * filt, s=(n, n, n))
+            r = r * (sigma / r.std())
+            out[c][s] = (base[c] + r).to(dtype)
+    return out
+
+
+def hip_mem_used():
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        return None
+    free, total = C.c_size_t(), C.c_size_t()
+    if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        return None
+    return total.value - free.value
+
+
+def run_adjoint(variant, n, steps, ckpt, dt=0.02):
+    """Adjoint workload: ABC at n^3 with perturbed snapshots, dt = 0.02,
+    `steps` steps (T = steps dt), RK4 + cubic B-spline, backward FTLE of the
+    window; loss = mean FTLE over [0, pi]^3; gradient with respect to every
+    snapshot through the adjoint C-ABI pair with checkpoints every `ckpt`
+    steps.  Returns wall times of the forward and backward calls."""
+    import torch
+
+    import libcfd2lcs_amd as L
+    lib = L.Cfd2lcs(variant)
+    lib.finalize()
+    X, Y, Z = abc_grid(n)
+    flag = np.zeros(X.shape, dtype=np.int32)
+    lib.init(X, Y, Z, flag)
+    lib.set_option("INTEGRATOR", L.RK4)
+    lib.set_option("INTERPOLATOR", L.CUBIC)
+    lib.set_param("CFL", 0.5)
+    dtype = torch.float64 if variant == "dp" else torch.float32
+    nsnap = steps + 1
+    mem0 = hip_mem_used()
+    u, v, w = abc_perturbed_snapshots(n, nsnap, dtype=dtype)
+    times = [dt * s for s in range(nsnap)]
+    dev = torch.device("cuda:0")
+    ftle = torch.empty((n, n, n), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc = lib.adjoint_forward(u.data_ptr(), v.data_ptr(), w.data_ptr(), nsnap, times, ckpt, ftle.data_ptr())
+    t_f = time.perf_counter() - t0
+    if rc != 0 or lib.error:
+        raise RuntimeError("adjoint forward failed")
+    h = 2 * np.pi / n
+    ax = torch.tensor(0.5 * h + np.arange(n) * h, device=dev)
+    m1 = (ax <= np.pi).to(torch.float64)
+    mask = m1[:, None, None] * m1[None, :, None] * m1[None, None, :]
+    dL = (mask / mask.sum()).contiguous()
+    loss = float((ftle * dL).sum())
+    gu, gv, gw = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rc = lib.adjoint_backward(dL.data_ptr(), gu.data_ptr(), gv.data_ptr(), gw.data_ptr())
+    t_b = time.perf_counter() - t0
+    if rc != 0 or lib.error:
+        raise RuntimeError("adjoint backward failed")
+    used = hip_mem_used()
+    st = lib.adjoint_stats()
+    gnorm = float(torch.sqrt((gu.double() ** 2).sum() + (gv.double() ** 2).sum() + (gw.double() ** 2).sum()))
+    lib.adjoint_release()
+    lib.finalize()
+    del u, v, w, gu, gv, gw
+    torch.cuda.empty_cache()
+    return {"forward_s": t_f, "backward_s": t_b, "bwd_fwd_ratio": t_b / t_f, "loss": loss, "grad_norm": gnorm,
+            "subcycles": int(st["subcycles"]), "tape_bytes": int(st["tape_bytes"]),
+            "hbm_used_bytes": (used - mem0) if (used is not None and mem0 is not None) else None,
+            "config": {"grid": [n] * 3, "steps": steps, "dt": dt, "T": steps * dt, "checkpoint_every": ckpt,
+                       "integrator": "RK4", "interpolator": "CUBIC", "precision": "f32" if variant == "sp" else "f64",
+                       "loss": "mean FTLE over [0, pi]^3",
+                       "snapshots": "ABC + seeded Gaussian field (sigma 0.05, smoothed over 4 cells)"}}
+
+
 def timers(lib):
     import ctypes as C
     sec = (C.c_double * 16)()
@@ -218,6 +319,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ftle", action="store_true", help="skip the full-ring FTLE fields/s measurement")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the adjoint workload (ABC 192^3, dt 0.02); 0 skips it")
+    ap.add_argument("--adjoint-n", type=int, default=192)
     ap.add_argument("--mixed", choices=["fp16", "bf16", "none"], default="fp16",
                     help="also run FP32 advection with 16-bit stage velocities and compensated positions")
     ap.add_argument("--calibrate", action="store_true",
@@ -255,6 +359,12 @@ def main():
                        {"VELOCITY_STORAGE": stor, "COMPENSATED": L.LCS_TRUE})
         print(f"[bench] mixed ({args.mixed} velocity, compensated FP32 positions): "
               f"{rmix['updates'] / rmix['elapsed']:.4e} updates/s", file=sys.stderr, flush=True)
+
+    radj = None
+    if args.adjoint_steps > 0:
+        radj = run_adjoint("sp", args.adjoint_n, args.adjoint_steps, 20)
+        print(f"[bench] adjoint: forward {radj['forward_s']:.2f} s, backward {radj['backward_s']:.2f} s",
+              file=sys.stderr, flush=True)
 
     def ftle_err(r):
         """FTLE of a run against the FP64 run at the same output (both
@@ -332,7 +442,8 @@ def main():
             "ftle_fields_per_s": u2(rmix), "ftle_error_vs_fp64": ftle_err(rmix),
             "kernel_ms": {k: round(1e3 * s_ / max(rmix["kernel_cnt"][k], 1), 4)
                           for k, s_ in rmix["kernel_sec"].items()}},
-        "bwd_fwd_ratio": None,
+        "bwd_fwd_ratio": None if radj is None else radj["bwd_fwd_ratio"],
+        "adjoint": radj,
         "kernel_ms": {k: round(1e3 * s / max(r32["kernel_cnt"][k], 1), 4) for k, s in ks.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -76,16 +76,64 @@ __device__ __forceinline__ void lin_setup_d(const GridDev<T>& g, const T q[3], c
   w.base = i0[0] + g.N[0] * (i0[1] + g.N[1] * i0[2]);
 }
 
+// relaxed floating-point adds through address-space-qualified pointers, so
+// that they compile to global_atomic_add / ds_add rather than flat atomics
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+using lptr = __attribute__((address_space(3))) T*;
+template <typename T>
+using vlptr = volatile __attribute__((address_space(3))) T*;
 template <typename T>
 __device__ __forceinline__ void atomic_add(T* p, T v) {
-  atomicAdd(p, v);
+  __hip_atomic_fetch_add((gptr<T>)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// ---- interpolants with value, Jacobian J[c][d] = d f_c / d q_d, and the
-// transposed weights (adj[idx] += w_c * weight) ----
+
+// ---- interpolants with value and Jacobian J[c][d] = d f_c / d q_d ----
+// Each adjoint interpolant also exposes its per-axis support: K_d weights
+// starting at coefficient base b_d, and a window of W_d cells starting at
+// node + B_d that holds every support of a point within about half a cell of
+// its node (the SL stage points under CFL <= 0.5).
+template <typename T>
+struct AdjNearest {
+  static constexpr int KX = 1, KY = 1, KZ = 1, WX = 1, WY = 1, WZ = 1, BX = 0, BY = 0, BZ = 0;
+  __device__ __forceinline__ void eval(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+                                       T J[3][3]) const {
+    const int l = gid(g, iclamp(nd[0], 0, g.n[0] + 1), iclamp(nd[1], 0, g.n[1] + 1), iclamp(nd[2], 0, g.n[2] + 1));
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      out[c] = f.c[c][l];
+      J[c][0] = J[c][1] = J[c][2] = T(0);
+    }
+  }
+  __device__ __forceinline__ int weights(const GridDev<T>& g, const T q[3], const int nd[3], int b[3], T wx[1],
+                                         T wy[1], T wz[1]) const {
+    b[0] = iclamp(nd[0], 0, g.n[0] + 1);
+    b[1] = iclamp(nd[1], 0, g.n[1] + 1);
+    b[2] = iclamp(nd[2], 0, g.n[2] + 1);
+    wx[0] = wy[0] = wz[0] = T(1);
+    return 0;
+  }
+};
+
 template <typename T>
 struct AdjLinear {
-  __device__ void eval(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+  static constexpr int KX = 2, KY = 2, KZ = 2, WX = 3, WY = 3, WZ = 3, BX = -1, BY = -1, BZ = -1;
+  __device__ __forceinline__ int weights(const GridDev<T>& g, const T q[3], const int nd[3], int b[3], T wx[2],
+                                         T wy[2], T wz[2]) const {
+    LinW<T> w;
+    T dt[3];
+    lin_setup_d(g, q, nd, w, dt);
+    b[0] = w.base % g.N[0];
+    b[1] = (w.base / g.N[0]) % g.N[1];
+    b[2] = w.base / (g.N[0] * g.N[1]);
+    wx[0] = w.mt[0]; wx[1] = w.t[0];
+    wy[0] = w.mt[1]; wy[1] = w.t[1];
+    wz[0] = w.mt[2]; wz[1] = w.t[2];
+    return 0;
+  }
+  __device__ __forceinline__ void eval(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
                        T J[3][3]) const {
     LinW<T> w;
     T dt[3];
@@ -109,32 +157,32 @@ struct AdjLinear {
       J[c][2] = az * dt[2];
     }
   }
-  __device__ void scatter(const GridDev<T>& g, const T q[3], const int nd[3], const T wv[3], F3<T> adj) const {
-    LinW<T> w;
-    T dt[3];
-    lin_setup_d(g, q, nd, w, dt);
-    const int b = w.base, sy = g.N[0], sz = g.N[0] * g.N[1];
-    const int idx[8] = {b, b + 1, b + sy, b + sy + 1, b + sz, b + sz + 1, b + sz + sy, b + sz + sy + 1};
-    const T ww[8] = {w.mt[0] * w.mt[1] * w.mt[2], w.t[0] * w.mt[1] * w.mt[2], w.mt[0] * w.t[1] * w.mt[2],
-                     w.t[0] * w.t[1] * w.mt[2],  w.mt[0] * w.mt[1] * w.t[2], w.t[0] * w.mt[1] * w.t[2],
-                     w.mt[0] * w.t[1] * w.t[2],  w.t[0] * w.t[1] * w.t[2]};
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      if (wv[c] == T(0)) continue;
-#pragma unroll
-      for (int e = 0; e < 8; e++)
-        if (ww[e] != T(0)) atomic_add(adj.c[c] + idx[e], wv[c] * ww[e]);
-    }
-  }
 };
 
-template <typename T, int KX, int KZ>
+template <typename T, int KX_, int KZ_>
 struct AdjSpline {
+  static constexpr int KX = KX_, KY = KX_, KZ = KZ_;
+  static constexpr int WX = KX + 1, WY = KX + 1, WZ = KZ + 1;
+  static constexpr int BX = (KX % 2 == 0) ? -KX / 2 : -(KX + 1) / 2, BY = BX;
+  static constexpr int BZ = (KZ % 2 == 0) ? -KZ / 2 : -(KZ + 1) / 2;
   struct W {
     int base;
     T wx[KX], wy[KX], wz[KZ], dx[KX], dy[KX], dz[KZ];
   };
-  __device__ int setup(const GridDev<T>& g, const T q[3], const int nd[3], W& s) const {
+  __device__ __forceinline__ int weights(const GridDev<T>& g, const T q[3], const int nd[3], int b[3], T wx[KX],
+                                         T wy[KY], T wz[KZ]) const {
+    W s;
+    if (setup(g, q, nd, s)) return 1;
+    b[0] = s.base % g.N[0];
+    b[1] = (s.base / g.N[0]) % g.N[1];
+    b[2] = s.base / (g.N[0] * g.N[1]);
+#pragma unroll
+    for (int a = 0; a < KX; a++) { wx[a] = s.wx[a]; wy[a] = s.wy[a]; }
+#pragma unroll
+    for (int a = 0; a < KZ; a++) wz[a] = s.wz[a];
+    return 0;
+  }
+  __device__ __forceinline__ int setup(const GridDev<T>& g, const T q[3], const int nd[3], W& s) const {
     int bx, by, bz;
     const int rx = bs_setup_d<T, KX>(g.kn[0], g.bpoly[0], g.N[0], q[0], true, nd[0] + (KX + 2) / 2, bx, s.wx, s.dx);
     const int ry = bs_setup_d<T, KX>(g.kn[1], g.bpoly[1], g.N[1], q[1], false, nd[1] + (KX + 2) / 2, by, s.wy, s.dy);
@@ -143,7 +191,7 @@ struct AdjSpline {
     s.base = bx + g.N[0] * (by + g.N[1] * bz);
     return 0;
   }
-  __device__ void eval(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+  __device__ __forceinline__ void eval(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
                        T J[3][3]) const {
     W s;
     if (setup(g, q, nd, s)) {
@@ -186,25 +234,6 @@ struct AdjSpline {
       J[c][2] = gz;
     }
   }
-  __device__ void scatter(const GridDev<T>& g, const T q[3], const int nd[3], const T wv[3], F3<T> adj) const {
-    W s;
-    if (setup(g, q, nd, s)) return;
-    const int sy = g.N[0], sz = g.N[0] * g.N[1];
-#pragma unroll
-    for (int cz = 0; cz < KZ; cz++)
-#pragma unroll
-      for (int by = 0; by < KX; by++) {
-        const T wyz = s.wy[by] * s.wz[cz];
-#pragma unroll
-        for (int a = 0; a < KX; a++) {
-          const T w = s.wx[a] * wyz;
-          const int idx = s.base + a + by * sy + cz * sz;
-#pragma unroll
-          for (int c = 0; c < 3; c++)
-            if (wv[c] != T(0) && w != T(0)) atomic_add(adj.c[c] + idx, wv[c] * w);
-        }
-      }
-  }
 };
 
 // dx_new(wrt q) of set_lp_bc: identity, a projection I - n n^T (MASK /
@@ -234,158 +263,302 @@ __device__ __forceinline__ int lp_bc_kind(const GridDev<T>& g, const int nd0[3],
   }
 }
 
-// One backward semi-Lagrangian sub-cycle in reverse.  lam: adjoint of the new
-// flowmap at the interior nodes (ghosted indexing).  Accumulates:
-//   adj_fm      transposed interpolation of the old flowmap (field the
-//               forward kernel read: values or spline coefficients)
-//   adj_v[b]    transposed interpolation of stage velocity buffer b; stage
-//               s >= 2 goes to buffer sbuf[s]
-//   dun, dunp1  first-stage (nodal value) adjoint, split by ct0
-template <typename T, class A>
-__global__ void __launch_bounds__(kBlock) k_sl_adjoint(GridDev<T> g, StageParams<T> sp, CF3<T> fmold, CF3<T> lam,
-                                                       F3<T> adj_fm, F3<T> adj_v0, F3<T> adj_v1, F3<T> adj_v2,
-                                                       int4 sbuf, F3<T> dun, F3<T> dunp1) {
-  const A ad{};
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.np) return;
-  int i, j, k;
-  decode_interior(g, p, i, j, k);
-  const int l = gid(g, i, j, k);
-  const int f = g.flag[l];
-  if (f == kMask || f == kInflow || f == kOutflow) return;
-  T gl3[3] = {lam.c[0][l], lam.c[1][l], lam.c[2][l]};
-  if (gl3[0] == T(0) && gl3[1] == T(0) && gl3[2] == T(0)) return;
-  const int nd[3] = {i, j, k};
-  const T x0[3] = {g.gl[0][i], g.gl[1][j], g.gl[2][k]};
-  const T dt = sp.dt;
-  const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
-  // ---- recompute the stages (same expressions as integrate()) ----
-  T xs[4][3], Js[4][3][3], up[3], kk[3], x1[3], xp[3];
-  CF3<T> vf[4];
+// ---------------------------------------------------------------------------
+// Atomic-free transposed scatter.  One wavefront owns a brick of BRX x BRY x
+// BRZ nodes.  Lane n scatters a point's K_x K_y K_z weights into the W_x W_y
+// W_z window that starts at its own node + (B_x, B_y, B_z), with zeros
+// outside the point's support; for a given window offset the 64 lanes then
+// write 64 distinct LDS cells, so plain read-modify-writes (volatile, in
+// program order) need no atomics.  A point whose support leaves its window
+// adds to an overflow field with global atomics instead.  The brick's tile is
+// flushed to HBM with plain read-modify-writes as well: bricks are processed
+// in 2 x 2 x 2 colours whose tiles never overlap, one launch per colour.
+// ---------------------------------------------------------------------------
+struct AdjLaunch {
+  int nbx, nby, nbz;  // bricks along each axis
+  int cx, cy, cz;     // colour: brick parity along each axis
+  int gx, gy;         // bricks of this colour along x and y
+};
+
+template <typename T, class A, int BRX, int BRY, int BRZ>
+struct WindowScatter {
+  static constexpr int TX = BRX + A::WX - 1, TY = BRY + A::WY - 1, TZ = BRZ + A::WZ - 1;
+  static constexpr int TILE = TX * TY * TZ;
+  vlptr<T> tile;  // [3][TZ][TY][TX]
+  int lx, ly, lz;    // lane's node in the brick
+  F3<T> ovf;
+  int N0, N1;
+  __device__ __forceinline__ void operator()(const A& ad, const GridDev<T>& g, const T q[3], const int nd[3],
+                                             const T wv[3]) const {
+    int b[3];
+    T wx[A::KX], wy[A::KY], wz[A::KZ];
+    if (ad.weights(g, q, nd, b, wx, wy, wz)) return;
+    const int sx = b[0] - (nd[0] + A::BX), sy = b[1] - (nd[1] + A::BY), sz = b[2] - (nd[2] + A::BZ);
+    if (sx < 0 || sx > A::WX - A::KX || sy < 0 || sy > A::WY - A::KY || sz < 0 || sz > A::WZ - A::KZ) {
 #pragma unroll
-  for (int a = 0; a < 4; a++) vf[a] = CF3<T>{{(const T*)sp.v[a][0], (const T*)sp.v[a][1], (const T*)sp.v[a][2]}};
-  // first stage: nodal blended value
-  {
+      for (int c = 0; c < A::KZ; c++)
+#pragma unroll
+        for (int bb = 0; bb < A::KY; bb++)
+#pragma unroll
+          for (int a = 0; a < A::KX; a++) {
+            const T w = wx[a] * wy[bb] * wz[c];
+            const int idx = (b[0] + a) + N0 * ((b[1] + bb) + N1 * (b[2] + c));
+#pragma unroll
+            for (int m = 0; m < 3; m++) atomic_add(ovf.c[m] + idx, wv[m] * w);
+          }
+      return;
+    }
+    T ex[A::WX], ey[A::WY], ez[A::WZ];
+#pragma unroll
+    for (int o = 0; o < A::WX; o++) {
+      T v = T(0);
+#pragma unroll
+      for (int a = 0; a < A::KX; a++) v = (o - a == sx) ? wx[a] : v;
+      ex[o] = v;
+    }
+#pragma unroll
+    for (int o = 0; o < A::WY; o++) {
+      T v = T(0);
+#pragma unroll
+      for (int a = 0; a < A::KY; a++) v = (o - a == sy) ? wy[a] : v;
+      ey[o] = v;
+    }
+#pragma unroll
+    for (int o = 0; o < A::WZ; o++) {
+      T v = T(0);
+#pragma unroll
+      for (int a = 0; a < A::KZ; a++) v = (o - a == sz) ? wz[a] : v;
+      ez[o] = v;
+    }
+#pragma unroll
+    for (int oz = 0; oz < A::WZ; oz++)
+#pragma unroll
+      for (int oy = 0; oy < A::WY; oy++) {
+        const T wyz = ey[oy] * ez[oz];
+        const int row = ((lz + oz) * TY + (ly + oy)) * TX + lx;
+#pragma unroll
+        for (int ox = 0; ox < A::WX; ox++) {
+          const T w = ex[ox] * wyz;
+#pragma unroll
+          for (int m = 0; m < 3; m++) tile[m * TILE + row + ox] += wv[m] * w;
+        }
+      }
+  }
+};
+
+// One backward semi-Lagrangian sub-cycle in reverse for the bricks of one
+// colour.  lam: adjoint of the new flowmap at the interior nodes (ghosted
+// indexing).  Each lane recomputes its node's stages and propagates the
+// adjoint of the departure point back through them; then, phase by phase,
+// the wave scatters the transposed interpolation weights of one destination
+// into its tile and flushes it.  Phase 0: the old flowmap (the field the
+// forward kernel read: values or spline coefficients); phase 1 + b: stage
+// velocity buffer b, which receives stage s >= 2 when sbuf[s] == b.  The
+// first stage's nodal adjoint goes to dun / dunp1, split by ct0.
+template <typename T, class A, int BRX, int BRY, int BRZ>
+__global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, StageParams<T> sp, CF3<T> fmold,
+                                                   CF3<T> lam, F3<T> adj_fm, F3<T> adj_v0, F3<T> adj_v1,
+                                                   F3<T> adj_v2, F3<T> ovf_fm, F3<T> ovf_v0, F3<T> ovf_v1,
+                                                   F3<T> ovf_v2, int4 sbuf, int nbuf, F3<T> dun, F3<T> dunp1) {
+  using WS = WindowScatter<T, A, BRX, BRY, BRZ>;
+  __shared__ T tile_mem[3 * WS::TILE];
+  const A ad{};
+  const int b = blockIdx.x;
+  const int ibx = 2 * (b % al.gx) + al.cx, iby = 2 * ((b / al.gx) % al.gy) + al.cy,
+            ibz = 2 * (b / (al.gx * al.gy)) + al.cz;
+  const int t = threadIdx.x;
+  WS ws_;
+  ws_.tile = (vlptr<T>)tile_mem;
+  ws_.lx = t % BRX;
+  ws_.ly = (t / BRX) % BRY;
+  ws_.lz = t / (BRX * BRY);
+  ws_.N0 = g.N[0];
+  ws_.N1 = g.N[1];
+  const int i = ibx * BRX + ws_.lx + 1, j = iby * BRY + ws_.ly + 1, k = ibz * BRZ + ws_.lz + 1;
+  // tile origin (ghosted index of window cell 0 of the brick's first node)
+  const int ox = ibx * BRX + 1 + A::BX, oy = iby * BRY + 1 + A::BY, oz = ibz * BRZ + 1 + A::BZ;
+
+  bool active = i <= g.n[0] && j <= g.n[1] && k <= g.n[2];
+  int l = 0;
+  T gl3[3] = {0, 0, 0};
+  if (active) {
+    l = gid(g, i, j, k);
+    const int f = g.flag[l];
+    gl3[0] = lam.c[0][l]; gl3[1] = lam.c[1][l]; gl3[2] = lam.c[2][l];
+    if (f == kMask || f == kInflow || f == kOutflow || (gl3[0] == T(0) && gl3[1] == T(0) && gl3[2] == T(0)))
+      active = false;
+  }
+  const int nd[3] = {i, j, k};
+  T xs[4][3], ws[4][3], xb[3];
+  int ns = 1;
+  bool stuck = false;
+  if (active) {
+    const T x0[3] = {g.gl[0][i], g.gl[1][j], g.gl[2][k]};
+    const T dt = sp.dt;
+    const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
+    // ---- recompute the stages (same expressions as integrate()) ----
+    T Js[4][3][3], up[3], kk[3], x1[3], xp[3];
+    CF3<T> vf[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) vf[a] = CF3<T>{{(const T*)sp.v[a][0], (const T*)sp.v[a][1], (const T*)sp.v[a][2]}};
+    {
+      const T w0 = T(1) - sp.ct0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) up[c] = w0 * sp.un.c[c][l] + sp.ct0 * sp.unp1.c[c][l];
+    }
+    T a_[4] = {0, 0, 0, 0}, b_[4] = {0, 0, 0, 0};
+    switch (sp.integ) {
+      case kEuler:
+      case kTrapezoidal:
+        ns = 1; b_[0] = T(1);
+#pragma unroll
+        for (int d = 0; d < 3; d++) xp[d] = x0[d] + dt * up[d];
+        break;
+      case kRK2:
+        ns = 2; a_[1] = HALF; b_[1] = T(1);
+#pragma unroll
+        for (int d = 0; d < 3; d++) xs[1][d] = x0[d] + HALF * dt * up[d];
+        ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) xp[d] = x0[d] + dt * up[d];
+        break;
+      case kRK3:
+        ns = 3; a_[1] = HALF; a_[2] = TWO; b_[0] = SIXTH; b_[1] = TWOTHIRD; b_[2] = SIXTH;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          kk[d] = dt * up[d];
+          x1[d] = x0[d] + SIXTH * kk[d];
+          xs[1][d] = x0[d] + HALF * kk[d];
+        }
+        ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          kk[d] = dt * up[d];
+          xs[2][d] = x0[d] + TWO * kk[d];
+          x1[d] = x1[d] + TWOTHIRD * kk[d];
+        }
+        ad.eval(g, xs[2], nd, vf[2], up, Js[2]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) xp[d] = x1[d] + SIXTH * (dt * up[d]);
+        break;
+      case kRK4:
+      default:
+        ns = 4; a_[1] = HALF; a_[2] = HALF; a_[3] = T(1);
+        b_[0] = SIXTH; b_[1] = THIRD; b_[2] = THIRD; b_[3] = SIXTH;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          kk[d] = dt * up[d];
+          x1[d] = x0[d] + SIXTH * kk[d];
+          xs[1][d] = x0[d] + HALF * kk[d];
+        }
+        ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          kk[d] = dt * up[d];
+          x1[d] = x1[d] + THIRD * kk[d];
+          xs[2][d] = x0[d] + HALF * kk[d];
+        }
+        ad.eval(g, xs[2], nd, vf[2], up, Js[2]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          kk[d] = dt * up[d];
+          x1[d] = x1[d] + THIRD * kk[d];
+          xs[3][d] = x0[d] + kk[d];
+        }
+        ad.eval(g, xs[3], nd, vf[3], up, Js[3]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) xp[d] = x1[d] + SIXTH * (dt * up[d]);
+        break;
+    }
+    // ---- BC of the departure point ----
+    int ndb[3] = {i, j, k};
+    xb[0] = xp[0]; xb[1] = xp[1]; xb[2] = xp[2];
+    project_flat(g, ndb, xb);
+    T nrm[3] = {0, 0, 0};
+    const int kind = lp_bc_kind(g, ndb, xb, nrm);
+    int err = 0;
+    if (set_lp_bc(g, ndb, xb, err)) {
+      xb[0] = x0[0]; xb[1] = x0[1]; xb[2] = x0[2];
+    }
+    // ---- reverse: fm_new = F(xb) + xb - x0 ----
+    T Fv[3], JF[3][3];
+    ad.eval(g, xb, nd, fmold, Fv, JF);
+    T xbar[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) xbar[d] = gl3[d] + JF[0][d] * gl3[0] + JF[1][d] * gl3[1] + JF[2][d] * gl3[2];
+    stuck = kind == 2;
+    if (kind == 1) {
+      const T dn = xbar[0] * nrm[0] + xbar[1] * nrm[1] + xbar[2] * nrm[2];
+#pragma unroll
+      for (int d = 0; d < 3; d++) xbar[d] -= nrm[d] * dn;
+    }
+    if (g.flat >= 0) xbar[g.flat] = T(0);
+    T kb[4][3];
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) kb[s][d] = stuck ? T(0) : b_[s] * xbar[d];
+#pragma unroll
+    for (int s = 3; s >= 1; s--) {
+      if (s < ns) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) ws[s][d] = dt * kb[s][d];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+          kb[s - 1][d] += a_[s] * (Js[s][0][d] * ws[s][0] + Js[s][1][d] * ws[s][1] + Js[s][2][d] * ws[s][2]);
+      }
+    }
     const T w0 = T(1) - sp.ct0;
 #pragma unroll
-    for (int c = 0; c < 3; c++) up[c] = w0 * sp.un.c[c][l] + sp.ct0 * sp.unp1.c[c][l];
+    for (int c = 0; c < 3; c++) {
+      const T kbar = dt * kb[0][c];
+      dun.c[c][l] += w0 * kbar;
+      dunp1.c[c][l] += sp.ct0 * kbar;
+    }
   }
-  int ns;
-  T a_[4] = {0, 0, 0, 0}, b_[4] = {0, 0, 0, 0};
-  switch (sp.integ) {
-    case kEuler:
-    case kTrapezoidal:
-      ns = 1; b_[0] = T(1);
+  // ---- scatter phases through the wave's tile ----
+  for (int ph = 0; ph <= nbuf; ph++) {
+    for (int e = t; e < 3 * WS::TILE; e += 64) ws_.tile[e] = T(0);
+    __syncthreads();
+    const F3<T> dst = ph == 0 ? adj_fm : (ph == 1 ? adj_v0 : (ph == 2 ? adj_v1 : adj_v2));
+    ws_.ovf = ph == 0 ? ovf_fm : (ph == 1 ? ovf_v0 : (ph == 2 ? ovf_v1 : ovf_v2));
+    if (active) {
+      if (ph == 0) {
+        ws_(ad, g, xb, nd, gl3);
+      } else if (!stuck) {
 #pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = x0[d] + dt * up[d];
-      break;
-    case kRK2:
-      ns = 2; a_[1] = HALF; b_[1] = T(1);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xs[1][d] = x0[d] + HALF * dt * up[d];
-      ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = x0[d] + dt * up[d];
-      break;
-    case kRK3:
-      ns = 3; a_[1] = HALF; a_[2] = TWO; b_[0] = SIXTH; b_[1] = TWOTHIRD; b_[2] = SIXTH;
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        kk[d] = dt * up[d];
-        x1[d] = x0[d] + SIXTH * kk[d];
-        xs[1][d] = x0[d] + HALF * kk[d];
+        for (int s = 3; s >= 1; s--) {
+          const int bsel = s == 1 ? sbuf.y : (s == 2 ? sbuf.z : sbuf.w);
+          if (s < ns && bsel == ph - 1) ws_(ad, g, xs[s], nd, ws[s]);
+        }
       }
-      ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        kk[d] = dt * up[d];
-        xs[2][d] = x0[d] + TWO * kk[d];
-        x1[d] = x1[d] + TWOTHIRD * kk[d];
-      }
-      ad.eval(g, xs[2], nd, vf[2], up, Js[2]);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = x1[d] + SIXTH * (dt * up[d]);
-      break;
-    case kRK4:
-    default:
-      ns = 4; a_[1] = HALF; a_[2] = HALF; a_[3] = T(1);
-      b_[0] = SIXTH; b_[1] = THIRD; b_[2] = THIRD; b_[3] = SIXTH;
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        kk[d] = dt * up[d];
-        x1[d] = x0[d] + SIXTH * kk[d];
-        xs[1][d] = x0[d] + HALF * kk[d];
-      }
-      ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        kk[d] = dt * up[d];
-        x1[d] = x1[d] + THIRD * kk[d];
-        xs[2][d] = x0[d] + HALF * kk[d];
-      }
-      ad.eval(g, xs[2], nd, vf[2], up, Js[2]);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        kk[d] = dt * up[d];
-        x1[d] = x1[d] + THIRD * kk[d];
-        xs[3][d] = x0[d] + kk[d];
-      }
-      ad.eval(g, xs[3], nd, vf[3], up, Js[3]);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = x1[d] + SIXTH * (dt * up[d]);
-      break;
+    }
+    __syncthreads();
+    for (int e = t; e < 3 * WS::TILE; e += 64) {
+      const T v = ws_.tile[e];
+      if (v == T(0)) continue;
+      const int m = e / WS::TILE, r = e - m * WS::TILE;
+      const int cx = ox + r % WS::TX, cy = oy + (r / WS::TX) % WS::TY, cz = oz + r / (WS::TX * WS::TY);
+      if (cx < 0 || cy < 0 || cz < 0 || cx >= g.N[0] || cy >= g.N[1] || cz >= g.N[2]) continue;
+      T* d = dst.c[m] + cx + g.N[0] * (cy + g.N[1] * cz);
+      *d = *d + v;
+    }
+    __syncthreads();
   }
-  // ---- BC of the departure point ----
-  int ndb[3] = {i, j, k};
-  T xb[3] = {xp[0], xp[1], xp[2]};
-  project_flat(g, ndb, xb);
-  T nrm[3] = {0, 0, 0};
-  const int kind = lp_bc_kind(g, ndb, xb, nrm);
-  int err = 0;
-  if (set_lp_bc(g, ndb, xb, err)) {
-    xb[0] = x0[0]; xb[1] = x0[1]; xb[2] = x0[2];
-  }
-  // ---- reverse ----
-  // fm_new = F(xb) + xb - x0
-  T Fv[3], JF[3][3];
-  const int no[3] = {i, j, k};
-  ad.eval(g, xb, no, fmold, Fv, JF);
-  ad.scatter(g, xb, no, gl3, adj_fm);
-  T xbar[3];
+}
+
+// dst += ovf, ovf = 0
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_merge_overflow(F3<T> dst, F3<T> ovf, long long nt) {
+  for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < nt; l += (long long)gridDim.x * blockDim.x) {
 #pragma unroll
-  for (int d = 0; d < 3; d++) xbar[d] = gl3[d] + JF[0][d] * gl3[0] + JF[1][d] * gl3[1] + JF[2][d] * gl3[2];
-  if (kind == 2) return;  // stuck: x_dep does not depend on the velocity
-  if (kind == 1) {
-    const T dn = xbar[0] * nrm[0] + xbar[1] * nrm[1] + xbar[2] * nrm[2];
-#pragma unroll
-    for (int d = 0; d < 3; d++) xbar[d] -= nrm[d] * dn;
-  }
-  if (g.flat >= 0) xbar[g.flat] = T(0);
-  // k_bar_s = b_s * xbar; x_s = x0 + a_s k_{s-1}
-  T kb[4][3];
-#pragma unroll
-  for (int s = 0; s < 4; s++)
-#pragma unroll
-    for (int d = 0; d < 3; d++) kb[s][d] = b_[s] * xbar[d];
-  F3<T> av[3] = {adj_v0, adj_v1, adj_v2};
-  const int sb[4] = {sbuf.x, sbuf.y, sbuf.z, sbuf.w};
-  for (int s = ns - 1; s >= 1; s--) {
-    // k_s = dt * v_s(x_s)
-    T w[3] = {dt * kb[s][0], dt * kb[s][1], dt * kb[s][2]};
-    ad.scatter(g, xs[s], nd, w, av[sb[s]]);
-    T xsb[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) xsb[d] = Js[s][0][d] * w[0] + Js[s][1][d] * w[1] + Js[s][2][d] * w[2];
-#pragma unroll
-    for (int d = 0; d < 3; d++) kb[s - 1][d] += a_[s] * xsb[d];
-  }
-  // first stage: k_1 = dt * ((1-ct0) u_n + ct0 u_np1) at the node
-  const T w0 = T(1) - sp.ct0;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const T kbar = dt * kb[0][c];
-    dun.c[c][l] += w0 * kbar;
-    dunp1.c[c][l] += sp.ct0 * kbar;
+    for (int c = 0; c < 3; c++) {
+      const T v = ovf.c[c][l];
+      if (v != T(0)) {
+        dst.c[c][l] += v;
+        ovf.c[c][l] = T(0);
+      }
+    }
   }
 }
 
@@ -403,53 +576,6 @@ __global__ void __launch_bounds__(kBlock) k_adj_unblend(F3<T> b, T ct, F3<T> an,
         b.c[c][l] = T(0);
       }
     }
-  }
-}
-
-// Transposed banded solve of one line, A^T y = b with A = L U (dbnfac):
-// U^T z = b forward, then L^T y = z backward; band element (r, c) at
-// (r - c + K - 1) + (2K - 1) c.
-template <typename T, int K>
-__global__ void __launch_bounds__(kBlock) k_fit_T_axis(F3<T> a, const T* __restrict__ W, int n, int axis, int N0,
-                                                       int N1, int nlines) {
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nlines) return;
-  long long start, stride;
-  if (axis == 0) {
-    start = (long long)N0 * id; stride = 1;
-  } else if (axis == 1) {
-    const int i = id % N0, kk = id / N0;
-    start = i + (long long)N0 * N1 * kk; stride = N0;
-  } else {
-    start = id; stride = (long long)N0 * N1;
-  }
-  T* b = a.c[blockIdx.y] + start;
-  constexpr int M = K - 1, NB = 2 * K - 1;
-  T win[M > 0 ? M : 1];
-#pragma unroll
-  for (int m = 0; m < M; m++) win[m] = T(0);
-  for (int r = 0; r < n; r++) {
-    T v = b[r * stride];
-#pragma unroll
-    for (int jj = M; jj >= 1; jj--)
-      if (r - jj >= 0) v = v - win[jj - 1] * W[(M - jj) + NB * r];
-    v = v / W[M + NB * r];
-#pragma unroll
-    for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
-    win[0] = v;
-    b[r * stride] = v;
-  }
-#pragma unroll
-  for (int m = 0; m < M; m++) win[m] = T(0);
-  for (int r = n - 1; r >= 0; r--) {
-    T v = b[r * stride];
-#pragma unroll
-    for (int jj = M; jj >= 1; jj--)
-      if (r + jj <= n - 1) v = v - win[jj - 1] * W[(jj + M) + NB * r];
-#pragma unroll
-    for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
-    win[0] = v;
-    b[r * stride] = v;
   }
 }
 
@@ -621,50 +747,56 @@ __global__ void __launch_bounds__(kBlock) k_interior(GridDev<T> g, CF3<T> src, F
 // ===========================================================================
 static inline dim3 agrid(long long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
+template <typename T, class A, int BRX, int BRY, int BRZ>
+static void sl_adjoint_colours(const GridDev<T>& g, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
+                               F3<T> adj_fm, F3<T> const adj_v[3], F3<T> const ovf[4], const int sbuf[4], int nbuf,
+                               F3<T> dun, F3<T> dunp1, hipStream_t s) {
+  AdjLaunch al;
+  al.nbx = (g.n[0] + BRX - 1) / BRX;
+  al.nby = (g.n[1] + BRY - 1) / BRY;
+  al.nbz = (g.n[2] + BRZ - 1) / BRZ;
+  const int4 sb = make_int4(sbuf[0], sbuf[1], sbuf[2], sbuf[3]);
+  for (int c = 0; c < 8; c++) {
+    al.cx = c & 1; al.cy = (c >> 1) & 1; al.cz = c >> 2;
+    al.gx = (al.nbx - al.cx + 1) / 2;
+    al.gy = (al.nby - al.cy + 1) / 2;
+    const int gz = (al.nbz - al.cz + 1) / 2;
+    if (al.gx <= 0 || al.gy <= 0 || gz <= 0) continue;
+    hipLaunchKernelGGL((k_sl_adjoint<T, A, BRX, BRY, BRZ>), dim3((unsigned)(al.gx * al.gy * gz)), dim3(64), 0, s, g,
+                       al, sp, fmold, lam, adj_fm, adj_v[0], adj_v[1], adj_v[2], ovf[0], ovf[1], ovf[2], ovf[3], sb,
+                       nbuf, dun, dunp1);
+  }
+  const long long nb = std::min<long long>((g.nt + kBlock - 1) / kBlock, 4096);
+  F3<T> dsts[4] = {adj_fm, adj_v[0], adj_v[1], adj_v[2]};
+  for (int ph = 0; ph <= nbuf; ph++)
+    hipLaunchKernelGGL((k_merge_overflow<T>), dim3((unsigned)nb), dim3(kBlock), 0, s, dsts[ph], ovf[ph], g.nt);
+}
+
 template <typename T>
 void launch_sl_adjoint(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
-                       F3<T> adj_fm, F3<T> const adj_v[3], const int sbuf[4], F3<T> dun, F3<T> dunp1,
-                       hipStream_t s) {
-  const int4 sb = make_int4(sbuf[0], sbuf[1], sbuf[2], sbuf[3]);
-  if (interp == kLinear || interp == kNearest) {
-    hipLaunchKernelGGL((k_sl_adjoint<T, AdjLinear<T>>), agrid(g.np), dim3(kBlock), 0, s, g, sp, fmold, lam, adj_fm,
-                       adj_v[0], adj_v[1], adj_v[2], sb, dun, dunp1);
-    return;
-  }
+                       F3<T> adj_fm, F3<T> const adj_v[3], F3<T> const ovf[4], const int sbuf[4], int nbuf,
+                       F3<T> dun, F3<T> dunp1, hipStream_t s) {
+  const bool flat = g.n[2] == 1;
+  auto go = [&](auto tag) {
+    using A = decltype(tag);
+    if (flat)
+      sl_adjoint_colours<T, A, 8, 8, 1>(g, sp, fmold, lam, adj_fm, adj_v, ovf, sbuf, nbuf, dun, dunp1, s);
+    else
+      sl_adjoint_colours<T, A, 4, 4, 4>(g, sp, fmold, lam, adj_fm, adj_v, ovf, sbuf, nbuf, dun, dunp1, s);
+  };
+  if (interp == kNearest) return go(AdjNearest<T>{});
+  if (interp == kLinear) return go(AdjLinear<T>{});
   const int kx = g.kord[0], kz = g.kord[2];
-  if (kx == 4 && kz == 4)
-    hipLaunchKernelGGL((k_sl_adjoint<T, AdjSpline<T, 4, 4>>), agrid(g.np), dim3(kBlock), 0, s, g, sp, fmold, lam,
-                       adj_fm, adj_v[0], adj_v[1], adj_v[2], sb, dun, dunp1);
-  else if (kx == 4 && kz == 2)
-    hipLaunchKernelGGL((k_sl_adjoint<T, AdjSpline<T, 4, 2>>), agrid(g.np), dim3(kBlock), 0, s, g, sp, fmold, lam,
-                       adj_fm, adj_v[0], adj_v[1], adj_v[2], sb, dun, dunp1);
-  else if (kx == 3 && kz == 3)
-    hipLaunchKernelGGL((k_sl_adjoint<T, AdjSpline<T, 3, 3>>), agrid(g.np), dim3(kBlock), 0, s, g, sp, fmold, lam,
-                       adj_fm, adj_v[0], adj_v[1], adj_v[2], sb, dun, dunp1);
-  else
-    hipLaunchKernelGGL((k_sl_adjoint<T, AdjSpline<T, 3, 2>>), agrid(g.np), dim3(kBlock), 0, s, g, sp, fmold, lam,
-                       adj_fm, adj_v[0], adj_v[1], adj_v[2], sb, dun, dunp1);
+  if (kx == 4 && kz == 4) go(AdjSpline<T, 4, 4>{});
+  else if (kx == 4 && kz == 2) go(AdjSpline<T, 4, 2>{});
+  else if (kx == 3 && kz == 3) go(AdjSpline<T, 3, 3>{});
+  else go(AdjSpline<T, 3, 2>{});
 }
 
 template <typename T>
 void launch_adj_unblend(const GridDev<T>& g, F3<T> b, T ct, F3<T> an, F3<T> anp1, hipStream_t s) {
   const long long nb = std::min<long long>((g.nt + kBlock - 1) / kBlock, 8192);
   hipLaunchKernelGGL((k_adj_unblend<T>), dim3((unsigned)nb), dim3(kBlock), 0, s, b, ct, an, anp1, g.nt);
-}
-
-// transpose of launch_spline_fit: z, y, then x pass of A^T solves
-template <typename T>
-void launch_spline_fit_T(const GridDev<T>& g, F3<T> a, hipStream_t s) {
-  for (int axis = 2; axis >= 0; axis--) {
-    const int n = g.N[axis];
-    const int nlines = (int)(g.nt / n);
-    dim3 grid((nlines + kBlock - 1) / kBlock, 3);
-    switch (g.kord[axis]) {
-      case 2: hipLaunchKernelGGL((k_fit_T_axis<T, 2>), grid, dim3(kBlock), 0, s, a, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
-      case 3: hipLaunchKernelGGL((k_fit_T_axis<T, 3>), grid, dim3(kBlock), 0, s, a, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
-      default: hipLaunchKernelGGL((k_fit_T_axis<T, 4>), grid, dim3(kBlock), 0, s, a, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
-    }
-  }
 }
 
 template <typename T>
@@ -692,9 +824,8 @@ void launch_interior(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStream_t s) 
 
 #define INST_ADJ(T, U)                                                                                          \
   template void launch_sl_adjoint<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, CF3<T>, F3<T>,      \
-                                     F3<T> const*, const int*, F3<T>, F3<T>, hipStream_t);                      \
+                                     F3<T> const*, F3<T> const*, const int*, int, F3<T>, F3<T>, hipStream_t);   \
   template void launch_adj_unblend<T>(const GridDev<T>&, F3<T>, T, F3<T>, F3<T>, hipStream_t);                  \
-  template void launch_spline_fit_T<T>(const GridDev<T>&, F3<T>, hipStream_t);                                 \
   template void launch_fold_ghost<T>(const GridDev<T>&, F3<T>, int, hipStream_t);                               \
   template void launch_ftle_adjoint<T>(const GridDev<T>&, CF3<T>, double, int, double, const double*, F3<T>,    \
                                        hipStream_t);                                                            \

@@ -903,7 +903,7 @@ struct Engine : EngineBase {
     U t_n = 0, t_np1 = 0;
     Blend blends[6];
     // state
-    DevBuf fm[2], fmcoef, compfm, dxout, ftle, lam, adjfm, adjv[3], dcn, dcnp1, dun, dunp1;
+    DevBuf fm[2], fmcoef, compfm, dxout, ftle, lam, adjfm, adjv[3], dcn, dcnp1, dun, dunp1, ovf[4];
     int fcur = 0;
     std::vector<DevBuf> ckpts;   // fm (ghosted values) after steps 0, ckpt, 2 ckpt, ...
     std::vector<DevBuf> segf;    // per sub-cycle of a segment: the field k_sl_step read
@@ -1122,7 +1122,7 @@ struct Engine : EngineBase {
     const size_t fb = 3 * fbytes(), np = (size_t)g.np;
     const int nsnap = tape.nsnap;
     for (DevBuf* b : {&tape.lam, &tape.adjfm, &tape.adjv[0], &tape.adjv[1], &tape.adjv[2], &tape.dcn, &tape.dcnp1,
-                      &tape.dun, &tape.dunp1})
+                      &tape.dun, &tape.dunp1, &tape.ovf[0], &tape.ovf[1], &tape.ovf[2], &tape.ovf[3]})
       if (!ensure(*b, fb, true)) return 1;
     for (U* o : {gu, gv, gw}) HIPCHK_R(hipMemsetAsync(o, 0, sizeof(U) * np * nsnap, stream), 1);
     const double tiny = sizeof(U) == 8 ? DBL_MIN : (double)FLT_MIN;
@@ -1198,8 +1198,9 @@ struct Engine : EngineBase {
           HIPCHK_R(hipMemsetAsync(tape.adjfm.p, 0, fb, stream), 1);
           const DevBuf& fread = tape.segf[(size_t)(s - s0) * maxsub + k];
           F3<T> av[3] = {f3(tape.adjv[0]), f3(tape.adjv[1]), f3(tape.adjv[2])};
+          F3<T> ov[4] = {f3(tape.ovf[0]), f3(tape.ovf[1]), f3(tape.ovf[2]), f3(tape.ovf[3])};
           launch_sl_adjoint(g, tape.interp, sp, cf3(const_cast<DevBuf&>(fread)), cf3(tape.lam), f3(tape.adjfm), av,
-                            sbuf, f3(tape.dun), f3(tape.dunp1), stream);
+                            ov, sbuf, nb, f3(tape.dun), f3(tape.dunp1), stream);
           for (int b = 0; b < nb; b++)
             launch_adj_unblend(g, f3(tape.adjv[b]), (T)bct[b], f3(tape.dcn), f3(tape.dcnp1), stream);
           if (tape.spl) launch_spline_fit_T(g, f3(tape.adjfm), stream);

@@ -451,9 +451,32 @@ __global__ void __launch_bounds__(kBlock) k_ftle(GridDev<T> g, CF3<T> fm, double
 // dependent chain runs on registers only; the arithmetic and its order are
 // those of the unpipelined loop.
 constexpr int kFitPf = 8;
-template <typename T, int K, class At>
+// Recurrence steps of A x = b with A = L U in band storage (element (r, c) at
+// (r - c + K - 1) + (2K - 1) c): forward with the unit lower factor, backward
+// with the upper one.  TR solves A^T x = b instead (the adjoint of the fit):
+// forward with U^T (dividing by the diagonal), backward with the unit L^T.
+template <typename T, int K, bool TR>
+__device__ __forceinline__ T fwd_elim(int r, T v, const T* win, const T* __restrict__ W) {
+  constexpr int M = K - 1, NB = 2 * K - 1;
+#pragma unroll
+  for (int jj = M; jj >= 1; jj--)
+    if (r - jj >= 0) v = v - win[jj - 1] * (TR ? W[(M - jj) + NB * r] : W[(jj + M) + NB * (r - jj)]);
+  if (TR) v = v / W[M + NB * r];
+  return v;
+}
+template <typename T, int K, bool TR>
+__device__ __forceinline__ T bwd_elim(int r, int n, T v, const T* win, const T* __restrict__ W) {
+  constexpr int M = K - 1, NB = 2 * K - 1;
+#pragma unroll
+  for (int jj = M; jj >= 1; jj--)
+    if (r + jj <= n - 1) v = v - win[jj - 1] * (TR ? W[(jj + M) + NB * r] : W[(M - jj) + NB * (r + jj)]);
+  if (!TR) v = v / W[M + NB * r];
+  return v;
+}
+
+template <typename T, int K, bool TR, class At>
 __device__ __forceinline__ void band_solve(At at, const T* __restrict__ W, int n) {
-  constexpr int M = K - 1, NB = 2 * K - 1, PF = kFitPf;
+  constexpr int M = K - 1, PF = kFitPf;
   T win[M > 0 ? M : 1];  // forward: win[0] = b(r-1), ...; backward: win[0] = b(r+1), ...
 #pragma unroll
   for (int m = 0; m < M; m++) win[m] = T(0);
@@ -467,10 +490,7 @@ __device__ __forceinline__ void band_solve(At at, const T* __restrict__ W, int n
     for (int u = 0; u < PF; u++) {
       const int r = r0 + u;
       if (r < n) {
-        T v = cur[u];
-#pragma unroll
-        for (int jj = M; jj >= 1; jj--)
-          if (r - jj >= 0) v = v - win[jj - 1] * W[(jj + M) + NB * (r - jj)];
+        const T v = fwd_elim<T, K, TR>(r, cur[u], win, W);
 #pragma unroll
         for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
         win[0] = v;
@@ -491,11 +511,7 @@ __device__ __forceinline__ void band_solve(At at, const T* __restrict__ W, int n
     for (int u = 0; u < PF; u++) {
       const int r = r0 - u;
       if (r >= 0) {
-        T v = cur[u];
-#pragma unroll
-        for (int jj = M; jj >= 1; jj--)
-          if (r + jj <= n - 1) v = v - win[jj - 1] * W[(M - jj) + NB * (r + jj)];
-        v = v / W[M + NB * r];
+        const T v = bwd_elim<T, K, TR>(r, n, cur[u], win, W);
 #pragma unroll
         for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
         win[0] = v;
@@ -507,7 +523,7 @@ __device__ __forceinline__ void band_solve(At at, const T* __restrict__ W, int n
   }
 }
 
-template <typename T, int K>
+template <typename T, int K, bool TR>
 __global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restrict__ W, int n, int axis, int N0,
                                                      int N1, int nlines) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
@@ -523,7 +539,7 @@ __global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restric
     start = id; stride = (long long)N0 * N1;
   }
   T* b = a.c[comp] + start;
-  band_solve<T, K>([&](int r) -> T& { return b[r * stride]; }, W, n);
+  band_solve<T, K, TR>([&](int r) -> T& { return b[r * stride]; }, W, n);
 }
 
 // x-axis pass of the fit: the lines are contiguous rows.  One wavefront
@@ -535,10 +551,10 @@ __global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restric
 // back, as the strided passes do.  Same arithmetic, in the same order, as
 // band_solve.
 constexpr int kXC = 16;
-template <typename T, int K>
+template <typename T, int K, bool TR>
 __global__ void __launch_bounds__(64) k_fit_x(F3<T> a, const T* __restrict__ W, int n, int nlines) {
   __shared__ T tile[64][kXC + 1];
-  constexpr int M = K - 1, NB = 2 * K - 1;
+  constexpr int M = K - 1;
   const int lane = threadIdx.x;
   const long long line0 = (long long)blockIdx.x * 64;
   const int nl = (int)min((long long)64, nlines - line0);
@@ -580,10 +596,7 @@ __global__ void __launch_bounds__(64) k_fit_x(F3<T> a, const T* __restrict__ W, 
     for (int u = 0; u < kXC; u++) {
       const int r = ch * kXC + u;
       if (r < n) {
-        T v = tile[lane][u];
-#pragma unroll
-        for (int jj = M; jj >= 1; jj--)
-          if (r - jj >= 0) v = v - win[jj - 1] * W[(jj + M) + NB * (r - jj)];
+        const T v = fwd_elim<T, K, TR>(r, tile[lane][u], win, W);
 #pragma unroll
         for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
         win[0] = v;
@@ -606,11 +619,7 @@ __global__ void __launch_bounds__(64) k_fit_x(F3<T> a, const T* __restrict__ W, 
     for (int u = kXC - 1; u >= 0; u--) {
       const int r = ch * kXC + u;
       if (r < n) {
-        T v = tile[lane][u];
-#pragma unroll
-        for (int jj = M; jj >= 1; jj--)
-          if (r + jj <= n - 1) v = v - win[jj - 1] * W[(M - jj) + NB * (r + jj)];
-        v = v / W[M + NB * r];
+        const T v = bwd_elim<T, K, TR>(r, n, tile[lane][u], win, W);
 #pragma unroll
         for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
         win[0] = v;
@@ -672,28 +681,38 @@ void launch_field_from_origin(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStr
   hipLaunchKernelGGL((k_field_from_origin<T>), grid_for(g.nt), dim3(kBlock), 0, s, g, src, dst);
 }
 
+template <typename T, bool TR>
+static void fit_pass(const GridDev<T>& g, F3<T> coef, int ncomp, int axis, hipStream_t s) {
+  const int n = g.N[axis];
+  const int nlines = (int)(g.nt / n);
+  if (axis == 0) {
+    dim3 gx((nlines + 63) / 64, ncomp);
+    switch (g.kord[0]) {
+      case 2: hipLaunchKernelGGL((k_fit_x<T, 2, TR>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
+      case 3: hipLaunchKernelGGL((k_fit_x<T, 3, TR>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
+      default: hipLaunchKernelGGL((k_fit_x<T, 4, TR>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
+    }
+    return;
+  }
+  dim3 grid((nlines + kBlock - 1) / kBlock, ncomp);
+  switch (g.kord[axis]) {
+    case 2: hipLaunchKernelGGL((k_fit_axis<T, 2, TR>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+    case 3: hipLaunchKernelGGL((k_fit_axis<T, 3, TR>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+    default: hipLaunchKernelGGL((k_fit_axis<T, 4, TR>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+  }
+}
+
 // fits in place: coef holds the data on entry and the coefficients on exit
 template <typename T>
 void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t s) {
-  for (int axis = 0; axis < 3; axis++) {
-    const int n = g.N[axis];
-    const int nlines = (int)(g.nt / n);
-    if (axis == 0) {
-      dim3 gx((nlines + 63) / 64, ncomp);
-      switch (g.kord[0]) {
-        case 2: hipLaunchKernelGGL((k_fit_x<T, 2>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
-        case 3: hipLaunchKernelGGL((k_fit_x<T, 3>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
-        default: hipLaunchKernelGGL((k_fit_x<T, 4>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
-      }
-      continue;
-    }
-    dim3 grid((nlines + kBlock - 1) / kBlock, ncomp);
-    switch (g.kord[axis]) {
-      case 2: hipLaunchKernelGGL((k_fit_axis<T, 2>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
-      case 3: hipLaunchKernelGGL((k_fit_axis<T, 3>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
-      default: hipLaunchKernelGGL((k_fit_axis<T, 4>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
-    }
-  }
+  for (int axis = 0; axis < 3; axis++) fit_pass<T, false>(g, coef, ncomp, axis, s);
+}
+
+// transpose of launch_spline_fit (three components, in place): A^T solves in
+// the reverse order of the passes
+template <typename T>
+void launch_spline_fit_T(const GridDev<T>& g, F3<T> a, hipStream_t s) {
+  for (int axis = 2; axis >= 0; axis--) fit_pass<T, true>(g, a, 3, axis, s);
 }
 
 // dispatch over interpolator and spline orders (KX = orders of x and y, KZ of z)
@@ -825,6 +844,7 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
   template void launch_fm_ghost<T>(const GridDev<T>&, F3<T>, CF3<T>, DevStatus*, hipStream_t);                \
   template void launch_field_from_origin<T>(const GridDev<T>&, CF3<T>, F3<T>, hipStream_t);                   \
   template void launch_spline_fit<T>(const GridDev<T>&, F3<T>, int, hipStream_t);                             \
+  template void launch_spline_fit_T<T>(const GridDev<T>&, F3<T>, hipStream_t);                                \
   template void launch_blend<T>(const GridDev<T>&, void* const*, int, CF3<T>, CF3<T>, T, hipStream_t);     \
   template void launch_sl_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, F3<T>, DevStatus*,  \
                                   hipStream_t);                                                               \

@@ -146,11 +146,12 @@ void launch_interior_to_user(const GridDev<T>& g, CF3<T> src, int ghosted, U* a,
 
 // ---- adjoint launchers (adjoint.hip) ----
 // one semi-Lagrangian sub-cycle in reverse; stage s >= 2 scatters into
-// adj_v[sbuf[s]]; the first stage's nodal adjoint goes to dun / dunp1
+// adj_v[sbuf[s]]; the first stage's nodal adjoint goes to dun / dunp1; ovf[4]
+// (zero on entry and exit) take the rare scatters outside a node's window
 template <typename T>
 void launch_sl_adjoint(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
-                       F3<T> adj_fm, F3<T> const adj_v[3], const int sbuf[4], F3<T> dun, F3<T> dunp1,
-                       hipStream_t stream);
+                       F3<T> adj_fm, F3<T> const adj_v[3], F3<T> const ovf[4], const int sbuf[4], int nbuf,
+                       F3<T> dun, F3<T> dunp1, hipStream_t stream);
 // an += (1-ct) b, anp1 += ct b, b = 0
 template <typename T>
 void launch_adj_unblend(const GridDev<T>& g, F3<T> b, T ct, F3<T> an, F3<T> anp1, hipStream_t stream);
