@@ -288,6 +288,77 @@ struct WindowScatter {
   int lx, ly, lz;    // lane's node in the brick
   F3<T> ovf;
   int N0, N1;
+  // several points of one lane into the same window: their contributions are
+  // summed in registers, one read-modify-write per window cell
+  template <int NP>
+  __device__ __forceinline__ void multi(const A& ad, const GridDev<T>& g, const T (*qs)[3], const bool* use,
+                                        const int nd[3], const T (*wvs)[3]) const {
+    T ex[NP][A::WX], ey[NP][A::WY], ez[NP][A::WZ];
+    bool any = false;
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      int b[3];
+      T wx[A::KX], wy[A::KY], wz[A::KZ];
+      int sx = 0, sy = 0, sz = 0;
+      bool ok = use[p] && !ad.weights(g, qs[p], nd, b, wx, wy, wz);
+      if (ok) {
+        sx = b[0] - (nd[0] + A::BX);
+        sy = b[1] - (nd[1] + A::BY);
+        sz = b[2] - (nd[2] + A::BZ);
+        if (sx < 0 || sx > A::WX - A::KX || sy < 0 || sy > A::WY - A::KY || sz < 0 || sz > A::WZ - A::KZ) {
+          (*this)(ad, g, qs[p], nd, wvs[p]);  // overflow path
+          ok = false;
+        }
+      }
+      any = any || ok;
+#pragma unroll
+      for (int o = 0; o < A::WX; o++) {
+        T v = T(0);
+#pragma unroll
+        for (int a = 0; a < A::KX; a++) v = (ok && o - a == sx) ? wx[a] : v;
+        ex[p][o] = v;
+      }
+#pragma unroll
+      for (int o = 0; o < A::WY; o++) {
+        T v = T(0);
+#pragma unroll
+        for (int a = 0; a < A::KY; a++) v = (ok && o - a == sy) ? wy[a] : v;
+        ey[p][o] = v;
+      }
+#pragma unroll
+      for (int o = 0; o < A::WZ; o++) {
+        T v = T(0);
+#pragma unroll
+        for (int a = 0; a < A::KZ; a++) v = (ok && o - a == sz) ? wz[a] : v;
+        ez[p][o] = v;
+      }
+    }
+    if (!any) return;
+#pragma unroll
+    for (int oz = 0; oz < A::WZ; oz++)
+#pragma unroll
+      for (int oy = 0; oy < A::WY; oy++) {
+        T wyz[NP][3];
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+          const T w = ey[p][oy] * ez[p][oz];
+#pragma unroll
+          for (int m = 0; m < 3; m++) wyz[p][m] = w * wvs[p][m];
+        }
+        const int row = ((lz + oz) * TY + (ly + oy)) * TX + lx;
+#pragma unroll
+        for (int ox = 0; ox < A::WX; ox++) {
+#pragma unroll
+          for (int m = 0; m < 3; m++) {
+            T add = T(0);
+#pragma unroll
+            for (int p = 0; p < NP; p++) add += ex[p][ox] * wyz[p][m];
+            tile[m * TILE + row + ox] += add;
+          }
+        }
+      }
+  }
+
   __device__ __forceinline__ void operator()(const A& ad, const GridDev<T>& g, const T q[3], const int nd[3],
                                              const T wv[3]) const {
     int b[3];
@@ -389,7 +460,7 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
       active = false;
   }
   const int nd[3] = {i, j, k};
-  T xs[4][3], ws[4][3], xb[3];
+  T xs[4][3] = {}, ws[4][3] = {}, xb[3] = {};
   int ns = 1;
   bool stuck = false;
   if (active) {
@@ -526,11 +597,9 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
       if (ph == 0) {
         ws_(ad, g, xb, nd, gl3);
       } else if (!stuck) {
-#pragma unroll
-        for (int s = 3; s >= 1; s--) {
-          const int bsel = s == 1 ? sbuf.y : (s == 2 ? sbuf.z : sbuf.w);
-          if (s < ns && bsel == ph - 1) ws_(ad, g, xs[s], nd, ws[s]);
-        }
+        // stages 1..3 (indices) that feed this buffer, in one window pass
+        const bool use[3] = {1 < ns && sbuf.y == ph - 1, 2 < ns && sbuf.z == ph - 1, 3 < ns && sbuf.w == ph - 1};
+        ws_.template multi<3>(ad, g, xs + 1, use, nd, ws + 1);
       }
     }
     __syncthreads();
