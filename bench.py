@@ -241,6 +241,80 @@ def run_adjoint(variant, n, steps, ckpt, dt=0.02):
                        "snapshots": "ABC + seeded Gaussian field (sigma 0.05, smoothed over 4 cells)"}}
 
 
+def run_turbulence(variant, opts, n, steps, dt=0.005, T=1.0, h=0.1, kmax=42, seed=0):
+    """Precision-study workload: forward FTLE (T = 1, h = 0.1, RK3, cubic
+    B-spline, CFL 0.5) of the random-Fourier turbulent field at n^3 over
+    `steps` steps of dt; the velocity is generated on the device each step
+    (not timed).  Returns the FTLE of the last output and the update rate."""
+    import torch
+
+    import libcfd2lcs_amd as L
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from flows import RandomFourierField
+    field = RandomFourierField(n, kmax=kmax, seed=seed)
+    lib = L.Cfd2lcs(variant)
+    lib.finalize()
+    hgrid = 2 * np.pi / n
+    a = 0.5 * hgrid + np.arange(n) * hgrid
+    X, Y, Z = np.meshgrid(a, a, a, indexing="ij")
+    lib.init(X, Y, Z, np.zeros(X.shape, dtype=np.int32))
+    hd = lib.diagnostic_init(L.FTLE_FWD, 0, T, h, "fwdFTLE")
+    lib.set_option("INTEGRATOR", L.RK3)
+    lib.set_option("INTERPOLATOR", L.CUBIC)
+    lib.set_param("CFL", 0.5)
+    for k, v in (opts or {}).items():
+        lib.set_option(k, v)
+    dtype = torch.float64 if variant == "dp" else torch.float32
+    busy, upd = 0.0, 0
+    st0 = lib.stats()
+    for s in range(steps + 1):
+        u, v, w = (x.to(dtype).contiguous() for x in field.at(s * dt))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lib.update_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), s * dt)
+        torch.cuda.synchronize()
+        busy += time.perf_counter() - t0
+        if lib.error:
+            raise RuntimeError("update failed in the turbulence workload")
+        if s % 50 == 0:
+            print(f"[bench] turbulence {variant} {opts}: step {s}", file=sys.stderr, flush=True)
+    st1 = lib.stats()
+    upd = st1["integrations_fwd"] - st0["integrations_fwd"]
+    ftle = np.asarray(lib.ftle(hd), dtype=np.float64)
+    nout = lib.noutput(hd)
+    lib.finalize()
+    torch.cuda.empty_cache()
+    return {"ftle": ftle, "updates_per_s": upd / busy, "outputs": nout}
+
+
+def precision_study(n, steps):
+    import libcfd2lcs_amd as L
+    runs = [("fp64", "dp", {}), ("fp32", "sp", {}),
+            ("fp16", "sp", {"VELOCITY_STORAGE": L.LCS_FP16, "COMPENSATED": L.LCS_TRUE}),
+            ("bf16", "sp", {"VELOCITY_STORAGE": L.LCS_BF16, "COMPENSATED": L.LCS_TRUE})]
+    res = {}
+    for name, var, opts in runs:
+        res[name] = run_turbulence(var, opts, n, steps)
+    ref = res["fp64"]["ftle"]
+    scale = float(np.abs(ref).max())
+    out = {}
+    for name, r in res.items():
+        e = np.abs(r["ftle"] - ref)
+        out[name] = {"updates_per_s": r["updates_per_s"], "ftle_outputs": r["outputs"],
+                     "max_abs": float(e.max()), "rms": float(np.sqrt((e ** 2).mean())),
+                     "p99_abs": float(np.quantile(e, 0.99)), "max_rel": float(e.max()) / scale,
+                     "rms_rel": float(np.sqrt((e ** 2).mean())) / scale}
+    return {"metric": "FTLE error against the FP64 run by velocity storage (precision study)",
+            "value": out["fp32"]["max_rel"], "unit": "max |FTLE error| / max |FTLE| (fp32 storage)",
+            "higher_is_better": False, "n_gpus": 1,
+            "data": "synthetic: divergence-free random Fourier field, |k| <= 42, E(k) ~ k^4 exp(-2 (k/4)^2), "
+                    "unit RMS, phases advancing at |k|^(2/3), seed 0",
+            "config": {"workload": f"turbulence{n}_rk3_cubic_fwd", "grid": [n] * 3, "dt": 0.005, "steps": steps,
+                       "T": 1.0, "h": 0.1, "integrator": "RK3", "interpolator": "CUBIC",
+                       "compare": "FTLE of the last output"},
+            "ftle_scale": scale, "by_storage": out}
+
+
 def timers(lib):
     import ctypes as C
     sec = (C.c_double * 16)()
@@ -319,6 +393,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ftle", action="store_true", help="skip the full-ring FTLE fields/s measurement")
+    ap.add_argument("--workload", choices=["abc", "turbulence"], default="abc",
+                    help="abc: the headline benchmark; turbulence: the velocity-storage precision study")
+    ap.add_argument("--turb-steps", type=int, default=400)
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the adjoint workload (ABC 192^3, dt 0.02); 0 skips it")
     ap.add_argument("--adjoint-n", type=int, default=192)
@@ -335,6 +412,9 @@ def main():
         print("bench.py: this build measures one GPU (the whole domain on one MI355X); "
               "multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "turbulence":
+        print(json.dumps(precision_study(args.n, args.turb_steps)), flush=True)
+        return
     dt, T, h = 0.02, 5.0, 0.5
     if args.calibrate:
         import torch

@@ -567,7 +567,9 @@ struct Engine : EngineBase {
       if (ct == (U)0) return whole(A);
       if (ct == (U)1) return whole(B);
     }
-    const size_t eb = storage_bytes(stor, sizeof(T));
+    // per-component stride of the stage field (0 for the packed records)
+    const size_t eb = stor == kStoreNative ? sizeof(T) : 0;
+    const size_t fbytes_stage = stage_field_bytes(stor, sizeof(T), (size_t)g.nt);
     Blend* victim = &blends[0];
     for (auto& b : blends) {
       if (b.valid && b.ct == ct && b.storage == stor && b.spl == spl) {
@@ -580,7 +582,7 @@ struct Engine : EngineBase {
         else if (b.valid == victim->valid && b.stamp < victim->stamp) victim = &b;
       }
     }
-    if ((!victim->buf.p || victim->buf.bytes < 3 * eb * (size_t)g.nt) && !victim->buf.alloc(3 * fbytes())) {
+    if ((!victim->buf.p || victim->buf.bytes < fbytes_stage) && !victim->buf.alloc(3 * fbytes())) {
       fail("hipMalloc failed (blended velocity)");
       return whole(A);
     }

@@ -182,6 +182,24 @@ __device__ __forceinline__ bf16_t narrow<bf16_t, float>(float x) {
   u += 0x7fffu + ((u >> 16) & 1u);
   return bf16_t{(uint16_t)(u >> 16)};
 }
+// packed 16-bit records: the element type and its widening
+template <typename S> struct Packed { static constexpr bool value = false; };
+template <> struct Packed<half4_t> { static constexpr bool value = true; using E = half_t; };
+template <> struct Packed<bf164_t> { static constexpr bool value = true; using E = bf16_t; };
+template <typename P>
+__device__ __forceinline__ void widen3(const P& r, float v[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) v[c] = widen(r.e[c]);
+}
+// K consecutive packed records (one x row of a spline stencil), 8 K bytes
+template <int K, typename P>
+__device__ __forceinline__ void load_row3(const P* __restrict__ row, float v[K][3]) {
+  struct __attribute__((aligned(8))) R { P e[K]; } r;
+  __builtin_memcpy(&r, row, sizeof(R));
+#pragma unroll
+  for (int a = 0; a < K; a++) widen3(r.e[a], v[a]);
+}
+
 // K consecutive stored elements (one x row of a spline stencil); 16-bit rows
 // are fetched as one unaligned 2K-byte load
 template <typename T, int K, typename S>
@@ -250,8 +268,15 @@ struct IntNearest {
   __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
                              unsigned long long&) const {
     int l = gid(g, iclamp(nd[0], 0, g.n[0] + 1), iclamp(nd[1], 0, g.n[1] + 1), iclamp(nd[2], 0, g.n[2] + 1));
+    if constexpr (Packed<S>::value) {
+      float v[3];
+      widen3(f.c[0][l], v);
 #pragma unroll
-    for (int c = 0; c < 3; c++) out[c] = T(widen(f.c[c][l]));
+      for (int c = 0; c < 3; c++) out[c] = T(v[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) out[c] = T(widen(f.c[c][l]));
+    }
   }
 };
 
@@ -263,12 +288,24 @@ struct IntLinear {
     LinW<T> w;
     lin_setup(g, q, nd, w);
     const int b = w.base, sy = g.N[0], sz = g.N[0] * g.N[1];
+    if constexpr (Packed<S>::value) {
+      const S* A = f.c[0];
+      float v[8][3];
+      const int o[8] = {b, b + 1, b + sy, b + sy + 1, b + sz, b + sz + 1, b + sz + sy, b + sz + sy + 1};
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const S* A = f.c[c];
-      out[c] = trilin(w, T(widen(A[b])), T(widen(A[b + 1])), T(widen(A[b + sy])), T(widen(A[b + sy + 1])),
-                      T(widen(A[b + sz])), T(widen(A[b + sz + 1])), T(widen(A[b + sz + sy])),
-                      T(widen(A[b + sz + sy + 1])));
+      for (int e = 0; e < 8; e++) widen3(A[o[e]], v[e]);
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        out[c] = trilin(w, T(v[0][c]), T(v[1][c]), T(v[2][c]), T(v[3][c]), T(v[4][c]), T(v[5][c]), T(v[6][c]),
+                        T(v[7][c]));
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const S* A = f.c[c];
+        out[c] = trilin(w, T(widen(A[b])), T(widen(A[b + 1])), T(widen(A[b + sy])), T(widen(A[b + sy + 1])),
+                        T(widen(A[b + sz])), T(widen(A[b + sz + 1])), T(widen(A[b + sz + sy])),
+                        T(widen(A[b + sz + sy + 1])));
+      }
     }
   }
 };
@@ -285,8 +322,35 @@ struct IntSpline {
       out[0] = out[1] = out[2] = T(0);
       return;
     }
+    if constexpr (Packed<S>::value) {
+      // all three components from the packed rows, same sum order per
+      // component as spline_sum
+      const int sy = g.N[0], sz = g.N[0] * g.N[1];
+      T acc[3] = {T(0), T(0), T(0)};
 #pragma unroll
-    for (int c = 0; c < 3; c++) out[c] = spline_sum(g, s, f.c[c]);
+      for (int cz = 0; cz < KZ; cz++) {
+        T accy[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int by = 0; by < KX; by++) {
+          float row[KX][3];
+          load_row3<KX>(f.c[0] + s.base + by * sy + cz * sz, row);
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            T accx = T(0);
+#pragma unroll
+            for (int a = 0; a < KX; a++) accx += s.wx[a] * T(row[a][c]);
+            accy[c] += s.wy[by] * accx;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[c] += s.wz[cz] * accy[c];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) out[c] = acc[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) out[c] = spline_sum(g, s, f.c[c]);
+    }
   }
 };
 

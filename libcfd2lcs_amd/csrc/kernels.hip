@@ -226,8 +226,17 @@ __global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b,
 #pragma clang fp contract(off)
   const T w0 = T(1) - ct;
   for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < nt; l += (long long)gridDim.x * blockDim.x) {
+    if constexpr (Packed<S>::value) {
+      using E = typename Packed<S>::E;
+      S r;
 #pragma unroll
-    for (int c = 0; c < 3; c++) out.c[c][l] = narrow<S>(w0 * a.c[c][l] + ct * b.c[c][l]);
+      for (int c = 0; c < 3; c++) r.e[c] = narrow<E>(w0 * a.c[c][l] + ct * b.c[c][l]);
+      r.e[3] = E{0};
+      out.c[0][l] = r;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) out.c[c][l] = narrow<S>(w0 * a.c[c][l] + ct * b.c[c][l]);
+    }
   }
 }
 
@@ -736,8 +745,8 @@ void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a
     hipLaunchKernelGGL((k_blend<T, S>), dim3((unsigned)nb), dim3(kBlock), 0, s, o, a, b, ct, g.nt);
   };
   if constexpr (std::is_same<T, float>::value) {
-    if (storage == kStoreF16) return go(half_t{});
-    if (storage == kStoreBF16) return go(bf16_t{});
+    if (storage == kStoreF16) return go(half4_t{});
+    if (storage == kStoreBF16) return go(bf164_t{});
   }
   go(T{});
 }
@@ -746,8 +755,8 @@ void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a
 template <typename T, class F>
 void with_storage(int storage, F&& f) {
   if constexpr (std::is_same<T, float>::value) {
-    if (storage == kStoreF16) return f(half_t{});
-    if (storage == kStoreBF16) return f(bf16_t{});
+    if (storage == kStoreF16) return f(half4_t{});
+    if (storage == kStoreBF16) return f(bf164_t{});
   }
   f(T{});
 }

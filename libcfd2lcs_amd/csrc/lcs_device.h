@@ -58,12 +58,20 @@ struct DevStatus {
 };
 
 // Storage of the gathered stage velocities.  kStoreNative keeps the compute
-// type; the 16-bit formats (FP32 compute only) halve the bytes of every
-// velocity gather and are widened to FP32 in registers.
+// type, one array per component.  The 16-bit formats (FP32 compute only)
+// pack the three components of a node into one 8-byte record (u, v, w, 0),
+// so a node's velocity is a single aligned load and a 4-node spline row two
+// 16-byte loads (against three for the FP32 component arrays); values are
+// widened to FP32 in registers.
 enum Storage { kStoreNative = 0, kStoreF16 = 1, kStoreBF16 = 2 };
 struct half_t { uint16_t u; };  // IEEE binary16
 struct bf16_t { uint16_t u; };  // bfloat16 (upper half of a binary32)
-inline size_t storage_bytes(int storage, size_t native) { return storage == kStoreNative ? native : 2; }
+struct __attribute__((aligned(8))) half4_t { half_t e[4]; };
+struct __attribute__((aligned(8))) bf164_t { bf16_t e[4]; };
+// bytes of one stage velocity field (all three components) over nt nodes
+inline size_t stage_field_bytes(int storage, size_t native, size_t nt) {
+  return storage == kStoreNative ? 3 * native * nt : 8 * nt;
+}
 
 // Per-subcycle integration parameters (lp_motion_m.f90:129-360).
 template <typename T>

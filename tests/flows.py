@@ -56,3 +56,71 @@ def double_gyre_velocity(X, Y, t, A=0.1, eps=0.1, omega=2 * np.pi / 10):
     v = np.pi * A * np.cos(np.pi * f) * np.sin(np.pi * Y) * dfdx
     w = np.zeros_like(X)
     return u, v, w
+
+
+class RandomFourierField:
+    """Synthetic divergence-free turbulent velocity on the periodic
+    cell-centred grid x_i = (i + 1/2) 2 pi / n: a sum of Fourier modes k
+    (integer wavevectors, 0 < |k| <= kmax) with seeded uniform phases and
+    seeded random directions projected perpendicular to k, amplitudes from
+    E(k) ~ k^4 exp(-2 (k/4)^2) shared over each shell, the mode -k the
+    complex conjugate of k (a real field), normalised to unit RMS of |u| at
+    t = 0; each mode's phase advances at omega_k = |k|^(2/3).  Evaluated on
+    the device with an inverse FFT (torch, FP64); at(t) returns (u, v, w) of
+    shape (n, n, n) in C order of (k, j, i)."""
+
+    def __init__(self, n, kmax=42, seed=0, device="cuda:0"):
+        import torch
+        self.torch = torch
+        self.n = n
+        dev = torch.device(device)
+        rng = np.random.default_rng(seed)
+        k1 = np.fft.fftfreq(n, d=1.0 / n)
+        KZ, KY, KX = np.meshgrid(k1, k1, k1, indexing="ij")
+        kk = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
+        sel = (kk > 0) & (kk <= kmax)
+        # one representative of each +-k pair: first nonzero component > 0
+        half = (KX > 0) | ((KX == 0) & (KY > 0)) | ((KX == 0) & (KY == 0) & (KZ > 0))
+        rep = sel & half
+        idx = np.nonzero(rep)
+        kx, ky, kz, km = KX[idx], KY[idx], KZ[idx], kk[idx]
+        nm = kx.size
+        d = rng.standard_normal((nm, 3))
+        kv = np.stack([kx, ky, kz], 1) / km[:, None]
+        d -= (d * kv).sum(1)[:, None] * kv
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        phase = rng.uniform(0, 2 * np.pi, nm)
+        shell = np.rint(km).astype(int)
+        counts = np.bincount(shell, minlength=kmax + 2).astype(float)
+        E = km ** 4 * np.exp(-2.0 * (km / 4.0) ** 2)
+        amp = np.sqrt(E / counts[shell])
+        h = 2 * np.pi / n
+        # cell-centred offset e^{i k . h/2}
+        coef = amp * np.exp(1j * (phase + 0.5 * h * (kx + ky + kz)))
+        self.C = []
+        for c in range(3):
+            F = np.zeros((n, n, n), dtype=np.complex128)
+            F[idx] = coef * d[:, c]
+            mi = ((-idx[0]) % n, (-idx[1]) % n, (-idx[2]) % n)
+            F[mi] = np.conj(coef * d[:, c])
+            self.C.append(torch.tensor(F, device=dev))
+        mi = ((-idx[0]) % n, (-idx[1]) % n, (-idx[2]) % n)
+        # the phase of k advances by +omega t and that of its mirror -k by
+        # -omega t, so the field stays real
+        sgn = np.zeros((n, n, n))
+        sgn[idx] = 1.0
+        sgn[mi] = -1.0
+        self.omega = torch.tensor(sgn * np.where(sel, kk, 0.0) ** (2.0 / 3.0), device=dev)
+        self.scale = 1.0
+        u = self.at(0.0)
+        rms = float(torch.sqrt((u[0] ** 2 + u[1] ** 2 + u[2] ** 2).mean()))
+        self.scale = 1.0 / rms
+
+    def at(self, t):
+        torch = self.torch
+        rot = torch.exp(1j * self.omega * t)
+        out = []
+        for c in range(3):
+            f = torch.fft.ifftn(self.C[c] * rot).real * (self.n ** 3) * self.scale
+            out.append(f.contiguous())
+        return out
