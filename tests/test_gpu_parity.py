@@ -205,3 +205,13 @@ def test_16bit_storage_needs_fp32_compute(dp):
     assert dp.error != 0
     dp.finalize()
     assert dp.error == 0
+
+
+def test_double_gyre_512x256_walls_rk2_linear_fp64(dp):
+    """The reference's CPU configuration at its full grid size: double gyre
+    512 x 256 x 1 cell-centred on [0,2] x [0,1] with walls, dt = 0.01, RK2 +
+    LINEAR, FWD and BKWD, on a window shortened to T = 1 (h = 0.5) so that the
+    FP64 oracle finishes in seconds."""
+    recs, hs = run_both(dp, "gyre", n=(512, 256), integrator=F.RK2, interpolator=F.LINEAR, T=1.0, h=0.5,
+                        dt=0.01, nsteps=102, every_step=25)
+    assert _check(recs, hs, 1e-10, 1e-9) >= 4
