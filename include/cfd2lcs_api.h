@@ -108,6 +108,24 @@ void cfd2lcs_finalize_c(void);
 /* Sticky error flag (CFD2LCS_ERROR, data_m.f90:381): 0 when healthy. */
 int cfd2lcs_error_c(void);
 
+/* Output files.  As the reference does (write_lcs, io_nohdf5_m.f90:47-332),
+ * every FTLE output writes ./cfd2lcs_output/<label>_<index>.dat, a Tecplot
+ * ASCII file with X, Y, Z, FM-X..Z (option WRITE_FLOWMAP), U-X..Z
+ * (WRITE_VELOCITY), FLAG (WRITE_BCFLAG) and FTLE in BLOCK packing, ES14.6,
+ * ten values per line.  Files are written by a background thread from host
+ * copies of the results (each under <name>.part until complete);
+ * cfd2lcs_update_c returns without waiting for them.  This call waits until
+ * every file requested so far is on disk; it returns 0, or -1 when a write
+ * failed (the error flag is then set).  cfd2lcs_finalize_c waits as well. */
+int cfd2lcs_flush_output_c(void);
+
+/* The number formatting of those files: n values in the Fortran edit
+ * descriptor ESw.d (ES14.6 for data, ES18.4 for SOLUTIONTIME), each exactly
+ * w characters, concatenated into out (n*w bytes, no terminator).  Returns 0,
+ * or -1 for unsupported w, d.  Host-only; for tools that compare or produce
+ * such files. */
+int cfd2lcs_format_es_c(const double *v, long long n, int w, int d, char *out);
+
 /* Same as cfd2lcs_update_c, but u, v, w are device (HBM) pointers of n[0]*
  * n[1]*n[2] lcsdata_t each, Fortran order; no host staging is involved. */
 void cfd2lcs_update_device_c(int n[3], const lcsdata_t *d_u,

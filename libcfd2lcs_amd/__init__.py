@@ -35,7 +35,7 @@ EXPORTED = (
     "cfd2lcs_get_ftle_device_c", "cfd2lcs_get_noutput_c", "cfd2lcs_get_stats_c",
     "cfd2lcs_get_timers_c", "cfd2lcs_debug_grid_c",
     "cfd2lcs_adjoint_forward_c", "cfd2lcs_adjoint_backward_c", "cfd2lcs_adjoint_release_c",
-    "cfd2lcs_adjoint_stats_c",
+    "cfd2lcs_adjoint_stats_c", "cfd2lcs_flush_output_c", "cfd2lcs_format_es_c",
     "cfd2lcs_init_", "cfd2lcs_diagnostic_init_", "cfd2lcs_update_",
     "cfd2lcs_diagnostic_destroy_", "cfd2lcs_set_option_", "cfd2lcs_set_param_",
     "cfd2lcs_finalize_",
@@ -91,6 +91,10 @@ def load(variant="dp"):
     L.cfd2lcs_adjoint_backward_c.restype = C.c_int
     L.cfd2lcs_adjoint_release_c.argtypes = []
     L.cfd2lcs_adjoint_stats_c.argtypes = [C.POINTER(C.c_double)]
+    L.cfd2lcs_flush_output_c.argtypes = []
+    L.cfd2lcs_flush_output_c.restype = C.c_int
+    L.cfd2lcs_format_es_c.argtypes = [C.POINTER(C.c_double), C.c_longlong, C.c_int, C.c_int, C.c_char_p]
+    L.cfd2lcs_format_es_c.restype = C.c_int
     _libs[variant] = L
     return L
 
@@ -99,11 +103,29 @@ def _fortran(a, dtype):
     return np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F"))
 
 
+def format_es(values, w=14, d=6, variant="dp"):
+    """The Fortran ESw.d strings the output files use, one per value (host
+    only; no GPU needed)."""
+    L = load(variant)
+    v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).ravel())
+    out = C.create_string_buffer(max(1, v.size * w))
+    if L.cfd2lcs_format_es_c(v.ctypes.data_as(C.POINTER(C.c_double)), v.size, w, d, out) != 0:
+        raise ValueError(f"unsupported edit descriptor ES{w}.{d}")
+    raw = out.raw[:v.size * w].decode()
+    return [raw[i * w:(i + 1) * w] for i in range(v.size)]
+
+
 class Cfd2lcs:
     """The reference's API on one GPU.  The library is process-global like the
-    reference's module state; ``finalize`` resets it (cfd2lcs.f90:388-434)."""
+    reference's module state; ``finalize`` resets it (cfd2lcs.f90:388-434).
 
-    def __init__(self, variant="dp"):
+    The C library writes the reference's output files (``./cfd2lcs_output``)
+    at every FTLE output unless the option "WRITE_LCS" is LCS_FALSE; this
+    binding sets it to LCS_FALSE in ``init`` unless ``write_output`` is true,
+    since its callers read the results from memory."""
+
+    def __init__(self, variant="dp", write_output=False):
+        self.write_output = write_output
         self.variant = variant
         self.L = load(variant)
         self.dtype = np.float64 if variant == "dp" else np.float32
@@ -121,6 +143,8 @@ class Cfd2lcs:
         off = (C.c_int * 3)(*offset)
         xs, ys, zs = (_fortran(a, self.dtype) for a in (x, y, z))
         fl = _fortran(flag, np.int32)
+        if not self.write_output:
+            self.set_option("WRITE_LCS", LCS_FALSE)
         self.L.cfd2lcs_init_c(comm, n, off, self._rp(xs), self._rp(ys), self._rp(zs),
                               fl.ctypes.data_as(C.POINTER(C.c_int)))
 
@@ -148,6 +172,10 @@ class Cfd2lcs:
 
     def finalize(self):
         self.L.cfd2lcs_finalize_c()
+
+    def flush_output(self):
+        """Wait until every output file requested so far is written."""
+        return self.L.cfd2lcs_flush_output_c()
 
     # ---- additive ----
     @property

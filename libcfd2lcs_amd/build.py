@@ -47,6 +47,7 @@ def build(verbose=False):
         "particles": os.path.join(BUILD, "particles.o"),
         "adjoint": os.path.join(BUILD, "adjoint.o"),
         "grid": os.path.join(BUILD, "grid_host.o"),
+        "io": os.path.join(BUILD, "lcs_io.o"),
         "api_dp": os.path.join(BUILD, "api_dp.o"),
         "api_sp": os.path.join(BUILD, "api_sp.o"),
     }
@@ -57,6 +58,7 @@ def build(verbose=False):
         [HIPCC, *FLAGS, "-munsafe-fp-atomics", *inc, "-c", os.path.join(CSRC, "adjoint.hip"), "-o",
          objs["adjoint"]],
         [HIPCC, *FLAGS, *inc, "-c", os.path.join(CSRC, "grid_host.cpp"), "-o", objs["grid"]],
+        [HIPCC, *FLAGS, *inc, "-pthread", "-c", os.path.join(CSRC, "lcs_io.cpp"), "-o", objs["io"]],
         [HIPCC, *FLAGS, *inc, "-c", os.path.join(CSRC, "api.cpp"), "-o", objs["api_dp"]],
         [HIPCC, *FLAGS, *inc, "-DLCS_SINGLE", "-c", os.path.join(CSRC, "api.cpp"), "-o", objs["api_sp"]],
     ]
@@ -70,7 +72,7 @@ def build(verbose=False):
         tmp = out + ".tmp"
         links.append([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
                       objs[f"api_{tag}"], objs["kernels"], objs["particles"], objs["adjoint"],
-                      objs["grid"]])
+                      objs["grid"], objs["io"], "-pthread"])
     _run(links)
     for tag in ("dp", "sp"):
         out = os.path.join(LIB, f"libcfd2lcs_{tag}.so")

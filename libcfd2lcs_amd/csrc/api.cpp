@@ -19,6 +19,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #ifdef LCS_SINGLE
@@ -32,6 +33,7 @@ typedef double lcsdata_t;
 #include "../../include/cfd2lcs_api.h"
 #include "grid_host.h"
 #include "lcs_device.h"
+#include "lcs_io.h"
 
 using U = lcsdata_t;  // user-facing real type
 
@@ -52,6 +54,14 @@ struct Options {
   int particle_sort = 1;  // re-order forward particles by node every n updates (0: never)
   int vel_storage = kStoreNative;  // storage of the gathered stage velocities
   int compensated = 0;    // two-sum position/displacement updates of forward particles
+  // output files: FLOWMAP_IO, VELOCITY_IO, BCFLAG_IO (data_m.f90:44-54) pick
+  // the variables of the Tecplot FTLE file; write_lcs switches that file
+  // (always written by the reference) and write_slots the flowmap/velocity
+  // slot files that the reference keeps in ./cfd2lcs_tmp (here the slots
+  // live in HBM, so the files are optional)
+  int flowmap_io = 1, velocity_io = 1, bcflag_io = 0;
+  int write_lcs = 1;
+  int write_slots = 0;
   bool warn_opt = false, warn_par = false;  // SAVEd warn flags (cfd2lcs.f90:444,632)
 };
 
@@ -138,6 +148,7 @@ struct EngineBase {
   virtual int adjoint_backward(const double* d_dftle, U* gu, U* gv, U* gw) = 0;
   virtual void adjoint_release() = 0;
   virtual void adjoint_stats(double out[4]) = 0;
+  virtual int flush_output() = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -206,7 +217,20 @@ struct Engine : EngineBase {
   };
   std::vector<std::unique_ptr<Diag>> diags;
 
+  // output files (write_lcs, write_flowmap_substep)
+  std::unique_ptr<io::Writer> writer;
+  std::shared_ptr<std::vector<U>> out_xyz;   // interior coordinates, 3 * np
+  std::shared_ptr<std::vector<U>> out_flag;  // interior bcflag as reals, np
+  // velocity at h boundaries by slot step, shared by all diagnostics as the
+  // reference's ./cfd2lcs_tmp/<rank>_U_NP1_<step>.dat files are
+  // (flowmap_m.f90:138-151), read back for the forward FTLE's U columns
+  // (flowmap_m.f90:212-247)
+  std::vector<DevBuf> uring;
+  std::vector<char> uring_ok;
+  DevBuf vint;  // interior scratch, 3 * np
+
   ~Engine() override {
+    if (writer) writer->flush();
     if (stream) {
       (void)hipStreamSynchronize(stream);
       for (auto& t : pending) {
@@ -813,6 +837,9 @@ struct Engine : EngineBase {
     }
     dg.slot_ok[step] = 1;
     dg.slot_coef_ok[step] = 0;
+    if (!io_ok()) return;
+    if (g_opt.write_slots && !write_slot_files(dg, step)) return;
+    if (g_opt.write_lcs && g_opt.velocity_io && any_fwd() && !record_velocity(step)) return;
     const bool spl = is_spline(interp);
     if (spl) {
       if (!dg.slotcoef.p && !dg.slotcoef.alloc((size_t)dg.nslot * 3 * fb)) {
@@ -861,8 +888,207 @@ struct Engine : EngineBase {
     // the order list must stay valid until the kernel ran: wait here
     HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipMemcpy(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost));
-    if (st_host->err & kErrTrack) fail("cannot locate all particles after the maximum number of tracking iterations");
-    else if (st_host->err) fail("device error during composition");
+    if (st_host->err & kErrTrack) {
+      fail("cannot locate all particles after the maximum number of tracking iterations");
+      return;
+    } else if (st_host->err) {
+      fail("device error during composition");
+      return;
+    }
+    if (g_opt.write_lcs) write_lcs_file(dg, step);
+  }
+
+  // ---- output files ----------------------------------------------------
+  io::Writer& wr() {
+    if (!writer) writer = std::make_unique<io::Writer>(2);
+    return *writer;
+  }
+  // a failure of an earlier asynchronous write sets the error flag here
+  bool io_ok() {
+    std::string m;
+    if (writer && writer->failed(&m)) {
+      fail(m);
+      return false;
+    }
+    return true;
+  }
+  int flush_output() override {
+    if (writer) writer->flush();
+    return io_ok() ? 0 : -1;
+  }
+  bool any_fwd() const {
+    for (auto& d : diags)
+      if (d->dir > 0) return true;
+    return false;
+  }
+  // n device values of T (on the stream) as a host array of lcsdata_t
+  std::shared_ptr<std::vector<U>> host_copy(const T* d, size_t n) {
+    auto h = std::make_shared<std::vector<U>>(n);
+    if (std::is_same<T, U>::value) {
+      if (hipMemcpyAsync(h->data(), d, n * sizeof(T), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+          hipStreamSynchronize(stream) != hipSuccess)
+        return nullptr;
+    } else {
+      std::vector<T> t(n);
+      if (hipMemcpyAsync(t.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+          hipStreamSynchronize(stream) != hipSuccess)
+        return nullptr;
+      for (size_t a = 0; a < n; a++) (*h)[a] = (U)t[a];
+    }
+    return h;
+  }
+  // interior of a ghosted 3-component field, on the host
+  std::shared_ptr<std::vector<U>> host_interior(CF3<T> src) {
+    if (!vint.p && !vint.alloc(3 * sizeof(T) * (size_t)g.np)) return nullptr;
+    launch_interior(g, src, p3(vint), stream);
+    return host_copy((const T*)vint.p, 3 * (size_t)g.np);
+  }
+  static void add3(io::Job& j, const std::shared_ptr<std::vector<U>>& h, size_t np) {
+    for (int d = 0; d < 3; d++) j.arrays.push_back(io::Array{h->data() + d * np, np, sizeof(U) == 8});
+    j.keep.push_back(h);
+  }
+  // the reference's file names are character(len=128) variables
+  static std::string fname128(const std::string& s) { return s.size() > 128 ? s.substr(0, 128) : s; }
+
+  // write_flowmap_substep (flowmap_m.f90:106-156): the h-interval flowmap of
+  // slot `step` and, with VELOCITY_IO, u_np1, as unformatted files
+  bool write_slot_files(Diag& dg, int step) {
+    std::string e;
+    if (!io::ensure_dir("cfd2lcs_tmp", e)) {
+      fail(e);
+      return false;
+    }
+    const size_t np = (size_t)g.np;
+    T* sp = slot_ptr(dg.slots, step);
+    auto h = host_interior(CF3<T>{{sp, sp + g.nt, sp + 2 * g.nt}});
+    if (!h) {
+      fail("copy of a flowmap slot to the host failed");
+      return false;
+    }
+    io::Job j;
+    j.kind = io::Job::kUnformatted;
+    j.path = fname128("./cfd2lcs_tmp/" + io::i4(0) + "_" + dg.label + "-LP-FM_" + io::i4(step) + ".dat");
+    add3(j, h, np);
+    wr().submit(std::move(j));
+    if (g_opt.velocity_io) {
+      auto hv = host_interior(cf3(vel[cur]));
+      if (!hv) {
+        fail("copy of the velocity to the host failed");
+        return false;
+      }
+      io::Job jv;
+      jv.kind = io::Job::kUnformatted;
+      jv.path = fname128("./cfd2lcs_tmp/" + io::i4(0) + "_U_NP1_" + io::i4(step) + ".dat");
+      add3(jv, hv, np);
+      wr().submit(std::move(jv));
+    }
+    return true;
+  }
+
+  bool record_velocity(int step) {
+    if ((int)uring.size() <= step) {
+      uring.resize((size_t)step + 1);
+      uring_ok.resize((size_t)step + 1, 0);
+    }
+    if (!uring[step].p && !uring[step].alloc(3 * sizeof(T) * (size_t)g.np)) {
+      fail("hipMalloc failed (velocity ring for the FTLE output)");
+      return false;
+    }
+    launch_interior(g, cf3(vel[cur]), p3(uring[step]), stream);
+    uring_ok[step] = 1;
+    return true;
+  }
+
+  // write_lcs + write_ftle (io_nohdf5_m.f90:47-265): X, Y, Z, [FM-X..Z],
+  // [U-X..Z], [FLAG], FTLE of the latest composition
+  void write_lcs_file(Diag& dg, int step) {
+    const size_t np = (size_t)g.np;
+    long long findex = std::llround((double)(U)(t_np1 / dg.h));
+    if (dg.dir > 0) findex -= std::llround((double)(U)(dg.Tint / dg.h));
+    std::string fs;
+    if (!io::findex_string(findex, fs)) {
+      fail("unsupported range for file index, " + std::to_string(findex));
+      return;
+    }
+    std::string e;
+    if (!io::ensure_dir("cfd2lcs_output", e)) {
+      fail(e);
+      return;
+    }
+    if (!out_xyz) {
+      auto v = std::make_shared<std::vector<U>>(3 * np);
+      auto f = std::make_shared<std::vector<U>>(np);
+      for (int k = 1; k <= g.n[2]; k++)
+        for (int j = 1; j <= g.n[1]; j++)
+          for (int i = 1; i <= g.n[0]; i++) {
+            const long long p = hg.pid(i, j, k), l = hg.id(i, j, k);
+            for (int d = 0; d < 3; d++) (*v)[d * np + p] = (U)hg.g[d][l];
+            (*f)[p] = (U)hg.flag[l];
+          }
+      out_xyz = v;
+      out_flag = f;
+    }
+    io::Job j;
+    j.kind = io::Job::kTecplot;
+    j.path = fname128("./cfd2lcs_output/" + dg.label + "_" + fs + ".dat");
+    const std::string ft = fname128(dg.label + "-FTLE");
+    std::string hd = "TITLE = \"" + ft + "\"\nVARIABLES =\n\"X\"\n\"Y\"\n\"Z\"\n";
+    add3(j, out_xyz, np);
+    if (g_opt.flowmap_io) {
+      hd += "\"FM-X\"\n\"FM-Y\"\n\"FM-Z\"\n";
+      auto h = host_copy((const T*)dg.comp.p, 3 * np);
+      if (!h) {
+        fail("copy of the composed flowmap to the host failed");
+        return;
+      }
+      add3(j, h, np);
+    }
+    if (g_opt.velocity_io) {
+      hd += "\"U-X\"\n\"U-Y\"\n\"U-Z\"\n";
+      // lp%ugrid after reconstruct_flowmap: the velocity read back from slot
+      // `step` (BKWD, just written: u_np1) or step + 1 (FWD: the oldest
+      // snapshot of the ring); zero when that slot was never written
+      std::shared_ptr<std::vector<U>> h;
+      if (dg.dir < 0) {
+        h = host_interior(cf3(vel[cur]));
+      } else {
+        const int vs = step + 1;
+        if (vs < (int)uring.size() && uring_ok[vs]) h = host_copy((const T*)uring[vs].p, 3 * np);
+        else h = std::make_shared<std::vector<U>>(3 * np, (U)0);
+      }
+      if (!h) {
+        fail("copy of the velocity to the host failed");
+        return;
+      }
+      add3(j, h, np);
+    }
+    if (g_opt.bcflag_io) {
+      hd += "\"FLAG\"\n";
+      j.arrays.push_back(io::Array{out_flag->data(), np, sizeof(U) == 8});
+      j.keep.push_back(out_flag);
+    }
+    hd += "\"FTLE\"\nZONE\nT = \"" + ft + "\"\n";
+    char es[18];
+    io::format_es((double)t_np1, 18, 4, es);
+    hd += "STRANDID = 0000000001, SOLUTIONTIME = " + std::string(es, 18) + "\n";
+    char ijk[96];
+    std::snprintf(ijk, sizeof ijk, "I = %010d J = %010d K = %010d\n", g.n[0], g.n[1], g.n[2]);
+    hd += ijk;
+    hd += "ZONETYPE = Ordered, DATAPACKING = BLOCK\n";
+    j.header = hd;
+    {
+      std::vector<double> f(np);
+      if (hipMemcpyAsync(f.data(), dg.ftle.p, np * sizeof(double), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+          hipStreamSynchronize(stream) != hipSuccess) {
+        fail("copy of the FTLE field to the host failed");
+        return;
+      }
+      auto h = std::make_shared<std::vector<U>>(np);
+      for (size_t a = 0; a < np; a++) (*h)[a] = (U)f[a];
+      j.arrays.push_back(io::Array{h->data(), np, sizeof(U) == 8});
+      j.keep.push_back(h);
+    }
+    wr().submit(std::move(j));
   }
 
   bool sync_status() {
@@ -1456,7 +1682,11 @@ void cfd2lcs_set_option_c(char option[], int val) {
   if (o == "SYNCTIMER") boolopt(g_opt.sync_timer);
   else if (o == "DEBUG") boolopt(g_opt.verbose);
   else if (o == "INCOMPRESSIBLE") boolopt(g_opt.incompressible);
-  else if (o == "WRITE_FLOWMAP" || o == "WRITE_VELOCITY" || o == "WRITE_BCFLAG") boolopt(dummy);
+  else if (o == "WRITE_FLOWMAP") boolopt(g_opt.flowmap_io);
+  else if (o == "WRITE_VELOCITY") boolopt(g_opt.velocity_io);
+  else if (o == "WRITE_BCFLAG") boolopt(g_opt.bcflag_io);
+  else if (o == "WRITE_LCS") boolopt(g_opt.write_lcs);
+  else if (o == "WRITE_SLOTS") boolopt(g_opt.write_slots);
   else if (o == "VELOCITY_INVARIANTS" || o == "AUX_GRID") {
     boolopt(dummy);
     if (val == LCS_TRUE) {
@@ -1528,6 +1758,20 @@ void cfd2lcs_finalize_c(void) {
 }
 
 int cfd2lcs_error_c(void) { return g_error; }
+
+int cfd2lcs_flush_output_c(void) {
+  if (!g_eng) return g_error ? -1 : 0;
+  return g_eng->flush_output();
+}
+
+int cfd2lcs_format_es_c(const double* v, long long n, int w, int d, char* out) {
+  if (!v || !out || n < 0 || w < 1 || w > 64 || d < 0 || d > 30) return -1;
+  for (long long a = 0; a < n; a++) {
+    if (w == 14 && d == 6) io::format_es14_6(v[a], out + a * w);
+    else io::format_es(v[a], w, d, out + a * w);
+  }
+  return 0;
+}
 
 int cfd2lcs_get_flowmap_c(int lcs_handle, lcsdata_t* fx, lcsdata_t* fy, lcsdata_t* fz) {
   if (!g_eng) return 1;
