@@ -670,8 +670,9 @@ struct Engine : EngineBase {
       fail("VELOCITY_STORAGE = LCS_FP16 / LCS_BF16 needs PRECISION = LCS_FP32");
       return;
     }
-    if (interp != kNearest && interp != kLinear && interp != kQuadratic && interp != kCubic) {
-      fail("INTERPOLATOR TSE/TSE_LIMIT is not supported by this build");
+    if (interp != kNearest && interp != kLinear && interp != kQuadratic && interp != kCubic && interp != kTSE &&
+        interp != kTSELimit) {
+      fail("Unrecognized interpolator");
       return;
     }
     if (g_opt.integrator < kEuler || g_opt.integrator > kRK4) {
@@ -1258,7 +1259,7 @@ struct Engine : EngineBase {
       return 1;
     }
     if (interp != kNearest && interp != kLinear && interp != kQuadratic && interp != kCubic) {
-      fail("INTERPOLATOR TSE/TSE_LIMIT is not supported by this build");
+      fail("the adjoint supports INTERPOLATOR NEAREST_NBR, LINEAR, QUADRATIC and CUBIC");
       return 1;
     }
     for (int s = 1; s < nsnap; s++)

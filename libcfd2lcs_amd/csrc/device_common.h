@@ -280,6 +280,69 @@ struct IntNearest {
   }
 };
 
+// TSE / TSE_LIMIT (lp_tracking_m.f90:526-654): first-order Taylor expansion
+// about the node, phi_no + grad(phi)_no . (x - x_no), with the central
+// difference gradient of grad_sr1 (gradient_m.f90:11-66: ghosted
+// coordinates, the column of an axis with one node zeroed); LIMIT clamps the
+// result to the min/max of phi over the node's 3x3x3 neighbourhood.  The
+// gradient is formed at the node on the fly (7 or 27 gathers per component)
+// instead of as a 9-component field.
+template <typename T, bool LIMIT>
+struct IntTSE {
+  template <typename S>
+  __device__ static void at(const CF3<S>& f, int l, T v[3]) {
+    if constexpr (Packed<S>::value) {
+      float w[3];
+      widen3(f.c[0][l], w);
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[c] = T(w[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[c] = T(widen(f.c[c][l]));
+    }
+  }
+  template <typename S>
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
+                             unsigned long long&) const {
+    const int ix[3] = {iclamp(nd[0], 1, g.n[0]), iclamp(nd[1], 1, g.n[1]), iclamp(nd[2], 1, g.n[2])};
+    const int st[3] = {1, g.N[0], g.N[0] * g.N[1]};
+    const int l = gid(g, ix[0], ix[1], ix[2]);
+    T r[3];
+    at(f, l, r);
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c] = r[c];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if (g.n[d] == 1) continue;  // zeroed gradient column
+      const T* G = g.gl[d];
+      const T den = G[ix[d] + 1] - G[ix[d] - 1];
+      const T dl = q[d] - G[ix[d]];
+      T a[3], b[3];
+      at(f, l + st[d], a);
+      at(f, l - st[d], b);
+#pragma unroll
+      for (int c = 0; c < 3; c++) out[c] = out[c] + dl * ((a[c] - b[c]) / den);
+    }
+    if constexpr (LIMIT) {
+      T mn[3] = {r[0], r[1], r[2]}, mx[3] = {r[0], r[1], r[2]};
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+          for (int dx = -1; dx <= 1; dx++) {
+            T v[3];
+            at(f, l + dx + dy * st[1] + dz * st[2], v);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              mn[c] = fmin(mn[c], v[c]);
+              mx[c] = fmax(mx[c], v[c]);
+            }
+          }
+#pragma unroll
+      for (int c = 0; c < 3; c++) out[c] = fmin(fmax(out[c], mn[c]), mx[c]);
+    }
+  }
+};
+
 template <typename T>
 struct IntLinear {
   template <typename S>

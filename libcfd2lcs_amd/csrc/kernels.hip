@@ -771,6 +771,10 @@ void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, C
       hipLaunchKernelGGL((k_sl_step<T, S, IntLinear<T>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
     } else if (interp == kNearest) {
       hipLaunchKernelGGL((k_sl_step<T, S, IntNearest<T>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+    } else if (interp == kTSE) {
+      hipLaunchKernelGGL((k_sl_step<T, S, IntTSE<T, false>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+    } else if (interp == kTSELimit) {
+      hipLaunchKernelGGL((k_sl_step<T, S, IntTSE<T, true>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
     } else {
       dispatch_spline(g, [&](auto KX, auto KZ) {
         constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
@@ -792,6 +796,10 @@ void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, 
         hipLaunchKernelGGL((k_fwd_step<T, S, IntLinear<T>, cm>), grid, blk, 0, s, g, sp, ps, st);
       } else if (interp == kNearest) {
         hipLaunchKernelGGL((k_fwd_step<T, S, IntNearest<T>, cm>), grid, blk, 0, s, g, sp, ps, st);
+      } else if (interp == kTSE) {
+        hipLaunchKernelGGL((k_fwd_step<T, S, IntTSE<T, false>, cm>), grid, blk, 0, s, g, sp, ps, st);
+      } else if (interp == kTSELimit) {
+        hipLaunchKernelGGL((k_fwd_step<T, S, IntTSE<T, true>, cm>), grid, blk, 0, s, g, sp, ps, st);
       } else {
         dispatch_spline(g, [&](auto KX, auto KZ) {
           constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
@@ -817,6 +825,12 @@ void launch_compose(const GridDev<T>& g, int interp, const T* slots, long long s
     hipLaunchKernelGGL((k_compose<T, IntLinear<T>>), grid, blk, 0, s, g, slots, stride, order, nslot, dxout, st);
   } else if (interp == kNearest) {
     hipLaunchKernelGGL((k_compose<T, IntNearest<T>>), grid, blk, 0, s, g, slots, stride, order, nslot, dxout, st);
+  } else if (interp == kTSE) {
+    hipLaunchKernelGGL((k_compose<T, IntTSE<T, false>>), grid, blk, 0, s, g, slots, stride, order, nslot, dxout,
+                       st);
+  } else if (interp == kTSELimit) {
+    hipLaunchKernelGGL((k_compose<T, IntTSE<T, true>>), grid, blk, 0, s, g, slots, stride, order, nslot, dxout,
+                       st);
   } else {
     dispatch_spline(g, [&](auto KX, auto KZ) {
       constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;

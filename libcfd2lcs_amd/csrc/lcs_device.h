@@ -13,7 +13,7 @@ namespace lcs {
 
 enum BcFlag { kInternal = 0, kInflow = 1, kOutflow = 2, kWall = 3, kSlip = 4, k2D = 5, kMask = 6 };
 enum Integrator { kEuler = 0, kTrapezoidal = 1, kRK2 = 2, kRK3 = 3, kRK4 = 4 };
-enum Interpolator { kNearest = 0, kLinear = 1, kQuadratic = 2, kCubic = 3 };
+enum Interpolator { kNearest = 0, kLinear = 1, kQuadratic = 2, kCubic = 3, kTSE = 5, kTSELimit = 6 };
 
 // device error bits (folded into the sticky error flag on the host)
 enum DevErr { kErrBcFlag = 1, kErrTrack = 2, kErrSpline = 4 };

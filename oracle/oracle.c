@@ -883,6 +883,45 @@ static void spline_eval(ocfd *s, int interp, const double p[3], double out[3]) {
   }
 }
 
+/* interp_s2u_r1 TSE / TSE_LIMIT (lp_tracking_m.f90:526-654): the value at
+ * the node plus grad(f) at the node dotted with (q - x_node), the gradient
+ * being grad_sr1's central difference over the ghosted grid coordinates
+ * (gradient_m.f90:11-66, columns of one-node axes zeroed); TSE_LIMIT clamps
+ * to the min/max of f over the 3x3x3 neighbourhood (594-643). */
+static void interp_tse(const ocfd *s, double *const f[3], const double q[3], const int n3[3], int limit,
+                       double o[3]) {
+  const ogrid *g = &s->g;
+  const int i = iclamp(n3[0], 1, g->n[0]), j = iclamp(n3[1], 1, g->n[1]), k = iclamp(n3[2], 1, g->n[2]);
+  const long l = GID(g, i, j, k);
+  const long xp = GID(g, i + 1, j, k), xm = GID(g, i - 1, j, k);
+  const long yp = GID(g, i, j + 1, k), ym = GID(g, i, j - 1, k);
+  const long zp = GID(g, i, j, k + 1), zm = GID(g, i, j, k - 1);
+  const double dx = q[0] - g->g[0][l], dy = q[1] - g->g[1][l], dz = q[2] - g->g[2][l];
+  for (int c = 0; c < 3; c++) {
+    const double *F = f[c];
+    double gx = (F[xp] - F[xm]) / (g->g[0][xp] - g->g[0][xm]);
+    double gy = (F[yp] - F[ym]) / (g->g[1][yp] - g->g[1][ym]);
+    double gz = (F[zp] - F[zm]) / (g->g[2][zp] - g->g[2][zm]);
+    if (g->n[0] == 1) gx = 0.0;
+    if (g->n[1] == 1) gy = 0.0;
+    if (g->n[2] == 1) gz = 0.0;
+    double v = F[l] + dx * gx + dy * gy + dz * gz;
+    if (limit) {
+      double mn = F[l], mx = F[l];
+      for (int c3 = -1; c3 <= 1; c3++)
+        for (int b = -1; b <= 1; b++)
+          for (int a = -1; a <= 1; a++) {
+            const double w = F[GID(g, i + a, j + b, k + c3)];
+            if (w < mn) mn = w;
+            if (w > mx) mx = w;
+          }
+      if (v < mn) v = mn;
+      if (v > mx) v = mx;
+    }
+    o[c] = v;
+  }
+}
+
 /* Interpolate field f at all particle positions pos using node array nd.
  * For spline interpolators the field is fitted once (as the reference does
  * on every call, lp_tracking_m.f90:509-511). */
@@ -895,7 +934,7 @@ static int interp_field(ocfd *s, double *const f[3], double *const pos[3],
     int e = spline_fit(s, f, it);
     if (e) return e;
   }
-  if (it != NEAREST_NBR && it != LINEAR && it != QUADRATIC && it != CUBIC) return 1;
+  if (it != NEAREST_NBR && it != LINEAR && it != QUADRATIC && it != CUBIC && it != TSE && it != TSE_LIMIT) return 1;
   #pragma omp parallel for schedule(static)
   for (long p = 0; p < np; p++) {
     double q[3] = {pos[0][p], pos[1][p], pos[2][p]}, o[3];
@@ -908,6 +947,8 @@ static int interp_field(ocfd *s, double *const f[3], double *const pos[3],
       }
     } else if (it == LINEAR) {
       interp_linear(s, f, q, n3, o);
+    } else if (it == TSE || it == TSE_LIMIT) {
+      interp_tse(s, f, q, n3, it == TSE_LIMIT, o);
     } else {
       spline_eval(s, it, q, o);
     }

@@ -69,6 +69,16 @@ def test_gyre_slip_rk3_cubic_fp64(dp):
     _check(recs, hs, 1e-10, 1e-9)
 
 
+@pytest.mark.parametrize("interp", [F.TSE, F.TSE_LIMIT])
+@pytest.mark.parametrize("case,n", [("abc", 12), ("gyre", (48, 24))])
+def test_tse_fp64(dp, interp, case, n):
+    """Taylor-series interpolation (lp_tracking_m.f90:526-654) on the periodic
+    ABC box and on the walled double gyre."""
+    recs, hs = run_both(dp, case, n=n, integrator=F.RK2, interpolator=interp, T=0.6, h=0.2, dt=0.05,
+                        nsteps=16)
+    assert _check(recs, hs, 1e-10, 1e-9) >= 4
+
+
 def test_abc_rk4_cubic_fp64_multibrick(dp):
     """36^3: more than one workgroup per grid row and a partial last block."""
     recs, hs = run_both(dp, "abc", n=36, integrator=F.RK4, interpolator=F.CUBIC, T=0.2, h=0.1,
@@ -84,7 +94,7 @@ def sp():
     lib.finalize()
 
 
-@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC])
+@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC, F.TSE_LIMIT])
 def test_abc_fp32_against_fp64_oracle(sp, interp):
     """FP32 library (lcsdata_t = float, FP32 compute) against the FP64
     oracle: flowmaps within 2e-5 relative, FTLE within 1e-3 absolute."""

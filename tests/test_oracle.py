@@ -163,7 +163,7 @@ def test_wall_fakes_flags_and_normals_2d():
 
 # ---------------------------------------------------------------- closed forms
 @pytest.mark.parametrize("integ", [F.EULER, F.TRAPEZOIDAL, F.RK2, F.RK3, F.RK4])
-@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC])
+@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC, F.TSE, F.TSE_LIMIT])
 def test_uniform_flow_displacements(integ, interp):
     """u = c: every scheme moves by c dt per step; T-flowmaps are c T (FWD) and
     -c T (BKWD); forward particles wrap the periodic box."""
@@ -206,10 +206,13 @@ def amplification(integ, z):
 
 @pytest.mark.parametrize("integ", [F.EULER, F.RK2, F.RK3, F.RK4])
 @pytest.mark.parametrize("direction", [F.FTLE_FWD, F.FTLE_BKWD])
-def test_linear_strain_flowmap_and_ftle(integ, direction):
-    """u = (lam x, -lam y, 0): LINEAR interpolation is exact, so every node
-    whose trajectory stays inside the domain carries fm = (R^N - 1) x with R
-    the scheme's amplification factor, and the displacement-gradient FTLE is
+@pytest.mark.parametrize("interp", [F.LINEAR, F.TSE, F.TSE_LIMIT])
+def test_linear_strain_flowmap_and_ftle(integ, direction, interp):
+    """u = (lam x, -lam y, 0): LINEAR and TSE interpolation are exact (TSE's
+    central-difference gradient is exact for linear data, and the value stays
+    inside the limiter's 3x3x3 range), so every node whose trajectory stays
+    inside the domain carries fm = (R^N - 1) x with R the scheme's
+    amplification factor, and the displacement-gradient FTLE is
     log(max|R^N - 1|)/T (lcs_m.f90:137-176)."""
     lam, T, h, dt = 0.8, 0.5, 0.25, 0.05
     X, Y, Z = F.cell_grid((60, 60, 1), (-1, -1, 0), (1, 1, 1))
@@ -218,6 +221,7 @@ def test_linear_strain_flowmap_and_ftle(integ, direction):
     o = Oracle(X, Y, Z, fl)
     hd = o.diagnostic_init(direction, 0, T, h, "s")
     o.set_option("INTEGRATOR", integ)
+    o.set_option("INTERPOLATOR", interp)
     u, v, w = lam * X, -lam * Y, 0 * X
     nsteps = 16
     for s in range(nsteps):
@@ -240,6 +244,42 @@ def test_linear_strain_flowmap_and_ftle(integ, direction):
     ref = np.log(max(abs(cx), abs(cy))) / T
     assert np.abs(ftle[m] - ref).max() < 1e-10
     o.close()
+
+
+@pytest.mark.parametrize("interp", [F.TSE, F.TSE_LIMIT])
+def test_tse_equals_linear_on_general_linear_field(interp):
+    """u = M x with a general (non-symmetric, all entries nonzero) M on a 3-D
+    walled grid: trilinear and Taylor (TSE) interpolation both reproduce
+    linear data exactly, so away from the non-linear ghost layer the TSE
+    flowmaps equal the LINEAR ones.  Any dropped, transposed or mis-signed
+    gradient term breaks the equality."""
+    M = np.array([[0.3, -0.7, 0.4], [0.5, -0.2, -0.6], [-0.45, 0.35, 0.1]])
+    X, Y, Z = F.cell_grid((18, 16, 14), (-1, -1, -1), (1, 1, 1))
+    fl = np.full(X.shape, F.LCS_INTERNAL, dtype=np.int32)
+    for sl in ((0,), (-1,), (slice(None), 0), (slice(None), -1), (slice(None), slice(None), 0),
+               (slice(None), slice(None), -1)):
+        fl[sl] = F.LCS_SLIP
+    u = M[0, 0] * X + M[0, 1] * Y + M[0, 2] * Z
+    v = M[1, 0] * X + M[1, 1] * Y + M[1, 2] * Z
+    w = M[2, 0] * X + M[2, 1] * Y + M[2, 2] * Z
+    res = {}
+    for it in (F.LINEAR, interp):
+        o = Oracle(X, Y, Z, fl)
+        hf = o.diagnostic_init(F.FTLE_FWD, 0, 0.1, 0.05, "f")
+        hb = o.diagnostic_init(F.FTLE_BKWD, 0, 0.1, 0.05, "b")
+        o.set_option("INTEGRATOR", F.RK4)
+        o.set_option("INTERPOLATOR", it)
+        for s in range(4):
+            o.update(u, v, w, s * 0.025)
+            assert o.error == 0
+        assert o.noutput(hf) == 1 and o.noutput(hb) == 1
+        res[it] = (o.flowmap(hf), o.flowmap(hb), o.composed(hf), o.composed(hb))
+        o.close()
+    m = (np.abs(X) < 0.4) & (np.abs(Y) < 0.4) & (np.abs(Z) < 0.4)
+    for a, b in zip(res[F.LINEAR], res[interp]):
+        for d in range(3):
+            assert np.abs(a[d][m]).max() > 1e-3
+            assert np.abs(a[d][m] - b[d][m]).max() < 1e-13
 
 
 def test_error_paths():
