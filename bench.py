@@ -140,6 +140,77 @@ def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None):
     return res
 
 
+def run_output_cost(n, intervals=3, dt=0.02, T=5.0, h=0.5, outdir=None):
+    """Wall time per h interval of the FP32 FWD+BKWD pipeline (ABC, RK4,
+    cubic) without and with the reference's Tecplot FTLE files (X, Y, Z,
+    FM, U, FTLE per output and direction) written to outdir, plus the time
+    the writer still needs after the last update (its backlog)."""
+    import shutil
+    import tempfile
+
+    import torch
+
+    import libcfd2lcs_amd as L
+
+    X, Y, Z = abc_grid(n)
+    flag = np.zeros(X.shape, dtype=np.int32)
+    dev = torch.device("cuda:0")
+    ax = torch.tensor(X[:, 0, 0], dtype=torch.float64, device=dev)
+    zz, yy, xx = torch.meshgrid(ax, ax, ax, indexing="ij")
+    A, B, Cc = np.sqrt(3.0), np.sqrt(2.0), 1.0
+    u = (A * torch.sin(zz) + Cc * torch.cos(yy)).float().contiguous()
+    v = (B * torch.sin(xx) + A * torch.cos(zz)).float().contiguous()
+    w = (Cc * torch.sin(yy) + B * torch.cos(xx)).float().contiguous()
+    del xx, yy, zz
+    base = outdir or tempfile.gettempdir()
+    cwd = os.getcwd()
+    res = {"n": n, "intervals": intervals}
+    for write in (False, True):
+        wd = tempfile.mkdtemp(prefix="lcs_out_", dir=base)
+        os.chdir(wd)
+        try:
+            lib = L.Cfd2lcs("sp", write_output=write)
+            lib.finalize()
+            lib.init(X, Y, Z, flag)
+            hf = lib.diagnostic_init(L.FTLE_FWD, 0, T, h, "fwdFTLE")
+            hb = lib.diagnostic_init(L.FTLE_BKWD, 0, T, h, "bkwdFTLE")
+            lib.set_option("INTEGRATOR", L.RK4)
+            lib.set_option("INTERPOLATOR", L.CUBIC)
+            step, marks = 0, []
+            while len(marks) < intervals + 1:
+                before = lib.noutput(hb)
+                lib.update_device(u.data_ptr(), v.data_ptr(), w.data_ptr(), step * dt)
+                step += 1
+                if lib.error:
+                    raise RuntimeError("update failed in the output-cost run")
+                if lib.noutput(hb) > before:
+                    torch.cuda.synchronize()
+                    marks.append(time.perf_counter())
+            t0 = time.perf_counter()
+            if lib.flush_output() != 0:
+                raise RuntimeError("writing the output files failed")
+            flush = time.perf_counter() - t0
+            per = (marks[-1] - marks[0]) / intervals
+            nbytes = sum(os.path.getsize(os.path.join(wd, "cfd2lcs_output", f))
+                         for f in os.listdir(os.path.join(wd, "cfd2lcs_output"))) if write else 0
+            lib.finalize()
+        finally:
+            os.chdir(cwd)
+            shutil.rmtree(wd, ignore_errors=True)
+        key = "with_files" if write else "without_files"
+        res[key] = {"interval_s": per, "flush_s": flush, "file_bytes": nbytes}
+        print(f"[bench] output cost {key}: {per:.3f} s per h interval, flush {flush:.3f} s, "
+              f"{nbytes / 1e9:.2f} GB", file=sys.stderr, flush=True)
+    res["overhead"] = res["with_files"]["interval_s"] / res["without_files"]["interval_s"] - 1.0
+    nfiles = 2 * (intervals + 1)
+    res["bytes_per_file"] = res["with_files"]["file_bytes"] / nfiles
+    res["note"] = ("per h interval = %d updates at %d^3 with both directions; each output writes one "
+                   "Tecplot file per direction (10 variables, ES14.6)" % (int(round(h / dt)), n))
+    del u, v, w
+    torch.cuda.empty_cache()
+    return res
+
+
 def abc_perturbed_snapshots(n, nsnap, sigma=0.05, width=4.0, seed=0, dtype=None):
     """ABC (A = sqrt 3, B = sqrt 2, C = 1) on the periodic cell-centred n^3
     grid plus, per snapshot and component, a seeded Gaussian random field
@@ -405,6 +476,9 @@ def main():
                     help="first run a fill (write 206.1 MB) and a plain copy b = 1.0 * a (read 206.1 MB, "
                          "write 206.1 MB) of 3*258^3 floats, the reference dispatches for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--output-cost", type=int, default=0, metavar="N",
+                    help="only measure the cost of the reference's output files on the N^3 pipeline")
+    ap.add_argument("--output-dir", default=None, help="directory for --output-cost files (default: TMPDIR)")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -414,6 +488,9 @@ def main():
         sys.exit(2)
     if args.workload == "turbulence":
         print(json.dumps(precision_study(args.n, args.turb_steps)), flush=True)
+        return
+    if args.output_cost:
+        print(json.dumps(run_output_cost(args.output_cost, outdir=args.output_dir)), flush=True)
         return
     dt, T, h = 0.02, 5.0, 0.5
     if args.calibrate:

@@ -60,6 +60,7 @@ struct Options {
   // slot files that the reference keeps in ./cfd2lcs_tmp (here the slots
   // live in HBM, so the files are optional)
   int flowmap_io = 1, velocity_io = 1, bcflag_io = 0;
+  int invariants = 0;  // VELOCITY_INVARIANTS (data_m.f90:59): Q, C, D, H, L2 columns
   int write_lcs = 1;
   int write_slots = 0;
   bool warn_opt = false, warn_par = false;  // SAVEd warn flags (cfd2lcs.f90:444,632)
@@ -228,6 +229,7 @@ struct Engine : EngineBase {
   std::vector<DevBuf> uring;
   std::vector<char> uring_ok;
   DevBuf vint;  // interior scratch, 3 * np
+  DevBuf invbuf;  // velocity invariants, 5 * np
 
   ~Engine() override {
     if (writer) writer->flush();
@@ -840,7 +842,7 @@ struct Engine : EngineBase {
     dg.slot_coef_ok[step] = 0;
     if (!io_ok()) return;
     if (g_opt.write_slots && !write_slot_files(dg, step)) return;
-    if (g_opt.write_lcs && g_opt.velocity_io && any_fwd() && !record_velocity(step)) return;
+    if (g_opt.write_lcs && (g_opt.velocity_io || g_opt.invariants) && any_fwd() && !record_velocity(step)) return;
     const bool spl = is_spline(interp);
     if (spl) {
       if (!dg.slotcoef.p && !dg.slotcoef.alloc((size_t)dg.nslot * 3 * fb)) {
@@ -971,7 +973,7 @@ struct Engine : EngineBase {
     j.path = fname128("./cfd2lcs_tmp/" + io::i4(0) + "_" + dg.label + "-LP-FM_" + io::i4(step) + ".dat");
     add3(j, h, np);
     wr().submit(std::move(j));
-    if (g_opt.velocity_io) {
+    if (g_opt.velocity_io || g_opt.invariants) {
       auto hv = host_interior(cf3(vel[cur]));
       if (!hv) {
         fail("copy of the velocity to the host failed");
@@ -1044,19 +1046,27 @@ struct Engine : EngineBase {
       }
       add3(j, h, np);
     }
-    if (g_opt.velocity_io) {
-      hd += "\"U-X\"\n\"U-Y\"\n\"U-Z\"\n";
-      // lp%ugrid after reconstruct_flowmap: the velocity read back from slot
-      // `step` (BKWD, just written: u_np1) or step + 1 (FWD: the oldest
-      // snapshot of the ring); zero when that slot was never written
-      std::shared_ptr<std::vector<U>> h;
+    // lp%ugrid after reconstruct_flowmap (flowmap_m.f90:212-247): the
+    // velocity read back from slot `step` (BKWD, just written: u_np1) or
+    // step + 1 (FWD: the oldest snapshot of the ring); zero when that slot
+    // was never written
+    const T* du = nullptr;
+    if (g_opt.velocity_io || g_opt.invariants) {
       if (dg.dir < 0) {
-        h = host_interior(cf3(vel[cur]));
+        if (!vint.p && !vint.alloc(3 * sizeof(T) * np)) {
+          fail("hipMalloc failed (velocity for the FTLE output)");
+          return;
+        }
+        launch_interior(g, cf3(vel[cur]), p3(vint), stream);
+        du = (const T*)vint.p;
       } else {
         const int vs = step + 1;
-        if (vs < (int)uring.size() && uring_ok[vs]) h = host_copy((const T*)uring[vs].p, 3 * np);
-        else h = std::make_shared<std::vector<U>>(3 * np, (U)0);
+        if (vs < (int)uring.size() && uring_ok[vs]) du = (const T*)uring[vs].p;
       }
+    }
+    if (g_opt.velocity_io) {
+      hd += "\"U-X\"\n\"U-Y\"\n\"U-Z\"\n";
+      auto h = du ? host_copy(du, 3 * np) : std::make_shared<std::vector<U>>(3 * np, (U)0);
       if (!h) {
         fail("copy of the velocity to the host failed");
         return;
@@ -1067,6 +1077,22 @@ struct Engine : EngineBase {
       hd += "\"FLAG\"\n";
       j.arrays.push_back(io::Array{out_flag->data(), np, sizeof(U) == 8});
       j.keep.push_back(out_flag);
+    }
+    if (g_opt.invariants) {
+      // compute_invariants (invariants_m.f90:15-198) of the same velocity
+      hd += "\"Q\"\n\"C\"\n\"D\"\n\"H\"\n\"L2\"\n";
+      if (!invbuf.p && !invbuf.alloc(5 * sizeof(T) * np)) {
+        fail("hipMalloc failed (velocity invariants)");
+        return;
+      }
+      launch_invariants(g, du, (T*)invbuf.p, stream);
+      auto h = host_copy((const T*)invbuf.p, 5 * np);
+      if (!h) {
+        fail("copy of the velocity invariants to the host failed");
+        return;
+      }
+      for (int q = 0; q < 5; q++) j.arrays.push_back(io::Array{h->data() + q * np, np, sizeof(U) == 8});
+      j.keep.push_back(h);
     }
     hd += "\"FTLE\"\nZONE\nT = \"" + ft + "\"\n";
     char es[18];
@@ -1688,7 +1714,8 @@ void cfd2lcs_set_option_c(char option[], int val) {
   else if (o == "WRITE_BCFLAG") boolopt(g_opt.bcflag_io);
   else if (o == "WRITE_LCS") boolopt(g_opt.write_lcs);
   else if (o == "WRITE_SLOTS") boolopt(g_opt.write_slots);
-  else if (o == "VELOCITY_INVARIANTS" || o == "AUX_GRID") {
+  else if (o == "VELOCITY_INVARIANTS") boolopt(g_opt.invariants);
+  else if (o == "AUX_GRID") {
     boolopt(dummy);
     if (val == LCS_TRUE) {
       fail(o + " = LCS_TRUE is not supported by this build");

@@ -708,9 +708,11 @@ __device__ __forceinline__ void flush_status(DevStatus* st, int err, unsigned lo
   if (oob) atomicAdd(&st->oob, oob);
 }
 
-// Largest eigenvalue of a symmetric 3x3 matrix by cyclic Jacobi rotations in
-// FP64 (the reference calls LAPACK DSYEV, lcs_m.f90:160-177).
-__device__ inline double eig_sym3_max(double a00, double a01, double a02, double a11, double a12, double a22) {
+// Eigenvalues (unordered) of a symmetric 3x3 matrix by cyclic Jacobi
+// rotations in FP64 (the reference calls LAPACK DSYEV, lcs_m.f90:160-177,
+// invariants_m.f90:178-193).
+__device__ inline void eig_sym3(double a00, double a01, double a02, double a11, double a12, double a22,
+                                double w[3]) {
   double a[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
   for (int sweep = 0; sweep < 60; sweep++) {
     const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
@@ -744,7 +746,16 @@ __device__ inline double eig_sym3_max(double a00, double a01, double a02, double
       a[q][p] = 0.0;
     }
   }
-  return fmax(a[0][0], fmax(a[1][1], a[2][2]));
+  w[0] = a[0][0];
+  w[1] = a[1][1];
+  w[2] = a[2][2];
+}
+
+// largest eigenvalue
+__device__ inline double eig_sym3_max(double a00, double a01, double a02, double a11, double a12, double a22) {
+  double w[3];
+  eig_sym3(a00, a01, a02, a11, a12, a22, w);
+  return fmax(w[0], fmax(w[1], w[2]));
 }
 
 

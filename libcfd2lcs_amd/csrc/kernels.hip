@@ -840,6 +840,109 @@ void launch_compose(const GridDev<T>& g, int interp, const T* slots, long long s
   }
 }
 
+// ---------------------------------------------------------------------------
+// Velocity invariants (compute_invariants, invariants_m.f90:15-198) of the
+// velocity at T0 (lp%ugrid), one node per thread.  u holds the interior
+// velocity (np per component), or is null for the all-zero field the
+// reference keeps when no velocity slot was found.  Face neighbours outside
+// the interior are the periodic image on a periodic axis and zero otherwise
+// (ugrid's ghosts after reset_lp and the face exchange); the gradient is
+// grad_sr1's central difference over the ghosted coordinate lines, with the
+// column of a one-node axis zeroed.  With S = grad u + grad u^T and
+// W = (grad u)^T - grad u, out holds per node (np each):
+//   Q  = [QS + QW > 0],  QS = -tr(S S)/8, QW = -tr(W W)/8
+//   C  = [RS > 0],       RS = -tr(S S S) * (1/24 as a REAL(4) literal)
+//   D  = 1..4 from the signs of D = Q^3 + 27/4 R^2 and R,
+//        R = RS - (sum_ijk W_ij W_jk S_ki + W_23 W_31 (S_13 - S_12)) / 8
+//        (the reference's seventh row multiplies W_23 W_31 by S_13)
+//   H  = sqrt(-QW / QS)
+//   L2 = [middle eigenvalue of S S + W W < 0], the (3,3) entry taken as 0
+//        as the reference fills it (invariants_m.f90:173-175)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_invariants(GridDev<T> g, const T* __restrict__ u, T* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.np) return;
+  int ix[3];
+  decode_interior(g, p, ix[0], ix[1], ix[2]);
+  const int sp[3] = {1, g.n[0], g.n[0] * g.n[1]};
+  T du[3][3];  // du[c][d] = d u_c / d x_d
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int a = ix[d];
+    int pp = -1, pm = -1;
+    if (a < g.n[d]) pp = p + sp[d];
+    else if (g.per[d]) pp = p - (g.n[d] - 1) * sp[d];
+    if (a > 1) pm = p - sp[d];
+    else if (g.per[d]) pm = p + (g.n[d] - 1) * sp[d];
+    const T den = g.gl[d][a + 1] - g.gl[d][a - 1];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const T* F = u ? u + (size_t)c * g.np : nullptr;
+      const T vp = (F && pp >= 0) ? F[pp] : T(0);
+      const T vm = (F && pm >= 0) ? F[pm] : T(0);
+      du[c][d] = g.n[d] == 1 ? T(0) : (vp - vm) / den;
+    }
+  }
+  T S[3][3], W[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      S[a][b] = du[b][a] + du[a][b];
+      W[a][b] = du[b][a] - du[a][b];
+    }
+  const T eighth = T(0.125), c24 = T(1.0f / 24.0f), coef = T(6.75);
+  T ss = 0, ww = 0, sss = 0, wws = 0;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      ss += S[a][b] * S[b][a];
+      ww += W[a][b] * W[b][a];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        sss += S[a][b] * S[b][c] * S[c][a];
+        wws += W[a][b] * W[b][c] * S[c][a];
+      }
+    }
+  wws += W[1][2] * W[2][0] * (S[0][2] - S[0][1]);
+  const T qs = -eighth * ss;
+  const T rs = -c24 * sss;
+  const T q = qs - eighth * ww;
+  const T r = rs - eighth * wws;
+  const T qw = q - qs;
+  const T dd = q * q * q + coef * (r * r);
+  T cls;
+  if (dd > T(0)) cls = r > T(0) ? T(1) : T(2);
+  else cls = r > T(0) ? T(3) : T(4);
+  // S S + W W, upper part, (3,3) = 0
+  T A[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      T v = 0;
+#pragma unroll
+      for (int c = 0; c < 3; c++) v += S[a][c] * S[c][b] + W[a][c] * W[c][b];
+      A[a][b] = v;
+    }
+  double w[3];
+  eig_sym3(A[0][0], A[0][1], A[0][2], A[1][1], A[1][2], 0.0, w);
+  const double lmid = w[0] + w[1] + w[2] - fmax(w[0], fmax(w[1], w[2])) - fmin(w[0], fmin(w[1], w[2]));
+  const size_t np = (size_t)g.np;
+  out[p] = q > T(0) ? T(1) : T(0);
+  out[np + p] = rs > T(0) ? T(1) : T(0);
+  out[2 * np + p] = cls;
+  out[3 * np + p] = sqrt(-qw / qs);
+  out[4 * np + p] = lmid < 0.0 ? T(1) : T(0);
+}
+
+template <typename T>
+void launch_invariants(const GridDev<T>& g, const T* u, T* out, hipStream_t s) {
+  hipLaunchKernelGGL((k_invariants<T>), grid_for(g.np), dim3(kBlock), 0, s, g, u, out);
+}
+
 template <typename T>
 void launch_ftle(const GridDev<T>& g, CF3<T> fm, double T_int, int incompressible, double tiny, double* ftle,
                  F3<T> comp, hipStream_t s) {
@@ -877,6 +980,7 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
   template void launch_compose<T>(const GridDev<T>&, int, const T*, long long, const int*, int, F3<T>,       \
                                   DevStatus*, hipStream_t);                                                   \
   template void launch_ftle<T>(const GridDev<T>&, CF3<T>, double, int, double, double*, F3<T>, hipStream_t); \
+  template void launch_invariants<T>(const GridDev<T>&, const T*, T*, hipStream_t);                           \
   template void launch_ingest<float, T>(const GridDev<T>&, F3<T>, const float*, const float*, const float*,  \
                                         hipStream_t);                                                         \
   template void launch_ingest<double, T>(const GridDev<T>&, F3<T>, const double*, const double*,             \

@@ -145,6 +145,10 @@ void launch_field_from_origin(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStr
 template <typename T>
 void launch_compose(const GridDev<T>& g, int interp, const T* slots, long long slot_stride,
                     const int* order, int nslot, F3<T> dxout, DevStatus* st, hipStream_t stream);
+// velocity invariants Q, C, D, H, L2 (5 * np) of the interior velocity u
+// (3 * np, or null for zero) at T0
+template <typename T>
+void launch_invariants(const GridDev<T>& g, const T* u, T* out, hipStream_t stream);
 template <typename T>
 void launch_ftle(const GridDev<T>& g, CF3<T> fm, double T_int, int incompressible,
                  double tiny, double* ftle, F3<T> comp, hipStream_t stream);

@@ -711,7 +711,19 @@ double oc_bspline_eval3(double xv, double yv, double zv, const double *tx,
  * Eigen-solve: cyclic Jacobi on a symmetric 3x3 (stands in for LAPACK
  * DSYEV('V','U') at lcs_m.f90:160-177; only the largest eigenvalue is used).
  * ====================================================================== */
+static void eig_sym3_all(const double *ain, double w[3]);
+
 double oc_eig_sym3_max(const double *ain) {
+  double w[3];
+  eig_sym3_all(ain, w);
+  double m = w[0];
+  if (w[1] > m) m = w[1];
+  if (w[2] > m) m = w[2];
+  return m;
+}
+
+/* all three eigenvalues (unordered) */
+static void eig_sym3_all(const double *ain, double w[3]) {
   double a[3][3];
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) a[i][j] = 0.5 * (ain[3 * i + j] + ain[3 * j + i]);
@@ -744,10 +756,9 @@ double oc_eig_sym3_max(const double *ain) {
         a[p][q] = 0.0; a[q][p] = 0.0;
       }
   }
-  double m = a[0][0];
-  if (a[1][1] > m) m = a[1][1];
-  if (a[2][2] > m) m = a[2][2];
-  return m;
+  w[0] = a[0][0];
+  w[1] = a[1][1];
+  w[2] = a[2][2];
 }
 
 /* lcs_m.f90:137-145: cg = gradfm^T gradfm, g[3*i+j] = d fm_i / d x_j */
@@ -1661,4 +1672,117 @@ int oc_get_periodic_shift(const ocfd *s, double *ps, double *bb) {
   if (ps) memcpy(ps, s->g.ps, sizeof(s->g.ps));
   if (bb) memcpy(bb, s->g.bb, sizeof(s->g.bb));
   return 0;
+}
+
+/* ======================================================================
+ * compute_invariants (invariants_m.f90:15-198) for one velocity field at T0
+ * (lp%ugrid; u, v, w interior arrays or NULL for zero).  ugrid's ghosts are
+ * zero (reset_lp, lp_m.f90:288-290) except where the face exchange
+ * (invariants_m.f90:37) copies the periodic image; grad_sr1
+ * (gradient_m.f90:11-66) then gives G[c][d] = d u_c / d x_d.  With
+ * S = G + G^T (S_ab = d_a u_b + d_b u_a) and W_ab = d_a u_b - d_b u_a:
+ *   QS = -tr(SS)/8, RS = -tr(SSS) * c24 with c24 the REAL(4) value of 1./24.
+ *   Q  = QS - tr(WW)/8
+ *   R  = RS - (tr(WWS) + W_23 W_31 (S_13 - S_12))/8: the reference's sum
+ *        (invariants_m.f90:105-116) has S13 where tr(WWS) has S12 in the
+ *        W_23 W_31 term
+ *   outputs Q>0, RS>0, the D class of (Q^3 + 27/4 R^2, R), sqrt(-(Q-QS)/QS)
+ *   and [middle eigenvalue of SS+WW with its (3,3) entry zero < 0]
+ *   (invariants_m.f90:164-194).
+ * out: Q, C, D, H, L2 (np each).
+ * ====================================================================== */
+static void mat3_mul(const double a[3][3], const double b[3][3], double c[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double v = 0.0;
+      for (int k = 0; k < 3; k++) v += a[i][k] * b[k][j];
+      c[i][j] = v;
+    }
+}
+
+void oc_velocity_invariants(const ocfd *s, const double *u, const double *v, const double *w, double *out) {
+  const ogrid *g = &s->g;
+  const long nt = g->nt, np = g->np;
+  double *ug[3];
+  const double *src[3] = {u, v, w};
+  for (int c = 0; c < 3; c++) {
+    ug[c] = (double *)calloc((size_t)nt, sizeof(double));
+    if (!src[c]) continue;
+    for (int k = 1; k <= g->n[2]; k++)
+      for (int j = 1; j <= g->n[1]; j++)
+        for (int i = 1; i <= g->n[0]; i++)
+          ug[c][GID(g, i, j, k)] = src[c][(i - 1) + (long)g->n[0] * ((j - 1) + (long)g->n[1] * (k - 1))];
+    /* face ghosts: periodic images */
+    for (int k = 1; k <= g->n[2]; k++)
+      for (int j = 1; j <= g->n[1]; j++) {
+        if (g->per[0]) {
+          ug[c][GID(g, 0, j, k)] = ug[c][GID(g, g->n[0], j, k)];
+          ug[c][GID(g, g->n[0] + 1, j, k)] = ug[c][GID(g, 1, j, k)];
+        }
+      }
+    for (int k = 1; k <= g->n[2]; k++)
+      for (int i = 1; i <= g->n[0]; i++) {
+        if (g->per[1]) {
+          ug[c][GID(g, i, 0, k)] = ug[c][GID(g, i, g->n[1], k)];
+          ug[c][GID(g, i, g->n[1] + 1, k)] = ug[c][GID(g, i, 1, k)];
+        }
+      }
+    for (int j = 1; j <= g->n[1]; j++)
+      for (int i = 1; i <= g->n[0]; i++) {
+        if (g->per[2]) {
+          ug[c][GID(g, i, j, 0)] = ug[c][GID(g, i, j, g->n[2])];
+          ug[c][GID(g, i, j, g->n[2] + 1)] = ug[c][GID(g, i, j, 1)];
+        }
+      }
+  }
+  const double c24 = (double)(1.0f / 24.0f);
+  for (int k = 1; k <= g->n[2]; k++)
+    for (int j = 1; j <= g->n[1]; j++)
+      for (int i = 1; i <= g->n[0]; i++) {
+        const long p = (i - 1) + (long)g->n[0] * ((j - 1) + (long)g->n[1] * (k - 1));
+        const long nb[3][2] = {{GID(g, i + 1, j, k), GID(g, i - 1, j, k)},
+                               {GID(g, i, j + 1, k), GID(g, i, j - 1, k)},
+                               {GID(g, i, j, k + 1), GID(g, i, j, k - 1)}};
+        double G[3][3];
+        for (int c = 0; c < 3; c++)
+          for (int d = 0; d < 3; d++) {
+            const double den = g->g[d][nb[d][0]] - g->g[d][nb[d][1]];
+            G[c][d] = g->n[d] == 1 ? 0.0 : (ug[c][nb[d][0]] - ug[c][nb[d][1]]) / den;
+          }
+        double S[3][3], W[3][3], SS[3][3], WW[3][3], SSS[3][3], WWS[3][3];
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) {
+            S[a][b] = G[b][a] + G[a][b];
+            W[a][b] = G[b][a] - G[a][b];
+          }
+        mat3_mul(S, S, SS);
+        mat3_mul(W, W, WW);
+        mat3_mul(SS, S, SSS);
+        mat3_mul(WW, S, WWS);
+        const double trSS = SS[0][0] + SS[1][1] + SS[2][2];
+        const double trWW = WW[0][0] + WW[1][1] + WW[2][2];
+        const double trSSS = SSS[0][0] + SSS[1][1] + SSS[2][2];
+        const double trWWS = WWS[0][0] + WWS[1][1] + WWS[2][2] + W[1][2] * W[2][0] * (S[0][2] - S[0][1]);
+        const double qs = -0.125 * trSS, rs = -c24 * trSSS;
+        const double q = qs - 0.125 * trWW, r = rs - 0.125 * trWWS;
+        const double dd = q * q * q + 6.75 * (r * r);
+        double cls;
+        if (dd > 0.0) cls = r > 0.0 ? 1.0 : 2.0;
+        else cls = r > 0.0 ? 3.0 : 4.0;
+        double A[9] = {SS[0][0] + WW[0][0], SS[0][1] + WW[0][1], SS[0][2] + WW[0][2],
+                       SS[0][1] + WW[0][1], SS[1][1] + WW[1][1], SS[1][2] + WW[1][2],
+                       SS[0][2] + WW[0][2], SS[1][2] + WW[1][2], 0.0};
+        double ev[3];
+        eig_sym3_all(A, ev);
+        /* sort3 */
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2 - a; b++)
+            if (ev[b] > ev[b + 1]) { double t = ev[b]; ev[b] = ev[b + 1]; ev[b + 1] = t; }
+        out[p] = q > 0.0 ? 1.0 : 0.0;
+        out[np + p] = rs > 0.0 ? 1.0 : 0.0;
+        out[2 * np + p] = cls;
+        out[3 * np + p] = sqrt(-(q - qs) / qs);
+        out[4 * np + p] = ev[1] < 0.0 ? 1.0 : 0.0;
+      }
+  for (int c = 0; c < 3; c++) free(ug[c]);
 }

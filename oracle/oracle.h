@@ -74,6 +74,9 @@ double oc_bspline_eval3(double xv, double yv, double zv, const double *tx,
 double oc_eig_sym3_max(const double *a);
 /* C = G^T G for a 3x3 gradient g (row-major, g[3*i+j] = d fm_i / d x_j) */
 void oc_cauchy_green(const double *g, double *c);
+/* compute_invariants of an interior velocity field (u, v, w: np each, or
+ * NULL for zero); out: Q, C, D, H, L2, np each */
+void oc_velocity_invariants(const ocfd *s, const double *u, const double *v, const double *w, double *out);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,7 @@ def lib():
         L.oc_eig_sym3_max.restype = C.c_double
         L.oc_eig_sym3_max.argtypes = [_dp]
         L.oc_cauchy_green.argtypes = [_dp, _dp]
+        L.oc_velocity_invariants.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _dp]
         _lib = L
     return _lib
 
@@ -118,6 +119,16 @@ class Oracle:
 
     def noutput(self, handle):
         return lib().oc_get_noutput(self.h, handle)
+
+    def invariants(self, u=None, v=None, w=None):
+        """compute_invariants of the velocity (u, v, w) at T0 on this grid
+        (None: the zero field); returns Q, C, D, H, L2."""
+        npt = int(np.prod(self.shape))
+        arrs = [None if a is None else _f(a) for a in (u, v, w)]
+        ptr = [None if a is None else a.ctypes.data_as(C.c_void_p) for a in arrs]
+        out = np.zeros(5 * npt)
+        lib().oc_velocity_invariants(self.h, *ptr, out)
+        return [out[q * npt:(q + 1) * npt].reshape(self.shape, order="F") for q in range(5)]
 
     @property
     def nsub(self):

@@ -200,6 +200,54 @@ def test_tecplot_sp_and_index_padding(tmp_path, monkeypatch):
     lib.finalize()
 
 
+def test_velocity_invariant_columns(tmp_path, monkeypatch):
+    """VELOCITY_INVARIANTS (invariants_m.f90:15-198): Q, C, D, H, L2 columns
+    after FLAG, computed on the GPU from the same velocity at T0 as the U
+    columns, against the oracle's invariants of that velocity (zero field
+    when the forward diagnostic's slot step+1 was never written)."""
+    import libcfd2lcs_amd as L
+    from oracle_api import Oracle
+    monkeypatch.chdir(tmp_path)
+    lib = L.Cfd2lcs("dp", write_output=True)
+    lib.finalize()
+    X, Y, Z, fl = _grid()
+    o = Oracle(X, Y, Z, fl)
+    lib.init(X, Y, Z, fl)
+    T, h, dt = 0.6, 0.2, 0.05
+    hf = lib.diagnostic_init(L.FTLE_FWD, 0, T, h, "fw")
+    hb = lib.diagnostic_init(L.FTLE_BKWD, 0, T, h, "bk")
+    lib.set_option("VELOCITY_INVARIANTS", L.LCS_TRUE)
+    lib.set_option("WRITE_FLOWMAP", L.LCS_FALSE)
+    ring, expect = {}, []
+    for s in range(18):
+        t = s * dt
+        lib.update(*_vel(X, Y, Z, t), t)
+        assert lib.error == 0
+        for hd, label, fwd in ((hf, "fw", True), (hb, "bk", False)):
+            if lib.noutput(hd) == len([e for e in expect if e[0] == label]):
+                continue
+            step = _nint(math.fmod(t, T) / h)
+            ring[step] = t
+            tu = ring.get(step + 1) if fwd else t
+            inv = o.invariants(*_vel(X, Y, Z, tu)) if tu is not None else o.invariants()
+            idx = _nint(t / h) - (_nint(T / h) if fwd else 0)
+            expect.append((label, idx, inv))
+    assert lib.flush_output() == 0
+    assert len(expect) == 8
+    for label, idx, inv in expect:
+        _, vnames, data, _ = read_tecplot(os.path.join("cfd2lcs_output", f"{label}_{idx}.dat"))
+        assert vnames == ["X", "Y", "Z", "U-X", "U-Y", "U-Z", "Q", "C", "D", "H", "L2", "FTLE"]
+        for name, ref in zip(("Q", "C", "D", "L2"), (inv[0], inv[1], inv[2], inv[4])):
+            assert data[name] == _fields(ref, np.float64), (label, idx, name)
+        got_h = np.array([float(x) for x in data["H"]])
+        ref_h = inv[3].ravel(order="F")
+        assert np.array_equal(np.isnan(got_h), np.isnan(ref_h))
+        m = np.isfinite(ref_h)
+        assert np.all(np.abs(got_h[m] - ref_h[m]) <= 1e-6 * np.abs(ref_h[m])), (label, idx)
+    o.close()
+    lib.finalize()
+
+
 def test_output_directory_error(tmp_path, monkeypatch):
     """A path that cannot be a directory sets the sticky error flag, as the
     reference's failed mkdir does (cfd2lcs.f90:35-40)."""

@@ -282,6 +282,99 @@ def test_tse_equals_linear_on_general_linear_field(interp):
             assert np.abs(a[d][m] - b[d][m]).max() < 1e-13
 
 
+# ---------------------------------------------------------------- invariants
+def _invariants_closed_form(G, quirk=True):
+    """compute_invariants (invariants_m.f90:15-198) of one velocity gradient
+    G[c][d] = du_c/dx_d by matrix algebra: S = G^T + G, W = G^T - G,
+    QS = -tr(SS)/8, RS = -tr(SSS)/24 (1/24 as a REAL(4) literal),
+    Q = QS - tr(WW)/8, R = RS - (tr(WWS) [+ the reference's S13-for-S12
+    term])/8, L2 from the LAPACK-style upper triangle of SS + WW with a zero
+    (3,3) entry."""
+    S, W = G.T + G, G.T - G
+    c24 = float(np.float32(1.0) / np.float32(24.0))
+    qs = -np.trace(S @ S) / 8
+    rs = -c24 * np.trace(S @ S @ S)
+    q = qs - np.trace(W @ W) / 8
+    extra = W[1, 2] * W[2, 0] * (S[0, 2] - S[0, 1]) if quirk else 0.0
+    r = rs - (np.trace(W @ W @ S) + extra) / 8
+    d = q ** 3 + 6.75 * r * r
+    cls = (1 if r > 0 else 2) if d > 0 else (3 if r > 0 else 4)
+    A = S @ S + W @ W
+    A[2, 2] = 0.0
+    ev = np.linalg.eigvalsh(A, UPLO="U")
+    with np.errstate(divide="ignore", invalid="ignore"):
+        h = float(np.sqrt(-(q - qs) / qs))
+    return [float(q > 0), float(rs > 0), float(cls), h, float(ev[1] < 0)]
+
+
+@pytest.mark.parametrize("M", [
+    # the reference's seventh R term (S13 for S12) moves D from class 2 to 1 here
+    [[0.02, -0.33, -0.13], [0.84, -0.62, -0.7], [-0.98, 0.7, 0.98]],
+    [[0.3, -0.7, 0.4], [0.5, -0.2, -0.6], [-0.45, 0.35, 0.1]],
+    [[0.0, 1.0, 0.0], [-1.0, 0.0, 0.0], [0.0, 0.0, 0.0]],      # solid rotation: vortex
+    [[1.0, 0.0, 0.0], [0.0, -0.5, 0.0], [0.0, 0.0, -0.5]],     # axisymmetric strain
+])
+def test_invariants_of_linear_field(M):
+    """u = M x on a walled grid: grad_sr1 is exact at nodes off the boundary,
+    so every node there carries the closed-form invariants of G = M."""
+    M = np.array(M)
+    X, Y, Z = F.cell_grid((9, 8, 7), (-1, -1, -1), (1, 1, 1))
+    fl = np.full(X.shape, F.LCS_WALL, dtype=np.int32)
+    o = Oracle(X, Y, Z, fl)
+    u = M[0, 0] * X + M[0, 1] * Y + M[0, 2] * Z
+    v = M[1, 0] * X + M[1, 1] * Y + M[1, 2] * Z
+    w = M[2, 0] * X + M[2, 1] * Y + M[2, 2] * Z
+    got = o.invariants(u, v, w)
+    want = _invariants_closed_form(M)
+    inner = (slice(1, -1), slice(1, -1), slice(1, -1))
+    for q in (0, 1, 2, 4):
+        assert np.all(got[q][inner] == want[q]), q
+    if np.isfinite(want[3]):
+        assert np.abs(got[3][inner] - want[3]).max() <= 1e-12 * max(1.0, abs(want[3]))
+    o.close()
+    if M[0, 0] == 0.02:
+        assert _invariants_closed_form(M)[2] != _invariants_closed_form(M, quirk=False)[2]
+
+
+def test_invariants_boundary_and_periodic_gradients():
+    """Face neighbours: the periodic image on a periodic axis, zero beyond a
+    wall (ugrid's ghosts); coordinates from the ghosted grid.  Checked by an
+    independent numpy central difference of the same padded field."""
+    X, Y, Z = F.cell_grid((10, 9, 8), (0, 0, 0), (2 * np.pi, 2 * np.pi, 2.0))
+    fl = np.full(X.shape, F.LCS_INTERNAL, dtype=np.int32)
+    fl[:, :, 0] = F.LCS_WALL
+    fl[:, :, -1] = F.LCS_WALL
+    o = Oracle(X, Y, Z, fl)
+    vel = [np.sin(Y) + 0.3 * Z, np.cos(X) * (1 + Z), np.sin(X + Y) - 0.2 * Z * Z]
+    got = o.invariants(*vel)
+    (gx, gy, gz), _, _ = o.ghost_grid()
+    want = [np.zeros(X.shape) for _ in range(5)]
+    for i in range(10):
+        for j in range(9):
+            for k in range(8):
+                G = np.zeros((3, 3))
+                for c in range(3):
+                    f = vel[c]
+                    ip, im = (i + 1) % 10, (i - 1) % 10
+                    jp, jm = (j + 1) % 9, (j - 1) % 9
+                    G[c, 0] = (f[ip, j, k] - f[im, j, k]) / (gx[i + 2, j + 1, k + 1] - gx[i, j + 1, k + 1])
+                    G[c, 1] = (f[i, jp, k] - f[i, jm, k]) / (gy[i + 1, j + 2, k + 1] - gy[i + 1, j, k + 1])
+                    fp = f[i, j, k + 1] if k < 7 else 0.0
+                    fm = f[i, j, k - 1] if k > 0 else 0.0
+                    G[c, 2] = (fp - fm) / (gz[i + 1, j + 1, k + 2] - gz[i + 1, j + 1, k])
+                r = _invariants_closed_form(G)
+                for q in range(5):
+                    want[q][i, j, k] = r[q]
+    for q in (0, 1, 2, 4):
+        assert np.array_equal(got[q], want[q]), q
+    assert np.allclose(got[3], want[3], rtol=1e-12, atol=0)
+    # no velocity slot: the zero field (H = 0/0)
+    z = o.invariants()
+    assert np.all(z[0] == 0) and np.all(z[1] == 0) and np.all(z[2] == 4) and np.all(z[4] == 0)
+    assert np.all(np.isnan(z[3]))
+    o.close()
+
+
 def test_error_paths():
     X, Y, Z, fl = F.abc_grid(4)
     o = Oracle(X, Y, Z, fl)
