@@ -280,16 +280,72 @@ struct AdjLaunch {
   int gx, gy;         // bricks of this colour along x and y
 };
 
+// A DPP quad permute of a value: lane q of each group of four receives the
+// value of lane perm[q] (quad_perm control word)
+template <int CTRL>
+__device__ __forceinline__ float quad_perm(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double quad_perm(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// lane q receives from lane (q - d) & 3, resp. (q + d) & 3
+template <int D, typename T>
+__device__ __forceinline__ T from_minus(T v) {
+  if constexpr (D == 1) return quad_perm<0x93>(v);  // [3,0,1,2]
+  else if constexpr (D == 2) return quad_perm<0x4E>(v);  // [2,3,0,1]
+  else return quad_perm<0x39>(v);  // [1,2,3,0]
+}
+template <int D, typename T>
+__device__ __forceinline__ T from_plus(T v) {
+  if constexpr (D == 1) return quad_perm<0x39>(v);
+  else if constexpr (D == 2) return quad_perm<0x4E>(v);
+  else return quad_perm<0x93>(v);
+}
+
+// Transposed interpolation of one wave's brick into its LDS tile.  Every lane
+// owns the window of WX x WY x WZ cells that starts at its node (zero weight
+// outside its point's true support); a point whose support leaves the window
+// goes to the overflow field with global atomics.  At one window offset the
+// 64 lanes touch 64 distinct cells, and LDS performs one wave's operations in
+// order, so the only ordering the code needs is between offsets, which a
+// compiler barrier provides; each offset costs one LDS round trip.
+// With 4-wide bricks (3-D) the four x-lanes of a brick row are a DPP quad:
+// their WX offsets along x are folded by quad permutes into two cells per
+// lane (q and q + 4 of the 8-cell tile row), so that one (oz, oy) offset row
+// costs one round trip for all of x.
 template <typename T, class A, int BRX, int BRY, int BRZ>
 struct WindowScatter {
   static constexpr int TX = BRX + A::WX - 1, TY = BRY + A::WY - 1, TZ = BRZ + A::WZ - 1;
   static constexpr int TILE = TX * TY * TZ;
-  vlptr<T> tile;  // [3][TZ][TY][TX]
-  int lx, ly, lz;    // lane's node in the brick
+  static constexpr bool kQuad = BRX == 4 && TX <= 8;
+  lptr<T> tile;   // [3][TZ][TY][TX]
+  int lx, ly, lz; // lane's node in the brick
   F3<T> ovf;
   int N0, N1;
-  // several points of one lane into the same window: their contributions are
-  // summed in registers, one read-modify-write per window cell
+
+  __device__ __forceinline__ void overflow(const int b[3], const T* wx, const T* wy, const T* wz,
+                                           const T wv[3]) const {
+#pragma unroll
+    for (int c = 0; c < A::KZ; c++)
+#pragma unroll
+      for (int bb = 0; bb < A::KY; bb++)
+#pragma unroll
+        for (int a = 0; a < A::KX; a++) {
+          const T w = wx[a] * wy[bb] * wz[c];
+          const int idx = (b[0] + a) + N0 * ((b[1] + bb) + N1 * (b[2] + c));
+#pragma unroll
+          for (int m = 0; m < 3; m++) atomic_add(ovf.c[m] + idx, wv[m] * w);
+        }
+  }
+
+  // NP points of one lane (use[p] false: no contribution); their
+  // contributions are summed in registers before the tile is touched.  Every
+  // lane of the wave must call this (the quad permutes read all lanes).
   template <int NP>
   __device__ __forceinline__ void multi(const A& ad, const GridDev<T>& g, const T (*qs)[3], const bool* use,
                                         const int nd[3], const T (*wvs)[3]) const {
@@ -297,8 +353,8 @@ struct WindowScatter {
     bool any = false;
 #pragma unroll
     for (int p = 0; p < NP; p++) {
-      int b[3];
-      T wx[A::KX], wy[A::KY], wz[A::KZ];
+      int b[3] = {0, 0, 0};
+      T wx[A::KX] = {}, wy[A::KY] = {}, wz[A::KZ] = {};
       int sx = 0, sy = 0, sz = 0;
       bool ok = use[p] && !ad.weights(g, qs[p], nd, b, wx, wy, wz);
       if (ok) {
@@ -306,7 +362,7 @@ struct WindowScatter {
         sy = b[1] - (nd[1] + A::BY);
         sz = b[2] - (nd[2] + A::BZ);
         if (sx < 0 || sx > A::WX - A::KX || sy < 0 || sy > A::WY - A::KY || sz < 0 || sz > A::WZ - A::KZ) {
-          (*this)(ad, g, qs[p], nd, wvs[p]);  // overflow path
+          overflow(b, wx, wy, wz, wvs[p]);
           ok = false;
         }
       }
@@ -333,87 +389,104 @@ struct WindowScatter {
         ez[p][o] = v;
       }
     }
-    if (!any) return;
+    if constexpr (kQuad) {
+      if (!__any(any)) return;  // wave-uniform
+      const int q = lx;
 #pragma unroll
-    for (int oz = 0; oz < A::WZ; oz++)
+      for (int oz = 0; oz < A::WZ; oz++)
 #pragma unroll
-      for (int oy = 0; oy < A::WY; oy++) {
-        T wyz[NP][3];
-#pragma unroll
-        for (int p = 0; p < NP; p++) {
-          const T w = ey[p][oy] * ez[p][oz];
-#pragma unroll
-          for (int m = 0; m < 3; m++) wyz[p][m] = w * wvs[p][m];
-        }
-        const int row = ((lz + oz) * TY + (ly + oy)) * TX + lx;
-#pragma unroll
-        for (int ox = 0; ox < A::WX; ox++) {
+        for (int oy = 0; oy < A::WY; oy++) {
+          T lo[3], hi[3];
 #pragma unroll
           for (int m = 0; m < 3; m++) {
-            T add = T(0);
+            T v[A::WX];
 #pragma unroll
-            for (int p = 0; p < NP; p++) add += ex[p][ox] * wyz[p][m];
-            tile[m * TILE + row + ox] += add;
+            for (int o = 0; o < A::WX; o++) {
+              T a = T(0);
+#pragma unroll
+              for (int p = 0; p < NP; p++) a += ex[p][o] * (ey[p][oy] * ez[p][oz] * wvs[p][m]);
+              v[o] = a;
+            }
+            // cell q: lane q - d at offset d; cell q + 4: lane q + d at offset 4 - d
+            T l = v[0], h = T(0);
+            if constexpr (A::WX > 4) h = v[4];
+            if constexpr (A::WX > 1) {
+              const T r = from_minus<1>(v[1]);
+              if (q >= 1) l += r;
+            }
+            if constexpr (A::WX > 2) {
+              const T r = from_minus<2>(v[2]);
+              if (q >= 2) l += r;
+            }
+            if constexpr (A::WX > 3) {
+              const T r = from_minus<3>(v[3]);
+              if (q >= 3) l += r;
+              const T u = from_plus<1>(v[3]);
+              if (q <= 2) h += u;
+            }
+            if constexpr (A::WX > 2) {
+              const T u = from_plus<2>(v[2]);
+              if (q <= 1) h += u;
+            }
+            if constexpr (A::WX > 1) {
+              const T u = from_plus<3>(v[1]);
+              if (q == 0) h += u;
+            }
+            lo[m] = l;
+            hi[m] = h;
+          }
+          const int rb = ((lz + oz) * TY + (ly + oy)) * TX + q;
+          const bool has_hi = q + 4 < TX;
+          T r0 = tile[rb], r1 = tile[TILE + rb], r2 = tile[2 * TILE + rb];
+          T s0 = T(0), s1 = T(0), s2 = T(0);
+          if (has_hi) {
+            s0 = tile[rb + 4];
+            s1 = tile[TILE + rb + 4];
+            s2 = tile[2 * TILE + rb + 4];
+          }
+          tile[rb] = r0 + lo[0];
+          tile[TILE + rb] = r1 + lo[1];
+          tile[2 * TILE + rb] = r2 + lo[2];
+          if (has_hi) {
+            tile[rb + 4] = s0 + hi[0];
+            tile[TILE + rb + 4] = s1 + hi[1];
+            tile[2 * TILE + rb + 4] = s2 + hi[2];
+          }
+          __asm__ volatile("" ::: "memory");
+        }
+    } else {
+      if (!any) return;
+#pragma unroll
+      for (int oz = 0; oz < A::WZ; oz++)
+#pragma unroll
+        for (int oy = 0; oy < A::WY; oy++) {
+          T wyz[NP][3];
+#pragma unroll
+          for (int p = 0; p < NP; p++) {
+            const T w = ey[p][oy] * ez[p][oz];
+#pragma unroll
+            for (int m = 0; m < 3; m++) wyz[p][m] = w * wvs[p][m];
+          }
+          const int row = ((lz + oz) * TY + (ly + oy)) * TX + lx;
+#pragma unroll
+          for (int ox = 0; ox < A::WX; ox++) {
+            T add[3];
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+              T a = T(0);
+#pragma unroll
+              for (int p = 0; p < NP; p++) a += ex[p][ox] * wyz[p][m];
+              add[m] = a;
+            }
+            const int c = row + ox;
+            const T r0 = tile[c], r1 = tile[TILE + c], r2 = tile[2 * TILE + c];
+            tile[c] = r0 + add[0];
+            tile[TILE + c] = r1 + add[1];
+            tile[2 * TILE + c] = r2 + add[2];
+            __asm__ volatile("" ::: "memory");
           }
         }
-      }
-  }
-
-  __device__ __forceinline__ void operator()(const A& ad, const GridDev<T>& g, const T q[3], const int nd[3],
-                                             const T wv[3]) const {
-    int b[3];
-    T wx[A::KX], wy[A::KY], wz[A::KZ];
-    if (ad.weights(g, q, nd, b, wx, wy, wz)) return;
-    const int sx = b[0] - (nd[0] + A::BX), sy = b[1] - (nd[1] + A::BY), sz = b[2] - (nd[2] + A::BZ);
-    if (sx < 0 || sx > A::WX - A::KX || sy < 0 || sy > A::WY - A::KY || sz < 0 || sz > A::WZ - A::KZ) {
-#pragma unroll
-      for (int c = 0; c < A::KZ; c++)
-#pragma unroll
-        for (int bb = 0; bb < A::KY; bb++)
-#pragma unroll
-          for (int a = 0; a < A::KX; a++) {
-            const T w = wx[a] * wy[bb] * wz[c];
-            const int idx = (b[0] + a) + N0 * ((b[1] + bb) + N1 * (b[2] + c));
-#pragma unroll
-            for (int m = 0; m < 3; m++) atomic_add(ovf.c[m] + idx, wv[m] * w);
-          }
-      return;
     }
-    T ex[A::WX], ey[A::WY], ez[A::WZ];
-#pragma unroll
-    for (int o = 0; o < A::WX; o++) {
-      T v = T(0);
-#pragma unroll
-      for (int a = 0; a < A::KX; a++) v = (o - a == sx) ? wx[a] : v;
-      ex[o] = v;
-    }
-#pragma unroll
-    for (int o = 0; o < A::WY; o++) {
-      T v = T(0);
-#pragma unroll
-      for (int a = 0; a < A::KY; a++) v = (o - a == sy) ? wy[a] : v;
-      ey[o] = v;
-    }
-#pragma unroll
-    for (int o = 0; o < A::WZ; o++) {
-      T v = T(0);
-#pragma unroll
-      for (int a = 0; a < A::KZ; a++) v = (o - a == sz) ? wz[a] : v;
-      ez[o] = v;
-    }
-#pragma unroll
-    for (int oz = 0; oz < A::WZ; oz++)
-#pragma unroll
-      for (int oy = 0; oy < A::WY; oy++) {
-        const T wyz = ey[oy] * ez[oz];
-        const int row = ((lz + oz) * TY + (ly + oy)) * TX + lx;
-#pragma unroll
-        for (int ox = 0; ox < A::WX; ox++) {
-          const T w = ex[ox] * wyz;
-#pragma unroll
-          for (int m = 0; m < 3; m++) tile[m * TILE + row + ox] += wv[m] * w;
-        }
-      }
   }
 };
 
@@ -439,7 +512,7 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
             ibz = 2 * (b / (al.gx * al.gy)) + al.cz;
   const int t = threadIdx.x;
   WS ws_;
-  ws_.tile = (vlptr<T>)tile_mem;
+  ws_.tile = (lptr<T>)tile_mem;
   ws_.lx = t % BRX;
   ws_.ly = (t / BRX) % BRY;
   ws_.lz = t / (BRX * BRY);
@@ -593,14 +666,16 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
     __syncthreads();
     const F3<T> dst = ph == 0 ? adj_fm : (ph == 1 ? adj_v0 : (ph == 2 ? adj_v1 : adj_v2));
     ws_.ovf = ph == 0 ? ovf_fm : (ph == 1 ? ovf_v0 : (ph == 2 ? ovf_v1 : ovf_v2));
-    if (active) {
-      if (ph == 0) {
-        ws_(ad, g, xb, nd, gl3);
-      } else if (!stuck) {
-        // stages 1..3 (indices) that feed this buffer, in one window pass
-        const bool use[3] = {1 < ns && sbuf.y == ph - 1, 2 < ns && sbuf.z == ph - 1, 3 < ns && sbuf.w == ph - 1};
-        ws_.template multi<3>(ad, g, xs + 1, use, nd, ws + 1);
-      }
+    // every lane calls the scatter (inactive lanes contribute nothing)
+    if (ph == 0) {
+      const bool use0[1] = {active};
+      ws_.template multi<1>(ad, g, &xb, use0, nd, &gl3);
+    } else {
+      // stages 1..3 (indices) that feed this buffer, in one window pass
+      const bool on = active && !stuck;
+      const bool use[3] = {on && 1 < ns && sbuf.y == ph - 1, on && 2 < ns && sbuf.z == ph - 1,
+                           on && 3 < ns && sbuf.w == ph - 1};
+      ws_.template multi<3>(ad, g, xs + 1, use, nd, ws + 1);
     }
     __syncthreads();
     for (int e = t; e < 3 * WS::TILE; e += 64) {
