@@ -54,6 +54,7 @@ struct Options {
   int particle_sort = 1;  // re-order forward particles by node every n updates (0: never)
   int vel_storage = kStoreNative;  // storage of the gathered stage velocities
   int compensated = 0;    // two-sum position/displacement updates of forward particles
+  int sl_tiled = 0;       // LDS-tiled semi-Lagrangian B-spline step (results identical; slower here)
   // output files: FLOWMAP_IO, VELOCITY_IO, BCFLAG_IO (data_m.f90:44-54) pick
   // the variables of the Tecplot FTLE file; write_lcs switches that file
   // (always written by the reference) and write_slots the flowmap/velocity
@@ -788,7 +789,7 @@ struct Engine : EngineBase {
             src = cf3(dg.fmcoef);
           }
           const StageParams<T> sp = stage_params(t, dt, spl);
-          ktimed(9, [&] { launch_sl_step(g, interp, sp, src, f3(b), st, stream); });
+          ktimed(9, [&] { launch_sl_step(g, interp, sp, src, f3(b), st, stream, g_opt.sl_tiled); });
           ktimed(13, [&] { launch_fm_ghost(g, f3(b), cf3(a), st, stream); });
           dg.fcur = 1 - dg.fcur;
           t = t + dt;
@@ -1267,7 +1268,7 @@ struct Engine : EngineBase {
       HIPCHK_R(hipMemcpyAsync(keep->p, tape.spl ? tape.fmcoef.p : a.p, fb, hipMemcpyDeviceToDevice, stream), false);
     }
     const StageParams<T> sp = stage_params(t, dt, tape.spl);
-    launch_sl_step(g, tape.interp, sp, src, f3(b), st, stream);
+    launch_sl_step(g, tape.interp, sp, src, f3(b), st, stream, g_opt.sl_tiled);
     launch_fm_ghost(g, f3(b), cf3(a), st, stream);
     tape.fcur = 1 - tape.fcur;
     return true;
@@ -1639,7 +1640,9 @@ void cfd2lcs_init_c(MPI_Comm usercomm, int n[3], int offset[3], lcsdata_t* x, lc
     Z[p] = z[p];
   }
   auto hg = std::make_unique<HostGrid>();
-  std::string e = hg->build(n, X.data(), Y.data(), Z.data(), flag);
+  // ghost edges may depart from the coordinate lines by the rounding of the
+  // user's coordinates (lcsdata_t)
+  std::string e = hg->build(n, X.data(), Y.data(), Z.data(), flag, sizeof(lcsdata_t) == 8 ? 1e-9 : 1e-6);
   if (!e.empty()) {
     fail(e);
     return;
@@ -1732,6 +1735,8 @@ void cfd2lcs_set_option_c(char option[], int val) {
     else warn = true;
   } else if (o == "COMPENSATED") {
     boolopt(g_opt.compensated);
+  } else if (o == "SL_TILED") {
+    boolopt(g_opt.sl_tiled);
   } else if (o == "PARTICLE_SORT") {
     if (val < 0) warn = true;
     else g_opt.particle_sort = val;

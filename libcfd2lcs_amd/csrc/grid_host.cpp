@@ -67,7 +67,8 @@ void cross_unit(const double a[3], const double b[3], double out[3]) {
 
 }  // namespace
 
-std::string HostGrid::build(const int nn[3], const double* x, const double* y, const double* z, const int* fl) {
+std::string HostGrid::build(const int nn[3], const double* x, const double* y, const double* z, const int* fl,
+                            double edge_tol) {
   for (int d = 0; d < 3; d++) {
     n[d] = nn[d];
     N[d] = nn[d] + 2;
@@ -257,12 +258,14 @@ std::string HostGrid::build(const int nn[3], const double* x, const double* y, c
     for (int a = 0; a < N[d]; a++) line[d][a] = g[d][id(d == 0 ? a : 1, d == 1 ? a : 1, d == 2 ? a : 1)];
   }
   // Interior and face-ghost nodes must lie exactly on the lines.  Ghost edges
-  // and corners of periodic grids carry the rounding of the periodic shifts
-  // (one shift per neighbour direction), which only they see; the kernels read
-  // face ghosts only, so these may differ from the lines by rounding.
+  // and corners carry the rounding of the periodic shifts (one shift per
+  // neighbour direction), and where a periodic axis meets a wall the edge
+  // keeps its mirror value, built from the spacing at the other end of the
+  // axis; both differ from the lines by the rounding of the input coordinates
+  // only, and the kernels read face ghosts only.
   double ext = 0;
   for (int d = 0; d < 3; d++) ext = std::max(ext, std::fabs(bb[3 + d] - bb[d]));
-  const double tol = 1e-9 * std::max(ext, 1.0);
+  const double tol = edge_tol * std::max(ext, 1.0);
   for (int k = 0; k <= nk + 1 && separable; k++)
     for (int j = 0; j <= nj + 1 && separable; j++)
       for (int i = 0; i <= ni + 1; i++) {

@@ -30,8 +30,11 @@ struct HostGrid {
     return (i - 1) + (long long)n[0] * ((j - 1) + (long long)n[1] * (k - 1));
   }
   // Builds the grid from the user's interior arrays (Fortran order).
-  // Returns an empty string on success, else the reason.
-  std::string build(const int nn[3], const double* x, const double* y, const double* z, const int* fl);
+  // edge_tol: relative tolerance (times the domain extent) for ghost edges and
+  // corners against the coordinate lines; it follows the precision of the
+  // user's coordinates.  Returns an empty string on success, else the reason.
+  std::string build(const int nn[3], const double* x, const double* y, const double* z, const int* fl,
+                    double edge_tol = 1e-9);
 };
 
 // Knots (dbknot rule) and the banded LU factors of the collocation matrix

@@ -178,13 +178,9 @@ __global__ void __launch_bounds__(kBlock) k_field_from_origin(GridDev<T> g, CF3<
 // BC, old flowmap at the departure point, displacement form write.
 // ---------------------------------------------------------------------------
 template <typename T, typename S, class I>
-__global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
-                                                    DevStatus* st) {
-  const I interp{};
-  const int p = xcd_thread_index();
-  if (p >= g.np) return;
-  int i, j, k;
-  decode_interior(g, p, i, j, k);
+__device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T>& sp, const I& interp,
+                                        const CF3<T>& fmold, const F3<T>& fmnew, int i, int j, int k,
+                                        DevStatus* st) {
   int nd[3] = {i, j, k};
   const T x0[3] = {g.gl[0][i], g.gl[1][j], g.gl[2][k]};
   T xp[3] = {x0[0], x0[1], x0[2]};
@@ -217,6 +213,132 @@ __global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T>
     fmnew.c[d][l] = zero ? T(0) : v;
   }
   flush_status(st, err, oob);
+}
+
+template <typename T, typename S, class I>
+__global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
+                                                    DevStatus* st) {
+  const I interp{};
+  const int p = xcd_thread_index();
+  if (p >= g.np) return;
+  int i, j, k;
+  decode_interior(g, p, i, j, k);
+  sl_node<T, S>(g, sp, interp, fmold, fmnew, i, j, k, st);
+}
+
+// ---------------------------------------------------------------------------
+// Stage (a) with the brick's coefficient tiles in LDS (B-spline, 3-D): a
+// workgroup owns a BX x BY x BZ brick of nodes and stages, for every field
+// the step gathers from (the old flowmap's coefficients and the distinct
+// stage-velocity coefficient fields), the coefficients from kTileHalo cells
+// below to kTileHalo cells above the brick.  A point within about half a cell
+// of its node (the stage and departure points under CFL <= 0.5) finds its
+// whole support in the tile; any other point gathers from global memory.
+// The sums run in spline_sum's order, so the results equal the untiled
+// kernel's bit for bit.  Measured on ABC 256^3 (RK4, cubic) it is slower than
+// the gather kernel (4.0 vs 2.6 ms per sub-cycle): with a 12-float tile pitch
+// two lanes of each row pair share an LDS bank (bank-conflict cycles are
+// about half of the LDS-active cycles), the staging index arithmetic adds two
+// thirds to the VALU count, and 148 VGPRs leave 3 waves per SIMD, while the
+// gather kernel's float4 row loads already run the texture path at about
+// 70 % busy.  It is kept as the "SL_TILED" option.
+// ---------------------------------------------------------------------------
+constexpr int kTileHalo = 2;
+template <typename T>
+using lcptr = const __attribute__((address_space(3))) T*;
+
+template <typename T, int KX, int KZ, int TX, int TY, int TZ>
+struct IntSplineTiled {
+  static constexpr int TILE = TX * TY * TZ;
+  const T* key[3];  // component-0 pointer of each staged field (null: unused)
+  lcptr<T> tile;    // [field][component][TZ][TY][TX]
+  int o[3];         // ghosted index of tile cell (0, 0, 0)
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+                             unsigned long long& oob) const {
+    SplineW<T, KX, KZ> s;
+    const int r = spline_setup(g, q, nd, s);
+    if (r) {
+      if (r == 1) oob++;
+      out[0] = out[1] = out[2] = T(0);
+      return;
+    }
+    const int slot = f.c[0] == key[0] ? 0 : (f.c[0] == key[1] ? 1 : (f.c[0] == key[2] ? 2 : -1));
+    const int lx = s.b3[0] - o[0], ly = s.b3[1] - o[1], lz = s.b3[2] - o[2];
+    if (slot >= 0 && lx >= 0 && lx <= TX - KX && ly >= 0 && ly <= TY - KX && lz >= 0 && lz <= TZ - KZ) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        lcptr<T> tc = tile + (slot * 3 + c) * TILE + (lz * TY + ly) * TX + lx;
+        T acc = T(0);
+#pragma unroll
+        for (int cz = 0; cz < KZ; cz++) {
+          T accy = T(0);
+#pragma unroll
+          for (int by = 0; by < KX; by++) {
+            T accx = T(0);
+#pragma unroll
+            for (int a = 0; a < KX; a++) accx += s.wx[a] * tc[(cz * TY + by) * TX + a];
+            accy += s.wy[by] * accx;
+          }
+          acc += s.wz[cz] * accy;
+        }
+        out[c] = acc;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) out[c] = spline_sum(g, s, f.c[c]);
+    }
+  }
+};
+
+template <typename T, int KX, int KZ, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX * BY * BZ) k_sl_tiled(GridDev<T> g, StageParams<T> sp, CF3<T> fmold,
+                                                           F3<T> fmnew, DevStatus* st, int nbx, int nby) {
+  constexpr int NT = BX * BY * BZ;
+  constexpr int TX = BX + 2 * kTileHalo, TY = BY + 2 * kTileHalo, TZ = BZ + 2 * kTileHalo;
+  using I = IntSplineTiled<T, KX, KZ, TX, TY, TZ>;
+  __shared__ T tile_mem[9 * I::TILE];
+  const int b = xcd_brick(blockIdx.x, gridDim.x);
+  const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+  const int t = threadIdx.x;
+  I interp;
+  interp.tile = (lcptr<T>)tile_mem;
+  interp.o[0] = bx * BX + 1 - kTileHalo;
+  interp.o[1] = by * BY + 1 - kTileHalo;
+  interp.o[2] = bz * BZ + 1 - kTileHalo;
+  // the distinct gathered fields: the flowmap, then the stage fields in use
+  const int ns = sp.integ == kRK4 ? 4 : (sp.integ == kRK3 ? 3 : (sp.integ == kRK2 ? 2 : 1));
+  CF3<T> fb{{nullptr, nullptr, nullptr}}, fc{{nullptr, nullptr, nullptr}};
+#pragma unroll
+  for (int a = 1; a < 4; a++) {
+    const CF3<T> v{{(const T*)sp.v[a][0], (const T*)sp.v[a][1], (const T*)sp.v[a][2]}};
+    if (a >= ns || v.c[0] == fmold.c[0] || v.c[0] == fb.c[0] || v.c[0] == fc.c[0]) continue;
+    if (!fb.c[0]) fb = v;
+    else if (!fc.c[0]) fc = v;
+  }
+  interp.key[0] = fmold.c[0];
+  interp.key[1] = fb.c[0];
+  interp.key[2] = fc.c[0];
+  auto stage_in = [&](int f, const CF3<T>& src) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const T* S = src.c[c];
+      for (int e = t; e < I::TILE; e += NT) {
+        const int x = e % TX, y = (e / TX) % TY, z = e / (TX * TY);
+        const int gx = interp.o[0] + x, gy = interp.o[1] + y, gz = interp.o[2] + z;
+        T v = T(0);
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < g.N[0] && gy < g.N[1] && gz < g.N[2])
+          v = S[gx + g.N[0] * (gy + g.N[1] * gz)];
+        tile_mem[(f * 3 + c) * I::TILE + e] = v;
+      }
+    }
+  };
+  stage_in(0, fmold);
+  if (fb.c[0]) stage_in(1, fb);
+  if (fc.c[0]) stage_in(2, fc);
+  __syncthreads();
+  const int i = bx * BX + t % BX + 1, j = by * BY + (t / BX) % BY + 1, k = bz * BZ + t / (BX * BY) + 1;
+  if (i > g.n[0] || j > g.n[1] || k > g.n[2]) return;
+  sl_node<T, T>(g, sp, interp, fmold, fmnew, i, j, k, st);
 }
 
 // time blend u = (1-ct) u_n + ct u_np1 over the ghosted grid, in the
@@ -763,8 +885,25 @@ void with_storage(int storage, F&& f) {
 
 template <typename T>
 void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, F3<T> fmnew,
-                    DevStatus* st, hipStream_t s) {
+                    DevStatus* st, hipStream_t s, int tiled) {
   const dim3 grid = grid_for(g.np), blk(kBlock);
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32 B-spline step on a 3-D grid with native stage storage: LDS tiles
+    if (tiled && (interp == kQuadratic || interp == kCubic) && sp.storage == kStoreNative && g.flat < 0 &&
+        g.kord[0] == g.kord[1] && g.kord[1] == g.kord[2]) {
+      constexpr int BX = 8, BY = 8, BZ = 4;
+      const int nbx = (g.n[0] + BX - 1) / BX, nby = (g.n[1] + BY - 1) / BY, nbz = (g.n[2] + BZ - 1) / BZ;
+      const dim3 tgrid((unsigned)(nbx * nby * nbz)), tblk(BX * BY * BZ);
+      if (g.kord[0] == 4) {
+        hipLaunchKernelGGL((k_sl_tiled<T, 4, 4, BX, BY, BZ>), tgrid, tblk, 0, s, g, sp, fmold, fmnew, st, nbx, nby);
+        return;
+      }
+      if (g.kord[0] == 3) {
+        hipLaunchKernelGGL((k_sl_tiled<T, 3, 3, BX, BY, BZ>), tgrid, tblk, 0, s, g, sp, fmold, fmnew, st, nbx, nby);
+        return;
+      }
+    }
+  }
   with_storage<T>(sp.storage, [&](auto tag) {
     using S = decltype(tag);
     if (interp == kLinear) {
@@ -973,7 +1112,7 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
   template void launch_spline_fit_T<T>(const GridDev<T>&, F3<T>, hipStream_t);                                \
   template void launch_blend<T>(const GridDev<T>&, void* const*, int, CF3<T>, CF3<T>, T, hipStream_t);     \
   template void launch_sl_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, F3<T>, DevStatus*,  \
-                                  hipStream_t);                                                               \
+                                  hipStream_t, int);                                                          \
   template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, ParticleSet<T>, int,     \
                                    DevStatus*, hipStream_t);                                                   \
   template void launch_fwd_reset<T>(const GridDev<T>&, ParticleSet<T>, hipStream_t);                          \

@@ -120,9 +120,11 @@ void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t s
 template <typename T>
 void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a, CF3<T> b, T ct,
                   hipStream_t stream);
+// tiled: FP32 B-spline steps on 3-D grids stage the gathered coefficients
+// in LDS (bit-identical results)
 template <typename T>
 void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold,
-                    F3<T> fmnew, DevStatus* st, hipStream_t stream);
+                    F3<T> fmnew, DevStatus* st, hipStream_t stream, int tiled = 0);
 // compensated: two-sum updates of position and displacement (uses xl, dl)
 template <typename T>
 void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, ParticleSet<T> ps,

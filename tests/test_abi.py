@@ -9,6 +9,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -95,3 +96,29 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(L, "_libs", {})
     with pytest.raises(RuntimeError, match="not built"):
         L.load("dp")
+
+
+@pytest.mark.parametrize("variant", ["sp", "dp"])
+def test_single_precision_channel_grid_is_accepted(variant, capfd):
+    """A rectilinear grid with walls on x, slip faces on z and a periodic y
+    axis, given in lcsdata_t: the periodic face ghosts take the spacing at one
+    end of y and the wall edges keep the mirror spacing of the other end, so
+    with float coordinates they differ by float rounding.  The grid set-up
+    (host code, no GPU needed) must accept it; without a GPU, init then stops
+    at the device check."""
+    import flows as F
+    import libcfd2lcs_amd as L
+    try:
+        lib = L.Cfd2lcs(variant)
+    except RuntimeError as e:
+        pytest.skip(str(e))
+    X, Y, Z = F.cell_grid((24, 16, 12), (0, 0, 0), (2 * np.pi,) * 3)
+    fl = np.full(X.shape, F.LCS_INTERNAL, dtype=np.int32)
+    fl[0], fl[-1] = F.LCS_WALL, F.LCS_WALL
+    fl[:, :, 0], fl[:, :, -1] = F.LCS_SLIP, F.LCS_SLIP
+    lib.finalize()
+    lib.init(X, Y, Z, fl)
+    out = capfd.readouterr().out
+    assert "not separable" not in out and "rectilinear" not in out, out
+    assert lib.error == 0 or "no HIP device" in out, out
+    lib.finalize()
