@@ -678,14 +678,30 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
       ws_.template multi<3>(ad, g, xs + 1, use, nd, ws + 1);
     }
     __syncthreads();
-    for (int e = t; e < 3 * WS::TILE; e += 64) {
-      const T v = ws_.tile[e];
-      if (v == T(0)) continue;
-      const int m = e / WS::TILE, r = e - m * WS::TILE;
-      const int cx = ox + r % WS::TX, cy = oy + (r / WS::TX) % WS::TY, cz = oz + r / (WS::TX * WS::TY);
-      if (cx < 0 || cy < 0 || cz < 0 || cx >= g.N[0] || cy >= g.N[1] || cz >= g.N[2]) continue;
-      T* d = dst.c[m] + cx + g.N[0] * (cy + g.N[1] * cz);
-      *d = *d + v;
+    // flush: batches of 8 independent read-modify-writes per lane, so that
+    // the global loads of a batch are in flight together
+    constexpr int kFlushBatch = 8;
+    for (int e0 = t; e0 < 3 * WS::TILE; e0 += 64 * kFlushBatch) {
+      T v[kFlushBatch], o[kFlushBatch];
+      T* d[kFlushBatch];
+#pragma unroll
+      for (int q = 0; q < kFlushBatch; q++) {
+        const int e = e0 + 64 * q;
+        d[q] = nullptr;
+        v[q] = T(0);
+        if (e < 3 * WS::TILE) {
+          v[q] = ws_.tile[e];
+          const int m = e / WS::TILE, r = e - m * WS::TILE;
+          const int cx = ox + r % WS::TX, cy = oy + (r / WS::TX) % WS::TY, cz = oz + r / (WS::TX * WS::TY);
+          if (v[q] != T(0) && cx >= 0 && cy >= 0 && cz >= 0 && cx < g.N[0] && cy < g.N[1] && cz < g.N[2])
+            d[q] = dst.c[m] + cx + g.N[0] * (cy + g.N[1] * cz);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kFlushBatch; q++) o[q] = d[q] ? *d[q] : T(0);
+#pragma unroll
+      for (int q = 0; q < kFlushBatch; q++)
+        if (d[q]) *d[q] = o[q] + v[q];
     }
     __syncthreads();
   }
