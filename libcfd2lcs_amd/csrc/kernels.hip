@@ -681,7 +681,7 @@ __global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restric
 // forward sweep leaves its result in place and the backward sweep reads it
 // back, as the strided passes do.  Same arithmetic, in the same order, as
 // band_solve.
-constexpr int kXC = 16;
+constexpr int kXC = 32;
 template <typename T, int K, bool TR>
 __global__ void __launch_bounds__(64) k_fit_x(F3<T> a, const T* __restrict__ W, int n, int nlines) {
   __shared__ T tile[64][kXC + 1];
