@@ -525,11 +525,15 @@ def main():
 
     def ftle_err(r):
         """FTLE of a run against the FP64 run at the same output (both
-        directions): max and RMS absolute error and max error relative to
-        the largest FP64 FTLE magnitude."""
+        directions): max and RMS absolute error, max error relative to the
+        largest FP64 FTLE magnitude, and per direction the error quantiles
+        and the FP64 FTLE where the largest error sits (a very negative
+        FTLE is a node whose displacement gradient nearly vanishes, where
+        log(sigma)/T amplifies any rounding of the flowmap)."""
         if r is None or r64 is None or "ftle" not in r or "ftle" not in r64:
             return None
         mx, ss, cnt, scale = 0.0, 0.0, 0, 0.0
+        per = {}
         for d in ("fwd", "bkwd"):
             a, b = r["ftle"][d], r64["ftle"][d]
             e = np.abs(a - b)
@@ -537,7 +541,13 @@ def main():
             ss += float((e * e).sum())
             cnt += e.size
             scale = max(scale, float(np.abs(b).max()))
-        return {"max_abs": mx, "rms": (ss / cnt) ** 0.5, "max_rel": mx / scale, "ftle_scale": scale}
+            q = np.quantile(e, [0.5, 0.99, 0.999])
+            k = int(np.argmax(e))
+            per[d] = {"max_abs": float(e.max()), "p50": float(q[0]), "p99": float(q[1]), "p999": float(q[2]),
+                      "ftle64_at_max": float(b.ravel()[k]), "ftle64_min": float(b.min()),
+                      "frac_above_1e-4_rel": float((e > 1e-4 * float(np.abs(b).max())).mean())}
+        return {"max_abs": mx, "rms": (ss / cnt) ** 0.5, "max_rel": mx / scale, "ftle_scale": scale,
+                "per_direction": per}
     # dominant kernel of the FP32 run
     ks = r32["kernel_sec"]
     dom = max(("k_sl_step", "k_fwd_step"), key=lambda k: ks[k])

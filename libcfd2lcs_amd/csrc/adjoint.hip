@@ -540,7 +540,8 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
     const T x0[3] = {g.gl[0][i], g.gl[1][j], g.gl[2][k]};
     const T dt = sp.dt;
     const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
-    // ---- recompute the stages (same expressions as integrate()) ----
+    // ---- recompute the stages (same expressions as integrate_inc(); the
+    // departure point is x0 plus the summed increments) ----
     T Js[4][3][3], up[3], kk[3], x1[3], xp[3];
     CF3<T> vf[4];
 #pragma unroll
@@ -571,7 +572,7 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           kk[d] = dt * up[d];
-          x1[d] = x0[d] + SIXTH * kk[d];
+          x1[d] = SIXTH * kk[d];
           xs[1][d] = x0[d] + HALF * kk[d];
         }
         ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
@@ -583,7 +584,7 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
         }
         ad.eval(g, xs[2], nd, vf[2], up, Js[2]);
 #pragma unroll
-        for (int d = 0; d < 3; d++) xp[d] = x1[d] + SIXTH * (dt * up[d]);
+        for (int d = 0; d < 3; d++) xp[d] = x0[d] + (x1[d] + SIXTH * (dt * up[d]));
         break;
       case kRK4:
       default:
@@ -592,7 +593,7 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           kk[d] = dt * up[d];
-          x1[d] = x0[d] + SIXTH * kk[d];
+          x1[d] = SIXTH * kk[d];
           xs[1][d] = x0[d] + HALF * kk[d];
         }
         ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
@@ -611,7 +612,7 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
         }
         ad.eval(g, xs[3], nd, vf[3], up, Js[3]);
 #pragma unroll
-        for (int d = 0; d < 3; d++) xp[d] = x1[d] + SIXTH * (dt * up[d]);
+        for (int d = 0; d < 3; d++) xp[d] = x0[d] + (x1[d] + SIXTH * (dt * up[d]));
         break;
     }
     // ---- BC of the departure point ----

@@ -421,103 +421,18 @@ struct IntSpline {
 // Particle operations
 // ---------------------------------------------------------------------------
 
-// integrate_lp stages (lp_motion_m.f90:129-328); RK3 evaluates k3 at
-// xn + 2 k2 as the reference does (lp_motion_m.f90:239-247).  The first
-// stage of every scheme samples the velocity at the start position xn, which
-// first(up) supplies (an interpolation, or the nodal value when xn is a node).
+// stage velocity field i of a step (time-blended values or coefficients)
 template <typename S, typename T>
 __device__ __forceinline__ CF3<S> stage(const StageParams<T>& sp, int i) {
   return CF3<S>{{(const S*)sp.v[i][0], (const S*)sp.v[i][1], (const S*)sp.v[i][2]}};
 }
 
-template <typename S, typename T, class V, class V0>
-__device__ __forceinline__ void integrate(const GridDev<T>& g, const StageParams<T>& sp, const V& vel,
-                                          const V0& first, const int nd[3], const T xn[3], T xp[3],
-                                          unsigned long long& oob) {
-  const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
-  const T dt = sp.dt;
-  T up[3], k[3], x1[3];
-  switch (sp.integ) {
-    case kEuler:
-    case kTrapezoidal:
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = xp[d] + dt * up[d];
-      break;
-    case kRK2:
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = xp[d] + HALF * dt * up[d];
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = xn[d] + dt * up[d];
-      break;
-    case kRK3:
-#pragma unroll
-      for (int d = 0; d < 3; d++) x1[d] = xn[d];
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        x1[d] = x1[d] + SIXTH * k[d];
-        xp[d] = xn[d] + HALF * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        xp[d] = xn[d] + TWO * k[d];
-        x1[d] = x1[d] + TWOTHIRD * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        xp[d] = x1[d] + SIXTH * k[d];
-      }
-      break;
-    case kRK4:
-    default:
-#pragma unroll
-      for (int d = 0; d < 3; d++) x1[d] = xn[d];
-      first(up);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        x1[d] = x1[d] + SIXTH * k[d];
-        xp[d] = xn[d] + HALF * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        x1[d] = x1[d] + THIRD * k[d];
-        xp[d] = xn[d] + HALF * k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        x1[d] = x1[d] + THIRD * k[d];
-        xp[d] = xn[d] + k[d];
-      }
-      vel(g, xp, nd, stage<S>(sp, 3), up, oob);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        k[d] = dt * up[d];
-        xp[d] = x1[d] + SIXTH * k[d];
-      }
-      break;
-  }
-}
-
-// project_lp2plane on a flat separable grid (lp_m.f90:418-538): the plane
-// normal is the collapsed axis, the node index on it becomes 1 and the
-// particle's coordinate loses its offset from the node plane.
-
-// The same stages in increment form for compensated updates: inc = x_new - xn
-// summed from the stage terms alone, so that no term is rounded against |xn|
-// (same stage positions and weights as integrate()).
+// integrate_lp stages (lp_motion_m.f90:129-328) in increment form:
+// inc = x_new - xn summed from the stage terms alone, so that no term is
+// rounded against |xn|.  RK3 evaluates k3 at xn + 2 k2 as the reference does
+// (lp_motion_m.f90:239-247).  The first stage of every scheme samples the
+// velocity at the start position xn, which first(up) supplies (an
+// interpolation, or the nodal value when xn is a node).
 template <typename S, typename T, class V, class V0>
 __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StageParams<T>& sp, const V& vel,
                                               const V0& first, const int nd[3], const T xn[3], T inc[3],
@@ -598,6 +513,9 @@ __device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
   e = (a - (s - bb)) + (b - bb);
 }
 
+// project_lp2plane on a flat separable grid (lp_m.f90:418-538): the plane
+// normal is the collapsed axis, the node index on it becomes 1 and the
+// particle's coordinate loses its offset from the node plane.
 template <typename T>
 __device__ __forceinline__ void project_flat(const GridDev<T>& g, int nd[3], T q[3]) {
   if (g.flat < 0) return;

@@ -196,7 +196,16 @@ __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T
 #pragma unroll
     for (int c = 0; c < 3; c++) up[c] = w0 * sp.un.c[c][l] + sp.ct0 * sp.unp1.c[c][l];
   };
-  integrate<S>(g, sp, interp, first, nd, x0, xp, oob);
+  // the departure point as x0 plus the summed stage increments: the
+  // displacement keeps the increments' precision instead of that of a
+  // difference of two rounded |x|-sized positions
+  T inc[3], xr[3];
+  integrate_inc<S>(g, sp, interp, first, nd, x0, inc, oob);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    xr[d] = x0[d] + inc[d];
+    xp[d] = xr[d];
+  }
   project_flat(g, nd, xp);
   if (set_lp_bc(g, nd, xp, err)) {
     xp[0] = x0[0]; xp[1] = x0[1]; xp[2] = x0[2];
@@ -209,7 +218,9 @@ __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T
   const bool zero = (f == kMask || f == kInflow || f == kOutflow);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    T v = (fmp[d] + (xp[d] - x0[d])) * mask;
+    // a boundary projection or stick moved the point: its displacement
+    const T disp = xp[d] == xr[d] ? inc[d] : xp[d] - x0[d];
+    T v = (fmp[d] + disp) * mask;
     fmnew.c[d][l] = zero ? T(0) : v;
   }
   flush_status(st, err, oob);
@@ -379,13 +390,20 @@ __global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T
   const I interp{};
   auto first = [&](T up[3]) { interp(g, xn, nd, stage<S>(sp, 0), up, oob); };
   if constexpr (!COMP) {
-    integrate<S>(g, sp, interp, first, nd, xn, q, oob);
+    // displacement from the summed stage increments (see sl_node)
+    T inc[3], qr[3];
+    integrate_inc<S>(g, sp, interp, first, nd, xn, inc, oob);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      qr[d] = xn[d] + inc[d];
+      q[d] = qr[d];
+    }
     project_flat(g, nd, q);
     if (set_lp_bc(g, nd, q, err)) {
       q[0] = xn[0]; q[1] = xn[1]; q[2] = xn[2];
     }
 #pragma unroll
-    for (int d = 0; d < 3; d++) ps.d[d][p] = ps.d[d][p] + (q[d] - xn[d]);
+    for (int d = 0; d < 3; d++) ps.d[d][p] = ps.d[d][p] + (q[d] == qr[d] ? inc[d] : q[d] - xn[d]);
     track_pass(g, q, nd);
     wrap(g, nd, q);
   } else {
