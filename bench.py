@@ -470,8 +470,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the adjoint workload (ABC 192^3, dt 0.02); 0 skips it")
     ap.add_argument("--adjoint-n", type=int, default=192)
-    ap.add_argument("--mixed", choices=["fp16", "bf16", "none"], default="fp16",
-                    help="also run FP32 advection with 16-bit stage velocities and compensated positions")
+    ap.add_argument("--mixed", choices=["fp16", "bf16", "fp32", "none"], default="fp16",
+                    help="also run FP32 advection with compensated positions and fp16/bf16 (or fp32) stage "
+                         "velocities")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run a fill (write 206.1 MB) and a plain copy b = 1.0 * a (read 206.1 MB, "
                          "write 206.1 MB) of 3*258^3 floats, the reference dispatches for scaling the "
@@ -511,7 +512,7 @@ def main():
     rmix = None
     if args.mixed != "none":
         import libcfd2lcs_amd as L
-        stor = {"fp16": L.LCS_FP16, "bf16": L.LCS_BF16}[args.mixed]
+        stor = {"fp16": L.LCS_FP16, "bf16": L.LCS_BF16, "fp32": L.LCS_FP32}[args.mixed]
         rmix = run_gpu("sp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle,
                        {"VELOCITY_STORAGE": stor, "COMPENSATED": L.LCS_TRUE})
         print(f"[bench] mixed ({args.mixed} velocity, compensated FP32 positions): "

@@ -99,11 +99,21 @@ struct LinW {
   T t[3], mt[3];
 };
 
+// A point known as a grid coordinate plus an offset (q = qb + off before
+// rounding): the interpolants form their local differences q - x_a as
+// (qb - x_a) + off, which is exact or nearly so for x_a a neighbouring grid
+// coordinate, instead of from the rounded |q|-sized position.
+template <typename T>
+struct Split {
+  T qb[3], off[3];
+};
+
 // LINEAR bracket relative to the tracked node, local coordinate clamped to
 // [0,1] (lp_tracking_m.f90:381-396,438-448).  A bracket that would leave the
 // ghosted range is shifted inside.
 template <typename T>
-__device__ __forceinline__ void lin_setup(const GridDev<T>& g, const T q[3], const int nd[3], LinW<T>& w) {
+__device__ __forceinline__ void lin_setup(const GridDev<T>& g, const T q[3], const int nd[3], LinW<T>& w,
+                                          const Split<T>* sq = nullptr) {
   int i0[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -112,7 +122,8 @@ __device__ __forceinline__ void lin_setup(const GridDev<T>& g, const T q[3], con
     int a = (q[d] > xl[n0]) ? n0 : n0 - 1;
     a = iclamp(a, 0, g.n[d]);
     T x0 = xl[a], x1 = xl[a + 1];
-    T tt = clamp01((q[d] - x0) / (x1 - x0));
+    const T dq = sq ? (sq->qb[d] - x0) + sq->off[d] : q[d] - x0;
+    T tt = clamp01(dq / (x1 - x0));
     w.t[d] = tt;
     w.mt[d] = T(1) - tt;
     i0[d] = a;
@@ -140,7 +151,7 @@ __device__ __forceinline__ T trilin(const LinW<T>& w, T f0, T f1, T f2, T f3, T 
 // bspline_sub_module.f90:634-636).
 template <typename T, int K>
 __device__ __forceinline__ int bs_setup(const T* __restrict__ t, const T* __restrict__ poly, int n, T x, bool xaxis,
-                                        int hint, int& base, T v[K]) {
+                                        int hint, int& base, T v[K], const T* xs = nullptr) {
   const int L = n + K;
   if (!(x >= t[0]) || x > t[L - 1]) return 1;
   int left;
@@ -153,7 +164,8 @@ __device__ __forceinline__ int bs_setup(const T* __restrict__ t, const T* __rest
     while (left > K && x < t[left - 1]) left--;
   }
   const T* P = poly + left * (K * K + 1);
-  const T s = (x - t[left - 1]) * P[K * K];
+  // xs: {base coordinate, offset} of x (Split)
+  const T s = (xs ? (xs[0] - t[left - 1]) + xs[1] : x - t[left - 1]) * P[K * K];
 #pragma unroll
   for (int a = 0; a < K; a++) {
     T w = P[a * K + K - 1];
@@ -225,11 +237,22 @@ struct SplineW {
 // nd: node next to q (tracked node), used as the interval hint
 template <typename T, int KX, int KZ>
 __device__ __forceinline__ int spline_setup(const GridDev<T>& g, const T q[3], const int nd[3],
-                                            SplineW<T, KX, KZ>& s) {
+                                            SplineW<T, KX, KZ>& s, const Split<T>* sq = nullptr) {
   int bx, by, bz;
-  int rx = bs_setup<T, KX>(g.kn[0], g.bpoly[0], g.N[0], q[0], true, nd[0] + (KX + 2) / 2, bx, s.wx);
-  int ry = bs_setup<T, KX>(g.kn[1], g.bpoly[1], g.N[1], q[1], false, nd[1] + (KX + 2) / 2, by, s.wy);
-  int rz = bs_setup<T, KZ>(g.kn[2], g.bpoly[2], g.N[2], q[2], false, nd[2] + (KZ + 2) / 2, bz, s.wz);
+  T xs[3][2];
+  if (sq) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      xs[d][0] = sq->qb[d];
+      xs[d][1] = sq->off[d];
+    }
+  }
+  int rx = bs_setup<T, KX>(g.kn[0], g.bpoly[0], g.N[0], q[0], true, nd[0] + (KX + 2) / 2, bx, s.wx,
+                           sq ? xs[0] : nullptr);
+  int ry = bs_setup<T, KX>(g.kn[1], g.bpoly[1], g.N[1], q[1], false, nd[1] + (KX + 2) / 2, by, s.wy,
+                           sq ? xs[1] : nullptr);
+  int rz = bs_setup<T, KZ>(g.kn[2], g.bpoly[2], g.N[2], q[2], false, nd[2] + (KZ + 2) / 2, bz, s.wz,
+                           sq ? xs[2] : nullptr);
   if (rx == 1 || ry == 1 || rz == 1) return 1;
   if (ry || rz) return 2;
   s.base = bx + g.N[0] * (by + g.N[1] * bz);
@@ -266,7 +289,7 @@ template <typename T>
 struct IntNearest {
   template <typename S>
   __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
-                             unsigned long long&) const {
+                             unsigned long long&, const Split<T>* = nullptr) const {
     int l = gid(g, iclamp(nd[0], 0, g.n[0] + 1), iclamp(nd[1], 0, g.n[1] + 1), iclamp(nd[2], 0, g.n[2] + 1));
     if constexpr (Packed<S>::value) {
       float v[3];
@@ -303,7 +326,7 @@ struct IntTSE {
   }
   template <typename S>
   __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
-                             unsigned long long&) const {
+                             unsigned long long&, const Split<T>* sq = nullptr) const {
     const int ix[3] = {iclamp(nd[0], 1, g.n[0]), iclamp(nd[1], 1, g.n[1]), iclamp(nd[2], 1, g.n[2])};
     const int st[3] = {1, g.N[0], g.N[0] * g.N[1]};
     const int l = gid(g, ix[0], ix[1], ix[2]);
@@ -316,7 +339,7 @@ struct IntTSE {
       if (g.n[d] == 1) continue;  // zeroed gradient column
       const T* G = g.gl[d];
       const T den = G[ix[d] + 1] - G[ix[d] - 1];
-      const T dl = q[d] - G[ix[d]];
+      const T dl = sq ? (sq->qb[d] - G[ix[d]]) + sq->off[d] : q[d] - G[ix[d]];
       T a[3], b[3];
       at(f, l + st[d], a);
       at(f, l - st[d], b);
@@ -347,9 +370,9 @@ template <typename T>
 struct IntLinear {
   template <typename S>
   __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
-                             unsigned long long&) const {
+                             unsigned long long&, const Split<T>* sq = nullptr) const {
     LinW<T> w;
-    lin_setup(g, q, nd, w);
+    lin_setup(g, q, nd, w, sq);
     const int b = w.base, sy = g.N[0], sz = g.N[0] * g.N[1];
     if constexpr (Packed<S>::value) {
       const S* A = f.c[0];
@@ -377,9 +400,9 @@ template <typename T, int KX, int KZ>
 struct IntSpline {
   template <typename S>
   __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
-                             unsigned long long& oob) const {
+                             unsigned long long& oob, const Split<T>* sq = nullptr) const {
     SplineW<T, KX, KZ> s;
-    int r = spline_setup(g, q, nd, s);
+    int r = spline_setup(g, q, nd, s, sq);
     if (r) {
       if (r == 1) oob++;
       out[0] = out[1] = out[2] = T(0);
@@ -436,10 +459,22 @@ __device__ __forceinline__ CF3<S> stage(const StageParams<T>& sp, int i) {
 template <typename S, typename T, class V, class V0>
 __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StageParams<T>& sp, const V& vel,
                                               const V0& first, const int nd[3], const T xn[3], T inc[3],
-                                              unsigned long long& oob) {
+                                              unsigned long long& oob, const T* xnl = nullptr) {
   const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
   const T dt = sp.dt;
   T up[3], k[3], xp[3];
+  // stage point xn + c k, handed to the interpolant as xn plus an offset
+  // (plus xnl, the low part of a compensated position)
+  Split<T> sq;
+  auto at = [&](T c, const T kk[3]) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const T o = c * kk[d];
+      xp[d] = xn[d] + o;
+      sq.qb[d] = xn[d];
+      sq.off[d] = xnl ? xnl[d] + o : o;
+    }
+  };
   switch (sp.integ) {
     case kEuler:
     case kTrapezoidal:
@@ -450,8 +485,9 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
     case kRK2:
       first(up);
 #pragma unroll
-      for (int d = 0; d < 3; d++) xp[d] = xn[d] + HALF * dt * up[d];
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
+      for (int d = 0; d < 3; d++) k[d] = dt * up[d];
+      at(HALF, k);
+      vel(g, xp, nd, stage<S>(sp, 1), up, oob, &sq);
 #pragma unroll
       for (int d = 0; d < 3; d++) inc[d] = dt * up[d];
       break;
@@ -461,16 +497,16 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         inc[d] = SIXTH * k[d];
-        xp[d] = xn[d] + HALF * k[d];
       }
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
+      at(HALF, k);
+      vel(g, xp, nd, stage<S>(sp, 1), up, oob, &sq);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
-        xp[d] = xn[d] + TWO * k[d];
         inc[d] = inc[d] + TWOTHIRD * k[d];
       }
-      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
+      at(TWO, k);
+      vel(g, xp, nd, stage<S>(sp, 2), up, oob, &sq);
 #pragma unroll
       for (int d = 0; d < 3; d++) inc[d] = inc[d] + SIXTH * (dt * up[d]);
       break;
@@ -481,23 +517,23 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         inc[d] = SIXTH * k[d];
-        xp[d] = xn[d] + HALF * k[d];
       }
-      vel(g, xp, nd, stage<S>(sp, 1), up, oob);
+      at(HALF, k);
+      vel(g, xp, nd, stage<S>(sp, 1), up, oob, &sq);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         inc[d] = inc[d] + THIRD * k[d];
-        xp[d] = xn[d] + HALF * k[d];
       }
-      vel(g, xp, nd, stage<S>(sp, 2), up, oob);
+      at(HALF, k);
+      vel(g, xp, nd, stage<S>(sp, 2), up, oob, &sq);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
         inc[d] = inc[d] + THIRD * k[d];
-        xp[d] = xn[d] + k[d];
       }
-      vel(g, xp, nd, stage<S>(sp, 3), up, oob);
+      at(T(1), k);
+      vel(g, xp, nd, stage<S>(sp, 3), up, oob, &sq);
 #pragma unroll
       for (int d = 0; d < 3; d++) inc[d] = inc[d] + SIXTH * (dt * up[d]);
       break;

@@ -212,7 +212,11 @@ __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T
   }
   int no[3] = {i, j, k};
   T fmp[3];
-  interp(g, xp, no, fmold, fmp, oob);
+  // the old flowmap at the departure point, located from the node and the
+  // increment unless a boundary condition moved the point
+  const Split<T> sq{{x0[0], x0[1], x0[2]}, {inc[0], inc[1], inc[2]}};
+  const bool split = xp[0] == xr[0] && xp[1] == xr[1] && xp[2] == xr[2];
+  interp(g, xp, no, fmold, fmp, oob, split ? &sq : nullptr);
   const int f = g.flag[l];
   const T mask = (f != kMask) ? T(1) : T(0);
   const bool zero = (f == kMask || f == kInflow || f == kOutflow);
@@ -265,9 +269,9 @@ struct IntSplineTiled {
   lcptr<T> tile;    // [field][component][TZ][TY][TX]
   int o[3];         // ghosted index of tile cell (0, 0, 0)
   __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
-                             unsigned long long& oob) const {
+                             unsigned long long& oob, const Split<T>* sq = nullptr) const {
     SplineW<T, KX, KZ> s;
-    const int r = spline_setup(g, q, nd, s);
+    const int r = spline_setup(g, q, nd, s, sq);
     if (r) {
       if (r == 1) oob++;
       out[0] = out[1] = out[2] = T(0);
@@ -388,7 +392,14 @@ __global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T
   unsigned long long oob = 0;
   int err = 0;
   const I interp{};
-  auto first = [&](T up[3]) { interp(g, xn, nd, stage<S>(sp, 0), up, oob); };
+  // compensated positions: the low parts, which every sample point carries
+  T xnl[3] = {T(0), T(0), T(0)};
+  if constexpr (COMP) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) xnl[d] = ps.xl[d][p];
+  }
+  const Split<T> s0{{xn[0], xn[1], xn[2]}, {xnl[0], xnl[1], xnl[2]}};
+  auto first = [&](T up[3]) { interp(g, xn, nd, stage<S>(sp, 0), up, oob, COMP ? &s0 : nullptr); };
   if constexpr (!COMP) {
     // displacement from the summed stage increments (see sl_node)
     T inc[3], qr[3];
@@ -409,9 +420,8 @@ __global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T
   } else {
     // position and displacement as unevaluated sums hi + lo, advanced with
     // two-sum; BC moves and stick reset the low part of what they change
-    const T xnl[3] = {ps.xl[0][p], ps.xl[1][p], ps.xl[2][p]};
     T ql[3], inc[3];
-    integrate_inc<S>(g, sp, interp, first, nd, xn, inc, oob);
+    integrate_inc<S>(g, sp, interp, first, nd, xn, inc, oob, xnl);
 #pragma unroll
     for (int d = 0; d < 3; d++) two_sum(xn[d], inc[d] + xnl[d], q[d], ql[d]);
     const T qb[3] = {q[0], q[1], q[2]};
