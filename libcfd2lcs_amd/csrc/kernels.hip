@@ -500,18 +500,31 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __res
   int nd[3] = {i, j, k};
   T q[3] = {g.gl[0][i], g.gl[1][j], g.gl[2][k]};
   T dxx[3] = {T(0), T(0), T(0)};
+  // the particle as base + offset: the base is its node coordinate (exact)
+  // until a boundary condition or a periodic relocation moves it, and the
+  // offset accumulates the slot displacements, so that interpolation sees
+  // the position without the rounding of an |x|-sized sum
+  Split<T> sq{{q[0], q[1], q[2]}, {T(0), T(0), T(0)}};
   unsigned long long oob = 0;
   int err = 0;
   for (int s = 0; s < nslot; s++) {
     const T* base = slots + (long long)order[s] * 3 * stride;
     const CF3<T> f{{base, base + stride, base + 2 * stride}};
+    const T qr[3] = {q[0], q[1], q[2]};
     project_flat(g, nd, q);
     set_lp_bc(g, nd, q, err);
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      if (q[d] != qr[d]) {
+        sq.qb[d] = q[d];
+        sq.off[d] = T(0);
+      }
     T up[3];
-    I{}(g, q, nd, f, up, oob);
+    I{}(g, q, nd, f, up, oob, &sq);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      q[d] = q[d] + up[d];
+      sq.off[d] = sq.off[d] + up[d];
+      q[d] = sq.qb[d] + sq.off[d];
       dxx[d] = dxx[d] + up[d];
     }
     // relocation to the bounding box or its first periodic image
@@ -521,6 +534,13 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __res
       for (int c = 0; c < 27 && !in; c++) {
         const T y0 = q[0] - g.ps[c][0], y1 = q[1] - g.ps[c][1], y2 = q[2] - g.ps[c][2];
         if (y0 >= bb[0] && y0 <= bb[3] && y1 >= bb[1] && y1 <= bb[4] && y2 >= bb[2] && y2 <= bb[5]) {
+          const T y[3] = {y0, y1, y2};
+#pragma unroll
+          for (int d = 0; d < 3; d++)
+            if (y[d] != q[d]) {
+              sq.qb[d] = y[d];
+              sq.off[d] = T(0);
+            }
           q[0] = y0; q[1] = y1; q[2] = y2;
           in = true;
         }
@@ -538,7 +558,14 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __res
         err |= kErrTrack;
         break;
       }
-      wrap(g, nd, q);
+      T sh[3];
+      wrap(g, nd, q, sh);
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        if (sh[d] != T(0)) {
+          sq.qb[d] = q[d];
+          sq.off[d] = T(0);
+        }
       if (!moved) break;
     }
   }
