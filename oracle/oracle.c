@@ -73,6 +73,19 @@ typedef struct {
 
 #define GID(g, i, j, k) \
   ((long)(i) + (long)(g)->N[0] * ((long)(j) + (long)(g)->N[1] * (long)(k)))
+#define PIDG(g, i, j, k) \
+  ((long)((i) - 1) + (long)(g)->n[0] * ((long)((j) - 1) + (long)(g)->n[1] * (long)((k) - 1)))
+
+/* A grid with its interpolation lines and spline work arrays: the flow grid
+ * (scfd%sgrid), or the grid of a diagnostic's particle set when that is not
+ * the flow grid (resolution != 0, new_sgrid_from_sgrid, sgrid_m.f90:445-761;
+ * auxiliary grids, create_aux_grid, lcs_m.f90:227-435). */
+typedef struct {
+  ogrid g;
+  double *xl[3];  /* coordinate lines with 2-D overrides (lp_tracking_m.f90:362-378) */
+  double *bst[3]; /* spline knots */
+  double *bsc[3]; /* spline coefficients */
+} ofg;
 
 /* single-rank SELF_COMM test for direction (i,j,k) (comms_m.f90:258-283) */
 static int self_comm(const ogrid *g, int i, int j, int k) {
@@ -779,42 +792,48 @@ void oc_cauchy_green(const double *g, double *c) {
 /* ======================================================================
  * Library state (scfd_t, lp_t, lcs_t: data_m.f90:317-377)
  * ====================================================================== */
-typedef struct {
+/* One particle set (lp_t, data_m.f90:317-337) with the diagnostic state
+ * that goes with it.  A diagnostic owns one set on its grid and, with
+ * AUX_GRID, six more on the shifted auxiliary grids (lcs_m.f90:51-53). */
+typedef struct odiag_ {
   int id, type, dir;
   double T, h, dt_factor;
   char label[132];
+  ofg *L;          /* lcs%sgrid / lp%sgrid: the flow grid or the set's own */
+  int own_grid;    /* L was allocated for this set */
+  long np;         /* particles = nodes of L */
   /* particles in origin order */
   double *xp[3], *dx[3], *up[3];
   int *no[3], *ns[3], *pflag;
-  double *fm[3];   /* lp%fm (ghosted) */
+  double *fm[3];   /* lp%fm (ghosted, on L) */
   int nstep, nslot;
   double **slot;   /* nslot * 3 interior fields */
   int *slot_ok;
   double *comp[3]; /* composed T flowmap (interior) */
   double *ftle;
   int noutput;
+  int naux;                /* 6 with AUX_GRID, else 0 */
+  struct odiag_ *auxs[6];  /* lpX0, lpY0, lpZ0, lpX1, lpY1, lpZ1 */
 } odiag;
 
 struct ocfd {
-  ogrid g;
+  ofg F;                  /* the flow grid (scfd%sgrid) */
   double t_n, t_np1;
   int first_call;
   double *un[3], *unp1[3];
   int ndiag;
   odiag *d[NMAX_STRUCT];
   int integrator, interpolator, incompressible, error;
+  int aux_grid;           /* AUX_GRID (data_m.f90:39) */
   double cfl_max;
   int nsub_last;
   long long integ_fwd, integ_bkwd;
   int warn_opt, warn_par; /* SAVEd warn flags (cfd2lcs.f90:444,632) */
-  double *xl[3];          /* coordinate lines with 2-D overrides (lp_tracking_m.f90:362-378) */
   double *work[3];        /* blended velocity */
-  double *bsc[3];         /* spline coefficients */
-  double *bst[3];         /* spline knots */
   long oob;               /* out-of-range spline evaluations */
 };
 
-#define PID(s, i, j, k) ((long)((i) - 1) + (long)(s)->g.n[0] * ((long)((j) - 1) + (long)(s)->g.n[1] * (long)((k) - 1)))
+#define AUX_GRID_SCALING 0.2 /* data_m.f90:92 */
 
 /* ---------------- interpolation (interp_s2u_r1, lp_tracking_m.f90:279-662) */
 typedef struct {
@@ -826,13 +845,13 @@ typedef struct {
 /* LINEAR (lp_tracking_m.f90:342-474).  The bracket follows the tracked node;
  * an index that would leave the ghosted range (the reference reads out of
  * bounds there) is shifted back inside. */
-static void interp_linear(const ocfd *s, double *const f[3], const double p[3],
+static void interp_linear(const ofg *G, double *const f[3], const double p[3],
                           const int nd[3], double out[3]) {
-  const ogrid *g = &s->g;
+  const ogrid *g = &G->g;
   int i0[3], i1[3];
   double tt[3], mt[3];
   for (int d = 0; d < 3; d++) {
-    const double *xl = s->xl[d];
+    const double *xl = G->xl[d];
     int n0 = nd[d];
     if (n0 < 0) n0 = 0;
     if (n0 > g->n[d] + 1) n0 = g->n[d] + 1;
@@ -867,26 +886,26 @@ static void spline_orders(const ogrid *g, int interp, int k[3]) {
   }
 }
 
-/* fit (db3ink) the three components of a ghosted field into s->bsc */
-static int spline_fit(ocfd *s, double *const f[3], int interp) {
-  const ogrid *g = &s->g;
+/* fit (db3ink) the three components of a ghosted field into G->bsc */
+static int spline_fit(ofg *G, double *const f[3], int interp) {
+  const ogrid *g = &G->g;
   int k[3];
   spline_orders(g, interp, k);
   for (int c = 0; c < 3; c++) {
-    int e = oc_bspline_fit3(s->xl[0], g->N[0], s->xl[1], g->N[1], s->xl[2], g->N[2], k[0], k[1],
-                            k[2], f[c], s->bst[0], s->bst[1], s->bst[2], s->bsc[c]);
+    int e = oc_bspline_fit3(G->xl[0], g->N[0], G->xl[1], g->N[1], G->xl[2], g->N[2], k[0], k[1],
+                            k[2], f[c], G->bst[0], G->bst[1], G->bst[2], G->bsc[c]);
     if (e) return e;
   }
   return 0;
 }
 
-static void spline_eval(ocfd *s, int interp, const double p[3], double out[3]) {
-  const ogrid *g = &s->g;
+static void spline_eval(ocfd *s, const ofg *G, int interp, const double p[3], double out[3]) {
+  const ogrid *g = &G->g;
   int k[3], fl;
   spline_orders(g, interp, k);
   for (int c = 0; c < 3; c++) {
-    out[c] = oc_bspline_eval3(p[0], p[1], p[2], s->bst[0], s->bst[1], s->bst[2], g->N[0], g->N[1],
-                              g->N[2], k[0], k[1], k[2], s->bsc[c], &fl);
+    out[c] = oc_bspline_eval3(p[0], p[1], p[2], G->bst[0], G->bst[1], G->bst[2], g->N[0], g->N[1],
+                              g->N[2], k[0], k[1], k[2], G->bsc[c], &fl);
     if (fl > 0) {
       #pragma omp atomic
       s->oob++;
@@ -899,9 +918,9 @@ static void spline_eval(ocfd *s, int interp, const double p[3], double out[3]) {
  * being grad_sr1's central difference over the ghosted grid coordinates
  * (gradient_m.f90:11-66, columns of one-node axes zeroed); TSE_LIMIT clamps
  * to the min/max of f over the 3x3x3 neighbourhood (594-643). */
-static void interp_tse(const ocfd *s, double *const f[3], const double q[3], const int n3[3], int limit,
+static void interp_tse(const ofg *G, double *const f[3], const double q[3], const int n3[3], int limit,
                        double o[3]) {
-  const ogrid *g = &s->g;
+  const ogrid *g = &G->g;
   const int i = iclamp(n3[0], 1, g->n[0]), j = iclamp(n3[1], 1, g->n[1]), k = iclamp(n3[2], 1, g->n[2]);
   const long l = GID(g, i, j, k);
   const long xp = GID(g, i + 1, j, k), xm = GID(g, i - 1, j, k);
@@ -933,16 +952,15 @@ static void interp_tse(const ocfd *s, double *const f[3], const double q[3], con
   }
 }
 
-/* Interpolate field f at all particle positions pos using node array nd.
- * For spline interpolators the field is fitted once (as the reference does
- * on every call, lp_tracking_m.f90:509-511). */
-static int interp_field(ocfd *s, double *const f[3], double *const pos[3],
+/* Interpolate field f (on grid G) at the np particle positions pos using
+ * node array nd.  For spline interpolators the field is fitted once (as the
+ * reference does on every call, lp_tracking_m.f90:509-511). */
+static int interp_field(ocfd *s, ofg *G, long np, double *const f[3], double *const pos[3],
                         int *const nd[3], double *const out[3]) {
-  const ogrid *g = &s->g;
-  long np = g->np;
+  const ogrid *g = &G->g;
   int it = s->interpolator;
   if (it == QUADRATIC || it == CUBIC) {
-    int e = spline_fit(s, f, it);
+    int e = spline_fit(G, f, it);
     if (e) return e;
   }
   if (it != NEAREST_NBR && it != LINEAR && it != QUADRATIC && it != CUBIC && it != TSE && it != TSE_LIMIT) return 1;
@@ -957,11 +975,11 @@ static int interp_field(ocfd *s, double *const f[3], double *const pos[3],
         o[c] = f[c][GID(g, a, b, cc)];
       }
     } else if (it == LINEAR) {
-      interp_linear(s, f, q, n3, o);
+      interp_linear(G, f, q, n3, o);
     } else if (it == TSE || it == TSE_LIMIT) {
-      interp_tse(s, f, q, n3, it == TSE_LIMIT, o);
+      interp_tse(G, f, q, n3, it == TSE_LIMIT, o);
     } else {
-      spline_eval(s, it, q, o);
+      spline_eval(s, G, it, q, o);
     }
     for (int c = 0; c < 3; c++) out[c][p] = o[c];
   }
@@ -970,18 +988,12 @@ static int interp_field(ocfd *s, double *const f[3], double *const pos[3],
 
 /* ---------------- particle operations */
 
-/* project_lp2plane (lp_m.f90:418-538) */
-static void project_lp2plane(ocfd *s, odiag *dg, int *const nd[3]) {
-  const ogrid *g = &s->g;
+/* project_lp2plane's move of one point q onto the data plane through the
+ * node n3 of a 2-D grid g (lp_m.f90:450-536) */
+static void project_point(const ogrid *g, const int n3[3], double q[3]) {
   const int ni = g->n[0], nj = g->n[1], nk = g->n[2];
-  if (ni > 1 && nj > 1 && nk > 1) return;
-  long np = g->np;
-  if (ni == 1) for (long p = 0; p < np; p++) { dg->no[0][p] = 1; dg->ns[0][p] = 1; }
-  if (nj == 1) for (long p = 0; p < np; p++) { dg->no[1][p] = 1; dg->ns[1][p] = 1; }
-  if (nk == 1) for (long p = 0; p < np; p++) { dg->no[2][p] = 1; dg->ns[2][p] = 1; }
-  #pragma omp parallel for schedule(static)
-  for (long p = 0; p < np; p++) {
-    int i = iclamp(nd[0][p], 1, ni), j = iclamp(nd[1][p], 1, nj), k = iclamp(nd[2][p], 1, nk);
+  {
+    int i = iclamp(n3[0], 1, ni), j = iclamp(n3[1], 1, nj), k = iclamp(n3[2], 1, nk);
     double v1[3], v2[3], nrm[3];
     for (int d = 0; d < 3; d++) {
       const double *G = g->g[d];
@@ -1005,23 +1017,40 @@ static void project_lp2plane(ocfd *s, odiag *dg, int *const nd[3]) {
     for (int d = 0; d < 3; d++) nrm[d] = nrm[d] / mag;
     long l = GID(g, i, j, k);
     double d2p[3];
-    for (int d = 0; d < 3; d++) d2p[d] = dg->xp[d][p] - g->g[d][l];
+    for (int d = 0; d < 3; d++) d2p[d] = q[d] - g->g[d][l];
     double nm = d2p[0] * nrm[0] + d2p[1] * nrm[1] + d2p[2] * nrm[2];
     if (nm < 0.0) {
       nm = -nm;
       for (int d = 0; d < 3; d++) nrm[d] = -nrm[d];
     }
-    for (int d = 0; d < 3; d++) dg->xp[d][p] = dg->xp[d][p] - nm * nrm[d];
+    for (int d = 0; d < 3; d++) q[d] = q[d] - nm * nrm[d];
+  }
+}
+
+/* project_lp2plane (lp_m.f90:418-538) of a particle set against grid g with
+ * node array nd; both node arrays get index 1 along the one-node axis */
+static void project_lp2plane(const ogrid *g, odiag *dg, int *const nd[3]) {
+  const int ni = g->n[0], nj = g->n[1], nk = g->n[2];
+  if (ni > 1 && nj > 1 && nk > 1) return;
+  long np = dg->np;
+  if (ni == 1) for (long p = 0; p < np; p++) { dg->no[0][p] = 1; dg->ns[0][p] = 1; }
+  if (nj == 1) for (long p = 0; p < np; p++) { dg->no[1][p] = 1; dg->ns[1][p] = 1; }
+  if (nk == 1) for (long p = 0; p < np; p++) { dg->no[2][p] = 1; dg->ns[2][p] = 1; }
+  #pragma omp parallel for schedule(static)
+  for (long p = 0; p < np; p++) {
+    const int n3[3] = {nd[0][p], nd[1][p], nd[2][p]};
+    double q[3] = {dg->xp[0][p], dg->xp[1][p], dg->xp[2][p]};
+    project_point(g, n3, q);
+    for (int d = 0; d < 3; d++) dg->xp[d][p] = q[d];
   }
 }
 
 /* set_lp_bc (lp_tracking_m.f90:185-277) */
-static int set_lp_bc(ocfd *s, odiag *dg, int *const nd[3]) {
-  const ogrid *g = &s->g;
+static int set_lp_bc(const ogrid *g, odiag *dg, int *const nd[3]) {
   const int ni = g->n[0], nj = g->n[1], nk = g->n[2];
   int err = 0;
   #pragma omp parallel for schedule(static) reduction(| : err)
-  for (long p = 0; p < g->np; p++) {
+  for (long p = 0; p < dg->np; p++) {
     int i = iclamp(nd[0][p], 1, ni), j = iclamp(nd[1][p], 1, nj), k = iclamp(nd[2][p], 1, nk);
     nd[0][p] = i; nd[1][p] = j; nd[2][p] = k;
     dg->pflag[p] = LP_IB;
@@ -1076,10 +1105,11 @@ static int track_pass(const ogrid *g, const double q[3], int n3[3]) {
   return off[0] != 0 || off[1] != 0 || off[2] != 0;
 }
 
-/* exchange_lpdata on one rank (comms_m.f90:872-1189): a particle whose
- * tracked node left [1,n] along periodic directions is shifted by
- * periodic_shift and both node indices are wrapped. */
-static void wrap_particle(const ogrid *g, odiag *dg, long p, const int n3[3]) {
+/* exchange_lpdata on one rank (comms_m.f90:872-1189) for tracking on grid
+ * g: a particle whose tracked node n3 left [1,n] along periodic directions
+ * is shifted by g's periodic_shift; lp%no is wrapped by g's extents and
+ * lp%no_scfd by the flow grid's (comms_m.f90:1045-1073). */
+static void wrap_particle(const ogrid *g, const ogrid *gf, odiag *dg, long p, const int n3[3]) {
   int c[3];
   for (int d = 0; d < 3; d++) c[d] = n3[d] < 1 ? -1 : (n3[d] > g->n[d] ? 1 : 0);
   if (c[0] == 0 && c[1] == 0 && c[2] == 0) return;
@@ -1092,40 +1122,62 @@ static void wrap_particle(const ogrid *g, odiag *dg, long p, const int n3[3]) {
     if (v < 1) v += g->n[d];
     dg->no[d][p] = v;
     v = dg->ns[d][p];
-    if (v > g->n[d]) v -= g->n[d];
-    if (v < 1) v += g->n[d];
+    if (v > gf->n[d]) v -= gf->n[d];
+    if (v < 1) v += gf->n[d];
     dg->ns[d][p] = v;
   }
   dg->pflag[p] = LP_IB;
 }
 
-/* reset_lp (lp_m.f90:228-294) for resolution 0 */
+/* reset_lp (lp_m.f90:228-294): particles on the nodes of the set's grid, the
+ * flow-grid node guessed as nint(real(i*n_flow)/real(n)) (default REAL) */
 static void reset_lp(ocfd *s, odiag *dg) {
-  const ogrid *g = &s->g;
+  const ogrid *g = &dg->L->g, *gf = &s->F.g;
   for (int k = 1; k <= g->n[2]; k++)
     for (int j = 1; j <= g->n[1]; j++)
       for (int i = 1; i <= g->n[0]; i++) {
-        long p = PID(s, i, j, k), l = GID(g, i, j, k);
+        long p = PIDG(g, i, j, k), l = GID(g, i, j, k);
         int ijk[3] = {i, j, k};
         for (int d = 0; d < 3; d++) {
           dg->xp[d][p] = g->g[d][l];
           dg->up[d][p] = 0; dg->dx[d][p] = 0;
           dg->no[d][p] = ijk[d];
-          /* nint(real(i*n)/real(n)) with default REAL */
-          float r = (float)(ijk[d] * g->n[d]) / (float)g->n[d];
-          dg->ns[d][p] = iclamp(fnint((double)r), 1, g->n[d]);
+          float r = (float)(ijk[d] * gf->n[d]) / (float)g->n[d];
+          dg->ns[d][p] = iclamp(fnint((double)r), 1, gf->n[d]);
         }
         dg->pflag[p] = LP_UNKNOWN;
       }
   for (int d = 0; d < 3; d++) memset(dg->fm[d], 0, sizeof(double) * g->nt);
 }
 
-/* recursive track_lp2node of reset particles: every particle already sits on
- * its node, so no node moves and nothing is exchanged (lp_tracking_m.f90:16-183) */
+/* recursive track_lp2node of a reset set onto the flow grid
+ * (lp_tracking_m.f90:16-183), with exchange_lpdata after every pass for
+ * forward sets.  Passes repeat until no node moves; more than
+ * 2*max(n) passes is an error.  A particle that did not move stays unmoved
+ * in later passes, so each particle is run to its own fixed point.  On the
+ * flow grid itself every particle sits on its node and nothing moves. */
+static int track_to_flow(ocfd *s, odiag *dg) {
+  const ogrid *gf = &s->F.g;
+  const int max_track = 2 * imax3(gf->n[0], gf->n[1], gf->n[2]);
+  int err = 0;
+  #pragma omp parallel for schedule(static) reduction(| : err)
+  for (long p = 0; p < dg->np; p++) {
+    for (int pass = 1;; pass++) {
+      double q[3] = {dg->xp[0][p], dg->xp[1][p], dg->xp[2][p]};
+      int n3[3] = {dg->ns[0][p], dg->ns[1][p], dg->ns[2][p]};
+      int moved = track_pass(gf, q, n3);
+      for (int d = 0; d < 3; d++) dg->ns[d][p] = n3[d];
+      if (pass > max_track) { err |= 1; break; }
+      if (dg->dir == DIR_FWD) wrap_particle(gf, gf, dg, p, n3);
+      if (!moved) break;
+    }
+  }
+  return err;
+}
 
 /* set_dt (lp_motion_m.f90:56-127) */
 static double set_dt(ocfd *s, odiag *dg, int *nsub) {
-  const ogrid *g = &s->g;
+  const ogrid *g = &s->F.g;
   double dt_f = s->t_np1 - s->t_n;
   double mx = -DBL_MAX;
   for (int k = 1; k <= g->n[2]; k++)
@@ -1145,8 +1197,8 @@ static double set_dt(ocfd *s, odiag *dg, int *nsub) {
   double dt = dt_f / (double)n;
   if (dg->dir == DIR_BKWD) dt = -1.0 * dt;
   if (fabs(dt) > 0.0) {
-    if (dg->dir == DIR_FWD) s->integ_fwd += (long long)g->np * n;
-    if (dg->dir == DIR_BKWD) s->integ_bkwd += (long long)g->np * n;
+    if (dg->dir == DIR_FWD) s->integ_fwd += (long long)dg->np * n;
+    if (dg->dir == DIR_BKWD) s->integ_bkwd += (long long)dg->np * n;
   }
   s->nsub_last = n;
   return dt;
@@ -1155,7 +1207,7 @@ static double set_dt(ocfd *s, odiag *dg, int *nsub) {
 /* blended velocity u = (1-ct) u_n + ct u_np1 over the ghosted grid */
 static void blend(ocfd *s, double t, double c_dt) {
   double ct = clamp01((t + c_dt - s->t_n) / (s->t_np1 - s->t_n));
-  long nt = s->g.nt;
+  long nt = s->F.g.nt;
   for (int d = 0; d < 3; d++) {
     double *w = s->work[d];
     const double *a = s->un[d], *b = s->unp1[d];
@@ -1166,8 +1218,7 @@ static void blend(ocfd *s, double t, double c_dt) {
 
 /* integrate_lp (lp_motion_m.f90:129-360) */
 static int integrate_lp(ocfd *s, odiag *dg, double *t, double dt) {
-  const ogrid *g = &s->g;
-  const long np = g->np;
+  const long np = dg->np;
   const double ONEHALF = 1.0 / 2.0, ONETHIRD = 1.0 / 3.0, TWOTHIRD = 2.0 / 3.0,
                ONESIXTH = 1.0 / 6.0, TWO = 2.0;
   double *xn[3], *kk[3], *xnp1[3];
@@ -1180,7 +1231,7 @@ static int integrate_lp(ocfd *s, odiag *dg, double *t, double dt) {
   }
   int err = 0;
   double *const *u = s->work;
-#define INTERP() err |= interp_field(s, u, dg->xp, dg->ns, dg->up)
+#define INTERP() err |= interp_field(s, &s->F, np, u, dg->xp, dg->ns, dg->up)
 #define PLOOP for (long p = 0; p < np; p++) for (int d = 0; d < 3; d++)
   switch (s->integrator) {
   case EULER:
@@ -1233,8 +1284,9 @@ static int integrate_lp(ocfd *s, odiag *dg, double *t, double dt) {
 #undef INTERP
 #undef PLOOP
   *t = *t + dt;
-  project_lp2plane(s, dg, dg->ns);
-  err |= set_lp_bc(s, dg, dg->ns);
+  /* the flow grid for the BC (lp_motion_m.f90:333-337) */
+  project_lp2plane(&s->F.g, dg, dg->ns);
+  err |= set_lp_bc(&s->F.g, dg, dg->ns);
   for (long p = 0; p < np; p++) {
     if (dg->pflag[p] == LP_STICK)
       for (int d = 0; d < 3; d++) dg->xp[d][p] = xn[d][p];
@@ -1246,7 +1298,7 @@ static int integrate_lp(ocfd *s, odiag *dg, double *t, double dt) {
 
 /* update_lp (lp_motion_m.f90:15-54): forward particles */
 static int update_lp(ocfd *s, odiag *dg) {
-  const ogrid *g = &s->g;
+  const ogrid *g = &s->F.g;
   int nsub, err = 0;
   double dt = set_dt(s, dg, &nsub);
   double t = s->t_n;
@@ -1254,26 +1306,28 @@ static int update_lp(ocfd *s, odiag *dg) {
     err |= integrate_lp(s, dg, &t, dt);
     /* track_lp2node (non-recursive) + exchange_lpdata on no_scfd */
     #pragma omp parallel for schedule(static)
-    for (long p = 0; p < g->np; p++) {
+    for (long p = 0; p < dg->np; p++) {
       double q[3] = {dg->xp[0][p], dg->xp[1][p], dg->xp[2][p]};
       int n3[3] = {dg->ns[0][p], dg->ns[1][p], dg->ns[2][p]};
       track_pass(g, q, n3);
       for (int d = 0; d < 3; d++) dg->ns[d][p] = n3[d];
-      wrap_particle(g, dg, p, n3);
+      wrap_particle(g, g, dg, p, n3);
     }
   }
   return err;
 }
 
-static void fm_ghost_fill(ocfd *s, double *f[3]) {
-  for (int d = 0; d < 3; d++) exchange(&s->g, f[d], -1, NULL, 0);
-  if (set_flowmap_bc(&s->g, f)) s->error = 1;
+static void fm_ghost_fill(ocfd *s, const ogrid *g, double *f[3]) {
+  for (int d = 0; d < 3; d++) exchange(g, f[d], -1, NULL, 0);
+  if (set_flowmap_bc(g, f)) s->error = 1;
 }
 
-/* update_flowmap_sl (flowmap_m.f90:17-104): backward semi-Lagrangian step */
+/* update_flowmap_sl (flowmap_m.f90:17-104): backward semi-Lagrangian step.
+ * Velocity, BC and mask on the flow grid through lp%no_scfd; the flowmap on
+ * the set's grid through lp%no. */
 static int update_flowmap_sl(ocfd *s, odiag *dg) {
-  const ogrid *g = &s->g;
-  const long np = g->np;
+  const ogrid *g = &dg->L->g, *gf = &s->F.g;
+  const long np = dg->np;
   int nsub, err = 0;
   double *lpg[3], *fmp[3], *mask = malloc(np * sizeof(double));
   for (int d = 0; d < 3; d++) {
@@ -1283,17 +1337,17 @@ static int update_flowmap_sl(ocfd *s, odiag *dg) {
   }
   double dt = set_dt(s, dg, &nsub);
   for (long p = 0; p < np; p++)
-    mask[p] = g->flag[GID(g, dg->ns[0][p], dg->ns[1][p], dg->ns[2][p])] != LCS_MASK ? 1.0 : 0.0;
+    mask[p] = gf->flag[GID(gf, dg->ns[0][p], dg->ns[1][p], dg->ns[2][p])] != LCS_MASK ? 1.0 : 0.0;
   double t = s->t_np1;
   for (int sc = 0; sc < nsub; sc++) {
     err |= integrate_lp(s, dg, &t, dt);
-    err |= interp_field(s, dg->fm, dg->xp, dg->no, fmp);
+    err |= interp_field(s, dg->L, np, dg->fm, dg->xp, dg->no, fmp);
     for (long p = 0; p < np; p++) {
       long l = GID(g, dg->no[0][p], dg->no[1][p], dg->no[2][p]);
       for (int d = 0; d < 3; d++)
         dg->fm[d][l] = (fmp[d][p] + (dg->xp[d][p] - lpg[d][p])) * mask[p];
     }
-    fm_ghost_fill(s, dg->fm);
+    fm_ghost_fill(s, g, dg->fm);
     for (int d = 0; d < 3; d++) memcpy(dg->xp[d], lpg[d], np * sizeof(double));
   }
   for (int d = 0; d < 3; d++) { free(lpg[d]); free(fmp[d]); }
@@ -1305,7 +1359,7 @@ static int update_flowmap_sl(ocfd *s, odiag *dg) {
  * track_lp2node with exchange_lpdata after every pass (the direction is set
  * to FWD during reconstruction, flowmap_m.f90:211). */
 static int relocate_and_track(ocfd *s, odiag *dg, long p) {
-  const ogrid *g = &s->g;
+  const ogrid *g = &dg->L->g;
   const double *bb = g->bb;
 #define INBB(x, y, z) ((x) >= bb[0] && (x) <= bb[3] && (y) >= bb[1] && (y) <= bb[4] && (z) >= bb[2] && (z) <= bb[5])
   double x = dg->xp[0][p], y = dg->xp[1][p], z = dg->xp[2][p];
@@ -1337,7 +1391,7 @@ static int relocate_and_track(ocfd *s, odiag *dg, long p) {
     int moved = track_pass(g, q, n3);
     for (int d = 0; d < 3; d++) dg->no[d][p] = n3[d];
     if (pass > max_track) return 1;
-    wrap_particle(g, dg, p, n3);
+    wrap_particle(g, &s->F.g, dg, p, n3);
     if (!moved) break;
   }
   return 0;
@@ -1346,8 +1400,8 @@ static int relocate_and_track(ocfd *s, odiag *dg, long p) {
 /* reconstruct_flowmap (flowmap_m.f90:158-323) followed by exchange_lpmap
  * (comms_m.f90:1191-1376) and the flowmap BC */
 static int reconstruct_flowmap(ocfd *s, odiag *dg) {
-  const ogrid *g = &s->g;
-  const long np = g->np;
+  const ogrid *g = &dg->L->g;
+  const long np = dg->np;
   int err = 0;
   reset_lp(s, dg);
   int step = fnint(fmod(s->t_np1, dg->T) / dg->h);
@@ -1362,11 +1416,11 @@ static int reconstruct_flowmap(ocfd *s, odiag *dg) {
       for (int j = 1; j <= g->n[1]; j++)
         for (int i = 1; i <= g->n[0]; i++)
           for (int d = 0; d < 3; d++)
-            dg->fm[d][GID(g, i, j, k)] = dg->slot[3 * step + d][PID(s, i, j, k)];
-    fm_ghost_fill(s, dg->fm);
-    project_lp2plane(s, dg, dg->no);
-    err |= set_lp_bc(s, dg, dg->no);
-    err |= interp_field(s, dg->fm, dg->xp, dg->no, dg->up);
+            dg->fm[d][GID(g, i, j, k)] = dg->slot[3 * step + d][PIDG(g, i, j, k)];
+    fm_ghost_fill(s, g, dg->fm);
+    project_lp2plane(g, dg, dg->no);
+    err |= set_lp_bc(g, dg, dg->no);
+    err |= interp_field(s, dg->L, np, dg->fm, dg->xp, dg->no, dg->up);
     int terr = 0;
     #pragma omp parallel for schedule(static) reduction(| : terr)
     for (long p = 0; p < np; p++) {
@@ -1384,40 +1438,57 @@ static int reconstruct_flowmap(ocfd *s, odiag *dg) {
     for (int j = 1; j <= g->n[1]; j++)
       for (int i = 1; i <= g->n[0]; i++)
         for (int d = 0; d < 3; d++)
-          dg->fm[d][GID(g, i, j, k)] = dg->dx[d][PID(s, i, j, k)];
-  fm_ghost_fill(s, dg->fm);
+          dg->fm[d][GID(g, i, j, k)] = dg->dx[d][PIDG(g, i, j, k)];
+  fm_ghost_fill(s, g, dg->fm);
   return err;
 }
 
-/* compute_ftle (lcs_m.f90:66-193) with grad_sr1 (gradient_m.f90:11-66) */
+/* compute_ftle (lcs_m.f90:66-193).  The gradient of the composed
+ * displacement is grad_sr1 (gradient_m.f90:11-66) on the diagnostic's grid,
+ * or with AUX_GRID the difference of the X1/X0, Y1/Y0, Z1/Z0 auxiliary
+ * flowmaps over the difference of their node coordinates, a column left at
+ * zero where that difference is not above 100*tiny (lcs_m.f90:95-126). */
 static void compute_ftle(ocfd *s, odiag *dg) {
-  const ogrid *g = &s->g;
+  const ogrid *g = &dg->L->g;
   const int ni = g->n[0], nj = g->n[1], nk = g->n[2];
-  for (int d = 0; d < 3; d++) exchange(g, dg->fm[d], -1, NULL, 1);
+  if (!dg->naux)
+    for (int d = 0; d < 3; d++) exchange(g, dg->fm[d], -1, NULL, 1);
   double *const *F = dg->fm;
   double *const *X = g->g;
+  const double SMALL = 100.0 * DBL_MIN;
   #pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nk; k++)
     for (int j = 1; j <= nj; j++)
       for (int i = 1; i <= ni; i++) {
-        long ip = GID(g, i + 1, j, k), im = GID(g, i - 1, j, k);
-        long jp = GID(g, i, j + 1, k), jm = GID(g, i, j - 1, k);
-        long kp = GID(g, i, j, k + 1), km = GID(g, i, j, k - 1);
         double G[9];
-        for (int c = 0; c < 3; c++) {
-          G[3 * c + 0] = (F[c][ip] - F[c][im]) / (X[0][ip] - X[0][im]);
-          G[3 * c + 1] = (F[c][jp] - F[c][jm]) / (X[1][jp] - X[1][jm]);
-          G[3 * c + 2] = (F[c][kp] - F[c][km]) / (X[2][kp] - X[2][km]);
+        if (dg->naux) {
+          const long l = GID(g, i, j, k);
+          for (int a = 0; a < 9; a++) G[a] = 0.0;
+          for (int ax = 0; ax < 3; ax++) {
+            const odiag *A0 = dg->auxs[ax], *A1 = dg->auxs[3 + ax];
+            const double den = A1->L->g.g[ax][l] - A0->L->g.g[ax][l];
+            if (fabs(den) > SMALL)
+              for (int c = 0; c < 3; c++) G[3 * c + ax] = (A1->fm[c][l] - A0->fm[c][l]) / den;
+          }
+        } else {
+          long ip = GID(g, i + 1, j, k), im = GID(g, i - 1, j, k);
+          long jp = GID(g, i, j + 1, k), jm = GID(g, i, j - 1, k);
+          long kp = GID(g, i, j, k + 1), km = GID(g, i, j, k - 1);
+          for (int c = 0; c < 3; c++) {
+            G[3 * c + 0] = (F[c][ip] - F[c][im]) / (X[0][ip] - X[0][im]);
+            G[3 * c + 1] = (F[c][jp] - F[c][jm]) / (X[1][jp] - X[1][jm]);
+            G[3 * c + 2] = (F[c][kp] - F[c][km]) / (X[2][kp] - X[2][km]);
+          }
+          if (ni == 1) { G[0] = 0; G[3] = 0; G[6] = 0; }
+          if (nj == 1) { G[1] = 0; G[4] = 0; G[7] = 0; }
+          if (nk == 1) { G[2] = 0; G[5] = 0; G[8] = 0; }
         }
-        if (ni == 1) { G[0] = 0; G[3] = 0; G[6] = 0; }
-        if (nj == 1) { G[1] = 0; G[4] = 0; G[7] = 0; }
-        if (nk == 1) { G[2] = 0; G[5] = 0; G[8] = 0; }
         double C[9];
         oc_cauchy_green(G, C);
         double lam = oc_eig_sym3_max(C);
         double r = log(fmax(sqrt(lam), DBL_MIN)) / dg->T;
         if (s->incompressible) r = fmax(r, 0.0);
-        long p = PID(s, i, j, k);
+        long p = PIDG(g, i, j, k);
         dg->ftle[p] = r;
         for (int c = 0; c < 3; c++) dg->comp[c][p] = F[c][GID(g, i, j, k)];
       }
@@ -1427,20 +1498,47 @@ static void compute_ftle(ocfd *s, odiag *dg) {
 /* write_flowmap_substep (flowmap_m.f90:106-156): slot step =
  * nint(mod(t_np1,T)/h), not wrapped (slot nstep is written but never read) */
 static void write_slot(ocfd *s, odiag *dg) {
-  const ogrid *g = &s->g;
+  const ogrid *g = &dg->L->g;
   int step = fnint(fmod(s->t_np1, dg->T) / dg->h);
   if (step < 0 || step >= dg->nslot) { s->error = 1; return; }
   for (int k = 1; k <= g->n[2]; k++)
     for (int j = 1; j <= g->n[1]; j++)
       for (int i = 1; i <= g->n[0]; i++)
         for (int d = 0; d < 3; d++)
-          dg->slot[3 * step + d][PID(s, i, j, k)] = dg->fm[d][GID(g, i, j, k)];
+          dg->slot[3 * step + d][PIDG(g, i, j, k)] = dg->fm[d][GID(g, i, j, k)];
   dg->slot_ok[step] = 1;
 }
 
 /* ======================================================================
  * API
  * ====================================================================== */
+/* a grid bundle from interior coordinate arrays (init_sgrid, sgrid_m.f90:16-444);
+ * only rectilinear grids are restated */
+static int fg_init(ofg *G, const int n[3], const double *x, const double *y, const double *z,
+                   const int *flag) {
+  memset(G, 0, sizeof(*G));
+  if (grid_init(&G->g, n, x, y, z, flag)) return 1;
+  if (!G->g.rect) return 1;
+  const ogrid *g = &G->g;
+  for (int d = 0; d < 3; d++) {
+    G->bsc[d] = calloc(g->nt, sizeof(double));
+    G->bst[d] = calloc(g->N[d] + 8, sizeof(double));
+    G->xl[d] = calloc(g->N[d], sizeof(double));
+    for (int a = 0; a < g->N[d]; a++) {
+      int i = d == 0 ? a : 1, j = d == 1 ? a : 1, k = d == 2 ? a : 1;
+      G->xl[d][a] = g->g[d][GID(g, i, j, k)];
+    }
+    if (g->n[d] == 1) { G->xl[d][0] = G->xl[d][1] - 1.0; G->xl[d][2] = G->xl[d][1] + 1.0; }
+  }
+  return 0;
+}
+
+static void fg_free(ofg *G) {
+  if (!G->g.g[0]) return;
+  for (int d = 0; d < 3; d++) { free(G->bsc[d]); free(G->bst[d]); free(G->xl[d]); }
+  grid_free(&G->g);
+}
+
 ocfd *oc_init(const int n[3], const double *x, const double *y,
               const double *z, const int *flag) {
   ocfd *s = calloc(1, sizeof(ocfd));
@@ -1448,64 +1546,144 @@ ocfd *oc_init(const int n[3], const double *x, const double *y,
   s->interpolator = LINEAR;
   s->cfl_max = 0.5;
   s->first_call = 1;
-  int e = grid_init(&s->g, n, x, y, z, flag);
-  if (e) { s->error = 1; return s; }
-  if (!s->g.rect) { s->error = 1; return s; }
-  const ogrid *g = &s->g;
+  if (fg_init(&s->F, n, x, y, z, flag)) { s->error = 1; return s; }
+  const ogrid *g = &s->F.g;
   for (int d = 0; d < 3; d++) {
     s->un[d] = calloc(g->nt, sizeof(double));
     s->unp1[d] = calloc(g->nt, sizeof(double));
     s->work[d] = calloc(g->nt, sizeof(double));
-    s->bsc[d] = calloc(g->nt, sizeof(double));
-    s->bst[d] = calloc(g->N[d] + 8, sizeof(double));
-    s->xl[d] = calloc(g->N[d], sizeof(double));
-    for (int a = 0; a < g->N[d]; a++) {
-      int i = d == 0 ? a : 1, j = d == 1 ? a : 1, k = d == 2 ? a : 1;
-      s->xl[d][a] = g->g[d][GID(g, i, j, k)];
-    }
-    if (g->n[d] == 1) { s->xl[d][0] = s->xl[d][1] - 1.0; s->xl[d][2] = s->xl[d][1] + 1.0; }
   }
   return s;
 }
 
+static void set_free(odiag *dg) {
+  for (int d = 0; d < 3; d++) {
+    free(dg->xp[d]); free(dg->dx[d]); free(dg->up[d]); free(dg->no[d]); free(dg->ns[d]);
+    free(dg->fm[d]); free(dg->comp[d]);
+  }
+  for (int k = 0; k < 3 * dg->nslot; k++) free(dg->slot[k]);
+  free(dg->slot); free(dg->slot_ok); free(dg->pflag); free(dg->ftle);
+  for (int a = 0; a < dg->naux; a++) set_free(dg->auxs[a]);
+  if (dg->own_grid) { fg_free(dg->L); free(dg->L); }
+  free(dg);
+}
+
 void oc_free(ocfd *s) {
   if (!s) return;
-  for (int i = 0; i < s->ndiag; i++) {
-    odiag *dg = s->d[i];
-    for (int d = 0; d < 3; d++) {
-      free(dg->xp[d]); free(dg->dx[d]); free(dg->up[d]); free(dg->no[d]); free(dg->ns[d]);
-      free(dg->fm[d]); free(dg->comp[d]);
-    }
-    for (int k = 0; k < 3 * dg->nslot; k++) free(dg->slot[k]);
-    free(dg->slot); free(dg->slot_ok); free(dg->pflag); free(dg->ftle);
-    free(dg);
-  }
-  if (s->g.g[0]) {
-    for (int d = 0; d < 3; d++) {
-      free(s->un[d]); free(s->unp1[d]); free(s->work[d]); free(s->bsc[d]); free(s->bst[d]);
-      free(s->xl[d]);
-    }
-    grid_free(&s->g);
+  for (int i = 0; i < s->ndiag; i++) set_free(s->d[i]);
+  if (s->F.g.g[0]) {
+    for (int d = 0; d < 3; d++) { free(s->un[d]); free(s->unp1[d]); free(s->work[d]); }
+    fg_free(&s->F);
   }
   free(s);
 }
 
 int oc_error(const ocfd *s) { return s->error; }
 
-int oc_diagnostic_init(ocfd *s, int lcs_type, int resolution, double T,
-                       double h, const char *label) {
-  if (s->error) return 0;
-  if (s->ndiag >= NMAX_STRUCT) { s->error = 1; return 0; }
-  if (lcs_type != FTLE_FWD && lcs_type != FTLE_BKWD) { s->error = 1; return 0; }
-  if (resolution != 0) { s->error = 1; return 0; }
-  const ogrid *g = &s->g;
+/* new_sgrid_from_sgrid (sgrid_m.f90:445-761) on one rank for a rectilinear
+ * grid g.  res > 0 inserts res points into every cell, including the wrap
+ * cell after the last node of a periodic axis but not after the last node of
+ * a non-periodic one, at x(i) + (x(i+1) - x(i))/(res+1) * r with the flag of
+ * node i; res < 0 keeps the nodes whose 1-based index is a multiple of
+ * |res|+1.  An axis with one node is kept.  |res| >= n along an axis with
+ * n > 1 resets res to 0 (the flow grid itself).  Returns the effective res;
+ * the interior arrays are allocated here. */
+static int refine_grid(const ogrid *g, int res, int nn[3], double *xyz[3], int **flag) {
+  for (int d = 0; d < 3; d++)
+    if (g->n[d] > 1 && abs(res) >= g->n[d]) res = 0;
+  if (res == 0) return 0;
+  int *map[3], *sub[3];
+  for (int d = 0; d < 3; d++) {
+    const int n = g->n[d];
+    int m;
+    if (n == 1) m = 1;
+    else if (res > 0) m = g->per[d] ? n * (res + 1) : n * (res + 1) - res;
+    else m = n / (abs(res) + 1);
+    nn[d] = m;
+    map[d] = malloc(sizeof(int) * m);  /* source node of each new index */
+    sub[d] = malloc(sizeof(int) * m);  /* inserted point number r */
+    for (int a = 0; a < m; a++) {
+      if (n == 1) { map[d][a] = 1; sub[d][a] = 0; }
+      else if (res > 0) { map[d][a] = 1 + a / (res + 1); sub[d][a] = a % (res + 1); }
+      else { map[d][a] = (a + 1) * (abs(res) + 1); sub[d][a] = 0; }
+    }
+  }
+  const long np = (long)nn[0] * nn[1] * nn[2];
+  for (int d = 0; d < 3; d++) xyz[d] = malloc(sizeof(double) * np);
+  *flag = malloc(sizeof(int) * np);
+  for (int k = 0; k < nn[2]; k++)
+    for (int j = 0; j < nn[1]; j++)
+      for (int i = 0; i < nn[0]; i++) {
+        const long p = (long)i + (long)nn[0] * ((long)j + (long)nn[1] * k);
+        const int src[3] = {map[0][i], map[1][j], map[2][k]}, r[3] = {sub[0][i], sub[1][j], sub[2][k]};
+        const long l = GID(g, src[0], src[1], src[2]);
+        for (int d = 0; d < 3; d++) {
+          if (res > 0) {
+            int up[3] = {src[0], src[1], src[2]};
+            up[d] += 1;
+            const double dx = (g->g[d][GID(g, up[0], up[1], up[2])] - g->g[d][l]) / (double)(res + 1);
+            xyz[d][p] = g->g[d][l] + dx * (double)r[d];
+          } else {
+            xyz[d][p] = g->g[d][l];
+          }
+        }
+        (*flag)[p] = g->flag[l];
+      }
+  for (int d = 0; d < 3; d++) { free(map[d]); free(sub[d]); }
+  return res;
+}
+
+/* create_aux_grid (lcs_m.f90:227-435): node coordinates of auxiliary grid
+ * a = 0..5 (X0, Y0, Z0, X1, Y1, Z1) of grid g, shifted along axis a % 3 by
+ * -/+ aux_delta, aux_delta being AUX_GRID_SCALING times the mean distance to
+ * the in-range face neighbours (faces in NBR order), then projected onto
+ * the data plane of a 2-D grid.  The shift starts from the node coordinate
+ * (the reference starts from the particle, which the tracking onto the flow
+ * grid leaves on its node except where a forward particle of a refined
+ * periodic grid is wrapped by a period). */
+static void aux_coords(const ogrid *g, int a, double *xyz[3]) {
+  const int ni = g->n[0], nj = g->n[1], nk = g->n[2];
+  const int flat = ni == 1 || nj == 1 || nk == 1;
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        const long l = GID(g, i, j, k), p = PIDG(g, i, j, k);
+        double nbrsum = 0.0, distsum = 0.0;
+        for (int nb = 1; nb <= 6; nb++) {
+          const int ii = i + NBR[nb][0], jj = j + NBR[nb][1], kk = k + NBR[nb][2];
+          if (ii > 0 && jj > 0 && kk > 0 && ii <= ni && jj <= nj && kk <= nk) {
+            const long m = GID(g, ii, jj, kk);
+            nbrsum = nbrsum + 1.0;
+            const double dx = g->g[0][l] - g->g[0][m], dy = g->g[1][l] - g->g[1][m],
+                         dz = g->g[2][l] - g->g[2][m];
+            distsum = distsum + sqrt(dx * dx + dy * dy + dz * dz);
+          }
+        }
+        const double delta = distsum / nbrsum * AUX_GRID_SCALING;
+        double q[3] = {g->g[0][l], g->g[1][l], g->g[2][l]};
+        const int ax = a % 3;
+        q[ax] = a < 3 ? q[ax] - delta : q[ax] + delta;
+        if (flat) {
+          const int n3[3] = {i, j, k};
+          project_point(g, n3, q);
+        }
+        for (int d = 0; d < 3; d++) xyz[d][p] = q[d];
+      }
+}
+
+/* one particle set on grid L (init_lp, lp_m.f90:13-114): seeded, tracked
+ * onto the flow grid (init_ftle / create_aux_grid) */
+static odiag *new_set(ocfd *s, ofg *L, int own, int lcs_type, double T, double h, double dt_factor,
+                      const char *label) {
+  const ogrid *g = &L->g;
   odiag *dg = calloc(1, sizeof(odiag));
-  s->d[s->ndiag++] = dg;
-  dg->id = s->ndiag;
   dg->type = lcs_type;
   dg->dir = lcs_type == FTLE_FWD ? DIR_FWD : DIR_BKWD;
   dg->T = T; dg->h = h;
-  dg->dt_factor = dg->dir == DIR_BKWD ? 1.0 / (double)(1 + resolution > 1 ? 1 + resolution : 1) : 1.0;
+  dg->dt_factor = dt_factor;
+  dg->L = L;
+  dg->own_grid = own;
+  dg->np = g->np;
   snprintf(dg->label, sizeof(dg->label), "%s", label ? label : "");
   for (int d = 0; d < 3; d++) {
     dg->xp[d] = calloc(g->np, sizeof(double));
@@ -1523,8 +1701,59 @@ int oc_diagnostic_init(ocfd *s, int lcs_type, int resolution, double T,
   dg->slot = calloc(3 * dg->nslot, sizeof(double *));
   for (int k = 0; k < 3 * dg->nslot; k++) dg->slot[k] = calloc(g->np, sizeof(double));
   dg->slot_ok = calloc(dg->nslot, sizeof(int));
-  reset_lp(s, dg); /* init_lp seeds the same state (lp_m.f90:13-114) */
-  return dg->id;
+  reset_lp(s, dg);
+  if (track_to_flow(s, dg)) s->error = 1;
+  return dg;
+}
+
+int oc_diagnostic_init(ocfd *s, int lcs_type, int resolution, double T,
+                       double h, const char *label) {
+  if (s->error) return 0;
+  if (s->ndiag >= NMAX_STRUCT) { s->error = 1; return 0; }
+  if (lcs_type != FTLE_FWD && lcs_type != FTLE_BKWD) { s->error = 1; return 0; }
+  /* the diagnostic's grid (cfd2lcs.f90:298-310) */
+  ofg *L = &s->F;
+  int own = 0;
+  if (resolution != 0) {
+    int nn[3], *fl = NULL;
+    double *xyz[3] = {NULL, NULL, NULL};
+    resolution = refine_grid(&s->F.g, resolution, nn, xyz, &fl);
+    if (resolution != 0) {
+      L = calloc(1, sizeof(ofg));
+      own = 1;
+      const int e = fg_init(L, nn, xyz[0], xyz[1], xyz[2], fl);
+      for (int d = 0; d < 3; d++) free(xyz[d]);
+      free(fl);
+      if (e) { free(L); s->error = 1; return 0; }
+    }
+  }
+  const int dir = lcs_type == FTLE_FWD ? DIR_FWD : DIR_BKWD;
+  const double dtf = dir == DIR_BKWD ? 1.0 / (double)(1 + resolution > 1 ? 1 + resolution : 1) : 1.0;
+  char lab[140];
+  snprintf(lab, sizeof(lab), "%s", label ? label : "");
+  odiag *dg = new_set(s, L, own, lcs_type, T, h, dtf, lab);
+  s->d[s->ndiag++] = dg;
+  dg->id = s->ndiag;
+  if (s->aux_grid) {
+    const ogrid *g = &L->g;
+    for (int a = 0; a < 6; a++) {
+      double *xyz[3];
+      int *fl = malloc(sizeof(int) * g->np);
+      for (int d = 0; d < 3; d++) xyz[d] = malloc(sizeof(double) * g->np);
+      aux_coords(g, a, xyz);
+      for (int k = 1; k <= g->n[2]; k++)
+        for (int j = 1; j <= g->n[1]; j++)
+          for (int i = 1; i <= g->n[0]; i++) fl[PIDG(g, i, j, k)] = g->flag[GID(g, i, j, k)];
+      ofg *A = calloc(1, sizeof(ofg));
+      const int e = fg_init(A, g->n, xyz[0], xyz[1], xyz[2], fl);
+      for (int d = 0; d < 3; d++) free(xyz[d]);
+      free(fl);
+      if (e) { free(A); s->error = 1; return 0; }
+      dg->auxs[a] = new_set(s, A, 1, lcs_type, T, h, dtf, lab);
+      dg->naux = a + 1;
+    }
+  }
+  return s->error ? 0 : dg->id;
 }
 
 void oc_set_option(ocfd *s, const char *option, int val) {
@@ -1535,7 +1764,7 @@ void oc_set_option(ocfd *s, const char *option, int val) {
       !strcmp(option, "WRITE_VELOCITY") || !strcmp(option, "WRITE_BCFLAG") ||
       !strcmp(option, "AUX_GRID")) {
     if (val != LCS_TRUE && val != LCS_FALSE) warn = 1;
-    if (!strcmp(option, "AUX_GRID") && val == LCS_TRUE) warn = 1; /* not restated */
+    if (!strcmp(option, "AUX_GRID")) s->aux_grid = val == LCS_TRUE;
     if (!strcmp(option, "VELOCITY_INVARIANTS") && val == LCS_TRUE) warn = 1;
   } else if (!strcmp(option, "INCOMPRESSIBLE")) {
     if (val == LCS_TRUE) s->incompressible = 1;
@@ -1568,10 +1797,15 @@ void oc_set_param(ocfd *s, const char *param, double val) {
   if (s->warn_par) s->error = 1;
 }
 
+/* the sets of diagnostic dg: its own, then its auxiliary sets (the order of
+ * lp_c, init_ftle + create_aux_grid) */
+static int nsets(const odiag *dg) { return 1 + dg->naux; }
+static odiag *set_at(odiag *dg, int a) { return a == 0 ? dg : dg->auxs[a - 1]; }
+
 void oc_update(ocfd *s, const int n[3], const double *u, const double *v,
                const double *w, double time) {
   if (s->error) return;
-  ogrid *g = &s->g;
+  ogrid *g = &s->F.g;
   if (n[0] != g->n[0] || n[1] != g->n[1] || n[2] != g->n[2]) { s->error = 1; return; }
   if (s->first_call) { s->t_n = time; s->t_np1 = time; s->first_call = 0; }
   else { s->t_n = s->t_np1; s->t_np1 = time; }
@@ -1580,29 +1814,39 @@ void oc_update(ocfd *s, const int n[3], const double *u, const double *v,
   for (int k = 1; k <= g->n[2]; k++)
     for (int j = 1; j <= g->n[1]; j++)
       for (int i = 1; i <= g->n[0]; i++)
-        for (int d = 0; d < 3; d++) s->unp1[d][GID(g, i, j, k)] = src[d][PID(s, i, j, k)];
+        for (int d = 0; d < 3; d++) s->unp1[d][GID(g, i, j, k)] = src[d][PIDG(g, i, j, k)];
   for (int d = 0; d < 3; d++) exchange(g, s->unp1[d], -1, NULL, 0);
   if (set_velocity_bc(g, s->unp1)) { s->error = 1; return; }
-  for (int i = 0; i < s->ndiag; i++) {
-    odiag *dg = s->d[i];
-    int e = dg->dir == DIR_FWD ? update_lp(s, dg) : update_flowmap_sl(s, dg);
-    if (e) { s->error = 1; return; }
-  }
-  for (int i = 0; i < s->ndiag; i++) {
-    odiag *dg = s->d[i];
-    if ((long)(s->t_np1 / dg->h) == (long)(s->t_n / dg->h)) continue;
-    if (dg->dir == DIR_FWD) {
-      for (int k = 1; k <= g->n[2]; k++)
-        for (int j = 1; j <= g->n[1]; j++)
-          for (int ii = 1; ii <= g->n[0]; ii++)
-            for (int d = 0; d < 3; d++)
-              dg->fm[d][GID(g, ii, j, k)] = dg->dx[d][PID(s, ii, j, k)];
+  for (int i = 0; i < s->ndiag; i++)
+    for (int a = 0; a < nsets(s->d[i]); a++) {
+      odiag *dg = set_at(s->d[i], a);
+      int e = dg->dir == DIR_FWD ? update_lp(s, dg) : update_flowmap_sl(s, dg);
+      if (e) { s->error = 1; return; }
     }
-    write_slot(s, dg);
-    if (s->error) return;
-    if (reconstruct_flowmap(s, dg)) { s->error = 1; return; }
-    compute_ftle(s, dg);
-    reset_lp(s, dg);
+  for (int i = 0; i < s->ndiag; i++) {
+    odiag *main = s->d[i];
+    if ((long)(s->t_np1 / main->h) == (long)(s->t_n / main->h)) continue;
+    for (int a = 0; a < nsets(main); a++) {
+      odiag *dg = set_at(main, a);
+      const ogrid *L = &dg->L->g;
+      if (dg->dir == DIR_FWD) {
+        for (int k = 1; k <= L->n[2]; k++)
+          for (int j = 1; j <= L->n[1]; j++)
+            for (int ii = 1; ii <= L->n[0]; ii++)
+              for (int d = 0; d < 3; d++)
+                dg->fm[d][GID(L, ii, j, k)] = dg->dx[d][PIDG(L, ii, j, k)];
+      }
+      write_slot(s, dg);
+      if (s->error) return;
+    }
+    for (int a = 0; a < nsets(main); a++)
+      if (reconstruct_flowmap(s, set_at(main, a))) { s->error = 1; return; }
+    compute_ftle(s, main);
+    for (int a = 0; a < nsets(main); a++) {
+      odiag *dg = set_at(main, a);
+      reset_lp(s, dg);
+      if (track_to_flow(s, dg)) { s->error = 1; return; }
+    }
   }
 }
 
@@ -1614,13 +1858,13 @@ static odiag *get_diag(const ocfd *s, int handle) {
 int oc_get_flowmap(const ocfd *s, int handle, double *fx, double *fy, double *fz) {
   odiag *dg = get_diag(s, handle);
   if (!dg) return 1;
-  const ogrid *g = &s->g;
+  const ogrid *g = &dg->L->g;
   double *o[3] = {fx, fy, fz};
   for (int k = 1; k <= g->n[2]; k++)
     for (int j = 1; j <= g->n[1]; j++)
       for (int i = 1; i <= g->n[0]; i++)
         for (int d = 0; d < 3; d++) {
-          long p = PID(s, i, j, k);
+          long p = PIDG(g, i, j, k);
           o[d][p] = dg->dir == DIR_FWD ? dg->dx[d][p] : dg->fm[d][GID(g, i, j, k)];
         }
   return 0;
@@ -1629,16 +1873,31 @@ int oc_get_flowmap(const ocfd *s, int handle, double *fx, double *fy, double *fz
 int oc_get_composed(const ocfd *s, int handle, double *fx, double *fy, double *fz) {
   odiag *dg = get_diag(s, handle);
   if (!dg) return 1;
-  memcpy(fx, dg->comp[0], s->g.np * sizeof(double));
-  memcpy(fy, dg->comp[1], s->g.np * sizeof(double));
-  memcpy(fz, dg->comp[2], s->g.np * sizeof(double));
+  memcpy(fx, dg->comp[0], dg->np * sizeof(double));
+  memcpy(fy, dg->comp[1], dg->np * sizeof(double));
+  memcpy(fz, dg->comp[2], dg->np * sizeof(double));
   return 0;
 }
 
 int oc_get_ftle(const ocfd *s, int handle, double *ftle) {
   odiag *dg = get_diag(s, handle);
   if (!dg) return 1;
-  memcpy(ftle, dg->ftle, s->g.np * sizeof(double));
+  memcpy(ftle, dg->ftle, dg->np * sizeof(double));
+  return 0;
+}
+
+int oc_get_lcs_grid(const ocfd *s, int handle, int set, int n[3], double *x, double *y, double *z) {
+  odiag *dg = get_diag(s, handle);
+  if (!dg || set < 0 || set > dg->naux) return 1;
+  dg = set_at(dg, set);
+  const ogrid *g = &dg->L->g;
+  for (int d = 0; d < 3; d++) n[d] = g->n[d];
+  double *o[3] = {x, y, z};
+  for (int k = 1; k <= g->n[2]; k++)
+    for (int j = 1; j <= g->n[1]; j++)
+      for (int i = 1; i <= g->n[0]; i++)
+        for (int d = 0; d < 3; d++)
+          if (o[d]) o[d][PIDG(g, i, j, k)] = g->g[d][GID(g, i, j, k)];
   return 0;
 }
 
@@ -1655,7 +1914,7 @@ long long oc_get_integrations(const ocfd *s, int direction) {
 
 int oc_get_ghost_grid(const ocfd *s, double *gx, double *gy, double *gz,
                       int *flag, double *bnx, double *bny, double *bnz) {
-  const ogrid *g = &s->g;
+  const ogrid *g = &s->F.g;
   if (!g->g[0]) return 1;
   size_t b = g->nt * sizeof(double);
   if (gx) memcpy(gx, g->g[0], b);
@@ -1669,8 +1928,8 @@ int oc_get_ghost_grid(const ocfd *s, double *gx, double *gy, double *gz,
 }
 
 int oc_get_periodic_shift(const ocfd *s, double *ps, double *bb) {
-  if (ps) memcpy(ps, s->g.ps, sizeof(s->g.ps));
-  if (bb) memcpy(bb, s->g.bb, sizeof(s->g.bb));
+  if (ps) memcpy(ps, s->F.g.ps, sizeof(s->F.g.ps));
+  if (bb) memcpy(bb, s->F.g.bb, sizeof(s->F.g.bb));
   return 0;
 }
 
@@ -1701,7 +1960,7 @@ static void mat3_mul(const double a[3][3], const double b[3][3], double c[3][3])
 }
 
 void oc_velocity_invariants(const ocfd *s, const double *u, const double *v, const double *w, double *out) {
-  const ogrid *g = &s->g;
+  const ogrid *g = &s->F.g;
   const long nt = g->nt, np = g->np;
   double *ug[3];
   const double *src[3] = {u, v, w};

@@ -48,6 +48,9 @@ int oc_get_flowmap(const ocfd *s, int handle, double *fx, double *fy,
 int oc_get_composed(const ocfd *s, int handle, double *fx, double *fy,
                     double *fz);
 int oc_get_ftle(const ocfd *s, int handle, double *ftle);
+/* the grid of a diagnostic's particle set (set 0: its own; 1..6: the
+ * AUX_GRID sets X0, Y0, Z0, X1, Y1, Z1): extents and interior coordinates */
+int oc_get_lcs_grid(const ocfd *s, int handle, int set, int n[3], double *x, double *y, double *z);
 int oc_get_noutput(const ocfd *s, int handle);
 /* CFL sub-cycle count of the latest set_dt call and cumulative counters */
 int oc_get_nsub(const ocfd *s);

@@ -37,6 +37,8 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.oc_get_ftle.restype = C.c_int
         L.oc_get_ftle.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.oc_get_lcs_grid.restype = C.c_int
+        L.oc_get_lcs_grid.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oc_get_noutput.restype = C.c_int
         L.oc_get_noutput.argtypes = [C.c_void_p, C.c_int]
         L.oc_get_nsub.restype = C.c_int
@@ -100,11 +102,26 @@ class Oracle:
     def error(self):
         return lib().oc_error(self.h)
 
+    def lcs_grid(self, handle, aux_set=0):
+        """extents and interior coordinates (x, y, z) of a diagnostic's grid
+        (aux_set 1..6: the AUX_GRID sets X0, Y0, Z0, X1, Y1, Z1)"""
+        n = np.zeros(3, dtype=np.int32)
+        assert lib().oc_get_lcs_grid(self.h, handle, aux_set, n, None, None, None) == 0
+        shape = tuple(int(a) for a in n)
+        xyz = [np.zeros(int(np.prod(shape))) for _ in range(3)]
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        assert lib().oc_get_lcs_grid(self.h, handle, aux_set, n, *[ptr(a) for a in xyz]) == 0
+        return shape, [a.reshape(shape, order="F") for a in xyz]
+
+    def _shape_of(self, handle):
+        return self.lcs_grid(handle)[0]
+
     def _get3(self, fn, handle):
-        out = [np.zeros(int(np.prod(self.shape))) for _ in range(3)]
+        shape = self._shape_of(handle)
+        out = [np.zeros(int(np.prod(shape))) for _ in range(3)]
         rc = fn(self.h, handle, *out)
         assert rc == 0
-        return [o.reshape(self.shape, order="F") for o in out]
+        return [o.reshape(shape, order="F") for o in out]
 
     def flowmap(self, handle):
         return self._get3(lib().oc_get_flowmap, handle)
@@ -113,9 +130,10 @@ class Oracle:
         return self._get3(lib().oc_get_composed, handle)
 
     def ftle(self, handle):
-        out = np.zeros(int(np.prod(self.shape)))
+        shape = self._shape_of(handle)
+        out = np.zeros(int(np.prod(shape)))
         assert lib().oc_get_ftle(self.h, handle, out) == 0
-        return out.reshape(self.shape, order="F")
+        return out.reshape(shape, order="F")
 
     def noutput(self, handle):
         return lib().oc_get_noutput(self.h, handle)

@@ -399,3 +399,125 @@ def test_error_paths():
     o.update(np.zeros((4, 4, 5)), np.zeros((4, 4, 5)), np.zeros((4, 4, 5)), 0.0)
     assert o.error == 1
     o.close()
+
+
+# ------------------------------------- diagnostic grids: resolution != 0, AUX_GRID
+def test_refined_and_coarsened_grid_coordinates():
+    """new_sgrid_from_sgrid (sgrid_m.f90:445-761) on uniform grids: res > 0
+    puts res points into every cell at fractions r/(res+1), including the
+    periodic wrap cell after the last node but not after the last node of a
+    wall-bounded axis; res < 0 keeps every (|res|+1)-th node; an axis with
+    one node is kept; |res| >= n falls back to the flow grid."""
+    h = 2 * np.pi / 8
+    X, Y, Z, fl = F.abc_grid(8)
+    o = Oracle(X, Y, Z, fl)
+    h2 = o.diagnostic_init(F.FTLE_BKWD, 2, 1.0, 0.5, "r2")
+    hm = o.diagnostic_init(F.FTLE_FWD, -1, 1.0, 0.5, "m1")
+    hz = o.diagnostic_init(F.FTLE_FWD, 8, 1.0, 0.5, "z8")
+    assert o.error == 0
+    shape, (x, y, z) = o.lcs_grid(h2)
+    assert shape == (24, 24, 24)
+    assert np.abs(x[:, 3, 5] - (0.5 * h + np.arange(24) * h / 3)).max() < 1e-14
+    assert np.abs(z[2, 3, :] - (0.5 * h + np.arange(24) * h / 3)).max() < 1e-14
+    shape, (x, y, z) = o.lcs_grid(hm)
+    assert shape == (4, 4, 4)
+    assert np.array_equal(y[1, :, 2], Y[0, 1::2, 0])
+    assert o.lcs_grid(hz)[0] == (8, 8, 8)
+    o.close()
+    # wall-bounded 2-D: n (res+1) - res points, z kept
+    X, Y, Z, fl = F.double_gyre_grid(16, 8)
+    o = Oracle(X, Y, Z, fl)
+    hd = o.diagnostic_init(F.FTLE_FWD, 1, 1.0, 0.5, "w")
+    shape, (x, y, z) = o.lcs_grid(hd)
+    assert shape == (31, 15, 1)
+    assert x[-1, 0, 0] == X[-1, 0, 0] and x[0, 0, 0] == X[0, 0, 0]
+    assert np.abs(np.diff(x[:, 0, 0]) - 1.0 / 16).max() < 1e-14
+    assert np.all(z == Z[0, 0, 0])
+    o.close()
+
+
+def test_aux_grid_coordinates():
+    """create_aux_grid (lcs_m.f90:227-435) on a uniform grid: every node of
+    set X0/X1 (Y0/Y1, Z0/Z1) is the node shifted by -/+ 0.2 h along x (y, z),
+    the mean face-neighbour distance being h at every node; on a 2-D grid the
+    out-of-plane sets are projected back onto the plane."""
+    X, Y, Z, fl = F.abc_grid(6)
+    h = 2 * np.pi / 6
+    o = Oracle(X, Y, Z, fl)
+    o.set_option("AUX_GRID", 1)
+    hd = o.diagnostic_init(F.FTLE_FWD, 0, 1.0, 0.5, "a")
+    assert o.error == 0
+    base = (X, Y, Z)
+    for a in range(6):
+        shape, xyz = o.lcs_grid(hd, a + 1)
+        ax, sgn = a % 3, (-1 if a < 3 else 1)
+        for d in range(3):
+            want = base[d] + (sgn * 0.2 * h if d == ax else 0.0)
+            assert np.abs(xyz[d] - want).max() < 1e-14
+    o.close()
+    X, Y, Z, fl = F.double_gyre_grid(10, 5)
+    o = Oracle(X, Y, Z, fl)
+    o.set_option("AUX_GRID", 1)
+    hd = o.diagnostic_init(F.FTLE_BKWD, 0, 1.0, 0.5, "a2")
+    for a in (2, 5):
+        _, xyz = o.lcs_grid(hd, a + 1)
+        assert np.array_equal(xyz[2], Z) and np.array_equal(xyz[0], X)
+    o.close()
+
+
+@pytest.mark.parametrize("res", [0, 1, 2, -1])
+@pytest.mark.parametrize("aux", [False, True])
+@pytest.mark.parametrize("direction", [F.FTLE_FWD, F.FTLE_BKWD])
+def test_diagnostic_grid_linear_flow_closed_form(res, aux, direction):
+    """u = s (a x, b y, 0) with s = -1 forward (particles move inward) and
+    s = +1 backward (departure points move inward), so no interpolation ever
+    reads the wall ghosts: with LINEAR interpolation and RK2 every node of the
+    diagnostic's grid (refined, coarsened or auxiliary) carries
+    fm = (R^N - 1) x, R the amplification factor of the sub-cycle, N the
+    number of sub-cycles over T (dt_factor = 1/(1+res) backward).  The FTLE
+    is log(max|R^N - 1|)/T at every node with AUX_GRID and at every
+    non-boundary node with grad_sr1 (whose boundary stencil reads ghosts)."""
+    a, b, T, h, dt = 0.8, 0.3, 0.25, 0.125, 1.0 / 64
+    # coarsening keeps nodes 2, 4, ..: an odd count keeps both faces INTERNAL
+    # (one wall face only would be rejected as in init_sgrid)
+    n = 33 if res < 0 else 32
+    X, Y, Z = F.cell_grid((n, n, 1), (-1, -1, 0), (1, 1, 1))
+    fl = np.full(X.shape, F.LCS_INTERNAL, dtype=np.int32)
+    # one wall layer: on a 2-D grid every WALL node projects (the interior
+    # test k > 1 .and. k < nk never holds, lp_tracking_m.f90:238), so an
+    # inner wall layer would move particles
+    fl[0], fl[-1], fl[:, 0], fl[:, -1] = F.LCS_WALL, F.LCS_WALL, F.LCS_WALL, F.LCS_WALL
+    o = Oracle(X, Y, Z, fl)
+    o.set_option("AUX_GRID", 1 if aux else 0)
+    o.set_option("INTEGRATOR", F.RK2)
+    o.set_option("INTERPOLATOR", F.LINEAR)
+    hd = o.diagnostic_init(direction, res, T, h, "lin")
+    s = -1.0 if direction == F.FTLE_FWD else 1.0
+    for step in range(17):
+        o.update(s * a * X, s * b * Y, 0 * X, step * dt)
+        assert o.error == 0
+    assert o.noutput(hd) == 2
+    nsub = o.nsub
+    # set_dt with dt_factor = 1/(1+res) backward (lcs_m.f90:40-44): the CFL
+    # is (a|x| + b|y|)/h dt at the outermost nodes
+    cfl = (a + b) * abs(X).max() * n / 2 * dt
+    dtf = 1.0 / max(1 + res, 1) if direction == F.FTLE_BKWD else 1.0
+    assert nsub == max(int(np.ceil(cfl / (0.5 * dtf))), 1)
+    sdt = (dt if direction == F.FTLE_FWD else -dt) / nsub
+    cx = amplification(F.RK2, s * a * sdt) ** (16 * nsub) - 1
+    cy = amplification(F.RK2, s * b * sdt) ** (16 * nsub) - 1
+    shape, (x, y, z) = o.lcs_grid(hd)
+    comp = o.composed(hd)
+    # a particle off its tracked flow node near a corner is projected along
+    # that node's corner-leaning wall normal (set_lp_bc, lp_tracking_m.f90:
+    # 237-252): grids other than the flow grid are checked inside two layers
+    e = 2 if (res or aux) else 0
+    cut = (slice(e, shape[0] - e), slice(e, shape[1] - e))
+    assert np.abs(comp[0] - cx * x)[cut].max() < 1e-13
+    assert np.abs(comp[1] - cy * y)[cut].max() < 1e-13
+    ftle = o.ftle(hd)
+    ref = np.log(max(abs(cx), abs(cy))) / T
+    e = max(e, 0 if aux else 1)
+    inner = ftle[e:shape[0] - e, e:shape[1] - e]
+    assert np.abs(inner - ref).max() < 1e-10
+    o.close()
