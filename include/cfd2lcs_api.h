@@ -132,8 +132,10 @@ void cfd2lcs_update_device_c(int n[3], const lcsdata_t *d_u,
                              const lcsdata_t *d_v, const lcsdata_t *d_w,
                              lcsdata_t time);
 
-/* Results kept in HBM by the library; copied to host arrays of
- * n[0]*n[1]*n[2] values, Fortran order.  Return 0 on success.
+/* Results kept in HBM by the library; copied to host arrays of one value
+ * per node of the diagnostic's grid (the flow grid's n[0]*n[1]*n[2] unless
+ * resolution != 0, see cfd2lcs_get_lcs_grid_c), Fortran order.  Return 0 on
+ * success.
  *   flowmap : displacement over the current h interval (the BKWD
  *             semi-Lagrangian field lp%fm, or the FWD particles' dx by
  *             origin node)
@@ -145,6 +147,13 @@ int cfd2lcs_get_flowmap_c(int lcs_handle, lcsdata_t *fx, lcsdata_t *fy,
 int cfd2lcs_get_composed_c(int lcs_handle, lcsdata_t *fx, lcsdata_t *fy,
                            lcsdata_t *fz);
 int cfd2lcs_get_ftle_c(int lcs_handle, lcsdata_t *ftle);
+/* The grid of a diagnostic (lcs%sgrid): the flow grid, or with
+ * resolution != 0 the refined / coarsened grid of new_sgrid_from_sgrid
+ * (sgrid_m.f90:445-761), whose extents size the arrays of the three getters
+ * above.  Fills n[3] and, when x, y, z are not NULL, the node coordinates
+ * (n[0]*n[1]*n[2] each, Fortran order).  Returns 0 on success.  No
+ * counterpart in the reference, which keeps the grid internal. */
+int cfd2lcs_get_lcs_grid_c(int lcs_handle, int n[3], lcsdata_t *x, lcsdata_t *y, lcsdata_t *z);
 /* device pointer (double, n[0]*n[1]*n[2], Fortran order) to the FTLE field;
  * valid until the next update or finalize */
 const double *cfd2lcs_get_ftle_device_c(int lcs_handle);

@@ -31,7 +31,7 @@ EXPORTED = (
     "cfd2lcs_init_c", "cfd2lcs_diagnostic_init_c", "cfd2lcs_update_c",
     "cfd2lcs_diagnostic_destroy_c", "cfd2lcs_set_option_c", "cfd2lcs_set_param_c",
     "cfd2lcs_finalize_c", "cfd2lcs_error_c", "cfd2lcs_update_device_c",
-    "cfd2lcs_get_flowmap_c", "cfd2lcs_get_composed_c", "cfd2lcs_get_ftle_c",
+    "cfd2lcs_get_flowmap_c", "cfd2lcs_get_composed_c", "cfd2lcs_get_ftle_c", "cfd2lcs_get_lcs_grid_c",
     "cfd2lcs_get_ftle_device_c", "cfd2lcs_get_noutput_c", "cfd2lcs_get_stats_c",
     "cfd2lcs_get_timers_c", "cfd2lcs_debug_grid_c",
     "cfd2lcs_adjoint_forward_c", "cfd2lcs_adjoint_backward_c", "cfd2lcs_adjoint_release_c",
@@ -77,6 +77,8 @@ def load(variant="dp"):
         getattr(L, name).restype = C.c_int
     L.cfd2lcs_get_ftle_c.argtypes = [C.c_int, rp]
     L.cfd2lcs_get_ftle_c.restype = C.c_int
+    L.cfd2lcs_get_lcs_grid_c.argtypes = [C.c_int, ip, rp, rp, rp]
+    L.cfd2lcs_get_lcs_grid_c.restype = C.c_int
     L.cfd2lcs_get_ftle_device_c.argtypes = [C.c_int]
     L.cfd2lcs_get_ftle_device_c.restype = C.c_void_p
     L.cfd2lcs_get_noutput_c.argtypes = [C.c_int]
@@ -182,13 +184,28 @@ class Cfd2lcs:
     def error(self):
         return self.L.cfd2lcs_error_c()
 
+    def lcs_grid(self, handle, coords=True):
+        """Extents (and node coordinates x, y, z) of a diagnostic's grid: the
+        flow grid, or the refined / coarsened grid of resolution != 0."""
+        n = (C.c_int * 3)()
+        if self.L.cfd2lcs_get_lcs_grid_c(handle, n, None, None, None) != 0:
+            raise RuntimeError(f"grid query failed for handle {handle}")
+        shape = tuple(int(v) for v in n)
+        if not coords:
+            return shape
+        xyz = [np.zeros(int(np.prod(shape)), dtype=self.dtype) for _ in range(3)]
+        if self.L.cfd2lcs_get_lcs_grid_c(handle, n, *[self._rp(a) for a in xyz]) != 0:
+            raise RuntimeError(f"grid query failed for handle {handle}")
+        return shape, [a.reshape(shape, order="F") for a in xyz]
+
     def _get3(self, fn, handle):
-        npt = int(np.prod(self.shape))
+        shape = self.lcs_grid(handle, coords=False)
+        npt = int(np.prod(shape))
         out = [np.zeros(npt, dtype=self.dtype) for _ in range(3)]
         rc = fn(handle, *[self._rp(o) for o in out])
         if rc != 0:
             raise RuntimeError(f"getter failed for handle {handle}")
-        return [o.reshape(self.shape, order="F") for o in out]
+        return [o.reshape(shape, order="F") for o in out]
 
     def flowmap(self, handle):
         return self._get3(self.L.cfd2lcs_get_flowmap_c, handle)
@@ -197,10 +214,11 @@ class Cfd2lcs:
         return self._get3(self.L.cfd2lcs_get_composed_c, handle)
 
     def ftle(self, handle):
-        out = np.zeros(int(np.prod(self.shape)), dtype=self.dtype)
+        shape = self.lcs_grid(handle, coords=False)
+        out = np.zeros(int(np.prod(shape)), dtype=self.dtype)
         if self.L.cfd2lcs_get_ftle_c(handle, self._rp(out)) != 0:
             raise RuntimeError(f"ftle getter failed for handle {handle}")
-        return out.reshape(self.shape, order="F")
+        return out.reshape(shape, order="F")
 
     # ---- adjoint of the backward FTLE (device pointers) ----
     def adjoint_forward(self, du, dv, dw, nsnap, times, checkpoint, dftle):

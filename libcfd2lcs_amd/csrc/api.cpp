@@ -64,6 +64,9 @@ struct Options {
   int invariants = 0;  // VELOCITY_INVARIANTS (data_m.f90:59): Q, C, D, H, L2 columns
   int write_lcs = 1;
   int write_slots = 0;
+  // AUX_GRID (data_m.f90:39): six auxiliary particle sets per FTLE
+  // diagnostic, read when the diagnostic is created
+  int aux_grid = 0;
   bool warn_opt = false, warn_par = false;  // SAVEd warn flags (cfd2lcs.f90:444,632)
 };
 
@@ -119,6 +122,8 @@ struct DevBuf {
     p = nullptr;
     bytes = 0;
   }
+  // at least b bytes (the contents are lost when it grows)
+  bool ensure(size_t b) { return (p && bytes >= b) || alloc(b); }
   bool alloc(size_t b) {
     release();
     if (b == 0) return true;
@@ -133,11 +138,12 @@ struct DevBuf {
 
 struct EngineBase {
   virtual ~EngineBase() {}
-  virtual int add_diag(int type, U T, U h, const char* label) = 0;
+  virtual int add_diag(int type, int resolution, U T, U h, const char* label) = 0;
   virtual void update(const U* u, const U* v, const U* w, bool device, U time) = 0;
   virtual int get_flowmap(int h, U* a, U* b, U* c) = 0;
   virtual int get_composed(int h, U* a, U* b, U* c) = 0;
   virtual int get_ftle(int h, U* out) = 0;
+  virtual int get_grid(int h, int n[3], U* x, U* y, U* z) = 0;
   virtual const double* ftle_device(int h) = 0;
   virtual int noutput(int h) = 0;
   virtual void stats(double out[8]) = 0;
@@ -193,10 +199,29 @@ struct Engine : EngineBase {
   };
   std::vector<Timed> pending;
 
+  // The grid of a particle set other than the flow grid: a diagnostic with
+  // resolution != 0 (new_sgrid_from_sgrid) or an AUX_GRID set
+  // (create_aux_grid).  Own spline factors; per node the tracked flow node
+  // of a backward set (lp%no_scfd after init_lp / reset_lp + track_lp2node).
+  struct SetGrid {
+    HostGrid hg;
+    GridDev<T> g{};
+    DevBuf mem, spline_mem[3];
+    int spline_k = 0;
+    DevBuf nodes;  // 3 x np int16
+    std::shared_ptr<std::vector<U>> out_xyz, out_flag;  // output-file columns
+  };
+
   struct Diag {
     int id, type, dir;
     U Tint, h, dt_factor;
     std::string label;
+    std::string lp_label;  // lp%label: <label>-LP, AUX_GRID sets <label>-LP0 .. -LP5
+    std::unique_ptr<SetGrid> sg;  // null: the particles live on the flow grid
+    // with sg, backward: the tracked nodes are unclamped until the first
+    // sub-cycle after a reset (set_lp_bc clamps lp%no_scfd in place)
+    bool fresh_nodes = true;
+    std::vector<std::unique_ptr<Diag>> aux;  // AUX_GRID sets X0, Y0, Z0, X1, Y1, Z1
     int nstep, nslot;
     std::vector<char> slot_ok, slot_coef_ok;
     DevBuf part[2];        // FWD particle sets (current = part[pcur])
@@ -211,6 +236,7 @@ struct Engine : EngineBase {
     DevBuf fmcoef;         // spline coefficients of fm[fcur]
     DevBuf slots, slotcoef;  // nslot * 3 ghosted fields
     DevBuf order;          // slot order of the composition
+    std::vector<int> order_h;  // its host copy (kept until the copy ran)
     DevBuf dxout;          // composed displacement by origin (3*np)
     DevBuf compfm;         // composed flowmap, ghosted
     DevBuf comp;           // composed flowmap interior (3*np)
@@ -218,6 +244,17 @@ struct Engine : EngineBase {
     int noutput = 0;
   };
   std::vector<std::unique_ptr<Diag>> diags;
+
+  // a diagnostic's own set, then its AUX_GRID sets (the order of lp_c)
+  template <class F>
+  void for_sets(Diag& dg, F&& f) {
+    f(dg);
+    for (auto& a : dg.aux) f(*a);
+  }
+  GridDev<T>& G(Diag& dg) { return dg.sg ? dg.sg->g : g; }
+  const HostGrid& HG(Diag& dg) { return dg.sg ? dg.sg->hg : hg; }
+  long long NP(Diag& dg) { return G(dg).np; }
+  long long NT(Diag& dg) { return G(dg).nt; }
 
   // output files (write_lcs, write_flowmap_substep)
   std::unique_ptr<io::Writer> writer;
@@ -230,7 +267,10 @@ struct Engine : EngineBase {
   std::vector<DevBuf> uring;
   std::vector<char> uring_ok;
   DevBuf vint;  // interior scratch, 3 * np
+  DevBuf iscratch;  // interior scratch of any grid (host copies)
   DevBuf invbuf;  // velocity invariants, 5 * np
+  DevBuf uscratch[2];  // velocity at T0 of a diagnostic grid: ghosted field on
+                       // the flow grid (and its spline fit), then 3 * np
 
   ~Engine() override {
     if (writer) writer->flush();
@@ -258,22 +298,27 @@ struct Engine : EngineBase {
     return m;
   }
 
-  size_t fbytes() const { return sizeof(T) * (size_t)g.nt; }
-  F3<T> f3(DevBuf& b) {
+  // ghosted 3-component fields of nt values per component (default: the
+  // flow grid's)
+  size_t fbytes(long long nt) const { return sizeof(T) * (size_t)nt; }
+  size_t fbytes() const { return fbytes(g.nt); }
+  F3<T> f3(DevBuf& b, long long nt) {
     T* p = (T*)b.p;
-    return F3<T>{{p, p + g.nt, p + 2 * g.nt}};
+    return F3<T>{{p, p + nt, p + 2 * nt}};
   }
-  CF3<T> cf3(DevBuf& b) {
+  CF3<T> cf3(DevBuf& b, long long nt) {
     const T* p = (const T*)b.p;
-    return CF3<T>{{p, p + g.nt, p + 2 * g.nt}};
+    return CF3<T>{{p, p + nt, p + 2 * nt}};
   }
+  F3<T> f3(DevBuf& b) { return f3(b, g.nt); }
+  CF3<T> cf3(DevBuf& b) { return cf3(b, g.nt); }
   // particle-set layout inside one allocation: x[3], d[3] (T), n[3]
   // (int16), org (int), each array aligned to 256 bytes
-  size_t pset_bytes() const {
+  size_t pset_bytes(long long np) const {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    return 12 * al(sizeof(T) * (size_t)g.np) + 3 * al(sizeof(int16_t) * (size_t)g.np) + al(sizeof(int) * (size_t)g.np);
+    return 12 * al(sizeof(T) * (size_t)np) + 3 * al(sizeof(int16_t) * (size_t)np) + al(sizeof(int) * (size_t)np);
   }
-  ParticleSet<T> pset(DevBuf& b) {
+  ParticleSet<T> pset(DevBuf& b, long long np) {
     char* base = (char*)b.p;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -282,42 +327,44 @@ struct Engine : EngineBase {
       return q;
     };
     ParticleSet<T> ps;
-    for (int d = 0; d < 3; d++) ps.x[d] = (T*)take(sizeof(T) * (size_t)g.np);
-    for (int d = 0; d < 3; d++) ps.d[d] = (T*)take(sizeof(T) * (size_t)g.np);
-    for (int d = 0; d < 3; d++) ps.xl[d] = (T*)take(sizeof(T) * (size_t)g.np);
-    for (int d = 0; d < 3; d++) ps.dl[d] = (T*)take(sizeof(T) * (size_t)g.np);
-    for (int d = 0; d < 3; d++) ps.n[d] = (int16_t*)take(sizeof(int16_t) * (size_t)g.np);
-    ps.org = (int*)take(sizeof(int) * (size_t)g.np);
+    for (int d = 0; d < 3; d++) ps.x[d] = (T*)take(sizeof(T) * (size_t)np);
+    for (int d = 0; d < 3; d++) ps.d[d] = (T*)take(sizeof(T) * (size_t)np);
+    for (int d = 0; d < 3; d++) ps.xl[d] = (T*)take(sizeof(T) * (size_t)np);
+    for (int d = 0; d < 3; d++) ps.dl[d] = (T*)take(sizeof(T) * (size_t)np);
+    for (int d = 0; d < 3; d++) ps.n[d] = (int16_t*)take(sizeof(int16_t) * (size_t)np);
+    ps.org = (int*)take(sizeof(int) * (size_t)np);
     return ps;
   }
   // forward displacement by origin node: the current set's dx, or (when the
   // set is sorted, or dx is a compensated sum) its scatter to origin order in
   // the spare set's dx arrays
   CF3<T> fwd_dx_by_origin(Diag& dg) {
-    ParticleSet<T> a = pset(dg.part[dg.pcur]);
+    const long long n = NP(dg);
+    ParticleSet<T> a = pset(dg.part[dg.pcur], n);
     if (!dg.sorted && !dg.cmode) return CF3<T>{{a.d[0], a.d[1], a.d[2]}};
-    if (!dg.part[1].p && !dg.part[1].alloc(pset_bytes())) {
+    if (!dg.part[1].p && !dg.part[1].alloc(pset_bytes(n))) {
       fail("hipMalloc failed (particle set)");
       return CF3<T>{{a.d[0], a.d[1], a.d[2]}};
     }
-    ParticleSet<T> b = pset(dg.part[1 - dg.pcur]);
+    ParticleSet<T> b = pset(dg.part[1 - dg.pcur], n);
     const T* lo[3] = {a.dl[0], a.dl[1], a.dl[2]};
-    launch_unpermute(g, a.org, CF3<T>{{a.d[0], a.d[1], a.d[2]}}, dg.cmode ? lo : nullptr,
+    launch_unpermute(n, a.org, CF3<T>{{a.d[0], a.d[1], a.d[2]}}, dg.cmode ? lo : nullptr,
                      F3<T>{{b.d[0], b.d[1], b.d[2]}}, stream);
     return CF3<T>{{b.d[0], b.d[1], b.d[2]}};
   }
   bool sort_particles(Diag& dg) {
-    const size_t np = (size_t)g.np;
+    const long long n = NP(dg);
+    const size_t np = (size_t)n;
     if (!dg.stemp.p) {
-      dg.stemp_bytes = particle_sort_temp_bytes(g.np);
-      if (!dg.stemp_bytes || (!dg.part[1].p && !dg.part[1].alloc(pset_bytes())) ||
+      dg.stemp_bytes = particle_sort_temp_bytes(n, g.np);
+      if (!dg.stemp_bytes || (!dg.part[1].p && !dg.part[1].alloc(pset_bytes(n))) ||
           !dg.skeys.alloc(2 * sizeof(unsigned) * np) ||
           !dg.svals.alloc(2 * sizeof(unsigned) * np) || !dg.stemp.alloc(dg.stemp_bytes)) {
         fail("hipMalloc failed (particle sort buffers)");
         return false;
       }
     }
-    if (launch_particle_sort(g, pset(dg.part[dg.pcur]), pset(dg.part[1 - dg.pcur]), (unsigned*)dg.skeys.p,
+    if (launch_particle_sort(g, pset(dg.part[dg.pcur], n), pset(dg.part[1 - dg.pcur], n), n, (unsigned*)dg.skeys.p,
                              (unsigned*)dg.svals.p, dg.stemp.p, dg.stemp_bytes, stream)) {
       fail("particle sort failed");
       return false;
@@ -327,42 +374,34 @@ struct Engine : EngineBase {
     dg.since_sort = 0;
     return true;
   }
-  F3<T> p3(DevBuf& b) {
-    T* p = (T*)b.p;
-    return F3<T>{{p, p + g.np, p + 2 * g.np}};
-  }
-  CF3<T> cp3(DevBuf& b) {
-    const T* p = (const T*)b.p;
-    return CF3<T>{{p, p + g.np, p + 2 * g.np}};
-  }
+  F3<T> p3(DevBuf& b, long long np) { return f3(b, np); }
+  CF3<T> cp3(DevBuf& b, long long np) { return cf3(b, np); }
+  F3<T> p3(DevBuf& b) { return p3(b, g.np); }
+  CF3<T> cp3(DevBuf& b) { return cp3(b, g.np); }
 
-  bool init(const HostGrid& h) {
-    hg = h;
-    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
-      fail("cannot create a HIP stream (no GPU visible?)");
-      return false;
-    }
+  // device copy of a grid: coordinate lines, interpolation lines, normals,
+  // 1/delta and flags in one allocation
+  bool upload_grid(const HostGrid& h, GridDev<T>& gd, DevBuf& mem) {
     for (int d = 0; d < 3; d++) {
-      g.n[d] = hg.n[d];
-      g.N[d] = hg.N[d];
-      g.per[d] = hg.per[d];
-      g.kord[d] = 0;
-      g.kn[d] = nullptr;
-      g.band[d] = nullptr;
-      g.bpoly[d] = nullptr;
+      gd.n[d] = h.n[d];
+      gd.N[d] = h.N[d];
+      gd.per[d] = h.per[d];
+      gd.kord[d] = 0;
+      gd.kn[d] = nullptr;
+      gd.band[d] = nullptr;
+      gd.bpoly[d] = nullptr;
     }
-    g.nt = hg.nt;
-    g.np = hg.np;
-    g.flat = hg.flat;
-    g.max_track = 2 * std::max(hg.n[0], std::max(hg.n[1], hg.n[2]));
+    gd.nt = h.nt;
+    gd.np = h.np;
+    gd.flat = h.flat;
+    gd.max_track = 2 * std::max(h.n[0], std::max(h.n[1], h.n[2]));
     for (int c = 0; c < 27; c++)
-      for (int d = 0; d < 3; d++) g.ps[c][d] = (T)hg.ps[c][d];
-    for (int d = 0; d < 6; d++) g.bb[d] = (T)hg.bb[d];
-    // one allocation: lines (2 per axis), normals, deltas, flags
-    const size_t nl = (size_t)hg.N[0] + hg.N[1] + hg.N[2];
-    const size_t reals = 2 * nl + 6 * (size_t)hg.np;
-    const size_t bytes = reals * sizeof(T) + (size_t)hg.nt + 256;
-    if (!grid_mem.alloc(bytes)) {
+      for (int d = 0; d < 3; d++) gd.ps[c][d] = (T)h.ps[c][d];
+    for (int d = 0; d < 6; d++) gd.bb[d] = (T)h.bb[d];
+    const size_t nl = (size_t)h.N[0] + h.N[1] + h.N[2];
+    const size_t reals = 2 * nl + 6 * (size_t)h.np;
+    const size_t bytes = reals * sizeof(T) + (size_t)h.nt + 256;
+    if (!mem.alloc(bytes)) {
       fail("hipMalloc failed for the grid");
       return false;
     }
@@ -371,29 +410,39 @@ struct Engine : EngineBase {
     size_t off_gl[3], off_xl[3], off_bn[3], off_dl[3];
     for (int d = 0; d < 3; d++) {
       off_gl[d] = o;
-      for (int a = 0; a < hg.N[d]; a++) hr[o++] = (T)hg.line[d][a];
+      for (int a = 0; a < h.N[d]; a++) hr[o++] = (T)h.line[d][a];
       off_xl[d] = o;
-      for (int a = 0; a < hg.N[d]; a++) hr[o++] = (T)hg.xl[d][a];
+      for (int a = 0; a < h.N[d]; a++) hr[o++] = (T)h.xl[d][a];
     }
     for (int d = 0; d < 3; d++) {
       off_bn[d] = o;
-      for (long long p = 0; p < hg.np; p++) hr[o++] = (T)hg.bn[d][p];
+      for (long long p = 0; p < h.np; p++) hr[o++] = (T)h.bn[d][p];
       off_dl[d] = o;
-      for (long long p = 0; p < hg.np; p++) hr[o++] = (T)hg.dl[d][p];
+      for (long long p = 0; p < h.np; p++) hr[o++] = (T)h.dl[d][p];
     }
-    T* base = (T*)grid_mem.p;
+    T* base = (T*)mem.p;
     if (hipMemcpy(base, hr.data(), reals * sizeof(T), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy((char*)(base + reals), hg.flag.data(), (size_t)hg.nt, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy((char*)(base + reals), h.flag.data(), (size_t)h.nt, hipMemcpyHostToDevice) != hipSuccess) {
       fail("upload of the grid failed");
       return false;
     }
     for (int d = 0; d < 3; d++) {
-      g.gl[d] = base + off_gl[d];
-      g.xl[d] = base + off_xl[d];
-      g.bn[d] = base + off_bn[d];
-      g.dl[d] = base + off_dl[d];
+      gd.gl[d] = base + off_gl[d];
+      gd.xl[d] = base + off_xl[d];
+      gd.bn[d] = base + off_bn[d];
+      gd.dl[d] = base + off_dl[d];
     }
-    g.flag = (const int8_t*)(base + reals);
+    gd.flag = (const int8_t*)(base + reals);
+    return true;
+  }
+
+  bool init(const HostGrid& h) {
+    hg = h;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+      fail("cannot create a HIP stream (no GPU visible?)");
+      return false;
+    }
+    if (!upload_grid(hg, g, grid_mem)) return false;
     if (!status_mem.alloc(sizeof(DevStatus))) {
       fail("hipMalloc failed (status)");
       return false;
@@ -421,13 +470,14 @@ struct Engine : EngineBase {
   // spline order of the active interpolator, with the order clamp of
   // lp_tracking_m.f90:506-508
   int spline_order_for(int interp) const { return interp == kQuadratic ? 3 : 4; }
-  bool ensure_spline(int interp) {
+  // knots and collocation factors of grid h for the active interpolator
+  bool grid_factors(const HostGrid& h, GridDev<T>& gd, DevBuf sm[3], int& have, int interp) {
     const int k = spline_order_for(interp);
-    if (spline_k == k) return true;
+    if (have == k) return true;
     for (int d = 0; d < 3; d++) {
-      int kk = std::max(std::min(k, hg.N[d] - 1), 2);
+      int kk = std::max(std::min(k, h.N[d] - 1), 2);
       SplineAxis ax;
-      std::string e = spline_axis(hg.xl[d], kk, ax);
+      std::string e = spline_axis(h.xl[d], kk, ax);
       if (!e.empty()) {
         fail(e);
         return false;
@@ -437,23 +487,43 @@ struct Engine : EngineBase {
       for (size_t a = 0; a < ax.knots.size(); a++) hv[a] = (T)ax.knots[a];
       for (size_t a = 0; a < ax.band.size(); a++) hv[ax.knots.size() + a] = (T)ax.band[a];
       for (size_t a = 0; a < ax.poly.size(); a++) hv[ax.knots.size() + ax.band.size() + a] = (T)ax.poly[a];
-      if (!spline_mem[d].alloc(nr * sizeof(T)) ||
-          hipMemcpy(spline_mem[d].p, hv.data(), nr * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+      if (!sm[d].alloc(nr * sizeof(T)) ||
+          hipMemcpy(sm[d].p, hv.data(), nr * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
         fail("upload of the spline factors failed");
         return false;
       }
-      g.kord[d] = kk;
-      g.kn[d] = (const T*)spline_mem[d].p;
-      g.band[d] = (const T*)spline_mem[d].p + ax.knots.size();
-      g.bpoly[d] = (const T*)spline_mem[d].p + ax.knots.size() + ax.band.size();
+      gd.kord[d] = kk;
+      gd.kn[d] = (const T*)sm[d].p;
+      gd.band[d] = (const T*)sm[d].p + ax.knots.size();
+      gd.bpoly[d] = (const T*)sm[d].p + ax.knots.size() + ax.band.size();
     }
-    if (!(g.kord[0] == g.kord[1] && (g.kord[2] == g.kord[0] || g.kord[2] == 2))) {
+    if (!(gd.kord[0] == gd.kord[1] && (gd.kord[2] == gd.kord[0] || gd.kord[2] == 2))) {
       fail("spline orders per axis not supported by this build (need equal x/y orders)");
       return false;
     }
-    spline_k = k;
+    have = k;
+    return true;
+  }
+  bool ensure_spline(int interp) {
+    const int k = spline_order_for(interp);
+    if (spline_k == k) return true;
+    if (!grid_factors(hg, g, spline_mem, spline_k, interp)) return false;
     vcoef_ok[0] = vcoef_ok[1] = false;
-    for (auto& dg : diags) std::fill(dg->slot_coef_ok.begin(), dg->slot_coef_ok.end(), 0);
+    for (auto& dg : diags)
+      for_sets(*dg, [&](Diag& d) { std::fill(d.slot_coef_ok.begin(), d.slot_coef_ok.end(), 0); });
+    return true;
+  }
+  // the spline factors of a set's own grid (same orders as the flow grid's,
+  // which the gather kernels are instantiated for)
+  bool ensure_set_spline(Diag& dg, int interp) {
+    if (!dg.sg) return true;
+    SetGrid& sg = *dg.sg;
+    if (!grid_factors(sg.hg, sg.g, sg.spline_mem, sg.spline_k, interp)) return false;
+    for (int d = 0; d < 3; d++)
+      if (sg.g.kord[d] != g.kord[d]) {
+        fail("spline order of a diagnostic grid differs from the flow grid's (grid too small along an axis)");
+        return false;
+      }
     return true;
   }
   bool is_spline(int interp) const { return interp == kQuadratic || interp == kCubic; }
@@ -503,63 +573,140 @@ struct Engine : EngineBase {
     pending.clear();
   }
 
-  int add_diag(int type, U T_, U h, const char* label) override {
+  // a particle set's own grid from interior coordinates and flags
+  // (init_sgrid), uploaded to the device
+  std::unique_ptr<SetGrid> make_set_grid(const int nn[3], const std::vector<double> xyz[3],
+                                         const std::vector<int>& fl) {
+    auto sg = std::make_unique<SetGrid>();
+    // generated nodes may leave the coordinate lines by a few ulp
+    const double ulp = sizeof(U) == 8 ? DBL_EPSILON : (double)FLT_EPSILON;
+    std::string e = sg->hg.build(nn, xyz[0].data(), xyz[1].data(), xyz[2].data(), fl.data(),
+                                 sizeof(U) == 8 ? 1e-9 : 1e-6, 64 * ulp);
+    if (!e.empty()) {
+      fail("diagnostic grid: " + e);
+      return nullptr;
+    }
+    if (sg->hg.n[0] > 32767 || sg->hg.n[1] > 32767 || sg->hg.n[2] > 32767) {
+      fail("diagnostic grid too large (more than 32767 nodes along an axis)");
+      return nullptr;
+    }
+    if (!upload_grid(sg->hg, sg->g, sg->mem)) return nullptr;
+    return sg;
+  }
+
+  // one particle set with its flowmap / slot / composition state
+  // (init_lp, lp_m.f90:13-114)
+  std::unique_ptr<Diag> new_set(int type, U T_, U h, U dtf, const std::string& label, const std::string& lp_label,
+                                std::unique_ptr<SetGrid> sg) {
     auto dg = std::make_unique<Diag>();
-    dg->id = (int)diags.size() + 1;
     dg->type = type;
     dg->dir = type == 0 ? 1 : -1;
     dg->Tint = T_;
     dg->h = h;
-    dg->dt_factor = dg->dir < 0 ? (U)1 / (U)std::max(1 + 0, 1) : (U)1;
-    dg->label = label ? label : "";
+    dg->dt_factor = dtf;
+    dg->label = label;
+    dg->lp_label = lp_label;
+    dg->sg = std::move(sg);
     // nstep = nint(T/h); slots 0..nstep (the write index may equal nstep)
     dg->nstep = (int)std::lround((double)(T_ / h));
     if (dg->nstep < 1 || dg->nstep > 100000) {
       fail("T/h out of range");
-      return -1;
+      return nullptr;
     }
     dg->nslot = dg->nstep + 1;
     dg->slot_ok.assign(dg->nslot, 0);
     dg->slot_coef_ok.assign(dg->nslot, 0);
-    const size_t fb = fbytes(), pb = sizeof(T) * (size_t)g.np;
+    const long long np = NP(*dg);
+    const size_t fb = fbytes(NT(*dg)), pb = sizeof(T) * (size_t)np;
     bool ok = true;
     if (dg->dir > 0) {
-      ok = ok && dg->part[0].alloc(pset_bytes());
+      ok = ok && dg->part[0].alloc(pset_bytes(np));
     } else {
       ok = ok && dg->fm[0].alloc(3 * fb) && dg->fm[1].alloc(3 * fb);
     }
     ok = ok && dg->slots.alloc((size_t)dg->nslot * 3 * fb) && dg->order.alloc(sizeof(int) * dg->nslot) &&
          dg->dxout.alloc(3 * pb) && dg->compfm.alloc(3 * fb) && dg->comp.alloc(3 * pb) &&
-         dg->ftle.alloc(sizeof(double) * (size_t)g.np);
+         dg->ftle.alloc(sizeof(double) * (size_t)np);
+    if (dg->sg) ok = ok && dg->sg->nodes.alloc(3 * sizeof(int16_t) * (size_t)np);
     if (!ok) {
       fail("hipMalloc failed for a diagnostic (out of device memory?)");
-      return -1;
+      return nullptr;
     }
     if (hipMemsetAsync(dg->compfm.p, 0, 3 * fb, stream) != hipSuccess ||
         hipMemsetAsync(dg->comp.p, 0, 3 * pb, stream) != hipSuccess ||
-        hipMemsetAsync(dg->ftle.p, 0, sizeof(double) * (size_t)g.np, stream) != hipSuccess) {
+        hipMemsetAsync(dg->ftle.p, 0, sizeof(double) * (size_t)np, stream) != hipSuccess) {
       fail("hipMemset failed");
-      return -1;
+      return nullptr;
     }
+    // the tracked flow nodes (without the periodic wrap) of the set's nodes:
+    // reset_lp puts the particles back on the same nodes with the same
+    // guess, so a backward set's are tracked once; they also locate the
+    // velocity at T0 on the set's grid
+    if (dg->sg) launch_track_nodes(g, dg->sg->g, (int16_t*)dg->sg->nodes.p, st, stream);
     reset_diag(*dg);
+    return dg;
+  }
+
+  int add_diag(int type, int resolution, U T_, U h, const char* label) override {
+    const std::string lab = label ? label : "";
+    // the diagnostic's grid (cfd2lcs.f90:298-310, new_sgrid_from_sgrid)
+    std::unique_ptr<SetGrid> sg;
+    if (resolution != 0) {
+      int nn[3];
+      std::vector<double> xyz[3];
+      std::vector<int> fl;
+      resolution = refine_grid<U>(hg, resolution, nn, xyz, fl);
+      if (resolution != 0) {
+        sg = make_set_grid(nn, xyz, fl);
+        if (!sg) return -1;
+      }
+    }
+    // dt_factor (lcs_m.f90:40-44)
+    const U dtf = type == 0 ? (U)1 : (U)1 / (U)std::max(1 + resolution, 1);
+    auto dg = new_set(type, T_, h, dtf, lab, lab + "-LP", std::move(sg));
+    if (!dg) return -1;
+    dg->id = (int)diags.size() + 1;
+    if (g_opt.aux_grid) {
+      // create_aux_grid (lcs_m.f90:227-435): six sets on shifted copies of
+      // the diagnostic's grid, with its flags
+      const HostGrid& base = HG(*dg);
+      std::vector<int> fl((size_t)base.np);
+      for (int k = 1; k <= base.n[2]; k++)
+        for (int j = 1; j <= base.n[1]; j++)
+          for (int i = 1; i <= base.n[0]; i++) fl[base.pid(i, j, k)] = base.flag[base.id(i, j, k)];
+      for (int a = 0; a < 6; a++) {
+        std::vector<double> xyz[3];
+        aux_grid_coords<U>(base, a, xyz);
+        auto ag = make_set_grid(base.n, xyz, fl);
+        if (!ag) return -1;
+        auto ad = new_set(type, T_, h, dtf, lab, lab + "-LP" + std::to_string(a), std::move(ag));
+        if (!ad) return -1;
+        dg->aux.push_back(std::move(ad));
+      }
+    }
     if (hipStreamSynchronize(stream) != hipSuccess) {
       fail("diagnostic initialisation failed on the device");
       return -1;
     }
+    if (!sync_status()) return -1;
     diags.push_back(std::move(dg));
     return (int)diags.size();
   }
 
-  // reset_lp (lp_m.f90:228-294): particles back on their nodes, fm = 0
+  // reset_lp (lp_m.f90:228-294) + track_lp2node onto the flow grid:
+  // particles back on their nodes, fm = 0
   void reset_diag(Diag& dg) {
+    const long long np = NP(dg);
     if (dg.dir > 0) {
-      launch_fwd_reset(g, pset(dg.part[dg.pcur]), stream);
+      if (dg.sg) launch_fwd_place(g, dg.sg->g, pset(dg.part[dg.pcur], np), st, stream);
+      else launch_fwd_reset(g, pset(dg.part[dg.pcur], np), stream);
       dg.sorted = false;
       dg.since_sort = 0;
       dg.cmode = g_opt.compensated;
     } else {
-      (void)hipMemsetAsync(dg.fm[0].p, 0, 3 * fbytes(), stream);
-      (void)hipMemsetAsync(dg.fm[1].p, 0, 3 * fbytes(), stream);
+      (void)hipMemsetAsync(dg.fm[0].p, 0, 3 * fbytes(NT(dg)), stream);
+      (void)hipMemsetAsync(dg.fm[1].p, 0, 3 * fbytes(NT(dg)), stream);
+      dg.fresh_nodes = true;
     }
   }
 
@@ -740,63 +887,10 @@ struct Engine : EngineBase {
     }
     const U dt_f = t_np1 - t_n;
     for (auto& dgp : diags) {
-      Diag& dg = *dgp;
-      // set_dt (lp_motion_m.f90:56-127)
-      const U cmax = (U)((T)cfl_rate) * dt_f;
-      const double r = std::ceil((double)(cmax / (g_opt.cfl_max * dg.dt_factor)));
-      int nsub = 1;
-      if (r > 1e6) {
-        fail("CFL sub-cycle count above 1e6 (velocity or time step out of range)");
-        return;
-      }
-      if (r >= 1) nsub = (int)r;
-      nsub_last = nsub;
-      U dt = dt_f / (U)nsub;
-      if (dg.dir < 0) dt = (U)-1 * dt;
-      if (std::fabs(dt) > 0) {
-        if (dg.dir > 0) integ_fwd += (long long)g.np * nsub;
-        else integ_bkwd += (long long)g.np * nsub;
-      }
-      Timed tm;
-      timed_begin(tm, dg.dir > 0 ? 0 : 1);
-      if (dg.dir > 0) {
-        U t = t_n;
-        if (g_opt.particle_sort > 0 && dg.since_sort >= g_opt.particle_sort) {
-          bool ok = true;
-          ktimed(15, [&] { ok = sort_particles(dg); });
-          if (!ok) return;
-        }
-        const ParticleSet<T> ps = pset(dg.part[dg.pcur]);
-        for (int s = 0; s < nsub; s++) {
-          const StageParams<T> sp = stage_params(t, dt, spl);
-          ktimed(8, [&] { launch_fwd_step(g, interp, sp, ps, dg.cmode, st, stream); });
-          t = t + dt;
-        }
-        if (std::fabs(dt) > 0) dg.since_sort++;
-      } else {
-        U t = t_np1;
-        if (spl && !dg.fmcoef.p && !dg.fmcoef.alloc(3 * fbytes())) {
-          fail("hipMalloc failed (flowmap coefficients)");
-          return;
-        }
-        for (int s = 0; s < nsub; s++) {
-          DevBuf& a = dg.fm[dg.fcur];
-          DevBuf& b = dg.fm[1 - dg.fcur];
-          CF3<T> src = cf3(a);
-          if (spl) {
-            HIPCHK(hipMemcpyAsync(dg.fmcoef.p, a.p, 3 * fbytes(), hipMemcpyDeviceToDevice, stream));
-            ktimed(10, [&] { launch_spline_fit(g, f3(dg.fmcoef), 3, stream); });
-            src = cf3(dg.fmcoef);
-          }
-          const StageParams<T> sp = stage_params(t, dt, spl);
-          ktimed(9, [&] { launch_sl_step(g, interp, sp, src, f3(b), st, stream, g_opt.sl_tiled); });
-          ktimed(13, [&] { launch_fm_ghost(g, f3(b), cf3(a), st, stream); });
-          dg.fcur = 1 - dg.fcur;
-          t = t + dt;
-        }
-      }
-      timed_end(tm);
-      HIPCHK(hipGetLastError());
+      for_sets(*dgp, [&](Diag& dg) {
+        if (!g_error) advect(dg, interp, spl, cfl_rate, dt_f);
+      });
+      if (g_error) return;
     }
     // diagnostics at h boundaries (cfd2lcs.f90:144-224)
     for (auto& dgp : diags) {
@@ -805,6 +899,79 @@ struct Engine : EngineBase {
       output(dg, interp);
       if (g_error) return;
     }
+  }
+
+  // one particle set through the CFD step: set_dt (lp_motion_m.f90:56-127),
+  // then update_lp (forward) or update_flowmap_sl (backward)
+  void advect(Diag& dg, int interp, bool spl, double cfl_rate, U dt_f) {
+    const U cmax = (U)((T)cfl_rate) * dt_f;
+    const double r = std::ceil((double)(cmax / (g_opt.cfl_max * dg.dt_factor)));
+    int nsub = 1;
+    if (r > 1e6) {
+      fail("CFL sub-cycle count above 1e6 (velocity or time step out of range)");
+      return;
+    }
+    if (r >= 1) nsub = (int)r;
+    nsub_last = nsub;
+    U dt = dt_f / (U)nsub;
+    if (dg.dir < 0) dt = (U)-1 * dt;
+    const long long np = NP(dg), nt = NT(dg);
+    if (std::fabs(dt) > 0) {
+      if (dg.dir > 0) integ_fwd += np * nsub;
+      else integ_bkwd += np * nsub;
+    }
+    Timed tm;
+    timed_begin(tm, dg.dir > 0 ? 0 : 1);
+    if (dg.dir > 0) {
+      U t = t_n;
+      if (g_opt.particle_sort > 0 && dg.since_sort >= g_opt.particle_sort) {
+        bool ok = true;
+        ktimed(15, [&] { ok = sort_particles(dg); });
+        if (!ok) return;
+      }
+      const ParticleSet<T> ps = pset(dg.part[dg.pcur], np);
+      for (int s = 0; s < nsub; s++) {
+        const StageParams<T> sp = stage_params(t, dt, spl);
+        ktimed(8, [&] { launch_fwd_step(g, interp, sp, ps, (int)np, dg.cmode, st, stream); });
+        t = t + dt;
+      }
+      if (std::fabs(dt) > 0) dg.since_sort++;
+    } else {
+      U t = t_np1;
+      GridDev<T>& lg = G(dg);
+      if (spl && !ensure_set_spline(dg, interp)) return;
+      if (spl && !dg.fmcoef.p && !dg.fmcoef.alloc(3 * fbytes(nt))) {
+        fail("hipMalloc failed (flowmap coefficients)");
+        return;
+      }
+      // the MASK test reads lp%no_scfd as it stands at the start of the update
+      const bool fresh_at_start = dg.fresh_nodes;
+      for (int s = 0; s < nsub; s++) {
+        DevBuf& a = dg.fm[dg.fcur];
+        DevBuf& b = dg.fm[1 - dg.fcur];
+        CF3<T> src = cf3(a, nt);
+        if (spl) {
+          HIPCHK(hipMemcpyAsync(dg.fmcoef.p, a.p, 3 * fbytes(nt), hipMemcpyDeviceToDevice, stream));
+          ktimed(10, [&] { launch_spline_fit(lg, f3(dg.fmcoef, nt), 3, stream); });
+          src = cf3(dg.fmcoef, nt);
+        }
+        const StageParams<T> sp = stage_params(t, dt, spl);
+        if (dg.sg) {
+          const int fresh = (dg.fresh_nodes ? 1 : 0) | (fresh_at_start ? 2 : 0);
+          ktimed(9, [&] {
+            launch_sl_step2(g, lg, (const int16_t*)dg.sg->nodes.p, fresh, interp, sp, src, f3(b, nt), st, stream);
+          });
+          dg.fresh_nodes = false;
+        } else {
+          ktimed(9, [&] { launch_sl_step(g, interp, sp, src, f3(b), st, stream, g_opt.sl_tiled); });
+        }
+        ktimed(13, [&] { launch_fm_ghost(lg, f3(b, nt), cf3(a, nt), st, stream); });
+        dg.fcur = 1 - dg.fcur;
+        t = t + dt;
+      }
+    }
+    timed_end(tm);
+    HIPCHK(hipGetLastError());
   }
 
   bool fit_vel(int b) {
@@ -821,9 +988,66 @@ struct Engine : EngineBase {
     return true;
   }
 
-  T* slot_ptr(DevBuf& b, int s) { return (T*)b.p + (size_t)s * 3 * g.nt; }
+  T* slot_ptr(DevBuf& b, int s, long long nt) { return (T*)b.p + (size_t)s * 3 * nt; }
 
-  // write_flowmap_substep + reconstruct_flowmap + compute_ftle + reset_lp
+  // write_flowmap_substep (flowmap_m.f90:106-156) of one set: the h-interval
+  // flowmap into ring slot `step` (forward sets: exchange_lpmap first)
+  void store_slot(Diag& d, int step) {
+    const long long nt = NT(d);
+    T* sp = slot_ptr(d.slots, step, nt);
+    F3<T> slot{{sp, sp + nt, sp + 2 * nt}};
+    if (d.dir > 0) {
+      launch_field_from_origin(G(d), fwd_dx_by_origin(d), slot, stream);
+    } else {
+      HIPCHK(hipMemcpyAsync(sp, d.fm[d.fcur].p, 3 * fbytes(nt), hipMemcpyDeviceToDevice, stream));
+    }
+    d.slot_ok[step] = 1;
+    d.slot_coef_ok[step] = 0;
+    if (g_opt.write_slots) (void)write_slot_files(d, step);
+  }
+
+  // reconstruct_flowmap (flowmap_m.f90:158-323) of one set on its grid:
+  // composed displacement by origin node into dxout, as a ghosted field in
+  // compfm
+  void compose_set(Diag& d, int interp, bool spl, int step) {
+    const long long nt = NT(d), np = NP(d);
+    GridDev<T>& lg = G(d);
+    if (spl) {
+      if (!ensure_set_spline(d, interp)) return;
+      if (!d.slotcoef.p && !d.slotcoef.alloc((size_t)d.nslot * 3 * fbytes(nt))) {
+        fail("hipMalloc failed (slot coefficients)");
+        return;
+      }
+      for (int s = 0; s < d.nslot; s++) {
+        if (!d.slot_ok[s] || d.slot_coef_ok[s]) continue;
+        T* cp = slot_ptr(d.slotcoef, s, nt);
+        HIPCHK(hipMemcpyAsync(cp, slot_ptr(d.slots, s, nt), 3 * fbytes(nt), hipMemcpyDeviceToDevice, stream));
+        ktimed(10, [&] { launch_spline_fit(lg, F3<T>{{cp, cp + nt, cp + 2 * nt}}, 3, stream); });
+        d.slot_coef_ok[s] = 1;
+      }
+    }
+    // slot order (flowmap_m.f90:202-268)
+    std::vector<int>& order = d.order_h;
+    order.clear();
+    int s = step;
+    const int inc = d.dir > 0 ? 1 : -1;
+    if (d.dir > 0) s = s + inc;
+    for (int it = 0; it < d.nstep; it++) {
+      if (s >= d.nstep) s = 0;
+      if (s < 0) s = d.nstep - 1;
+      if (d.slot_ok[s]) order.push_back(s);
+      s += inc;
+    }
+    HIPCHK(hipMemcpyAsync(d.order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream));
+    ktimed(11, [&] {
+      launch_compose(lg, interp, (const T*)(spl ? d.slotcoef.p : d.slots.p), nt, (const int*)d.order.p,
+                     (int)order.size(), p3(d.dxout, np), st, stream);
+    });
+    ktimed(13, [&] { launch_field_from_origin(lg, cp3(d.dxout, np), f3(d.compfm, nt), stream); });
+  }
+
+  // the diagnostic at an h boundary (cfd2lcs.f90:147-213): every set's
+  // substep and composition, compute_ftle, the output file, reset_lp
   void output(Diag& dg, int interp) {
     const double md = std::fmod((double)t_np1, (double)dg.Tint);
     const int step = (int)std::lround((double)((U)md / dg.h));
@@ -831,65 +1055,49 @@ struct Engine : EngineBase {
       fail("flowmap slot index out of range");
       return;
     }
-    const size_t fb = fbytes();
-    T* sp = slot_ptr(dg.slots, step);
-    F3<T> slot{{sp, sp + g.nt, sp + 2 * g.nt}};
-    if (dg.dir > 0) {
-      launch_field_from_origin(g, fwd_dx_by_origin(dg), slot, stream);
-    } else {
-      HIPCHK(hipMemcpyAsync(sp, dg.fm[dg.fcur].p, 3 * fb, hipMemcpyDeviceToDevice, stream));
-    }
-    dg.slot_ok[step] = 1;
-    dg.slot_coef_ok[step] = 0;
-    if (!io_ok()) return;
-    if (g_opt.write_slots && !write_slot_files(dg, step)) return;
+    for_sets(dg, [&](Diag& d) {
+      if (!g_error) store_slot(d, step);
+    });
+    if (g_error || !io_ok()) return;
     if (g_opt.write_lcs && (g_opt.velocity_io || g_opt.invariants) && any_fwd() && !record_velocity(step)) return;
     const bool spl = is_spline(interp);
-    if (spl) {
-      if (!dg.slotcoef.p && !dg.slotcoef.alloc((size_t)dg.nslot * 3 * fb)) {
-        fail("hipMalloc failed (slot coefficients)");
-        return;
-      }
-      for (int s = 0; s < dg.nslot; s++) {
-        if (!dg.slot_ok[s] || dg.slot_coef_ok[s]) continue;
-        T* cp = slot_ptr(dg.slotcoef, s);
-        HIPCHK(hipMemcpyAsync(cp, slot_ptr(dg.slots, s), 3 * fb, hipMemcpyDeviceToDevice, stream));
-        ktimed(10, [&] { launch_spline_fit(g, F3<T>{{cp, cp + g.nt, cp + 2 * g.nt}}, 3, stream); });
-        dg.slot_coef_ok[s] = 1;
-      }
-    }
-    // slot order (flowmap_m.f90:202-268)
-    std::vector<int> order;
-    int s = step;
-    const int inc = dg.dir > 0 ? 1 : -1;
-    if (dg.dir > 0) s = s + inc;
-    for (int it = 0; it < dg.nstep; it++) {
-      if (s >= dg.nstep) s = 0;
-      if (s < 0) s = dg.nstep - 1;
-      if (dg.slot_ok[s]) order.push_back(s);
-      s += inc;
-    }
-    HIPCHK(hipMemcpyAsync(dg.order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream));
     Timed tc;
     timed_begin(tc, 2);
-    ktimed(11, [&] {
-      launch_compose(g, interp, (const T*)(spl ? dg.slotcoef.p : dg.slots.p), g.nt, (const int*)dg.order.p,
-                     (int)order.size(), p3(dg.dxout), st, stream);
+    for_sets(dg, [&](Diag& d) {
+      if (!g_error) compose_set(d, interp, spl, step);
     });
-    ktimed(13, [&] { launch_field_from_origin(g, cp3(dg.dxout), f3(dg.compfm), stream); });
     timed_end(tc);
+    if (g_error) return;
     Timed tf;
     timed_begin(tf, 3);
     const double tiny = sizeof(U) == 8 ? DBL_MIN : (double)FLT_MIN;
-    ktimed(12, [&] {
-      launch_ftle(g, cf3(dg.compfm), (double)dg.Tint, g_opt.incompressible, tiny, (double*)dg.ftle.p, p3(dg.comp),
-                  stream);
-    });
+    GridDev<T>& lg = G(dg);
+    const long long np = NP(dg), nt = NT(dg);
+    if (dg.aux.empty()) {
+      ktimed(12, [&] {
+        launch_ftle(lg, cf3(dg.compfm, nt), (double)dg.Tint, g_opt.incompressible, tiny, (double*)dg.ftle.p,
+                    p3(dg.comp, np), stream);
+      });
+    } else {
+      // gradient from the six auxiliary sets (lcs_m.f90:95-126); the FM
+      // columns are the diagnostic's own composed flowmap
+      AuxSets<T> ax;
+      for (int a = 0; a < 6; a++) {
+        Diag& A = *dg.aux[a];
+        for (int c = 0; c < 3; c++) ax.disp[a][c] = (const T*)A.dxout.p + (size_t)c * np;
+        ax.line[a] = A.sg->g.gl[a % 3];
+      }
+      ktimed(12, [&] {
+        launch_ftle_aux(lg, ax, 100.0 * tiny, (double)dg.Tint, g_opt.incompressible, tiny, (double*)dg.ftle.p,
+                        stream);
+      });
+      HIPCHK(hipMemcpyAsync(dg.comp.p, dg.dxout.p, 3 * sizeof(T) * (size_t)np, hipMemcpyDeviceToDevice, stream));
+    }
     timed_end(tf);
-    reset_diag(dg);
+    for_sets(dg, [&](Diag& d) { reset_diag(d); });
     dg.noutput++;
     HIPCHK(hipGetLastError());
-    // the order list must stay valid until the kernel ran: wait here
+    // the order lists must stay valid until their copies ran: wait here
     HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipMemcpy(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost));
     if (st_host->err & kErrTrack) {
@@ -942,10 +1150,11 @@ struct Engine : EngineBase {
     return h;
   }
   // interior of a ghosted 3-component field, on the host
-  std::shared_ptr<std::vector<U>> host_interior(CF3<T> src) {
-    if (!vint.p && !vint.alloc(3 * sizeof(T) * (size_t)g.np)) return nullptr;
-    launch_interior(g, src, p3(vint), stream);
-    return host_copy((const T*)vint.p, 3 * (size_t)g.np);
+  std::shared_ptr<std::vector<U>> host_interior(CF3<T> src) { return host_interior(g, src); }
+  std::shared_ptr<std::vector<U>> host_interior(const GridDev<T>& gd, CF3<T> src) {
+    if (!iscratch.ensure(3 * sizeof(T) * (size_t)gd.np)) return nullptr;
+    launch_interior(gd, src, p3(iscratch, gd.np), stream);
+    return host_copy((const T*)iscratch.p, 3 * (size_t)gd.np);
   }
   static void add3(io::Job& j, const std::shared_ptr<std::vector<U>>& h, size_t np) {
     for (int d = 0; d < 3; d++) j.arrays.push_back(io::Array{h->data() + d * np, np, sizeof(U) == 8});
@@ -955,26 +1164,30 @@ struct Engine : EngineBase {
   static std::string fname128(const std::string& s) { return s.size() > 128 ? s.substr(0, 128) : s; }
 
   // write_flowmap_substep (flowmap_m.f90:106-156): the h-interval flowmap of
-  // slot `step` and, with VELOCITY_IO, u_np1, as unformatted files
+  // slot `step` (on the set's grid, file <lp%label>-FM) and, with
+  // VELOCITY_IO, u_np1 on the flow grid; every set of a diagnostic writes
+  // the same u_np1 file, so the diagnostic's own set writes it once
   bool write_slot_files(Diag& dg, int step) {
     std::string e;
     if (!io::ensure_dir("cfd2lcs_tmp", e)) {
       fail(e);
       return false;
     }
-    const size_t np = (size_t)g.np;
-    T* sp = slot_ptr(dg.slots, step);
-    auto h = host_interior(CF3<T>{{sp, sp + g.nt, sp + 2 * g.nt}});
+    const long long nt = NT(dg);
+    const size_t np = (size_t)NP(dg);
+    T* sp = slot_ptr(dg.slots, step, nt);
+    auto h = host_interior(G(dg), CF3<T>{{sp, sp + nt, sp + 2 * nt}});
     if (!h) {
       fail("copy of a flowmap slot to the host failed");
       return false;
     }
     io::Job j;
     j.kind = io::Job::kUnformatted;
-    j.path = fname128("./cfd2lcs_tmp/" + io::i4(0) + "_" + dg.label + "-LP-FM_" + io::i4(step) + ".dat");
+    j.path = fname128("./cfd2lcs_tmp/" + io::i4(0) + "_" + dg.lp_label + "-FM_" + io::i4(step) + ".dat");
     add3(j, h, np);
     wr().submit(std::move(j));
-    if (g_opt.velocity_io || g_opt.invariants) {
+    if ((g_opt.velocity_io || g_opt.invariants) && dg.lp_label == dg.label + "-LP") {
+      const size_t np = (size_t)g.np;
       auto hv = host_interior(cf3(vel[cur]));
       if (!hv) {
         fail("copy of the velocity to the host failed");
@@ -1005,8 +1218,30 @@ struct Engine : EngineBase {
 
   // write_lcs + write_ftle (io_nohdf5_m.f90:47-265): X, Y, Z, [FM-X..Z],
   // [U-X..Z], [FLAG], FTLE of the latest composition
+  // lp%ugrid of a set on its own grid (flowmap_m.f90:228-247): the interior
+  // flow-grid velocity du, read back with periodic ghosts and zero
+  // elsewhere, interpolated at the set's nodes through their tracked
+  // (clamped) flow nodes with the active interpolator
+  const T* velocity_on_set(Diag& dg, const T* du) {
+    const int interp = g_opt.interpolator;
+    if (!uscratch[0].ensure(3 * fbytes()) || !uscratch[1].ensure(3 * sizeof(T) * (size_t)NP(dg))) {
+      fail("hipMalloc failed (velocity on a diagnostic grid)");
+      return nullptr;
+    }
+    launch_expand_periodic(g, du, f3(uscratch[0]), stream);
+    if (is_spline(interp)) {
+      if (!ensure_spline(interp)) return nullptr;
+      launch_spline_fit(g, f3(uscratch[0]), 3, stream);
+    }
+    launch_field_at_nodes(g, G(dg), (const int16_t*)dg.sg->nodes.p, interp, cf3(uscratch[0]), (T*)uscratch[1].p,
+                          stream);
+    return (const T*)uscratch[1].p;
+  }
+
   void write_lcs_file(Diag& dg, int step) {
-    const size_t np = (size_t)g.np;
+    const HostGrid& hgd = HG(dg);
+    const GridDev<T>& lg = G(dg);
+    const size_t np = (size_t)lg.np;
     long long findex = std::llround((double)(U)(t_np1 / dg.h));
     if (dg.dir > 0) findex -= std::llround((double)(U)(dg.Tint / dg.h));
     std::string fs;
@@ -1019,25 +1254,27 @@ struct Engine : EngineBase {
       fail(e);
       return;
     }
-    if (!out_xyz) {
+    std::shared_ptr<std::vector<U>>& oxyz = dg.sg ? dg.sg->out_xyz : out_xyz;
+    std::shared_ptr<std::vector<U>>& oflag = dg.sg ? dg.sg->out_flag : out_flag;
+    if (!oxyz) {
       auto v = std::make_shared<std::vector<U>>(3 * np);
       auto f = std::make_shared<std::vector<U>>(np);
-      for (int k = 1; k <= g.n[2]; k++)
-        for (int j = 1; j <= g.n[1]; j++)
-          for (int i = 1; i <= g.n[0]; i++) {
-            const long long p = hg.pid(i, j, k), l = hg.id(i, j, k);
-            for (int d = 0; d < 3; d++) (*v)[d * np + p] = (U)hg.g[d][l];
-            (*f)[p] = (U)hg.flag[l];
+      for (int k = 1; k <= hgd.n[2]; k++)
+        for (int j = 1; j <= hgd.n[1]; j++)
+          for (int i = 1; i <= hgd.n[0]; i++) {
+            const long long p = hgd.pid(i, j, k), l = hgd.id(i, j, k);
+            for (int d = 0; d < 3; d++) (*v)[d * np + p] = (U)hgd.g[d][l];
+            (*f)[p] = (U)hgd.flag[l];
           }
-      out_xyz = v;
-      out_flag = f;
+      oxyz = v;
+      oflag = f;
     }
     io::Job j;
     j.kind = io::Job::kTecplot;
     j.path = fname128("./cfd2lcs_output/" + dg.label + "_" + fs + ".dat");
     const std::string ft = fname128(dg.label + "-FTLE");
     std::string hd = "TITLE = \"" + ft + "\"\nVARIABLES =\n\"X\"\n\"Y\"\n\"Z\"\n";
-    add3(j, out_xyz, np);
+    add3(j, oxyz, np);
     if (g_opt.flowmap_io) {
       hd += "\"FM-X\"\n\"FM-Y\"\n\"FM-Z\"\n";
       auto h = host_copy((const T*)dg.comp.p, 3 * np);
@@ -1054,7 +1291,7 @@ struct Engine : EngineBase {
     const T* du = nullptr;
     if (g_opt.velocity_io || g_opt.invariants) {
       if (dg.dir < 0) {
-        if (!vint.p && !vint.alloc(3 * sizeof(T) * np)) {
+        if (!vint.p && !vint.alloc(3 * sizeof(T) * (size_t)g.np)) {
           fail("hipMalloc failed (velocity for the FTLE output)");
           return;
         }
@@ -1064,6 +1301,9 @@ struct Engine : EngineBase {
         const int vs = step + 1;
         if (vs < (int)uring.size() && uring_ok[vs]) du = (const T*)uring[vs].p;
       }
+      // on a diagnostic grid: interpolated at its nodes (the reference reads
+      // the flow-grid file with the diagnostic grid's extents here)
+      if (du && dg.sg && !(du = velocity_on_set(dg, du))) return;
     }
     if (g_opt.velocity_io) {
       hd += "\"U-X\"\n\"U-Y\"\n\"U-Z\"\n";
@@ -1076,17 +1316,17 @@ struct Engine : EngineBase {
     }
     if (g_opt.bcflag_io) {
       hd += "\"FLAG\"\n";
-      j.arrays.push_back(io::Array{out_flag->data(), np, sizeof(U) == 8});
-      j.keep.push_back(out_flag);
+      j.arrays.push_back(io::Array{oflag->data(), np, sizeof(U) == 8});
+      j.keep.push_back(oflag);
     }
     if (g_opt.invariants) {
       // compute_invariants (invariants_m.f90:15-198) of the same velocity
       hd += "\"Q\"\n\"C\"\n\"D\"\n\"H\"\n\"L2\"\n";
-      if (!invbuf.p && !invbuf.alloc(5 * sizeof(T) * np)) {
+      if (!invbuf.ensure(5 * sizeof(T) * np)) {
         fail("hipMalloc failed (velocity invariants)");
         return;
       }
-      launch_invariants(g, du, (T*)invbuf.p, stream);
+      launch_invariants(lg, du, (T*)invbuf.p, stream);
       auto h = host_copy((const T*)invbuf.p, 5 * np);
       if (!h) {
         fail("copy of the velocity invariants to the host failed");
@@ -1100,7 +1340,7 @@ struct Engine : EngineBase {
     io::format_es((double)t_np1, 18, 4, es);
     hd += "STRANDID = 0000000001, SOLUTIONTIME = " + std::string(es, 18) + "\n";
     char ijk[96];
-    std::snprintf(ijk, sizeof ijk, "I = %010d J = %010d K = %010d\n", g.n[0], g.n[1], g.n[2]);
+    std::snprintf(ijk, sizeof ijk, "I = %010d J = %010d K = %010d\n", lg.n[0], lg.n[1], lg.n[2]);
     hd += ijk;
     hd += "ZONETYPE = Ordered, DATAPACKING = BLOCK\n";
     j.header = hd;
@@ -1507,35 +1747,52 @@ struct Engine : EngineBase {
     out[3] = (double)tot;
   }
 
-  int copy_out(CF3<T> src, int ghosted, U* a, U* b, U* c) {
-    if (!stage_out.p && !stage_out.alloc(3 * sizeof(U) * (size_t)g.np)) return 1;
+  int copy_out(CF3<T> src, int ghosted, U* a, U* b, U* c) { return copy_out(g, src, ghosted, a, b, c); }
+  int copy_out(const GridDev<T>& gd, CF3<T> src, int ghosted, U* a, U* b, U* c) {
+    const long long np = gd.np;
+    if (!stage_out.ensure(3 * sizeof(U) * (size_t)np)) return 1;
     U* o = (U*)stage_out.p;
-    launch_interior_to_user<T, U>(g, src, ghosted, o, o + g.np, o + 2 * g.np, stream);
-    const size_t b1 = sizeof(U) * (size_t)g.np;
+    launch_interior_to_user<T, U>(gd, src, ghosted, o, o + np, o + 2 * np, stream);
+    const size_t b1 = sizeof(U) * (size_t)np;
     if (hipMemcpyAsync(a, o, b1, hipMemcpyDeviceToHost, stream) != hipSuccess) return 1;
-    if (b && hipMemcpyAsync(b, o + g.np, b1, hipMemcpyDeviceToHost, stream) != hipSuccess) return 1;
-    if (c && hipMemcpyAsync(c, o + 2 * g.np, b1, hipMemcpyDeviceToHost, stream) != hipSuccess) return 1;
+    if (b && hipMemcpyAsync(b, o + np, b1, hipMemcpyDeviceToHost, stream) != hipSuccess) return 1;
+    if (c && hipMemcpyAsync(c, o + 2 * np, b1, hipMemcpyDeviceToHost, stream) != hipSuccess) return 1;
     return hipStreamSynchronize(stream) == hipSuccess ? 0 : 1;
   }
 
   int get_flowmap(int h, U* a, U* b, U* c) override {
     Diag* dg = diag(h);
     if (!dg) return 1;
-    if (dg->dir > 0) return copy_out(fwd_dx_by_origin(*dg), 0, a, b, c);
-    return copy_out(cf3(dg->fm[dg->fcur]), 1, a, b, c);
+    if (dg->dir > 0) return copy_out(G(*dg), fwd_dx_by_origin(*dg), 0, a, b, c);
+    return copy_out(G(*dg), cf3(dg->fm[dg->fcur], NT(*dg)), 1, a, b, c);
   }
   int get_composed(int h, U* a, U* b, U* c) override {
     Diag* dg = diag(h);
     if (!dg) return 1;
-    return copy_out(cp3(dg->comp), 0, a, b, c);
+    return copy_out(G(*dg), cp3(dg->comp, NP(*dg)), 0, a, b, c);
   }
   int get_ftle(int h, U* out) override {
     Diag* dg = diag(h);
     if (!dg) return 1;
-    std::vector<double> tmp(g.np);
+    const long long np = NP(*dg);
+    std::vector<double> tmp(np);
     if (hipStreamSynchronize(stream) != hipSuccess) return 1;
-    if (hipMemcpy(tmp.data(), dg->ftle.p, sizeof(double) * g.np, hipMemcpyDeviceToHost) != hipSuccess) return 1;
-    for (long long p = 0; p < g.np; p++) out[p] = (U)tmp[p];
+    if (hipMemcpy(tmp.data(), dg->ftle.p, sizeof(double) * np, hipMemcpyDeviceToHost) != hipSuccess) return 1;
+    for (long long p = 0; p < np; p++) out[p] = (U)tmp[p];
+    return 0;
+  }
+  // extents and (when x is given) node coordinates of a diagnostic's grid
+  int get_grid(int h, int n[3], U* x, U* y, U* z) override {
+    Diag* dg = diag(h);
+    if (!dg) return 1;
+    const HostGrid& hgd = HG(*dg);
+    for (int d = 0; d < 3; d++) n[d] = hgd.n[d];
+    U* o[3] = {x, y, z};
+    for (int k = 1; k <= hgd.n[2]; k++)
+      for (int j = 1; j <= hgd.n[1]; j++)
+        for (int i = 1; i <= hgd.n[0]; i++)
+          for (int d = 0; d < 3; d++)
+            if (o[d]) o[d][hgd.pid(i, j, k)] = (U)hgd.g[d][hgd.id(i, j, k)];
     return 0;
   }
   const double* ftle_device(int h) override {
@@ -1654,10 +1911,6 @@ void cfd2lcs_init_c(MPI_Comm usercomm, int n[3], int offset[3], lcsdata_t* x, lc
 int cfd2lcs_diagnostic_init_c(int lcs_type, int resolution, lcsdata_t t, lcsdata_t h, char label[]) {
   if (g_error) return 0;
   if (!ensure_engine()) return 0;
-  if (resolution != 0) {
-    fail("resolution != 0 (refined LCS grids) is not supported by this build");
-    return 0;
-  }
   if (lcs_type == LP_TRACER) {
     fail("LP_TRACER diagnostics are not supported by this build");
     return 0;
@@ -1670,7 +1923,7 @@ int cfd2lcs_diagnostic_init_c(int lcs_type, int resolution, lcsdata_t t, lcsdata
     fail("NLCS exceeds NMAX_STRUCT");
     return 0;
   }
-  return g_eng->add_diag(lcs_type, t, h, label);
+  return g_eng->add_diag(lcs_type, resolution, t, h, label);
 }
 
 void cfd2lcs_update_c(int n[3], lcsdata_t* u, lcsdata_t* v, lcsdata_t* w, lcsdata_t time) {
@@ -1708,7 +1961,6 @@ void cfd2lcs_set_option_c(char option[], int val) {
     else if (val == LCS_FALSE) dst = 0;
     else warn = true;
   };
-  int dummy = 0;
   if (o == "SYNCTIMER") boolopt(g_opt.sync_timer);
   else if (o == "DEBUG") boolopt(g_opt.verbose);
   else if (o == "INCOMPRESSIBLE") boolopt(g_opt.incompressible);
@@ -1718,13 +1970,8 @@ void cfd2lcs_set_option_c(char option[], int val) {
   else if (o == "WRITE_LCS") boolopt(g_opt.write_lcs);
   else if (o == "WRITE_SLOTS") boolopt(g_opt.write_slots);
   else if (o == "VELOCITY_INVARIANTS") boolopt(g_opt.invariants);
-  else if (o == "AUX_GRID") {
-    boolopt(dummy);
-    if (val == LCS_TRUE) {
-      fail(o + " = LCS_TRUE is not supported by this build");
-      return;
-    }
-  } else if (o == "KERNEL_TIMING") {
+  else if (o == "AUX_GRID") boolopt(g_opt.aux_grid);
+  else if (o == "KERNEL_TIMING") {
     boolopt(g_opt.kernel_timing);
   } else if (o == "VELOCITY_STORAGE") {
     // storage of the gathered stage velocities: the compute precision, or
@@ -1817,6 +2064,10 @@ int cfd2lcs_get_composed_c(int lcs_handle, lcsdata_t* fx, lcsdata_t* fy, lcsdata
 int cfd2lcs_get_ftle_c(int lcs_handle, lcsdata_t* ftle) {
   if (!g_eng) return 1;
   return g_eng->get_ftle(lcs_handle, ftle);
+}
+int cfd2lcs_get_lcs_grid_c(int lcs_handle, int n[3], lcsdata_t* x, lcsdata_t* y, lcsdata_t* z) {
+  if (!g_eng) return 1;
+  return g_eng->get_grid(lcs_handle, n, x, y, z);
 }
 const double* cfd2lcs_get_ftle_device_c(int lcs_handle) { return g_eng ? g_eng->ftle_device(lcs_handle) : nullptr; }
 int cfd2lcs_get_noutput_c(int lcs_handle) { return g_eng ? g_eng->noutput(lcs_handle) : -1; }

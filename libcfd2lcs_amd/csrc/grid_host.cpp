@@ -68,7 +68,7 @@ void cross_unit(const double a[3], const double b[3], double out[3]) {
 }  // namespace
 
 std::string HostGrid::build(const int nn[3], const double* x, const double* y, const double* z, const int* fl,
-                            double edge_tol) {
+                            double edge_tol, double node_tol) {
   for (int d = 0; d < 3; d++) {
     n[d] = nn[d];
     N[d] = nn[d] + 2;
@@ -257,7 +257,8 @@ std::string HostGrid::build(const int nn[3], const double* x, const double* y, c
     line[d].resize(N[d]);
     for (int a = 0; a < N[d]; a++) line[d][a] = g[d][id(d == 0 ? a : 1, d == 1 ? a : 1, d == 2 ? a : 1)];
   }
-  // Interior and face-ghost nodes must lie exactly on the lines.  Ghost edges
+  // Interior and face-ghost nodes must lie on the lines (within node_tol
+  // times the extent: exactly for user grids).  Ghost edges
   // and corners carry the rounding of the periodic shifts (one shift per
   // neighbour direction), and where a periodic axis meets a wall the edge
   // keeps its mirror value, built from the spacing at the other end of the
@@ -265,7 +266,7 @@ std::string HostGrid::build(const int nn[3], const double* x, const double* y, c
   // only, and the kernels read face ghosts only.
   double ext = 0;
   for (int d = 0; d < 3; d++) ext = std::max(ext, std::fabs(bb[3 + d] - bb[d]));
-  const double tol = edge_tol * std::max(ext, 1.0);
+  const double tol = edge_tol * std::max(ext, 1.0), ntol = node_tol * std::max(ext, 1.0);
   for (int k = 0; k <= nk + 1 && separable; k++)
     for (int j = 0; j <= nj + 1 && separable; j++)
       for (int i = 0; i <= ni + 1; i++) {
@@ -273,7 +274,7 @@ std::string HostGrid::build(const int nn[3], const double* x, const double* y, c
         const int ghosts = (i < 1 || i > ni) + (j < 1 || j > nj) + (k < 1 || k > nk);
         const double e = std::max(std::fabs(g[0][l] - line[0][i]),
                                   std::max(std::fabs(g[1][l] - line[1][j]), std::fabs(g[2][l] - line[2][k])));
-        if (ghosts <= 1 ? e != 0.0 : e > tol) {
+        if (ghosts <= 1 ? e > ntol : e > tol) {
           separable = false;
           break;
         }
@@ -474,5 +475,103 @@ std::string spline_axis(const std::vector<double>& x, int k, SplineAxis& out) {
   }
   return "";
 }
+
+// new_sgrid_from_sgrid on one rank, rectilinear case (sgrid_m.f90:484-620
+// counts, 635-665 coordinates, 693-740 removal).  The arithmetic runs in R,
+// the precision of the library's reals (LCSRP).
+template <typename R>
+int refine_grid(const HostGrid& src, int res, int nn[3], std::vector<double> xyz[3], std::vector<int>& flag) {
+  for (int d = 0; d < 3; d++)
+    if (src.n[d] > 1 && std::abs(res) >= src.n[d]) res = 0;
+  if (res == 0) return 0;
+  std::vector<int> map[3], sub[3];
+  for (int d = 0; d < 3; d++) {
+    const int n = src.n[d];
+    int m;
+    if (n == 1) m = 1;
+    else if (res > 0) m = src.per[d] ? n * (res + 1) : n * (res + 1) - res;
+    else m = n / (std::abs(res) + 1);
+    nn[d] = m;
+    map[d].resize(m);
+    sub[d].resize(m);
+    for (int a = 0; a < m; a++) {
+      if (n == 1) { map[d][a] = 1; sub[d][a] = 0; }
+      else if (res > 0) { map[d][a] = 1 + a / (res + 1); sub[d][a] = a % (res + 1); }
+      else { map[d][a] = (a + 1) * (std::abs(res) + 1); sub[d][a] = 0; }
+    }
+  }
+  // per axis: the new coordinate line (the grid is rectilinear)
+  std::vector<R> lin[3];
+  for (int d = 0; d < 3; d++) {
+    lin[d].resize(nn[d]);
+    for (int a = 0; a < nn[d]; a++) {
+      const int ii = map[d][a];
+      const R x0 = (R)src.line[d][ii];
+      if (res > 0) {
+        const R dx = ((R)src.line[d][ii + 1] - x0) / (R)(res + 1);
+        lin[d][a] = x0 + dx * (R)sub[d][a];
+      } else {
+        lin[d][a] = x0;
+      }
+    }
+  }
+  const long long np = (long long)nn[0] * nn[1] * nn[2];
+  for (int d = 0; d < 3; d++) xyz[d].resize(np);
+  flag.resize(np);
+  for (int k = 0; k < nn[2]; k++)
+    for (int j = 0; j < nn[1]; j++)
+      for (int i = 0; i < nn[0]; i++) {
+        const long long p = i + (long long)nn[0] * (j + (long long)nn[1] * k);
+        xyz[0][p] = (double)lin[0][i];
+        xyz[1][p] = (double)lin[1][j];
+        xyz[2][p] = (double)lin[2][k];
+        flag[p] = src.flag[src.id(map[0][i], map[1][j], map[2][k])];
+      }
+  return res;
+}
+
+// create_aux_grid (lcs_m.f90:264-300 offsets, 317-334 shifts, 339-344
+// projection): aux_delta is AUX_GRID_SCALING (0.2, data_m.f90:92) times the
+// mean distance to the in-range face neighbours, in NBR_OFFSET order; set a
+// moves every node by -/+ aux_delta along axis a % 3 (X0, Y0, Z0, X1, Y1, Z1),
+// and on a 2-D grid the node is put back onto the data plane.  The shift
+// starts from the node coordinate.
+template <typename R>
+void aux_grid_coords(const HostGrid& src, int a, std::vector<double> xyz[3]) {
+  static const int face[6][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  const int ni = src.n[0], nj = src.n[1], nk = src.n[2];
+  for (int d = 0; d < 3; d++) xyz[d].resize(src.np);
+  for (int k = 1; k <= nk; k++)
+    for (int j = 1; j <= nj; j++)
+      for (int i = 1; i <= ni; i++) {
+        const long long l = src.id(i, j, k), p = src.pid(i, j, k);
+        R nbrsum = 0, distsum = 0;
+        for (int f = 0; f < 6; f++) {
+          const int ii = i + face[f][0], jj = j + face[f][1], kk = k + face[f][2];
+          if (ii > 0 && jj > 0 && kk > 0 && ii <= ni && jj <= nj && kk <= nk) {
+            const long long m = src.id(ii, jj, kk);
+            nbrsum = nbrsum + (R)1;
+            const R dx = (R)src.g[0][l] - (R)src.g[0][m], dy = (R)src.g[1][l] - (R)src.g[1][m],
+                    dz = (R)src.g[2][l] - (R)src.g[2][m];
+            distsum = distsum + std::sqrt(dx * dx + dy * dy + dz * dz);
+          }
+        }
+        const R delta = distsum / nbrsum * (R)0.2;
+        R q[3] = {(R)src.g[0][l], (R)src.g[1][l], (R)src.g[2][l]};
+        const int ax = a % 3;
+        q[ax] = a < 3 ? q[ax] - delta : q[ax] + delta;
+        if (src.flat >= 0) {
+          const int f = src.flat;
+          const R plane = (R)src.g[f][l];
+          q[f] = q[f] - (q[f] - plane);
+        }
+        for (int d = 0; d < 3; d++) xyz[d][p] = (double)q[d];
+      }
+}
+
+template int refine_grid<float>(const HostGrid&, int, int*, std::vector<double>*, std::vector<int>&);
+template int refine_grid<double>(const HostGrid&, int, int*, std::vector<double>*, std::vector<int>&);
+template void aux_grid_coords<float>(const HostGrid&, int, std::vector<double>*);
+template void aux_grid_coords<double>(const HostGrid&, int, std::vector<double>*);
 
 }  // namespace lcs

@@ -32,9 +32,13 @@ struct HostGrid {
   // Builds the grid from the user's interior arrays (Fortran order).
   // edge_tol: relative tolerance (times the domain extent) for ghost edges and
   // corners against the coordinate lines; it follows the precision of the
-  // user's coordinates.  Returns an empty string on success, else the reason.
+  // user's coordinates.  node_tol: the same for interior and face-ghost nodes
+  // (0: exactly on the lines; a few ulp for generated grids whose nodes carry
+  // rounding, e.g. AUX_GRID shifts, which the reference's rectilinear test
+  // accepts within 1e-4, sgrid_m.f90:875).  Returns an empty string on
+  // success, else the reason.
   std::string build(const int nn[3], const double* x, const double* y, const double* z, const int* fl,
-                    double edge_tol = 1e-9);
+                    double edge_tol = 1e-9, double node_tol = 0.0);
 };
 
 // Knots (dbknot rule) and the banded LU factors of the collocation matrix
@@ -49,5 +53,16 @@ struct SplineAxis {
   std::vector<double> poly;
 };
 std::string spline_axis(const std::vector<double>& x, int k, SplineAxis& out);
+
+// The grid of a diagnostic with resolution res != 0 (new_sgrid_from_sgrid):
+// interior coordinates (Fortran order) and flags.  Returns the effective
+// res, 0 when |res| >= n along an axis with n > 1 (the flow grid is used).
+// R is the precision of the library's reals.
+template <typename R>
+int refine_grid(const HostGrid& src, int res, int nn[3], std::vector<double> xyz[3], std::vector<int>& flag);
+// Interior coordinates of AUX_GRID set a = 0..5 (X0, Y0, Z0, X1, Y1, Z1) of
+// grid src (create_aux_grid); its flags are those of src.
+template <typename R>
+void aux_grid_coords(const HostGrid& src, int a, std::vector<double> xyz[3]);
 
 }  // namespace lcs

@@ -382,10 +382,10 @@ __global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b,
 // integrate, BC, stick, dx accumulation, one tracking pass, periodic wrap.
 // ---------------------------------------------------------------------------
 template <typename T, typename S, class I, bool COMP>
-__global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T> sp, ParticleSet<T> ps,
+__global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T> sp, ParticleSet<T> ps, int np,
                                                      DevStatus* st) {
   const int p = xcd_thread_index();
-  if (p >= g.np) return;
+  if (p >= np) return;
   T q[3] = {ps.x[0][p], ps.x[1][p], ps.x[2][p]};
   int nd[3] = {ps.n[0][p], ps.n[1][p], ps.n[2][p]};
   const T xn[3] = {q[0], q[1], q[2]};
@@ -479,6 +479,189 @@ __global__ void __launch_bounds__(kBlock) k_fwd_reset(GridDev<T> g, ParticleSet<
   ps.dl[0][p] = T(0); ps.dl[1][p] = T(0); ps.dl[2][p] = T(0);
   ps.n[0][p] = (int16_t)i; ps.n[1][p] = (int16_t)j; ps.n[2][p] = (int16_t)k;
   ps.org[p] = p;
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostic grids other than the flow grid: resolution != 0
+// (new_sgrid_from_sgrid, sgrid_m.f90:445-761) and the AUX_GRID sets
+// (create_aux_grid, lcs_m.f90:227-435).  Particles sit on the nodes of the
+// diagnostic grid lg; their velocity, particle BCs and tracking use the flow
+// grid g through the tracked flow node (lp%no_scfd), the flowmap and its
+// interpolation the lg node (lp%no).
+// ---------------------------------------------------------------------------
+// reset_lp's flow-node guess nint(real(i*n_flow)/real(n)) in default REAL,
+// clamped to [1, n_flow] (lp_m.f90:274-277)
+__device__ __forceinline__ int guess_node(int i, int nf, int nl) {
+  const double r = (double)((float)(i * nf) / (float)nl);
+  const int v = (int)(r >= 0.0 ? floor(r + 0.5) : -floor(-r + 0.5));
+  return iclamp(v, 1, nf);
+}
+
+// recursive track_lp2node onto g (lp_tracking_m.f90:16-183) of the particle
+// at q from the node nd: passes until the node stays, each followed for
+// forward sets by exchange_lpdata's periodic wrap (the shifts summed into
+// sh).  A particle that did not move stays unmoved in later passes, so each
+// runs to its own fixed point; false after more than 2 max(n) passes.
+template <typename T>
+__device__ bool track_recursive(const GridDev<T>& g, bool fwd, T q[3], int nd[3], T sh[3]) {
+  for (int pass = 1;; pass++) {
+    const bool moved = track_pass(g, q, nd);
+    if (pass > g.max_track) return false;
+    if (fwd) {
+      T w[3];
+      wrap(g, nd, q, w);
+#pragma unroll
+      for (int d = 0; d < 3; d++) sh[d] = sh[d] + w[d];
+    }
+    if (!moved) return true;
+  }
+}
+
+// init_lp / reset_lp + track_lp2node of a backward set (no exchange): the
+// tracked flow node of every lg node, which may be a ghost node when the lg
+// node is nearer to a periodic ghost than to any interior node
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_track_nodes(GridDev<T> g, GridDev<T> lg, int16_t* __restrict__ nodes,
+                                                        DevStatus* st) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= lg.np) return;
+  int i, j, k;
+  decode_interior(lg, p, i, j, k);
+  T q[3] = {lg.gl[0][i], lg.gl[1][j], lg.gl[2][k]}, sh[3] = {T(0), T(0), T(0)};
+  int nd[3] = {guess_node(i, g.n[0], lg.n[0]), guess_node(j, g.n[1], lg.n[1]), guess_node(k, g.n[2], lg.n[2])};
+  if (!track_recursive(g, false, q, nd, sh)) atomicOr(&st->err, (int)kErrTrack);
+  nodes[p] = (int16_t)nd[0];
+  nodes[lg.np + p] = (int16_t)nd[1];
+  nodes[2LL * lg.np + p] = (int16_t)nd[2];
+}
+
+// reset_lp + track_lp2node of a forward set: particles on the lg nodes,
+// tracked onto g with exchange_lpdata's wrap after every pass
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fwd_place(GridDev<T> g, GridDev<T> lg, ParticleSet<T> ps,
+                                                      DevStatus* st) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= lg.np) return;
+  int i, j, k;
+  decode_interior(lg, p, i, j, k);
+  const T x0[3] = {lg.gl[0][i], lg.gl[1][j], lg.gl[2][k]};
+  T q[3] = {x0[0], x0[1], x0[2]}, sh[3] = {T(0), T(0), T(0)};
+  int nd[3] = {guess_node(i, g.n[0], lg.n[0]), guess_node(j, g.n[1], lg.n[1]), guess_node(k, g.n[2], lg.n[2])};
+  if (!track_recursive(g, true, q, nd, sh)) atomicOr(&st->err, (int)kErrTrack);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    ps.x[d][p] = q[d];
+    ps.d[d][p] = T(0);
+    ps.dl[d][p] = T(0);
+    // the low part of the node coordinate plus its periodic shifts
+    T hi, lo;
+    two_sum(x0[d], sh[d], hi, lo);
+    ps.xl[d][p] = hi == q[d] ? lo : T(0);
+    ps.n[d][p] = (int16_t)nd[d];
+  }
+  ps.org[p] = p;
+}
+
+// update_flowmap_sl (flowmap_m.f90:17-104) for one node of lg: the stages
+// interpolate the velocity on g through the tracked flow node, the BC and
+// the MASK test use it too (set_lp_bc clamps lp%no_scfd to the interior in
+// place, so the tracked node is used as it is only until the first
+// sub-cycle after a reset: bits of fresh), and the old flowmap is
+// interpolated on lg through the node itself.  The flowmap BC of lg zeroes
+// MASK/INFLOW/OUTFLOW nodes of lg (set_flowmap_bc, flowmap_m.f90:345-360).
+template <typename T, typename S, class I>
+__global__ void __launch_bounds__(kBlock) k_sl_step2(GridDev<T> g, GridDev<T> lg, const int16_t* __restrict__ nodes,
+                                                     int fresh, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
+                                                     DevStatus* st) {
+  const int p = xcd_thread_index();
+  if (p >= lg.np) return;
+  int i, j, k;
+  decode_interior(lg, p, i, j, k);
+  const I interp{};
+  const T x0[3] = {lg.gl[0][i], lg.gl[1][j], lg.gl[2][k]};
+  int nd[3], nm[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int t = nodes[(long long)d * lg.np + p];
+    const int c = iclamp(t, 1, g.n[d]);
+    nd[d] = (fresh & 1) ? t : c;
+    nm[d] = (fresh & 2) ? t : c;
+  }
+  const T mask = g.flag[gid(g, nm[0], nm[1], nm[2])] != kMask ? T(1) : T(0);
+  unsigned long long oob = 0;
+  int err = 0;
+  auto first = [&](T up[3]) { interp(g, x0, nd, stage<S>(sp, 0), up, oob); };
+  T inc[3], xr[3], xp[3];
+  integrate_inc<S>(g, sp, interp, first, nd, x0, inc, oob);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    xr[d] = x0[d] + inc[d];
+    xp[d] = xr[d];
+  }
+  project_flat(g, nd, xp);
+  if (set_lp_bc(g, nd, xp, err)) {
+    xp[0] = x0[0]; xp[1] = x0[1]; xp[2] = x0[2];
+  }
+  int no[3] = {i, j, k};
+  T fmp[3];
+  const Split<T> sq{{x0[0], x0[1], x0[2]}, {inc[0], inc[1], inc[2]}};
+  const bool split = xp[0] == xr[0] && xp[1] == xr[1] && xp[2] == xr[2];
+  interp(lg, xp, no, fmold, fmp, oob, split ? &sq : nullptr);
+  const int l = gid(lg, i, j, k);
+  const int f = lg.flag[l];
+  const bool zero = (f == kMask || f == kInflow || f == kOutflow);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const T disp = xp[d] == xr[d] ? inc[d] : xp[d] - x0[d];
+    const T v = (fmp[d] + disp) * mask;
+    fmnew.c[d][l] = zero ? T(0) : v;
+  }
+  flush_status(st, err, oob);
+}
+
+// a field read back from its interior values (structured_io + exchange_sdata
+// of a zeroed sr1, flowmap_m.f90:228-232): periodic ghosts are the images,
+// every other ghost zero
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_expand_periodic(GridDev<T> g, const T* __restrict__ src, F3<T> dst) {
+  const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= g.nt) return;
+  const int i = (int)(l % g.N[0]), j = (int)((l / g.N[0]) % g.N[1]), k = (int)(l / ((long long)g.N[0] * g.N[1]));
+  const int ei = i < 1 ? -1 : (i > g.n[0] ? 1 : 0), ej = j < 1 ? -1 : (j > g.n[1] ? 1 : 0),
+            ek = k < 1 ? -1 : (k > g.n[2] ? 1 : 0);
+  const bool ghost = ei || ej || ek;
+  if (ghost && !self_comm(g, ei, ej, ek)) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) dst.c[c][l] = T(0);
+    return;
+  }
+  const int si = ei < 0 ? g.n[0] : (ei > 0 ? 1 : i), sj = ej < 0 ? g.n[1] : (ej > 0 ? 1 : j),
+            sk = ek < 0 ? g.n[2] : (ek > 0 ? 1 : k);
+  const long long q = pid(g, si, sj, sk);
+#pragma unroll
+  for (int c = 0; c < 3; c++) dst.c[c][l] = src[(long long)c * g.np + q];
+}
+
+// interp_s2u_r1 of a field on g at the lg nodes through their clamped
+// tracked flow nodes (the velocity at T0 of a diagnostic grid,
+// flowmap_m.f90:234-241)
+template <typename T, class I>
+__global__ void __launch_bounds__(kBlock) k_field_at_nodes(GridDev<T> g, GridDev<T> lg,
+                                                           const int16_t* __restrict__ nodes, CF3<T> f,
+                                                           T* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= lg.np) return;
+  int i, j, k;
+  decode_interior(lg, p, i, j, k);
+  const T q[3] = {lg.gl[0][i], lg.gl[1][j], lg.gl[2][k]};
+  int nd[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) nd[d] = iclamp((int)nodes[(long long)d * lg.np + p], 1, g.n[d]);
+  unsigned long long oob = 0;
+  T o[3];
+  I{}(g, q, nd, f, o, oob);
+#pragma unroll
+  for (int c = 0; c < 3; c++) out[(long long)c * lg.np + p] = o[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -583,6 +766,20 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __res
 // (lcs_m.f90:168-185).  Gradient, tensor and eigen-solve run in FP64 for
 // both storage precisions.
 // ---------------------------------------------------------------------------
+// C = G^T G (no identity term), its largest eigenvalue, the FTLE
+__device__ __forceinline__ double ftle_of_grad(const double G[3][3], double Tint, int incomp, double tiny) {
+  const double cxx = G[0][0] * G[0][0] + G[1][0] * G[1][0] + G[2][0] * G[2][0];
+  const double cxy = G[0][0] * G[0][1] + G[1][0] * G[1][1] + G[2][0] * G[2][1];
+  const double cxz = G[0][0] * G[0][2] + G[1][0] * G[1][2] + G[2][0] * G[2][2];
+  const double cyy = G[0][1] * G[0][1] + G[1][1] * G[1][1] + G[2][1] * G[2][1];
+  const double cyz = G[0][1] * G[0][2] + G[1][1] * G[1][2] + G[2][1] * G[2][2];
+  const double czz = G[0][2] * G[0][2] + G[1][2] * G[1][2] + G[2][2] * G[2][2];
+  const double lam = eig_sym3_max(cxx, cxy, cxz, cyy, cyz, czz);
+  double r = log(fmax(sqrt(lam), tiny)) / Tint;
+  if (incomp) r = fmax(r, 0.0);
+  return r;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_ftle(GridDev<T> g, CF3<T> fm, double Tint, int incomp, double tiny,
                                                  double* __restrict__ ftle, F3<T> comp) {
@@ -612,16 +809,31 @@ __global__ void __launch_bounds__(kBlock) k_ftle(GridDev<T> g, CF3<T> fm, double
   if (g.n[0] == 1) { G[0][0] = 0; G[1][0] = 0; G[2][0] = 0; }
   if (g.n[1] == 1) { G[0][1] = 0; G[1][1] = 0; G[2][1] = 0; }
   if (g.n[2] == 1) { G[0][2] = 0; G[1][2] = 0; G[2][2] = 0; }
-  const double cxx = G[0][0] * G[0][0] + G[1][0] * G[1][0] + G[2][0] * G[2][0];
-  const double cxy = G[0][0] * G[0][1] + G[1][0] * G[1][1] + G[2][0] * G[2][1];
-  const double cxz = G[0][0] * G[0][2] + G[1][0] * G[1][2] + G[2][0] * G[2][2];
-  const double cyy = G[0][1] * G[0][1] + G[1][1] * G[1][1] + G[2][1] * G[2][1];
-  const double cyz = G[0][1] * G[0][2] + G[1][1] * G[1][2] + G[2][1] * G[2][2];
-  const double czz = G[0][2] * G[0][2] + G[1][2] * G[1][2] + G[2][2] * G[2][2];
-  const double lam = eig_sym3_max(cxx, cxy, cxz, cyy, cyz, czz);
-  double r = log(fmax(sqrt(lam), tiny)) / Tint;
-  if (incomp) r = fmax(r, 0.0);
-  ftle[p] = r;
+  ftle[p] = ftle_of_grad(G, Tint, incomp, tiny);
+}
+
+// compute_ftle with AUX_GRID (lcs_m.f90:95-126): column d of the gradient is
+// the difference of the d1 and d0 sets' composed displacements over the
+// difference of their node coordinates along d, left at zero where that
+// difference is not above small = 100 tiny(1.0) (per node of the main grid;
+// the six sets share its extents)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_ftle_aux(GridDev<T> lg, AuxSets<T> aux, double small, double Tint,
+                                                     int incomp, double tiny, double* __restrict__ ftle) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= lg.np) return;
+  int ix[3];
+  decode_interior(lg, p, ix[0], ix[1], ix[2]);
+  double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double den = (double)aux.line[3 + d][ix[d]] - (double)aux.line[d][ix[d]];
+    if (fabs(den) > small) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) G[c][d] = ((double)aux.disp[3 + d][c][p] - (double)aux.disp[d][c][p]) / den;
+    }
+  }
+  ftle[p] = ftle_of_grad(G, Tint, incomp, tiny);
 }
 
 // ---------------------------------------------------------------------------
@@ -979,25 +1191,25 @@ void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, C
 }
 
 template <typename T>
-void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, ParticleSet<T> ps, int comp,
+void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, ParticleSet<T> ps, int np, int comp,
                      DevStatus* st, hipStream_t s) {
-  const dim3 grid = grid_for(g.np), blk(kBlock);
+  const dim3 grid = grid_for(np), blk(kBlock);
   with_storage<T>(sp.storage, [&](auto tag) {
     using S = decltype(tag);
     auto go = [&](auto C) {
       constexpr bool cm = decltype(C)::value;
       if (interp == kLinear) {
-        hipLaunchKernelGGL((k_fwd_step<T, S, IntLinear<T>, cm>), grid, blk, 0, s, g, sp, ps, st);
+        hipLaunchKernelGGL((k_fwd_step<T, S, IntLinear<T>, cm>), grid, blk, 0, s, g, sp, ps, np, st);
       } else if (interp == kNearest) {
-        hipLaunchKernelGGL((k_fwd_step<T, S, IntNearest<T>, cm>), grid, blk, 0, s, g, sp, ps, st);
+        hipLaunchKernelGGL((k_fwd_step<T, S, IntNearest<T>, cm>), grid, blk, 0, s, g, sp, ps, np, st);
       } else if (interp == kTSE) {
-        hipLaunchKernelGGL((k_fwd_step<T, S, IntTSE<T, false>, cm>), grid, blk, 0, s, g, sp, ps, st);
+        hipLaunchKernelGGL((k_fwd_step<T, S, IntTSE<T, false>, cm>), grid, blk, 0, s, g, sp, ps, np, st);
       } else if (interp == kTSELimit) {
-        hipLaunchKernelGGL((k_fwd_step<T, S, IntTSE<T, true>, cm>), grid, blk, 0, s, g, sp, ps, st);
+        hipLaunchKernelGGL((k_fwd_step<T, S, IntTSE<T, true>, cm>), grid, blk, 0, s, g, sp, ps, np, st);
       } else {
         dispatch_spline(g, [&](auto KX, auto KZ) {
           constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
-          hipLaunchKernelGGL((k_fwd_step<T, S, IntSpline<T, kx, kz>, cm>), grid, blk, 0, s, g, sp, ps, st);
+          hipLaunchKernelGGL((k_fwd_step<T, S, IntSpline<T, kx, kz>, cm>), grid, blk, 0, s, g, sp, ps, np, st);
         });
       }
     };
@@ -1009,6 +1221,76 @@ void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, 
 template <typename T>
 void launch_fwd_reset(const GridDev<T>& g, ParticleSet<T> ps, hipStream_t s) {
   hipLaunchKernelGGL((k_fwd_reset<T>), grid_for(g.np), dim3(kBlock), 0, s, g, ps);
+}
+
+template <typename T>
+void launch_track_nodes(const GridDev<T>& g, const GridDev<T>& lg, int16_t* nodes, DevStatus* st, hipStream_t s) {
+  hipLaunchKernelGGL((k_track_nodes<T>), grid_for(lg.np), dim3(kBlock), 0, s, g, lg, nodes, st);
+}
+
+template <typename T>
+void launch_fwd_place(const GridDev<T>& g, const GridDev<T>& lg, ParticleSet<T> ps, DevStatus* st, hipStream_t s) {
+  hipLaunchKernelGGL((k_fwd_place<T>), grid_for(lg.np), dim3(kBlock), 0, s, g, lg, ps, st);
+}
+
+template <typename T>
+void launch_sl_step2(const GridDev<T>& g, const GridDev<T>& lg, const int16_t* nodes, int fresh, int interp,
+                     const StageParams<T>& sp, CF3<T> fmold, F3<T> fmnew, DevStatus* st, hipStream_t s) {
+  const dim3 grid = grid_for(lg.np), blk(kBlock);
+  with_storage<T>(sp.storage, [&](auto tag) {
+    using S = decltype(tag);
+    if (interp == kLinear) {
+      hipLaunchKernelGGL((k_sl_step2<T, S, IntLinear<T>>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold, fmnew,
+                         st);
+    } else if (interp == kNearest) {
+      hipLaunchKernelGGL((k_sl_step2<T, S, IntNearest<T>>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold, fmnew,
+                         st);
+    } else if (interp == kTSE) {
+      hipLaunchKernelGGL((k_sl_step2<T, S, IntTSE<T, false>>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold,
+                         fmnew, st);
+    } else if (interp == kTSELimit) {
+      hipLaunchKernelGGL((k_sl_step2<T, S, IntTSE<T, true>>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold,
+                         fmnew, st);
+    } else {
+      dispatch_spline(g, [&](auto KX, auto KZ) {
+        constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
+        hipLaunchKernelGGL((k_sl_step2<T, S, IntSpline<T, kx, kz>>), grid, blk, 0, s, g, lg, nodes, fresh, sp,
+                           fmold, fmnew, st);
+      });
+    }
+  });
+}
+
+template <typename T>
+void launch_expand_periodic(const GridDev<T>& g, const T* interior, F3<T> dst, hipStream_t s) {
+  hipLaunchKernelGGL((k_expand_periodic<T>), grid_for(g.nt), dim3(kBlock), 0, s, g, interior, dst);
+}
+
+template <typename T>
+void launch_field_at_nodes(const GridDev<T>& g, const GridDev<T>& lg, const int16_t* nodes, int interp, CF3<T> f,
+                           T* out, hipStream_t s) {
+  const dim3 grid = grid_for(lg.np), blk(kBlock);
+  if (interp == kLinear) {
+    hipLaunchKernelGGL((k_field_at_nodes<T, IntLinear<T>>), grid, blk, 0, s, g, lg, nodes, f, out);
+  } else if (interp == kNearest) {
+    hipLaunchKernelGGL((k_field_at_nodes<T, IntNearest<T>>), grid, blk, 0, s, g, lg, nodes, f, out);
+  } else if (interp == kTSE) {
+    hipLaunchKernelGGL((k_field_at_nodes<T, IntTSE<T, false>>), grid, blk, 0, s, g, lg, nodes, f, out);
+  } else if (interp == kTSELimit) {
+    hipLaunchKernelGGL((k_field_at_nodes<T, IntTSE<T, true>>), grid, blk, 0, s, g, lg, nodes, f, out);
+  } else {
+    dispatch_spline(g, [&](auto KX, auto KZ) {
+      constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
+      hipLaunchKernelGGL((k_field_at_nodes<T, IntSpline<T, kx, kz>>), grid, blk, 0, s, g, lg, nodes, f, out);
+    });
+  }
+}
+
+template <typename T>
+void launch_ftle_aux(const GridDev<T>& lg, const AuxSets<T>& aux, double small, double Tint, int incomp,
+                     double tiny, double* ftle, hipStream_t s) {
+  hipLaunchKernelGGL((k_ftle_aux<T>), grid_for(lg.np), dim3(kBlock), 0, s, lg, aux, small, Tint, incomp, tiny,
+                     ftle);
 }
 
 template <typename T>
@@ -1168,9 +1450,19 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
   template void launch_blend<T>(const GridDev<T>&, void* const*, int, CF3<T>, CF3<T>, T, hipStream_t);     \
   template void launch_sl_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, F3<T>, DevStatus*,  \
                                   hipStream_t, int);                                                          \
-  template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, ParticleSet<T>, int,     \
+  template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, ParticleSet<T>, int, int, \
                                    DevStatus*, hipStream_t);                                                   \
   template void launch_fwd_reset<T>(const GridDev<T>&, ParticleSet<T>, hipStream_t);                          \
+  template void launch_track_nodes<T>(const GridDev<T>&, const GridDev<T>&, int16_t*, DevStatus*, hipStream_t); \
+  template void launch_fwd_place<T>(const GridDev<T>&, const GridDev<T>&, ParticleSet<T>, DevStatus*,          \
+                                    hipStream_t);                                                             \
+  template void launch_sl_step2<T>(const GridDev<T>&, const GridDev<T>&, const int16_t*, int, int,             \
+                                   const StageParams<T>&, CF3<T>, F3<T>, DevStatus*, hipStream_t);            \
+  template void launch_expand_periodic<T>(const GridDev<T>&, const T*, F3<T>, hipStream_t);                    \
+  template void launch_field_at_nodes<T>(const GridDev<T>&, const GridDev<T>&, const int16_t*, int, CF3<T>,   \
+                                         T*, hipStream_t);                                                    \
+  template void launch_ftle_aux<T>(const GridDev<T>&, const AuxSets<T>&, double, double, int, double, double*, \
+                                   hipStream_t);                                                              \
   template void launch_compose<T>(const GridDev<T>&, int, const T*, long long, const int*, int, F3<T>,       \
                                   DevStatus*, hipStream_t);                                                   \
   template void launch_ftle<T>(const GridDev<T>&, CF3<T>, double, int, double, double*, F3<T>, hipStream_t); \

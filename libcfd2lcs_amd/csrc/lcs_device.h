@@ -125,22 +125,64 @@ void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a
 template <typename T>
 void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold,
                     F3<T> fmnew, DevStatus* st, hipStream_t stream, int tiled = 0);
-// compensated: two-sum updates of position and displacement (uses xl, dl)
+// compensated: two-sum updates of position and displacement (uses xl, dl);
+// np particles
 template <typename T>
-void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, ParticleSet<T> ps,
+void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, ParticleSet<T> ps, int np,
                      int compensated, DevStatus* st, hipStream_t stream);
 template <typename T>
 void launch_fwd_reset(const GridDev<T>& g, ParticleSet<T> ps, hipStream_t stream);
-// device temporary bytes of the particle sort for np particles (0 on error)
-size_t particle_sort_temp_bytes(long long np);
+
+// ---- diagnostic grids other than the flow grid g (resolution != 0, AUX_GRID):
+// particles on the nodes of lg, velocity / particle BCs / tracking on g
+// tracked flow node of every lg node (3 x lg.np int16, component-major),
+// from reset_lp's guess by recursive tracking without periodic wrap
+template <typename T>
+void launch_track_nodes(const GridDev<T>& g, const GridDev<T>& lg, int16_t* nodes, DevStatus* st,
+                        hipStream_t stream);
+// forward particles reset onto the lg nodes, tracked onto g with the
+// periodic wrap after every pass
+template <typename T>
+void launch_fwd_place(const GridDev<T>& g, const GridDev<T>& lg, ParticleSet<T> ps, DevStatus* st,
+                      hipStream_t stream);
+// one semi-Lagrangian sub-cycle of the flowmap on lg; fresh bit 0: the stage
+// velocities and the particle BC use the tracked nodes as they are (else
+// clamped to the interior), bit 1: the MASK test does
+template <typename T>
+void launch_sl_step2(const GridDev<T>& g, const GridDev<T>& lg, const int16_t* nodes, int fresh, int interp,
+                     const StageParams<T>& sp, CF3<T> fmold, F3<T> fmnew, DevStatus* st, hipStream_t stream);
+// ghosted field on g from an interior one: interior copied, periodic ghosts
+// from their images, other ghosts zero (exchange_sdata of a zeroed field)
+template <typename T>
+void launch_expand_periodic(const GridDev<T>& g, const T* interior, F3<T> dst, hipStream_t stream);
+// field (values or spline coefficients on g) interpolated at the lg nodes
+// through their clamped tracked nodes, into out (3 x lg.np)
+template <typename T>
+void launch_field_at_nodes(const GridDev<T>& g, const GridDev<T>& lg, const int16_t* nodes, int interp,
+                           CF3<T> field, T* out, hipStream_t stream);
+// FTLE from the six AUX_GRID sets: aux[a] holds set a's composed
+// displacement by origin node (3 x np), line[a] the coordinate line of set
+// a's grid along axis a % 3 (ghosted, as GridDev::gl)
+template <typename T>
+struct AuxSets {
+  const T* disp[6][3];
+  const T* line[6];
+};
+template <typename T>
+void launch_ftle_aux(const GridDev<T>& lg, const AuxSets<T>& aux, double small, double T_int, int incompressible,
+                     double tiny, double* ftle, hipStream_t stream);
+// device temporary bytes of the particle sort for np particles with keys
+// below nkeys (0 on error)
+size_t particle_sort_temp_bytes(long long np, long long nkeys);
 // sorts set a by tracked node into set b; keys and vals hold 2*np entries
+// (np particles; keys from their tracked nodes on g)
 template <typename T>
-int launch_particle_sort(const GridDev<T>& g, ParticleSet<T> a, ParticleSet<T> b, unsigned* keys,
+int launch_particle_sort(const GridDev<T>& g, ParticleSet<T> a, ParticleSet<T> b, long long np, unsigned* keys,
                          unsigned* vals, void* temp, size_t temp_bytes, hipStream_t stream);
-// dst[org[q]] = src[q] (+ lo[q] when lo is given): per-particle values back
-// to origin order
+// dst[org[q]] = src[q] (+ lo[q] when lo is given) for q < np: per-particle
+// values back to origin order
 template <typename T>
-void launch_unpermute(const GridDev<T>& g, const int* org, CF3<T> src, const T* const lo[3], F3<T> dst,
+void launch_unpermute(long long np, const int* org, CF3<T> src, const T* const lo[3], F3<T> dst,
                       hipStream_t stream);
 template <typename T>
 void launch_field_from_origin(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStream_t stream);

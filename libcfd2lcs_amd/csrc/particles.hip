@@ -62,43 +62,44 @@ static int key_bits(long long np) {
   return b;
 }
 
-size_t particle_sort_temp_bytes(long long np) {
+size_t particle_sort_temp_bytes(long long np, long long nkeys) {
   size_t bytes = 0;
   hipcub::DoubleBuffer<unsigned> k(nullptr, nullptr), v(nullptr, nullptr);
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)np, 0, key_bits(np)) != hipSuccess) return 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)np, 0, key_bits(nkeys)) != hipSuccess) return 0;
   return bytes;
 }
 
+// the keys index the flow grid g (tracked nodes); the np particles may live
+// on another grid
 template <typename T>
-int launch_particle_sort(const GridDev<T>& g, ParticleSet<T> a, ParticleSet<T> b, unsigned* keys, unsigned* vals,
-                         void* temp, size_t temp_bytes, hipStream_t s) {
-  const int np = (int)g.np;
+int launch_particle_sort(const GridDev<T>& g, ParticleSet<T> a, ParticleSet<T> b, long long npl, unsigned* keys,
+                         unsigned* vals, void* temp, size_t temp_bytes, hipStream_t s) {
+  const int np = (int)npl;
   const dim3 grid((np + kPBlock - 1) / kPBlock);
   hipLaunchKernelGGL(k_sort_keys, grid, dim3(kPBlock), 0, s, g.n[0], g.n[1], np, a.n[0], a.n[1], a.n[2], keys,
                      vals);
   hipcub::DoubleBuffer<unsigned> kb(keys, keys + np), vb(vals, vals + np);
   size_t tb = temp_bytes;
-  if (hipcub::DeviceRadixSort::SortPairs(temp, tb, kb, vb, np, 0, key_bits(np), s) != hipSuccess) return 1;
+  if (hipcub::DeviceRadixSort::SortPairs(temp, tb, kb, vb, np, 0, key_bits(g.np), s) != hipSuccess) return 1;
   hipLaunchKernelGGL((k_permute<T>), grid, dim3(kPBlock), 0, s, np, (const unsigned*)vb.Current(), a, b);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 template <typename T>
-void launch_unpermute(const GridDev<T>& g, const int* org, CF3<T> src, const T* const lo[3], F3<T> dst,
-                      hipStream_t s) {
-  const int np = (int)g.np;
+void launch_unpermute(long long npl, const int* org, CF3<T> src, const T* const lo[3], F3<T> dst, hipStream_t s) {
+  const int np = (int)npl;
   CF3<T> l{{lo ? lo[0] : nullptr, lo ? lo[1] : nullptr, lo ? lo[2] : nullptr}};
   hipLaunchKernelGGL((k_unpermute<T>), dim3((np + kPBlock - 1) / kPBlock), dim3(kPBlock), 0, s, np, org, src, l,
                      dst);
 }
 
-template int launch_particle_sort<float>(const GridDev<float>&, ParticleSet<float>, ParticleSet<float>, unsigned*,
-                                         unsigned*, void*, size_t, hipStream_t);
+template int launch_particle_sort<float>(const GridDev<float>&, ParticleSet<float>, ParticleSet<float>, long long,
+                                         unsigned*, unsigned*, void*, size_t, hipStream_t);
 template int launch_particle_sort<double>(const GridDev<double>&, ParticleSet<double>, ParticleSet<double>,
-                                          unsigned*, unsigned*, void*, size_t, hipStream_t);
-template void launch_unpermute<float>(const GridDev<float>&, const int*, CF3<float>, const float* const*,
-                                      F3<float>, hipStream_t);
-template void launch_unpermute<double>(const GridDev<double>&, const int*, CF3<double>, const double* const*,
-                                       F3<double>, hipStream_t);
+                                          long long, unsigned*, unsigned*, void*, size_t, hipStream_t);
+template void launch_unpermute<float>(long long, const int*, CF3<float>, const float* const*, F3<float>,
+                                      hipStream_t);
+template void launch_unpermute<double>(long long, const int*, CF3<double>, const double* const*, F3<double>,
+                                       hipStream_t);
 
 }  // namespace lcs

@@ -29,18 +29,31 @@ def make_case(name, n=None):
 
 
 def run_both(lib, case, n=None, integrator=F.RK2, interpolator=F.LINEAR, T=1.0, h=0.25, dt=0.05,
-             nsteps=30, diags=(F.FTLE_FWD, F.FTLE_BKWD), cfl=None, every_step=None):
-    """Returns a list of per-step records {step, fm: {handle: (gpu, oracle)}, ftle ...}."""
+             nsteps=30, diags=(F.FTLE_FWD, F.FTLE_BKWD), cfl=None, every_step=None, resolution=0,
+             aux=False, stats=None):
+    """Returns a list of per-step records {step, fm: {handle: (gpu, oracle)}, ftle ...}.
+    resolution / aux: the diagnostics' grid (new_sgrid_from_sgrid) and
+    AUX_GRID, both read when the diagnostics are created; stats, when a
+    list, receives the integration counters of both sides at the end."""
     X, Y, Z, fl, vel = make_case(case, n)
     o = Oracle(X, Y, Z, fl)
     lib.finalize()
     lib.init(X, Y, Z, fl)
+    if aux:
+        o.set_option("AUX_GRID", 1)
+        lib.set_option("AUX_GRID", 1)
     handles = []
     for d in diags:
         label = "fwd" if d == F.FTLE_FWD else "bkwd"
-        ho = o.diagnostic_init(d, 0, T, h, label)
-        hg = lib.diagnostic_init(d, 0, T, h, label)
+        ho = o.diagnostic_init(d, resolution, T, h, label)
+        hg = lib.diagnostic_init(d, resolution, T, h, label)
         assert ho == hg
+        assert lib.error == 0 and o.error == 0
+        shape_g, xyz_g = lib.lcs_grid(hg)
+        shape_o, xyz_o = o.lcs_grid(ho)
+        assert shape_g == shape_o
+        for c in range(3):
+            assert np.array_equal(xyz_g[c], xyz_o[c]) or lib.dtype != np.float64
         handles.append(hg)
     for name, val in (("INTEGRATOR", integrator), ("INTERPOLATOR", interpolator)):
         o.set_option(name, val)
@@ -69,6 +82,10 @@ def run_both(lib, case, n=None, integrator=F.RK2, interpolator=F.LINEAR, T=1.0, 
                 rec["out"][hd] = (lib.composed(hd), o.composed(hd), lib.ftle(hd), o.ftle(hd))
                 prev[hd] = o.noutput(hd)
         recs.append(rec)
+    if stats is not None:
+        st = lib.stats()
+        stats.append(((st["integrations_fwd"], st["integrations_bkwd"]),
+                      (o.integrations(1), o.integrations(-1))))
     o.close()
     return recs, handles
 
