@@ -93,3 +93,16 @@ def test_aux_grid_ftle_of_linear_strain(dp):
     f = dp.ftle(hd)
     assert np.abs(f[2:-2, 2:-2] - ref).max() < 1e-10
     dp.finalize()
+
+
+@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC])
+def test_refined_aux_fp32_against_fp64_oracle(interp):
+    """The single-precision library on a refined grid with AUX_GRID (grid
+    coordinates and shifts computed in REAL(4) as the reference's
+    single-precision build does), against the FP64 oracle."""
+    import libcfd2lcs_amd as L
+    sp = L.Cfd2lcs("sp")
+    recs, hs = run_both(sp, "abc", n=10, integrator=F.RK4, interpolator=interp, T=0.4, h=0.2, dt=0.05,
+                        nsteps=10, resolution=1, aux=True)
+    assert _check(recs, hs, 2e-5, 1e-3) >= 2
+    sp.finalize()

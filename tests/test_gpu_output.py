@@ -265,3 +265,64 @@ def test_output_directory_error(tmp_path, monkeypatch):
     assert lib.error == 1
     lib.finalize()
     assert lib.error == 0
+
+
+def test_refined_aux_grid_files(tmp_path, monkeypatch):
+    """resolution = 1 with AUX_GRID: the FTLE file is written on the refined
+    grid (I/J/K and X/Y/Z of new_sgrid_from_sgrid), its velocity columns are
+    the velocity at T0 at the refined nodes (exact here: the field is uniform
+    in space), every set writes its slot file <label>-LP-FM / -LP0..-LP5-FM
+    on the refined grid with the exact uniform displacement, and u_np1 is
+    written once on the flow grid.  The grid is isotropic: AUX_GRID shifts a
+    node by a fifth of its mean face-neighbour distance, which on an
+    anisotropic grid differs at the boundary nodes and leaves the auxiliary
+    grids non-rectilinear (rejected, as the reference's non-rectilinear path
+    is not built)."""
+    import libcfd2lcs_amd as L
+    monkeypatch.chdir(tmp_path)
+    lib = L.Cfd2lcs("dp", write_output=True)
+    lib.finalize()
+    L2 = 2.0 * np.pi
+    X, Y, Z = F.cell_grid((8, 6, 5), (0, 0, 0), (L2, L2 * 6 / 8, L2 * 5 / 8))
+    fl = np.full(X.shape, F.LCS_INTERNAL, dtype=np.int32)
+    lib.init(X, Y, Z, fl)
+    lib.set_option("AUX_GRID", L.LCS_TRUE)
+    T, h, dt = 0.4, 0.2, 0.05
+    hb = lib.diagnostic_init(L.FTLE_BKWD, 1, T, h, "rb")
+    lib.set_option("INTEGRATOR", L.RK2)
+    lib.set_option("INTERPOLATOR", L.LINEAR)
+    lib.set_option("WRITE_SLOTS", L.LCS_TRUE)
+    shape, xyz = lib.lcs_grid(hb)
+    assert shape == (16, 12, 10)
+    c0 = np.array([0.3, -0.2, 0.1])
+    outs = []
+    for s in range(10):
+        t = s * dt
+        u, v, w = (np.full(X.shape, c0[d] * (1.0 + t)) for d in range(3))
+        lib.update(u, v, w, t)
+        assert lib.error == 0
+        if lib.noutput(hb) > len(outs):
+            outs.append(t)
+    assert lib.flush_output() == 0
+    assert len(outs) == 2
+    nl = int(np.prod(shape))
+    for t in outs:
+        step = _nint(math.fmod(t, T) / h)
+        disp = c0 * (h + (t * t - (t - h) * (t - h)) / 2.0)
+        for suffix in ["-LP"] + [f"-LP{a}" for a in range(6)]:
+            recs = read_unformatted(os.path.join("cfd2lcs_tmp", f"0000_rb{suffix}-FM_{step:04d}.dat"), np.float64)
+            assert [r.size for r in recs] == [nl] * 3, suffix
+            for d in range(3):
+                assert np.abs(recs[d] + disp[d]).max() <= 1e-13, (suffix, t, d)
+        recs = read_unformatted(os.path.join("cfd2lcs_tmp", f"0000_U_NP1_{step:04d}.dat"), np.float64)
+        assert [r.size for r in recs] == [X.size] * 3
+        header, vnames, data, dims = read_tecplot(os.path.join("cfd2lcs_output", f"rb_{_nint(t / h)}.dat"))
+        assert vnames == ["X", "Y", "Z", "FM-X", "FM-Y", "FM-Z", "U-X", "U-Y", "U-Z", "FTLE"]
+        assert "I = 0000000016 J = 0000000012 K = 0000000010" in header
+        for d, n in enumerate("XYZ"):
+            assert data[n] == _fields(xyz[d], np.float64)
+            assert data["U-" + n] == _fields(np.full(nl, c0[d] * (1.0 + t)), np.float64)
+        assert dims == [16, 12, 10]
+        if t == outs[-1]:
+            assert data["FTLE"] == _fields(lib.ftle(hb), np.float64)
+    lib.finalize()
