@@ -36,7 +36,7 @@ def abc_grid(n):
     return X, Y, Z
 
 
-def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None):
+def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None, resolution=0, aux=False):
     import torch
 
     import libcfd2lcs_amd as L
@@ -48,8 +48,13 @@ def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None):
     lib.init(X, Y, Z, flag)
     if lib.error:
         raise RuntimeError("cfd2lcs_init failed")
-    hf = lib.diagnostic_init(L.FTLE_FWD, 0, T, h, "fwdFTLE")
-    hb = lib.diagnostic_init(L.FTLE_BKWD, 0, T, h, "bkwdFTLE")
+    if aux:
+        lib.set_option("AUX_GRID", L.LCS_TRUE)
+    hf = lib.diagnostic_init(L.FTLE_FWD, resolution, T, h, "fwdFTLE")
+    hb = lib.diagnostic_init(L.FTLE_BKWD, resolution, T, h, "bkwdFTLE")
+    if lib.error:
+        raise RuntimeError("cfd2lcs_diagnostic_init failed")
+    lcs_shape = lib.lcs_grid(hb, coords=False)
     lib.set_option("INTEGRATOR", L.RK4)
     lib.set_option("INTERPOLATOR", L.CUBIC)
     lib.set_option("KERNEL_TIMING", L.LCS_TRUE)
@@ -95,6 +100,7 @@ def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None):
         "kernel_sec": {TIMER_NAMES[k]: float(t1_sec[k] - t0_sec[k]) for k in TIMER_NAMES},
         "kernel_cnt": {TIMER_NAMES[k]: int(t1_cnt[k] - t0_cnt[k]) for k in TIMER_NAMES},
         "np": n ** 3,
+        "lcs_grid": list(lcs_shape),
     }
     # FTLE fields: continue the same run until both diagnostics have composed
     # outputs over a full ring of T/h sub-intervals, and time those outputs
@@ -480,6 +486,8 @@ def main():
     ap.add_argument("--output-cost", type=int, default=0, metavar="N",
                     help="only measure the cost of the reference's output files on the N^3 pipeline")
     ap.add_argument("--output-dir", default=None, help="directory for --output-cost files (default: TMPDIR)")
+    ap.add_argument("--grids-n", type=int, default=128,
+                    help="flow grid of the diagnostic-grid workloads (resolution 1; AUX_GRID); 0 skips them")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -517,6 +525,16 @@ def main():
                        {"VELOCITY_STORAGE": stor, "COMPENSATED": L.LCS_TRUE})
         print(f"[bench] mixed ({args.mixed} velocity, compensated FP32 positions): "
               f"{rmix['updates'] / rmix['elapsed']:.4e} updates/s", file=sys.stderr, flush=True)
+
+    # diagnostic grids: a refined LCS grid (resolution 1: twice the nodes per
+    # axis, backward sub-cycles halved by dt_factor) and AUX_GRID (seven
+    # particle sets per direction), FP32, same scheme, one h interval
+    rgrid = {}
+    if args.grids_n > 0:
+        for key, kw in (("refined", {"resolution": 1}), ("aux_grid", {"aux": True})):
+            r = run_gpu("sp", args.grids_n, args.steps, args.warmup, dt, T, h, False, **kw)
+            rgrid[key] = r
+            print(f"[bench] {key}: {r['updates'] / r['elapsed']:.4e} updates/s", file=sys.stderr, flush=True)
 
     radj = None
     if args.adjoint_steps > 0:
@@ -610,6 +628,18 @@ def main():
             "ftle_fields_per_s": u2(rmix), "ftle_error_vs_fp64": ftle_err(rmix),
             "kernel_ms": {k: round(1e3 * s_ / max(rmix["kernel_cnt"][k], 1), 4)
                           for k, s_ in rmix["kernel_sec"].items()}},
+        "diagnostic_grids": {
+            key: {"flow_grid": [args.grids_n] * 3, "lcs_grid": r["lcs_grid"],
+                  "sets_per_direction": 7 if key == "aux_grid" else 1,
+                  "value": r["updates"] / r["elapsed"], "unit": "grid-pt updates/s (all sets)",
+                  "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+                  "kernel_ms": {k: round(1e3 * s_ / max(r["kernel_cnt"][k], 1), 4)
+                                for k, s_ in r["kernel_sec"].items()}}
+            for key, r in rgrid.items()} or None,
+        "diagnostic_grids_note": "ABC on a %d^3 flow grid, FP32, RK4 + cubic, FWD + BKWD: refined = resolution 1 "
+                                 "(LCS grid 2n per axis, semi-Lagrangian step k_sl_step2 under the k_sl_step timer); "
+                                 "aux_grid = AUX_GRID (7 particle sets per direction, FTLE from their differences)"
+                                 % args.grids_n,
         "bwd_fwd_ratio": None if radj is None else radj["bwd_fwd_ratio"],
         "adjoint": radj,
         "kernel_ms": {k: round(1e3 * s / max(r32["kernel_cnt"][k], 1), 4) for k, s in ks.items()},
