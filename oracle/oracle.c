@@ -69,6 +69,10 @@ typedef struct {
   double ps[27][3]; /* periodic_shift(i,j,k,1:3) */
   double bb[6];  /* bounding box (ghost inclusive) */
   int rect;
+  double *lsw;   /* non-rectilinear grids: face least-squares gradient weights,
+                    [(f * 3 + d) * nt + l] for face f = NBR 1..6, interior l */
+  double pmin[3][3], pmax[3][3]; /* minloc / maxloc of x, y, z over the ghosted
+                                    arrays as 1-based array positions */
 } ogrid;
 
 #define GID(g, i, j, k) \
@@ -350,6 +354,22 @@ static int grid_init(ogrid *g, const int n[3], const double *x,
     g->bb[d] = mn; g->bb[3 + d] = mx;
   }
   g->rect = check_rectilinear(g);
+  /* minloc / maxloc of each coordinate over the ghosted arrays, first
+   * occurrence in array order, as 1-based positions (comms_m.f90:1625-1632) */
+  for (int d = 0; d < 3; d++) {
+    long amin = 0, amax = 0;
+    for (long l = 1; l < g->nt; l++) {
+      if (g->g[d][l] < g->g[d][amin]) amin = l;
+      if (g->g[d][l] > g->g[d][amax]) amax = l;
+    }
+    const long a2[2] = {amin, amax};
+    for (int m = 0; m < 2; m++) {
+      const long l = a2[m];
+      double pos[3] = {(double)(l % g->N[0] + 1), (double)((l / g->N[0]) % g->N[1] + 1),
+                       (double)(l / ((long)g->N[0] * g->N[1]) + 1)};
+      for (int e = 0; e < 3; e++) (m ? g->pmax : g->pmin)[d][e] = pos[e];
+    }
+  }
   /* bcnorm: normalised least-squares gradient of min(flag,1), 26 nbrs
    * (sgrid_m.f90:383-419, gradient_m.f90:114-217, 297-335) */
   {
@@ -401,6 +421,44 @@ static int grid_init(ogrid *g, const int n[3], const double *x,
         }
     free(bc);
   }
+  /* least-squares gradient weights of the face neighbours for a
+   * non-rectilinear grid (sgrid_m.f90:421-424, compute_lsgw with
+   * full_conn = .false., gradient_m.f90:114-217): inverse-distance weights
+   * and the closed-form solution of the 3x3 normal equations */
+  if (!g->rect) {
+    g->lsw = calloc((size_t)18 * g->nt, sizeof(double));
+    for (int k = 1; k <= nk; k++)
+      for (int j = 1; j <= nj; j++)
+        for (int i = 1; i <= ni; i++) {
+          long l = GID(g, i, j, k);
+          double sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0, dx[3];
+          for (int nb = 1; nb <= 6; nb++) {
+            long m = GID(g, i + NBR[nb][0], j + NBR[nb][1], k + NBR[nb][2]);
+            for (int d = 0; d < 3; d++) dx[d] = g->g[d][m] - g->g[d][l];
+            double idw = 1.0 / (dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+            sxx = sxx + idw * (dx[0] * dx[0]);
+            syy = syy + idw * (dx[1] * dx[1]);
+            szz = szz + idw * (dx[2] * dx[2]);
+            sxy = sxy + idw * dx[0] * dx[1];
+            sxz = sxz + idw * dx[0] * dx[2];
+            syz = syz + idw * dx[1] * dx[2];
+          }
+          double sden = 2.0 * sxy * sxz * syz + sxx * syy * szz - sxx * (syz * syz) -
+                        syy * (sxz * sxz) - szz * (sxy * sxy);
+          for (int nb = 1; nb <= 6; nb++) {
+            long m = GID(g, i + NBR[nb][0], j + NBR[nb][1], k + NBR[nb][2]);
+            for (int d = 0; d < 3; d++) dx[d] = g->g[d][m] - g->g[d][l];
+            double idw = 1.0 / (dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+            double *w = g->lsw + (size_t)(nb - 1) * 3 * g->nt;
+            w[l] = idw * ((syy * szz - syz * syz) * dx[0] + (sxz * syz - sxy * szz) * dx[1] +
+                          (sxy * syz - sxz * syy) * dx[2]) / sden;
+            w[g->nt + l] = idw * ((sxz * syz - sxy * szz) * dx[0] + (sxx * szz - sxz * sxz) * dx[1] +
+                                  (sxy * sxz - syz * sxx) * dx[2]) / sden;
+            w[2 * g->nt + l] = idw * ((sxy * syz - sxz * syy) * dx[0] + (sxy * sxz - syz * sxx) * dx[1] +
+                                      (sxx * syy - sxy * sxy) * dx[2]) / sden;
+          }
+        }
+  }
   /* 1/delta over the 3x3x3 neighbourhood (sgrid_m.f90:951-1014) */
   for (int k = 1; k <= nk; k++)
     for (int j = 1; j <= nj; j++)
@@ -425,6 +483,21 @@ static int grid_init(ogrid *g, const int n[3], const double *x,
 static void grid_free(ogrid *g) {
   for (int d = 0; d < 3; d++) { free(g->g[d]); free(g->bn[d]); free(g->dl[d]); }
   free(g->flag);
+  free(g->lsw);
+}
+
+/* grad_sr1_ls (gradient_m.f90:241-295) at interior node l of a
+ * non-rectilinear grid: sum over the face neighbours, in NBR order, of
+ * (F(nbr) - F(l)) times their weights; G[d] = dF/dx_d */
+static void grad_ls(const ogrid *g, const double *F, long l, const long nbr[6], double G[3]) {
+  G[0] = 0.0; G[1] = 0.0; G[2] = 0.0;
+  for (int f = 0; f < 6; f++) {
+    const double t = F[nbr[f]] - F[l];
+    const double *w = g->lsw + (size_t)f * 3 * g->nt;
+    G[0] = G[0] + t * w[l];
+    G[1] = G[1] + t * w[g->nt + l];
+    G[2] = G[2] + t * w[2 * g->nt + l];
+  }
 }
 
 /* set_velocity_bc (sgrid_m.f90:810-866) on an exchanged field */
@@ -927,14 +1000,22 @@ static void interp_tse(const ofg *G, double *const f[3], const double q[3], cons
   const long yp = GID(g, i, j + 1, k), ym = GID(g, i, j - 1, k);
   const long zp = GID(g, i, j, k + 1), zm = GID(g, i, j, k - 1);
   const double dx = q[0] - g->g[0][l], dy = q[1] - g->g[1][l], dz = q[2] - g->g[2][l];
+  const long nbr[6] = {xm, ym, zm, xp, yp, zp};
   for (int c = 0; c < 3; c++) {
     const double *F = f[c];
-    double gx = (F[xp] - F[xm]) / (g->g[0][xp] - g->g[0][xm]);
-    double gy = (F[yp] - F[ym]) / (g->g[1][yp] - g->g[1][ym]);
-    double gz = (F[zp] - F[zm]) / (g->g[2][zp] - g->g[2][zm]);
-    if (g->n[0] == 1) gx = 0.0;
-    if (g->n[1] == 1) gy = 0.0;
-    if (g->n[2] == 1) gz = 0.0;
+    double gx, gy, gz;
+    if (g->rect) {
+      gx = (F[xp] - F[xm]) / (g->g[0][xp] - g->g[0][xm]);
+      gy = (F[yp] - F[ym]) / (g->g[1][yp] - g->g[1][ym]);
+      gz = (F[zp] - F[zm]) / (g->g[2][zp] - g->g[2][zm]);
+      if (g->n[0] == 1) gx = 0.0;
+      if (g->n[1] == 1) gy = 0.0;
+      if (g->n[2] == 1) gz = 0.0;
+    } else {
+      double G3[3];
+      grad_ls(g, F, l, nbr, G3);
+      gx = G3[0]; gy = G3[1]; gz = G3[2];
+    }
     double v = F[l] + dx * gx + dy * gy + dz * gz;
     if (limit) {
       double mn = F[l], mx = F[l];
@@ -959,6 +1040,9 @@ static int interp_field(ocfd *s, ofg *G, long np, double *const f[3], double *co
                         int *const nd[3], double *const out[3]) {
   const ogrid *g = &G->g;
   int it = s->interpolator;
+  /* non-rectilinear grids interpolate with TSE or TSE_LIMIT
+   * (lp_tracking_m.f90:319-328) */
+  if (!g->rect && it != TSE) it = TSE_LIMIT;
   if (it == QUADRATIC || it == CUBIC) {
     int e = spline_fit(G, f, it);
     if (e) return e;
@@ -1376,10 +1460,23 @@ static int relocate_and_track(ocfd *s, odiag *dg, long p) {
         }
   }
 #undef INBB
+  double stu[3];
   for (int d = 0; d < 3; d++) {
     double sv = (dg->xp[d][p] - bb[d]) / (bb[3 + d] - bb[d]);
-    sv = fmax(fmin(sv, 1.0), 0.0);
-    int v = fnint((double)(float)g->n[d] * sv);
+    stu[d] = fmax(fmin(sv, 1.0), 0.0);
+  }
+  for (int d = 0; d < 3; d++) {
+    int v;
+    if (g->rect) {
+      v = fnint((double)(float)g->n[d] * stu[d]);
+    } else {
+      /* non-rectilinear guess from the positions of the coordinate extrema
+       * (comms_m.f90:1666-1681) */
+      const double third = 1.0 / 3.0;
+      v = fnint(third * (g->pmin[0][d] + g->pmin[1][d] + g->pmin[2][d])) +
+          fnint(third * (stu[0] * (g->pmax[0][d] - g->pmin[0][d]) + stu[1] * (g->pmax[1][d] - g->pmin[1][d]) +
+                         stu[2] * (g->pmax[2][d] - g->pmin[2][d])));
+    }
     dg->no[d][p] = iclamp(v, 1, g->n[d]);
     dg->up[d][p] = 0.0;
   }
@@ -1474,14 +1571,20 @@ static void compute_ftle(ocfd *s, odiag *dg) {
           long ip = GID(g, i + 1, j, k), im = GID(g, i - 1, j, k);
           long jp = GID(g, i, j + 1, k), jm = GID(g, i, j - 1, k);
           long kp = GID(g, i, j, k + 1), km = GID(g, i, j, k - 1);
-          for (int c = 0; c < 3; c++) {
-            G[3 * c + 0] = (F[c][ip] - F[c][im]) / (X[0][ip] - X[0][im]);
-            G[3 * c + 1] = (F[c][jp] - F[c][jm]) / (X[1][jp] - X[1][jm]);
-            G[3 * c + 2] = (F[c][kp] - F[c][km]) / (X[2][kp] - X[2][km]);
+          if (!g->rect) {
+            /* grad_sr1_ls on a non-rectilinear grid (lcs_m.f90:131-133) */
+            const long nbr[6] = {im, jm, km, ip, jp, kp};
+            for (int c = 0; c < 3; c++) grad_ls(g, F[c], GID(g, i, j, k), nbr, &G[3 * c]);
+          } else {
+            for (int c = 0; c < 3; c++) {
+              G[3 * c + 0] = (F[c][ip] - F[c][im]) / (X[0][ip] - X[0][im]);
+              G[3 * c + 1] = (F[c][jp] - F[c][jm]) / (X[1][jp] - X[1][jm]);
+              G[3 * c + 2] = (F[c][kp] - F[c][km]) / (X[2][kp] - X[2][km]);
+            }
+            if (ni == 1) { G[0] = 0; G[3] = 0; G[6] = 0; }
+            if (nj == 1) { G[1] = 0; G[4] = 0; G[7] = 0; }
+            if (nk == 1) { G[2] = 0; G[5] = 0; G[8] = 0; }
           }
-          if (ni == 1) { G[0] = 0; G[3] = 0; G[6] = 0; }
-          if (nj == 1) { G[1] = 0; G[4] = 0; G[7] = 0; }
-          if (nk == 1) { G[2] = 0; G[5] = 0; G[8] = 0; }
         }
         double C[9];
         oc_cauchy_green(G, C);
@@ -1512,13 +1615,11 @@ static void write_slot(ocfd *s, odiag *dg) {
 /* ======================================================================
  * API
  * ====================================================================== */
-/* a grid bundle from interior coordinate arrays (init_sgrid, sgrid_m.f90:16-444);
- * only rectilinear grids are restated */
+/* a grid bundle from interior coordinate arrays (init_sgrid, sgrid_m.f90:16-444) */
 static int fg_init(ofg *G, const int n[3], const double *x, const double *y, const double *z,
                    const int *flag) {
   memset(G, 0, sizeof(*G));
   if (grid_init(&G->g, n, x, y, z, flag)) return 1;
-  if (!G->g.rect) return 1;
   const ogrid *g = &G->g;
   for (int d = 0; d < 3; d++) {
     G->bsc[d] = calloc(g->nt, sizeof(double));
@@ -1714,6 +1815,9 @@ int oc_diagnostic_init(ocfd *s, int lcs_type, int resolution, double T,
   /* the diagnostic's grid (cfd2lcs.f90:298-310) */
   ofg *L = &s->F;
   int own = 0;
+  /* new_sgrid_from_sgrid's trilinear refinement of a non-rectilinear grid
+   * (sgrid_m.f90:666-683) is not restated */
+  if (resolution != 0 && !s->F.g.rect) { s->error = 1; return 0; }
   if (resolution != 0) {
     int nn[3], *fl = NULL;
     double *xyz[3] = {NULL, NULL, NULL};

@@ -521,3 +521,73 @@ def test_diagnostic_grid_linear_flow_closed_form(res, aux, direction):
     inner = ftle[e:shape[0] - e, e:shape[1] - e]
     assert np.abs(inner - ref).max() < 1e-10
     o.close()
+
+
+def warped_grid(n, eps=0.05):
+    """A non-rectilinear 2-D grid on about [-1, 1]^2 (nodes displaced by
+    eps sin(pi y) in x and eps sin(pi x) in y), walls on the outer lines."""
+    X, Y, Z = F.cell_grid((n, n, 1), (-1, -1, 0), (1, 1, 1))
+    Xw = X + eps * np.sin(np.pi * Y)
+    Yw = Y + eps * np.sin(np.pi * X)
+    fl = np.full(X.shape, F.LCS_INTERNAL, dtype=np.int32)
+    fl[0], fl[-1], fl[:, 0], fl[:, -1] = F.LCS_WALL, F.LCS_WALL, F.LCS_WALL, F.LCS_WALL
+    return Xw, Yw, Z, fl
+
+
+@pytest.mark.parametrize("direction", [F.FTLE_FWD, F.FTLE_BKWD])
+@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC, F.TSE])
+def test_non_rectilinear_linear_flow_closed_form(direction, interp):
+    """On a non-rectilinear grid every interpolation is TSE or TSE_LIMIT
+    (lp_tracking_m.f90:319-328) with the face least-squares gradient
+    (gradient_m.f90:114-295), and the FTLE gradient is least-squares too
+    (lcs_m.f90:131-133).  Both are exact for a linear field whatever the
+    node positions, so for u = s (a x, b y) (information moving inward, as
+    in the rectilinear closed-form test) the composed map is (R^N - 1) x
+    and the FTLE log(max|R^N - 1|)/T at nodes whose stencils stay clear of
+    the walls' ghost values, which the repeated stencils carry inward by
+    one node per sub-cycle."""
+    a, b, T, h, dt = 0.8, 0.3, 0.125, 0.125, 1.0 / 64
+    X, Y, Z, fl = warped_grid(48)
+    o = Oracle(X, Y, Z, fl)
+    o.set_option("INTEGRATOR", F.RK2)
+    o.set_option("INTERPOLATOR", interp)
+    hd = o.diagnostic_init(direction, 0, T, h, "w")
+    s = -1.0 if direction == F.FTLE_FWD else 1.0
+    for step in range(9):
+        o.update(s * a * X, s * b * Y, 0 * X, step * dt)
+        assert o.error == 0
+    assert o.noutput(hd) == 1
+    nsub = o.nsub
+    sdt = (dt if direction == F.FTLE_FWD else -dt) / nsub
+    cx = amplification(F.RK2, s * a * sdt) ** (8 * nsub) - 1
+    cy = amplification(F.RK2, s * b * sdt) ** (8 * nsub) - 1
+    comp = o.composed(hd)
+    e = 8 * nsub + 3
+    cut = (slice(e, 48 - e), slice(e, 48 - e))
+    assert np.abs(comp[0] - cx * X)[cut].max() < 1e-12
+    assert np.abs(comp[1] - cy * Y)[cut].max() < 1e-12
+    ref = np.log(max(abs(cx), abs(cy))) / T
+    assert np.abs(o.ftle(hd)[cut] - ref).max() < 1e-9
+    o.close()
+
+
+def test_non_rectilinear_interpolation_is_tse_limit():
+    """A LINEAR run on a non-rectilinear grid gives exactly the TSE_LIMIT
+    run (the interpolator is replaced), and differs from TSE on a curved
+    field (the limiter acts)."""
+    X, Y, Z, fl = warped_grid(24)
+    out = {}
+    for interp in (F.LINEAR, F.TSE_LIMIT, F.TSE):
+        o = Oracle(X, Y, Z, fl)
+        o.set_option("INTEGRATOR", F.RK2)
+        o.set_option("INTERPOLATOR", interp)
+        hd = o.diagnostic_init(F.FTLE_BKWD, 0, 0.2, 0.1, "w")
+        for step in range(6):
+            u, v, w = F.double_gyre_velocity(X + 1.0, Y + 0.5, step * 0.05)
+            o.update(u, v, w, step * 0.05)
+            assert o.error == 0
+        out[interp] = o.flowmap(hd)
+        o.close()
+    for c in range(3):
+        assert np.array_equal(out[F.LINEAR][c], out[F.TSE_LIMIT][c])
+    assert max(np.abs(out[F.TSE][c] - out[F.TSE_LIMIT][c]).max() for c in range(2)) > 0
