@@ -399,7 +399,9 @@ struct Engine : EngineBase {
       for (int d = 0; d < 3; d++) gd.ps[c][d] = (T)h.ps[c][d];
     for (int d = 0; d < 6; d++) gd.bb[d] = (T)h.bb[d];
     const size_t nl = (size_t)h.N[0] + h.N[1] + h.N[2];
-    const size_t reals = 2 * nl + 6 * (size_t)h.np;
+    // a non-rectilinear grid adds its full coordinates and face weights
+    const size_t nr = h.rect ? 0 : 3 * (size_t)h.nt + 18 * (size_t)h.np;
+    const size_t reals = 2 * nl + 6 * (size_t)h.np + nr;
     const size_t bytes = reals * sizeof(T) + (size_t)h.nt + 256;
     if (!mem.alloc(bytes)) {
       fail("hipMalloc failed for the grid");
@@ -420,6 +422,15 @@ struct Engine : EngineBase {
       off_dl[d] = o;
       for (long long p = 0; p < h.np; p++) hr[o++] = (T)h.dl[d][p];
     }
+    size_t off_gx[3] = {0, 0, 0}, off_lsw = 0;
+    if (!h.rect) {
+      for (int d = 0; d < 3; d++) {
+        off_gx[d] = o;
+        for (long long l = 0; l < h.nt; l++) hr[o++] = (T)h.g[d][l];
+      }
+      off_lsw = o;
+      for (size_t a = 0; a < h.lsw.size(); a++) hr[o++] = (T)h.lsw[a];
+    }
     T* base = (T*)mem.p;
     if (hipMemcpy(base, hr.data(), reals * sizeof(T), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy((char*)(base + reals), h.flag.data(), (size_t)h.nt, hipMemcpyHostToDevice) != hipSuccess) {
@@ -433,6 +444,14 @@ struct Engine : EngineBase {
       gd.dl[d] = base + off_dl[d];
     }
     gd.flag = (const int8_t*)(base + reals);
+    gd.rect = h.rect ? 1 : 0;
+    for (int d = 0; d < 3; d++) gd.gx[d] = h.rect ? nullptr : base + off_gx[d];
+    gd.lsw = h.rect ? nullptr : base + off_lsw;
+    for (int d = 0; d < 3; d++)
+      for (int e = 0; e < 3; e++) {
+        gd.pmin[d][e] = (T)h.pmin[d][e];
+        gd.pmax[d][e] = (T)h.pmax[d][e];
+      }
     return true;
   }
 
@@ -649,6 +668,10 @@ struct Engine : EngineBase {
 
   int add_diag(int type, int resolution, U T_, U h, const char* label) override {
     const std::string lab = label ? label : "";
+    if (!hg.rect && (resolution != 0 || g_opt.aux_grid)) {
+      fail("resolution != 0 and AUX_GRID on a non-rectilinear grid are not supported by this build");
+      return -1;
+    }
     // the diagnostic's grid (cfd2lcs.f90:298-310, new_sgrid_from_sgrid)
     std::unique_ptr<SetGrid> sg;
     if (resolution != 0) {
@@ -815,7 +838,7 @@ struct Engine : EngineBase {
   }
 
   void update(const U* u, const U* v, const U* w, bool device, U time) override {
-    const int interp = g_opt.interpolator;
+    int interp = g_opt.interpolator;
     if (sizeof(T) == 8 && g_opt.vel_storage != kStoreNative) {
       fail("VELOCITY_STORAGE = LCS_FP16 / LCS_BF16 needs PRECISION = LCS_FP32");
       return;
@@ -829,6 +852,9 @@ struct Engine : EngineBase {
       fail("unknown INTEGRATOR");
       return;
     }
+    // a non-rectilinear grid interpolates with TSE, or TSE_LIMIT for every
+    // other choice (lp_tracking_m.f90:319-328)
+    if (!hg.rect && interp != kTSE) interp = kTSELimit;
     const bool spl = is_spline(interp);
     if (spl && !ensure_spline(interp)) return;
     // time levels (cfd2lcs.f90:104-111) and u_n <- u_np1 (112)
@@ -1319,6 +1345,11 @@ struct Engine : EngineBase {
       j.arrays.push_back(io::Array{oflag->data(), np, sizeof(U) == 8});
       j.keep.push_back(oflag);
     }
+    if (g_opt.invariants && !lg.rect) {
+      fail("VELOCITY_INVARIANTS on a non-rectilinear grid (least-squares velocity gradient) is not supported "
+           "by this build");
+      return;
+    }
     if (g_opt.invariants) {
       // compute_invariants (invariants_m.f90:15-198) of the same velocity
       hd += "\"Q\"\n\"C\"\n\"D\"\n\"H\"\n\"L2\"\n";
@@ -1517,6 +1548,10 @@ struct Engine : EngineBase {
   int adjoint_forward(int nsnap, const U* du, const U* dv, const U* dw, const U* times, int ckpt,
                       double* d_ftle) override {
     const int interp = g_opt.interpolator;
+    if (!hg.rect) {
+      fail("the adjoint pair on a non-rectilinear grid is not supported by this build");
+      return 1;
+    }
     if (nsnap < 2 || !du || !dv || !dw || !times || ckpt < 1) {
       fail("cfd2lcs_adjoint_forward_c: need at least 2 snapshots, device arrays, times and checkpoint >= 1");
       return 1;

@@ -287,6 +287,7 @@ __device__ __forceinline__ T spline_sum(const GridDev<T>& g, const SplineW<T, KX
 // (fitted) flowmap. ----
 template <typename T>
 struct IntNearest {
+  static constexpr bool kNonRect = false;
   template <typename S>
   __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
                              unsigned long long&, const Split<T>* = nullptr) const {
@@ -312,6 +313,7 @@ struct IntNearest {
 // instead of as a 9-component field.
 template <typename T, bool LIMIT>
 struct IntTSE {
+  static constexpr bool kNonRect = false;
   template <typename S>
   __device__ static void at(const CF3<S>& f, int l, T v[3]) {
     if constexpr (Packed<S>::value) {
@@ -366,8 +368,69 @@ struct IntTSE {
   }
 };
 
+// TSE / TSE_LIMIT on a non-rectilinear grid, where every interpolation is
+// one of them (lp_tracking_m.f90:319-328): the gradient at the node is
+// grad_sr1_ls (gradient_m.f90:241-295), the sum over the face neighbours in
+// NBR order of (phi(nbr) - phi(node)) times the node's least-squares
+// weights, and x - x_no uses the full node coordinates.
+template <typename T, bool LIMIT>
+struct IntTSELS {
+  static constexpr bool kNonRect = true;
+  template <typename S>
+  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
+                             unsigned long long&, const Split<T>* sq = nullptr) const {
+    const int ix[3] = {iclamp(nd[0], 1, g.n[0]), iclamp(nd[1], 1, g.n[1]), iclamp(nd[2], 1, g.n[2])};
+    const int st[3] = {1, g.N[0], g.N[0] * g.N[1]};
+    const int l = gid(g, ix[0], ix[1], ix[2]);
+    const long long p = pid(g, ix[0], ix[1], ix[2]);
+    T r[3];
+    IntTSE<T, LIMIT>::at(f, l, r);
+    T G[3][3] = {{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
+    const int off[6] = {-st[0], -st[1], -st[2], st[0], st[1], st[2]};
+#pragma unroll
+    for (int fc = 0; fc < 6; fc++) {
+      T a[3];
+      IntTSE<T, LIMIT>::at(f, l + off[fc], a);
+      const T* w = g.lsw + (long long)fc * 3 * g.np;
+      const T w3[3] = {w[p], w[g.np + p], w[2 * g.np + p]};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const T t = a[c] - r[c];
+#pragma unroll
+        for (int d = 0; d < 3; d++) G[c][d] = G[c][d] + t * w3[d];
+      }
+    }
+    T dl[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const T xn = g.gx[d][l];
+      dl[d] = sq ? (sq->qb[d] - xn) + sq->off[d] : q[d] - xn;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c] = r[c] + dl[0] * G[c][0] + dl[1] * G[c][1] + dl[2] * G[c][2];
+    if constexpr (LIMIT) {
+      T mn[3] = {r[0], r[1], r[2]}, mx[3] = {r[0], r[1], r[2]};
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+          for (int dx = -1; dx <= 1; dx++) {
+            T v[3];
+            IntTSE<T, LIMIT>::at(f, l + dx + dy * st[1] + dz * st[2], v);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              mn[c] = fmin(mn[c], v[c]);
+              mx[c] = fmax(mx[c], v[c]);
+            }
+          }
+#pragma unroll
+      for (int c = 0; c < 3; c++) out[c] = fmin(fmax(out[c], mn[c]), mx[c]);
+    }
+  }
+};
+
 template <typename T>
 struct IntLinear {
+  static constexpr bool kNonRect = false;
   template <typename S>
   __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
                              unsigned long long&, const Split<T>* sq = nullptr) const {
@@ -398,6 +461,7 @@ struct IntLinear {
 
 template <typename T, int KX, int KZ>
 struct IntSpline {
+  static constexpr bool kNonRect = false;
   template <typename S>
   __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
                              unsigned long long& oob, const Split<T>* sq = nullptr) const {
@@ -552,16 +616,56 @@ __device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
 // project_lp2plane on a flat separable grid (lp_m.f90:418-538): the plane
 // normal is the collapsed axis, the node index on it becomes 1 and the
 // particle's coordinate loses its offset from the node plane.
-template <typename T>
+template <bool NR = false, typename T>
 __device__ __forceinline__ void project_flat(const GridDev<T>& g, int nd[3], T q[3]) {
   if (g.flat < 0) return;
   const int f = g.flat;
   nd[f] = 1;
-  q[f] = q[f] - (q[f] - g.gl[f][1]);
+  if constexpr (!NR) {
+    q[f] = q[f] - (q[f] - g.gl[f][1]);
+  } else {
+    // project_lp2plane (lp_m.f90:450-536): the plane through the node with
+    // the normal v1 x v2 of the in-plane central differences
+    const int i = iclamp(nd[0], 1, g.n[0]), j = iclamp(nd[1], 1, g.n[1]), k = iclamp(nd[2], 1, g.n[2]);
+    const int a0 = f == 0 ? 1 : 0, a1 = f == 2 ? 1 : 2;
+    int u0[3] = {i, j, k}, u1[3] = {i, j, k}, w0[3] = {i, j, k}, w1[3] = {i, j, k};
+    u0[a0] -= 1; u1[a0] += 1; w0[a1] -= 1; w1[a1] += 1;
+    T v1[3], v2[3], x[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      v1[d] = g.gx[d][gid(g, u1[0], u1[1], u1[2])] - g.gx[d][gid(g, u0[0], u0[1], u0[2])];
+      v2[d] = g.gx[d][gid(g, w1[0], w1[1], w1[2])] - g.gx[d][gid(g, w0[0], w0[1], w0[2])];
+      x[d] = g.gx[d][gid(g, i, j, k)];
+    }
+    T nrm[3] = {v1[1] * v2[2] - v1[2] * v2[1], v1[2] * v2[0] - v1[0] * v2[2], v1[0] * v2[1] - v1[1] * v2[0]};
+    const T mag = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+#pragma unroll
+    for (int d = 0; d < 3; d++) nrm[d] = nrm[d] / mag;
+    T nm = (q[0] - x[0]) * nrm[0] + (q[1] - x[1]) * nrm[1] + (q[2] - x[2]) * nrm[2];
+    if (nm < T(0)) {
+      nm = -nm;
+#pragma unroll
+      for (int d = 0; d < 3; d++) nrm[d] = -nrm[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) q[d] = q[d] - nm * nrm[d];
+  }
+}
+
+// node coordinates: the coordinate lines of a rectilinear grid, the full
+// coordinates of a non-rectilinear one (NR)
+template <bool NR, typename T>
+__device__ __forceinline__ void node_pos(const GridDev<T>& g, int i, int j, int k, T x[3]) {
+  if constexpr (NR) {
+    const int l = gid(g, i, j, k);
+    x[0] = g.gx[0][l]; x[1] = g.gx[1][l]; x[2] = g.gx[2][l];
+  } else {
+    x[0] = g.gl[0][i]; x[1] = g.gl[1][j]; x[2] = g.gl[2][k];
+  }
 }
 
 // set_lp_bc (lp_tracking_m.f90:185-277); returns true for LP_STICK
-template <typename T>
+template <bool NR = false, typename T>
 __device__ __forceinline__ bool set_lp_bc(const GridDev<T>& g, int nd[3], T q[3], int& err) {
   const int i = iclamp(nd[0], 1, g.n[0]), j = iclamp(nd[1], 1, g.n[1]), k = iclamp(nd[2], 1, g.n[2]);
   nd[0] = i; nd[1] = j; nd[2] = k;
@@ -578,7 +682,9 @@ __device__ __forceinline__ bool set_lp_bc(const GridDev<T>& g, int nd[3], T q[3]
     case kMask: {
       const int p = pid(g, i, j, k);
       T nrm[3] = {g.bn[0][p], g.bn[1][p], g.bn[2][p]};
-      T p2no[3] = {q[0] - g.gl[0][i], q[1] - g.gl[1][j], q[2] - g.gl[2][k]};
+      T xn[3];
+      node_pos<NR>(g, i, j, k, xn);
+      T p2no[3] = {q[0] - xn[0], q[1] - xn[1], q[2] - xn[2]};
       T dist = p2no[0] * nrm[0] + p2no[1] * nrm[1] + p2no[2] * nrm[2];
       if (dist > T(0)) {
 #pragma unroll
@@ -589,7 +695,7 @@ __device__ __forceinline__ bool set_lp_bc(const GridDev<T>& g, int nd[3], T q[3]
     case kInflow:
     case kOutflow:
       if (inner) return false;
-      q[0] = g.gl[0][i]; q[1] = g.gl[1][j]; q[2] = g.gl[2][k];
+      node_pos<NR>(g, i, j, k, q);
       return true;
     default:
       err |= kErrBcFlag;
@@ -601,16 +707,18 @@ __device__ __forceinline__ bool set_lp_bc(const GridDev<T>& g, int nd[3], T q[3]
 // hysteresis (lp_tracking_m.f90:72-109), in the reference's neighbour order.
 // The three candidate coordinates per axis are loaded once; contraction is
 // off so the squared distances round exactly as the CPU does.
-template <typename T>
+template <bool NR = false, typename T>
 __device__ __forceinline__ bool track_pass(const GridDev<T>& g, const T q[3], int nd[3]) {
 #pragma clang fp contract(off)
   T sq[3][3];
+  if constexpr (!NR) {
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
+    for (int d = 0; d < 3; d++) {
 #pragma unroll
-    for (int o = 0; o < 3; o++) {
-      const T c = g.gl[d][iclamp(nd[d] + o - 1, 0, g.n[d] + 1)] - q[d];
-      sq[d][o] = c * c;
+      for (int o = 0; o < 3; o++) {
+        const T c = g.gl[d][iclamp(nd[d] + o - 1, 0, g.n[d] + 1)] - q[d];
+        sq[d][o] = c * c;
+      }
     }
   }
   const bool fx = g.n[0] == 1, fy = g.n[1] == 1, fz = g.n[2] == 1;
@@ -626,7 +734,16 @@ __device__ __forceinline__ bool track_pass(const GridDev<T>& g, const T q[3], in
   for (int b = 0; b < 27; b++) {
     const int di = nb[b][0], dj = nb[b][1], dk = nb[b][2];
     if ((fx && di) || (fy && dj) || (fz && dk)) continue;
-    const T rsq = (sq[0][di + 1] + sq[1][dj + 1]) + sq[2][dk + 1];
+    T rsq;
+    if constexpr (NR) {
+      // full coordinates of the neighbour (indices clamped to the ghost layer)
+      const int l = gid(g, iclamp(nd[0] + di, 0, g.n[0] + 1), iclamp(nd[1] + dj, 0, g.n[1] + 1),
+                        iclamp(nd[2] + dk, 0, g.n[2] + 1));
+      const T a = g.gx[0][l] - q[0], b = g.gx[1][l] - q[1], c = g.gx[2][l] - q[2];
+      rsq = (a * a + b * b) + c * c;
+    } else {
+      rsq = (sq[0][di + 1] + sq[1][dj + 1]) + sq[2][dk + 1];
+    }
     if (rsq < T(0.99) * rmin) {
       rmin = rsq;
       oi = di; oj = dj; ok = dk;

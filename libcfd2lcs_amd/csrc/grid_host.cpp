@@ -250,9 +250,10 @@ std::string HostGrid::build(const int nn[3], const double* x, const double* y, c
         if (spread(0, i, i, j, j, 1, kst) > TOL || spread(1, i, i, j, j, 1, kst) > TOL) oz = false;
     rect = ox && oy && oz;
   }
-  if (!rect) return "non-rectilinear grids are not supported by this build";
-  // separable coordinates: the GPU path stores one line per axis
-  separable = true;
+  // separable coordinates: the rectilinear GPU path stores one line per axis
+  // (a non-rectilinear grid keeps its full coordinates; the lines through
+  // the first row are then unused)
+  separable = rect;
   for (int d = 0; d < 3; d++) {
     line[d].resize(N[d]);
     for (int a = 0; a < N[d]; a++) line[d][a] = g[d][id(d == 0 ? a : 1, d == 1 ? a : 1, d == 2 ? a : 1)];
@@ -279,7 +280,9 @@ std::string HostGrid::build(const int nn[3], const double* x, const double* y, c
           break;
         }
       }
-  if (!separable) return "the grid coordinates are not separable (x(i), y(j), z(k)); not supported by this build";
+  if (rect && !separable)
+    return "the grid coordinates are not separable (x(i), y(j), z(k)) although rectilinear within 1e-4; "
+           "not supported by this build";
   for (int d = 0; d < 3; d++) {
     xl[d] = line[d];
     if (n[d] == 1) {
@@ -331,6 +334,60 @@ std::string HostGrid::build(const int nn[3], const double* x, const double* y, c
           const long long p = pid(i, j, k);
           for (int d = 0; d < 3; d++) bn[d][p] = mag <= 0.0 ? 0.0 : gv[d] / std::sqrt(mag);
         }
+  }
+  // non-rectilinear grids: face least-squares gradient weights
+  // (compute_lsgw with full_conn = .false., sgrid_m.f90:421-424,
+  // gradient_m.f90:114-217), [(f * 3 + d) * np + p] for the faces in NBR
+  // order, and the positions of the coordinate extrema for the node guess
+  // of a relocated particle (comms_m.f90:1625-1632)
+  if (!rect) {
+    static const int face[6][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    lsw.assign((size_t)18 * np, 0.0);
+    for (int k = 1; k <= nk; k++)
+      for (int j = 1; j <= nj; j++)
+        for (int i = 1; i <= ni; i++) {
+          const long long l = id(i, j, k), p = pid(i, j, k);
+          double sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0, dx[3];
+          for (int f = 0; f < 6; f++) {
+            const long long m = id(i + face[f][0], j + face[f][1], k + face[f][2]);
+            for (int d = 0; d < 3; d++) dx[d] = g[d][m] - g[d][l];
+            const double idw = 1.0 / (dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+            sxx = sxx + idw * (dx[0] * dx[0]);
+            syy = syy + idw * (dx[1] * dx[1]);
+            szz = szz + idw * (dx[2] * dx[2]);
+            sxy = sxy + idw * dx[0] * dx[1];
+            sxz = sxz + idw * dx[0] * dx[2];
+            syz = syz + idw * dx[1] * dx[2];
+          }
+          const double den =
+              2.0 * sxy * sxz * syz + sxx * syy * szz - sxx * (syz * syz) - syy * (sxz * sxz) - szz * (sxy * sxy);
+          for (int f = 0; f < 6; f++) {
+            const long long m = id(i + face[f][0], j + face[f][1], k + face[f][2]);
+            for (int d = 0; d < 3; d++) dx[d] = g[d][m] - g[d][l];
+            const double idw = 1.0 / (dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+            double* w = &lsw[(size_t)f * 3 * np];
+            w[p] = idw * ((syy * szz - syz * syz) * dx[0] + (sxz * syz - sxy * szz) * dx[1] +
+                          (sxy * syz - sxz * syy) * dx[2]) / den;
+            w[np + p] = idw * ((sxz * syz - sxy * szz) * dx[0] + (sxx * szz - sxz * sxz) * dx[1] +
+                               (sxy * sxz - syz * sxx) * dx[2]) / den;
+            w[2 * np + p] = idw * ((sxy * syz - sxz * syy) * dx[0] + (sxy * sxz - syz * sxx) * dx[1] +
+                                   (sxx * syy - sxy * sxy) * dx[2]) / den;
+          }
+        }
+    for (int d = 0; d < 3; d++) {
+      long long amin = 0, amax = 0;
+      for (long long l = 1; l < nt; l++) {
+        if (g[d][l] < g[d][amin]) amin = l;
+        if (g[d][l] > g[d][amax]) amax = l;
+      }
+      const long long a2[2] = {amin, amax};
+      for (int m = 0; m < 2; m++) {
+        const long long l = a2[m];
+        const double pos[3] = {(double)(l % N[0] + 1), (double)((l / N[0]) % N[1] + 1),
+                               (double)(l / ((long long)N[0] * N[1]) + 1)};
+        for (int e = 0; e < 3; e++) (m ? pmax : pmin)[d][e] = pos[e];
+      }
+    }
   }
   // 1/delta over the 3x3x3 neighbourhood
   for (int k = 1; k <= nk; k++)

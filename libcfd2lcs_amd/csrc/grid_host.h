@@ -24,6 +24,11 @@ struct HostGrid {
   int flat = -1;                 // axis with n == 1
   std::vector<double> line[3];   // ghosted coordinate lines
   std::vector<double> xl[3];     // interpolation lines (2-D override)
+  // non-rectilinear grids: face least-squares weights (18 per interior
+  // node, [(f * 3 + d) * np + p]) and the 1-based array positions of the
+  // coordinate extrema (pmin[d][e]: component e of minloc of coordinate d)
+  std::vector<double> lsw;
+  double pmin[3][3] = {}, pmax[3][3] = {};
 
   long long id(int i, int j, int k) const { return i + (long long)N[0] * (j + (long long)N[1] * k); }
   long long pid(int i, int j, int k) const {

@@ -181,8 +181,10 @@ template <typename T, typename S, class I>
 __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T>& sp, const I& interp,
                                         const CF3<T>& fmold, const F3<T>& fmnew, int i, int j, int k,
                                         DevStatus* st) {
+  constexpr bool NR = I::kNonRect;
   int nd[3] = {i, j, k};
-  const T x0[3] = {g.gl[0][i], g.gl[1][j], g.gl[2][k]};
+  T x0[3];
+  node_pos<NR>(g, i, j, k, x0);
   T xp[3] = {x0[0], x0[1], x0[2]};
   unsigned long long oob = 0;
   int err = 0;
@@ -206,8 +208,8 @@ __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T
     xr[d] = x0[d] + inc[d];
     xp[d] = xr[d];
   }
-  project_flat(g, nd, xp);
-  if (set_lp_bc(g, nd, xp, err)) {
+  project_flat<NR>(g, nd, xp);
+  if (set_lp_bc<NR>(g, nd, xp, err)) {
     xp[0] = x0[0]; xp[1] = x0[1]; xp[2] = x0[2];
   }
   int no[3] = {i, j, k};
@@ -264,6 +266,7 @@ using lcptr = const __attribute__((address_space(3))) T*;
 
 template <typename T, int KX, int KZ, int TX, int TY, int TZ>
 struct IntSplineTiled {
+  static constexpr bool kNonRect = false;
   static constexpr int TILE = TX * TY * TZ;
   const T* key[3];  // component-0 pointer of each staged field (null: unused)
   lcptr<T> tile;    // [field][component][TZ][TY][TX]
@@ -384,6 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b,
 template <typename T, typename S, class I, bool COMP>
 __global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T> sp, ParticleSet<T> ps, int np,
                                                      DevStatus* st) {
+  constexpr bool NR = I::kNonRect;
   const int p = xcd_thread_index();
   if (p >= np) return;
   T q[3] = {ps.x[0][p], ps.x[1][p], ps.x[2][p]};
@@ -409,13 +413,13 @@ __global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T
       qr[d] = xn[d] + inc[d];
       q[d] = qr[d];
     }
-    project_flat(g, nd, q);
-    if (set_lp_bc(g, nd, q, err)) {
+    project_flat<NR>(g, nd, q);
+    if (set_lp_bc<NR>(g, nd, q, err)) {
       q[0] = xn[0]; q[1] = xn[1]; q[2] = xn[2];
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) ps.d[d][p] = ps.d[d][p] + (q[d] == qr[d] ? inc[d] : q[d] - xn[d]);
-    track_pass(g, q, nd);
+    track_pass<NR>(g, q, nd);
     wrap(g, nd, q);
   } else {
     // position and displacement as unevaluated sums hi + lo, advanced with
@@ -425,8 +429,8 @@ __global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T
 #pragma unroll
     for (int d = 0; d < 3; d++) two_sum(xn[d], inc[d] + xnl[d], q[d], ql[d]);
     const T qb[3] = {q[0], q[1], q[2]};
-    project_flat(g, nd, q);
-    bool stick = set_lp_bc(g, nd, q, err);
+    project_flat<NR>(g, nd, q);
+    bool stick = set_lp_bc<NR>(g, nd, q, err);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       if (stick) {
@@ -448,7 +452,7 @@ __global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T
       ps.d[d][p] = s;
       ps.dl[d][p] = e;
     }
-    track_pass(g, q, nd);
+    track_pass<NR>(g, q, nd);
     const T qw[3] = {q[0], q[1], q[2]};
     T sh[3];
     wrap(g, nd, q, sh);
@@ -473,7 +477,10 @@ __global__ void __launch_bounds__(kBlock) k_fwd_reset(GridDev<T> g, ParticleSet<
   if (p >= g.np) return;
   int i, j, k;
   decode_interior(g, p, i, j, k);
-  ps.x[0][p] = g.gl[0][i]; ps.x[1][p] = g.gl[1][j]; ps.x[2][p] = g.gl[2][k];
+  T x[3];
+  if (g.rect) node_pos<false>(g, i, j, k, x);
+  else node_pos<true>(g, i, j, k, x);
+  ps.x[0][p] = x[0]; ps.x[1][p] = x[1]; ps.x[2][p] = x[2];
   ps.d[0][p] = T(0); ps.d[1][p] = T(0); ps.d[2][p] = T(0);
   ps.xl[0][p] = T(0); ps.xl[1][p] = T(0); ps.xl[2][p] = T(0);
   ps.dl[0][p] = T(0); ps.dl[1][p] = T(0); ps.dl[2][p] = T(0);
@@ -676,12 +683,14 @@ template <typename T, class I>
 __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __restrict__ slots,
                                                     long long stride, const int* __restrict__ order, int nslot,
                                                     F3<T> dxout, DevStatus* st) {
+  constexpr bool NR = I::kNonRect;
   const int p = xcd_thread_index();
   if (p >= g.np) return;
   int i, j, k;
   decode_interior(g, p, i, j, k);
   int nd[3] = {i, j, k};
-  T q[3] = {g.gl[0][i], g.gl[1][j], g.gl[2][k]};
+  T q[3];
+  node_pos<NR>(g, i, j, k, q);
   T dxx[3] = {T(0), T(0), T(0)};
   // the particle as base + offset: the base is its node coordinate (exact)
   // until a boundary condition or a periodic relocation moves it, and the
@@ -694,8 +703,8 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __res
     const T* base = slots + (long long)order[s] * 3 * stride;
     const CF3<T> f{{base, base + stride, base + 2 * stride}};
     const T qr[3] = {q[0], q[1], q[2]};
-    project_flat(g, nd, q);
-    set_lp_bc(g, nd, q, err);
+    project_flat<NR>(g, nd, q);
+    set_lp_bc<NR>(g, nd, q, err);
 #pragma unroll
     for (int d = 0; d < 3; d++)
       if (q[d] != qr[d]) {
@@ -735,8 +744,23 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __res
       T v = (T)(float)g.n[d] * sv;
       nd[d] = iclamp((int)round(v), 1, g.n[d]);
     }
+    if constexpr (NR) {
+      // the node guess of a non-rectilinear grid from the positions of the
+      // coordinate extrema (comms_m.f90:1666-1681)
+      T stu[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) stu[d] = clamp01((q[d] - bb[d]) / (bb[3 + d] - bb[d]));
+      const T third = T(1) / T(3);
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const T a = third * (g.pmin[0][d] + g.pmin[1][d] + g.pmin[2][d]);
+        const T b = third * (stu[0] * (g.pmax[0][d] - g.pmin[0][d]) + stu[1] * (g.pmax[1][d] - g.pmin[1][d]) +
+                             stu[2] * (g.pmax[2][d] - g.pmin[2][d]));
+        nd[d] = iclamp((int)round(a) + (int)round(b), 1, g.n[d]);
+      }
+    }
     for (int pass = 1;; pass++) {
-      bool moved = track_pass(g, q, nd);
+      bool moved = track_pass<NR>(g, q, nd);
       if (pass > g.max_track) {
         err |= kErrTrack;
         break;
@@ -794,21 +818,41 @@ __global__ void __launch_bounds__(kBlock) k_ftle(GridDev<T> g, CF3<T> fm, double
   const int jm = (j == 1 && g.per[1]) ? gid(g, i, g.n[1], k) : gid(g, i, j - 1, k);
   const int kp = (k == g.n[2] && g.per[2]) ? gid(g, i, j, 1) : gid(g, i, j, k + 1);
   const int km = (k == 1 && g.per[2]) ? gid(g, i, j, g.n[2]) : gid(g, i, j, k - 1);
-  const double hx = (double)g.gl[0][i + 1] - (double)g.gl[0][i - 1];
-  const double hy = (double)g.gl[1][j + 1] - (double)g.gl[1][j - 1];
-  const double hz = (double)g.gl[2][k + 1] - (double)g.gl[2][k - 1];
   double G[3][3];
+  if (!g.rect) {
+    // grad_sr1_ls on a non-rectilinear grid (lcs_m.f90:131-133,
+    // gradient_m.f90:241-295): face neighbours in NBR order
+    const int nb[6] = {im, jm, km, ip, jp, kp};
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const T* F = fm.c[c];
-    G[c][0] = ((double)F[ip] - (double)F[im]) / hx;
-    G[c][1] = ((double)F[jp] - (double)F[jm]) / hy;
-    G[c][2] = ((double)F[kp] - (double)F[km]) / hz;
-    comp.c[c][p] = F[l];
+    for (int c = 0; c < 3; c++) {
+      const T* F = fm.c[c];
+      G[c][0] = G[c][1] = G[c][2] = 0.0;
+#pragma unroll
+      for (int f = 0; f < 6; f++) {
+        const double t = (double)F[nb[f]] - (double)F[l];
+        const T* w = g.lsw + (long long)f * 3 * g.np;
+        G[c][0] = G[c][0] + t * (double)w[p];
+        G[c][1] = G[c][1] + t * (double)w[g.np + p];
+        G[c][2] = G[c][2] + t * (double)w[2 * g.np + p];
+      }
+      comp.c[c][p] = F[l];
+    }
+  } else {
+    const double hx = (double)g.gl[0][i + 1] - (double)g.gl[0][i - 1];
+    const double hy = (double)g.gl[1][j + 1] - (double)g.gl[1][j - 1];
+    const double hz = (double)g.gl[2][k + 1] - (double)g.gl[2][k - 1];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const T* F = fm.c[c];
+      G[c][0] = ((double)F[ip] - (double)F[im]) / hx;
+      G[c][1] = ((double)F[jp] - (double)F[jm]) / hy;
+      G[c][2] = ((double)F[kp] - (double)F[km]) / hz;
+      comp.c[c][p] = F[l];
+    }
+    if (g.n[0] == 1) { G[0][0] = 0; G[1][0] = 0; G[2][0] = 0; }
+    if (g.n[1] == 1) { G[0][1] = 0; G[1][1] = 0; G[2][1] = 0; }
+    if (g.n[2] == 1) { G[0][2] = 0; G[1][2] = 0; G[2][2] = 0; }
   }
-  if (g.n[0] == 1) { G[0][0] = 0; G[1][0] = 0; G[2][0] = 0; }
-  if (g.n[1] == 1) { G[0][1] = 0; G[1][1] = 0; G[2][1] = 0; }
-  if (g.n[2] == 1) { G[0][2] = 0; G[1][2] = 0; G[2][2] = 0; }
   ftle[p] = ftle_of_grad(G, Tint, incomp, tiny);
 }
 
@@ -1150,10 +1194,36 @@ void with_storage(int storage, F&& f) {
   f(T{});
 }
 
+// A non-rectilinear grid interpolates with TSE, or TSE_LIMIT for every other
+// choice, with least-squares gradients (lp_tracking_m.f90:319-328): f gets
+// the interpolator; false on a rectilinear grid.
+template <typename T, class F>
+bool dispatch_nonrect(const GridDev<T>& g, int interp, F&& f) {
+  if (g.rect) return false;
+  if (interp == kTSE) f(IntTSELS<T, false>{});
+  else f(IntTSELS<T, true>{});
+  return true;
+}
+
+// with_storage and dispatch_nonrect together: f(storage tag, interpolator)
+// on a non-rectilinear grid, false (nothing run) on a rectilinear one
+template <typename T, class F>
+bool with_storage_nonrect(const GridDev<T>& g, int interp, int storage, F&& f) {
+  if (g.rect) return false;
+  with_storage<T>(storage, [&](auto tag) { dispatch_nonrect(g, interp, [&](auto in) { f(tag, in); }); });
+  return true;
+}
+
 template <typename T>
 void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, F3<T> fmnew,
                     DevStatus* st, hipStream_t s, int tiled) {
   const dim3 grid = grid_for(g.np), blk(kBlock);
+  if (with_storage_nonrect(g, interp, sp.storage, [&](auto tag, auto in) {
+        using S = decltype(tag);
+        using I = decltype(in);
+        hipLaunchKernelGGL((k_sl_step<T, S, I>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+      }))
+    return;
   if constexpr (std::is_same<T, float>::value) {
     // FP32 B-spline step on a 3-D grid with native stage storage: LDS tiles
     if (tiled && (interp == kQuadratic || interp == kCubic) && sp.storage == kStoreNative && g.flat < 0 &&
@@ -1198,7 +1268,11 @@ void launch_fwd_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, 
     using S = decltype(tag);
     auto go = [&](auto C) {
       constexpr bool cm = decltype(C)::value;
-      if (interp == kLinear) {
+      if (dispatch_nonrect(g, interp, [&](auto in) {
+            using I = decltype(in);
+            hipLaunchKernelGGL((k_fwd_step<T, S, I, cm>), grid, blk, 0, s, g, sp, ps, np, st);
+          })) {
+      } else if (interp == kLinear) {
         hipLaunchKernelGGL((k_fwd_step<T, S, IntLinear<T>, cm>), grid, blk, 0, s, g, sp, ps, np, st);
       } else if (interp == kNearest) {
         hipLaunchKernelGGL((k_fwd_step<T, S, IntNearest<T>, cm>), grid, blk, 0, s, g, sp, ps, np, st);
@@ -1297,7 +1371,11 @@ template <typename T>
 void launch_compose(const GridDev<T>& g, int interp, const T* slots, long long stride, const int* order, int nslot,
                     F3<T> dxout, DevStatus* st, hipStream_t s) {
   const dim3 grid = grid_for(g.np), blk(kBlock);
-  if (interp == kLinear) {
+  if (dispatch_nonrect(g, interp, [&](auto in) {
+        using I = decltype(in);
+        hipLaunchKernelGGL((k_compose<T, I>), grid, blk, 0, s, g, slots, stride, order, nslot, dxout, st);
+      })) {
+  } else if (interp == kLinear) {
     hipLaunchKernelGGL((k_compose<T, IntLinear<T>>), grid, blk, 0, s, g, slots, stride, order, nslot, dxout, st);
   } else if (interp == kNearest) {
     hipLaunchKernelGGL((k_compose<T, IntNearest<T>>), grid, blk, 0, s, g, slots, stride, order, nslot, dxout, st);

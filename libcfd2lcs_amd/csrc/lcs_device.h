@@ -39,6 +39,13 @@ struct GridDev {
   const T* band[3];  // banded LU of the collocation matrices, (2k-1)*N[d]
   const T* bpoly[3]; // per knot interval: basis polynomials in the normalised
                      // interval coordinate and 1/length, (N[d]+1)*(k*k+1)
+  // non-rectilinear grids (rect == 0): full ghosted coordinates (nt each),
+  // face least-squares gradient weights (18 x np, [(f*3+d)*np + p]) and the
+  // 1-based array positions of the coordinate extrema (comms_m.f90:1625-1632)
+  int rect;
+  const T* gx[3];
+  const T* lsw;
+  T pmin[3][3], pmax[3][3];
 };
 
 template <typename T>

@@ -23,6 +23,18 @@ def make_case(name, n=None):
         fl[:, 0, :] = F.LCS_SLIP
         fl[:, -1, :] = F.LCS_SLIP
         vel = lambda t: F.double_gyre_velocity(X, Y, t)
+    elif name == "gyre_warp":
+        # non-rectilinear: node lines bent by a few percent of the spacing
+        ni, nj = n or (48, 24)
+        X, Y, Z, fl = F.double_gyre_grid(ni, nj)
+        X, Y = X + 0.02 * np.sin(2 * np.pi * Y), Y + 0.01 * np.sin(np.pi * X)
+        vel = lambda t: F.double_gyre_velocity(X, Y, t)
+    elif name == "abc_warp":
+        # non-rectilinear and triply periodic: the warp has the box's period
+        n = n or 12
+        X, Y, Z, fl = F.abc_grid(n)
+        X, Y, Z = X + 0.05 * np.sin(Y), Y + 0.05 * np.sin(Z), Z + 0.05 * np.sin(X)
+        vel = lambda t: F.abc_velocity(X, Y, Z)
     else:
         raise ValueError(name)
     return X, Y, Z, fl, vel
