@@ -36,7 +36,7 @@ def abc_grid(n):
     return X, Y, Z
 
 
-def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None, resolution=0, aux=False):
+def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None, resolution=0, aux=False, warp=0.0):
     import torch
 
     import libcfd2lcs_amd as L
@@ -44,6 +44,9 @@ def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None, resolution=
     lib = L.Cfd2lcs(variant)
     lib.finalize()
     X, Y, Z = abc_grid(n)
+    if warp:
+        # non-rectilinear: node lines bent with the box's period
+        X, Y, Z = X + warp * np.sin(Y), Y + warp * np.sin(Z), Z + warp * np.sin(X)
     flag = np.zeros(X.shape, dtype=np.int32)
     lib.init(X, Y, Z, flag)
     if lib.error:
@@ -66,6 +69,8 @@ def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None, resolution=
     # velocity on the device, Fortran order (i fastest) = C order of (k, j, i)
     ax = torch.tensor(X[:, 0, 0], dtype=torch.float64, device=dev)
     zz, yy, xx = torch.meshgrid(ax, ax, ax, indexing="ij")
+    if warp:
+        xx, yy, zz = xx + warp * torch.sin(yy), yy + warp * torch.sin(zz), zz + warp * torch.sin(xx)
     A, B, Cc = np.sqrt(3.0), np.sqrt(2.0), 1.0
     u = (A * torch.sin(zz) + Cc * torch.cos(yy)).to(dtype).contiguous()
     v = (B * torch.sin(xx) + A * torch.cos(zz)).to(dtype).contiguous()
@@ -488,6 +493,8 @@ def main():
     ap.add_argument("--output-dir", default=None, help="directory for --output-cost files (default: TMPDIR)")
     ap.add_argument("--grids-n", type=int, default=128,
                     help="flow grid of the diagnostic-grid workloads (resolution 1; AUX_GRID); 0 skips them")
+    ap.add_argument("--no-nonrect", dest="nonrect", action="store_false",
+                    help="skip the non-rectilinear (warped) grid workload")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -535,6 +542,13 @@ def main():
             r = run_gpu("sp", args.grids_n, args.steps, args.warmup, dt, T, h, False, **kw)
             rgrid[key] = r
             print(f"[bench] {key}: {r['updates'] / r['elapsed']:.4e} updates/s", file=sys.stderr, flush=True)
+    # a non-rectilinear grid of the headline size: every interpolation is
+    # TSE_LIMIT with least-squares gradients
+    rwarp = None
+    if args.nonrect:
+        rwarp = run_gpu("sp", args.n, args.steps, args.warmup, dt, T, h, False, warp=0.05)
+        print(f"[bench] non-rectilinear: {rwarp['updates'] / rwarp['elapsed']:.4e} updates/s", file=sys.stderr,
+              flush=True)
 
     radj = None
     if args.adjoint_steps > 0:
@@ -640,6 +654,13 @@ def main():
                                  "(LCS grid 2n per axis, semi-Lagrangian step k_sl_step2 under the k_sl_step timer); "
                                  "aux_grid = AUX_GRID (7 particle sets per direction, FTLE from their differences)"
                                  % args.grids_n,
+        "non_rectilinear": None if rwarp is None else {
+            "grid": [args.n] * 3, "warp": "x + 0.05 sin(y), y + 0.05 sin(z), z + 0.05 sin(x)",
+            "interpolation": "TSE_LIMIT, least-squares gradients (the requested CUBIC is replaced)",
+            "value": rwarp["updates"] / rwarp["elapsed"], "unit": "grid-pt updates/s",
+            "ms_per_step": 1e3 * rwarp["elapsed"] / args.steps,
+            "kernel_ms": {k: round(1e3 * s_ / max(rwarp["kernel_cnt"][k], 1), 4)
+                          for k, s_ in rwarp["kernel_sec"].items()}},
         "bwd_fwd_ratio": None if radj is None else radj["bwd_fwd_ratio"],
         "adjoint": radj,
         "kernel_ms": {k: round(1e3 * s / max(r32["kernel_cnt"][k], 1), 4) for k, s in ks.items()},
