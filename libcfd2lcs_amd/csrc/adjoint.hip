@@ -83,8 +83,6 @@ using gptr = __attribute__((address_space(1))) T*;
 template <typename T>
 using lptr = __attribute__((address_space(3))) T*;
 template <typename T>
-using vlptr = volatile __attribute__((address_space(3))) T*;
-template <typename T>
 __device__ __forceinline__ void atomic_add(T* p, T v) {
   __hip_atomic_fetch_add((gptr<T>)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -264,16 +262,23 @@ __device__ __forceinline__ int lp_bc_kind(const GridDev<T>& g, const int nd0[3],
 }
 
 // ---------------------------------------------------------------------------
-// Atomic-free transposed scatter.  One wavefront owns a brick of BRX x BRY x
-// BRZ nodes.  Lane n scatters a point's K_x K_y K_z weights into the W_x W_y
-// W_z window that starts at its own node + (B_x, B_y, B_z), with zeros
-// outside the point's support; for a given window offset the 64 lanes then
-// write 64 distinct LDS cells, so plain read-modify-writes (volatile, in
-// program order) need no atomics.  A point whose support leaves its window
-// adds to an overflow field with global atomics instead.  The brick's tile is
-// flushed to HBM with plain read-modify-writes as well: bricks are processed
-// in 2 x 2 x 2 colours whose tiles never overlap, one launch per colour.
+// Brick-tiled transposed scatter.  One wavefront owns a brick of BRX x BRY x
+// BRZ nodes and an LDS tile that covers their windows.  Lane n scatters a
+// point's K_x K_y K_z weights into the W_x W_y W_z window that starts at its
+// own node + (B_x, B_y, B_z), with zeros outside the point's support; for a
+// given window offset the 64 lanes then update 64 distinct LDS cells, so
+// plain read-modify-writes in program order need no atomics (and sum
+// deterministically; LDS float atomic adds measured four times slower here).
+// A point whose support leaves its window adds to an overflow field with
+// global atomics instead.  The finished tile reaches HBM
+// either (tile store) as a plain store to the brick's own slot, which a
+// gather pass then sums per cell, or (colours) with plain read-modify-writes
+// of the fields, the bricks processed in 2 x 2 x 2 colours whose tiles never
+// overlap, one launch per colour.
 // ---------------------------------------------------------------------------
+// cells per component of a tile stored as eight 4 x 4 x 4 octants
+constexpr int kOct = 512;
+
 struct AdjLaunch {
   int nbx, nby, nbz;  // bricks along each axis
   int cx, cy, cz;     // colour: brick parity along each axis
@@ -293,18 +298,12 @@ __device__ __forceinline__ double quad_perm(double v) {
   const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-// lane q receives from lane (q - d) & 3, resp. (q + d) & 3
+// lane q receives from lane (q - d) & 3
 template <int D, typename T>
 __device__ __forceinline__ T from_minus(T v) {
   if constexpr (D == 1) return quad_perm<0x93>(v);  // [3,0,1,2]
   else if constexpr (D == 2) return quad_perm<0x4E>(v);  // [2,3,0,1]
   else return quad_perm<0x39>(v);  // [1,2,3,0]
-}
-template <int D, typename T>
-__device__ __forceinline__ T from_plus(T v) {
-  if constexpr (D == 1) return quad_perm<0x39>(v);
-  else if constexpr (D == 2) return quad_perm<0x4E>(v);
-  else return quad_perm<0x93>(v);
 }
 
 // Transposed interpolation of one wave's brick into its LDS tile.  Every lane
@@ -323,9 +322,20 @@ struct WindowScatter {
   static constexpr int TX = BRX + A::WX - 1, TY = BRY + A::WY - 1, TZ = BRZ + A::WZ - 1;
   static constexpr int TILE = TX * TY * TZ;
   static constexpr bool kQuad = BRX == 4 && TX <= 8;
+  // tile-store layout: octants (4 x 4 x 4 bricks) or row-major, and the
+  // values stored per brick and phase
+  static constexpr bool kOctStore = BRX == 4 && BRY == 4 && BRZ == 4 && TX <= 8 && TY <= 8 && TZ <= 8;
+  static constexpr int kStore = kOctStore ? 3 * kOct : 3 * TILE;
+  // LDS layout of a component: row-major [TZ][TY][TX].  (A bank-swizzled
+  // layout with a 128-cell z pitch, conflict-free for the quad accesses,
+  // measured slower: the kernel is not bound by LDS bank conflicts.)
+  static constexpr int LTILE = TILE;
+  __device__ static __forceinline__ int lidx(int x, int y, int z) { return (z * TY + y) * TX + x; }
   lptr<T> tile;   // [3][TZ][TY][TX]
   int lx, ly, lz; // lane's node in the brick
   F3<T> ovf;
+  int* oflag;  // set to oseq when a point overflows (tile-store mode)
+  int oseq;
   int N0, N1;
 
   __device__ __forceinline__ void overflow(const int b[3], const T* wx, const T* wy, const T* wz,
@@ -341,6 +351,7 @@ struct WindowScatter {
 #pragma unroll
           for (int m = 0; m < 3; m++) atomic_add(ovf.c[m] + idx, wv[m] * w);
         }
+    if (oflag) __hip_atomic_store(oflag, oseq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 
   // NP points of one lane (use[p] false: no contribution); their
@@ -392,10 +403,35 @@ struct WindowScatter {
     if constexpr (kQuad) {
       if (!__any(any)) return;  // wave-uniform
       const int q = lx;
+      T mk[4], mh[4];
+#pragma unroll
+      for (int d = 0; d < 4; d++) {
+        mk[d] = q >= d ? T(1) : T(0);
+        mh[d] = T(1) - mk[d];
+      }
+      // window rows (oy, oz) where every lane's weight is zero add nothing:
+      // the wave skips them (in smooth flow the nodes of a brick mostly sit
+      // at the same offset in their windows, leaving 4 x 4 of the rows)
+      bool ry[A::WY], rz[A::WZ];
+#pragma unroll
+      for (int o = 0; o < A::WY; o++) {
+        bool nz = false;
+#pragma unroll
+        for (int p = 0; p < NP; p++) nz = nz || ey[p][o] != T(0);
+        ry[o] = __any(nz);
+      }
+#pragma unroll
+      for (int o = 0; o < A::WZ; o++) {
+        bool nz = false;
+#pragma unroll
+        for (int p = 0; p < NP; p++) nz = nz || ez[p][o] != T(0);
+        rz[o] = __any(nz);
+      }
 #pragma unroll
       for (int oz = 0; oz < A::WZ; oz++)
 #pragma unroll
         for (int oy = 0; oy < A::WY; oy++) {
+          if (!rz[oz] || !ry[oy]) continue;  // wave-uniform
           T lo[3], hi[3];
 #pragma unroll
           for (int m = 0; m < 3; m++) {
@@ -407,50 +443,45 @@ struct WindowScatter {
               for (int p = 0; p < NP; p++) a += ex[p][o] * (ey[p][oy] * ez[p][oz] * wvs[p][m]);
               v[o] = a;
             }
-            // cell q: lane q - d at offset d; cell q + 4: lane q + d at offset 4 - d
+            // lane q receives offset d of lane (q - d) mod 4: for q >= d
+            // that is cell q (lo), otherwise cell q + 4 (hi) of the row; the
+            // 0/1 factors mk[d] = (q >= d) select the cell without branches
             T l = v[0], h = T(0);
             if constexpr (A::WX > 4) h = v[4];
             if constexpr (A::WX > 1) {
               const T r = from_minus<1>(v[1]);
-              if (q >= 1) l += r;
+              l = fma(mk[1], r, l);
+              h = fma(mh[1], r, h);
             }
             if constexpr (A::WX > 2) {
               const T r = from_minus<2>(v[2]);
-              if (q >= 2) l += r;
+              l = fma(mk[2], r, l);
+              h = fma(mh[2], r, h);
             }
             if constexpr (A::WX > 3) {
               const T r = from_minus<3>(v[3]);
-              if (q >= 3) l += r;
-              const T u = from_plus<1>(v[3]);
-              if (q <= 2) h += u;
-            }
-            if constexpr (A::WX > 2) {
-              const T u = from_plus<2>(v[2]);
-              if (q <= 1) h += u;
-            }
-            if constexpr (A::WX > 1) {
-              const T u = from_plus<3>(v[1]);
-              if (q == 0) h += u;
+              l = fma(mk[3], r, l);
+              h = fma(mh[3], r, h);
             }
             lo[m] = l;
             hi[m] = h;
           }
-          const int rb = ((lz + oz) * TY + (ly + oy)) * TX + q;
+          const int rb = lidx(q, ly + oy, lz + oz);
           const bool has_hi = q + 4 < TX;
-          T r0 = tile[rb], r1 = tile[TILE + rb], r2 = tile[2 * TILE + rb];
+          T r0 = tile[rb], r1 = tile[LTILE + rb], r2 = tile[2 * LTILE + rb];
           T s0 = T(0), s1 = T(0), s2 = T(0);
           if (has_hi) {
             s0 = tile[rb + 4];
-            s1 = tile[TILE + rb + 4];
-            s2 = tile[2 * TILE + rb + 4];
+            s1 = tile[LTILE + rb + 4];
+            s2 = tile[2 * LTILE + rb + 4];
           }
           tile[rb] = r0 + lo[0];
-          tile[TILE + rb] = r1 + lo[1];
-          tile[2 * TILE + rb] = r2 + lo[2];
+          tile[LTILE + rb] = r1 + lo[1];
+          tile[2 * LTILE + rb] = r2 + lo[2];
           if (has_hi) {
             tile[rb + 4] = s0 + hi[0];
-            tile[TILE + rb + 4] = s1 + hi[1];
-            tile[2 * TILE + rb + 4] = s2 + hi[2];
+            tile[LTILE + rb + 4] = s1 + hi[1];
+            tile[2 * LTILE + rb + 4] = s2 + hi[2];
           }
           __asm__ volatile("" ::: "memory");
         }
@@ -467,7 +498,7 @@ struct WindowScatter {
 #pragma unroll
             for (int m = 0; m < 3; m++) wyz[p][m] = w * wvs[p][m];
           }
-          const int row = ((lz + oz) * TY + (ly + oy)) * TX + lx;
+          const int row = lidx(lx, ly + oy, lz + oz);
 #pragma unroll
           for (int ox = 0; ox < A::WX; ox++) {
             T add[3];
@@ -479,16 +510,21 @@ struct WindowScatter {
               add[m] = a;
             }
             const int c = row + ox;
-            const T r0 = tile[c], r1 = tile[TILE + c], r2 = tile[2 * TILE + c];
+            const T r0 = tile[c], r1 = tile[LTILE + c], r2 = tile[2 * LTILE + c];
             tile[c] = r0 + add[0];
-            tile[TILE + c] = r1 + add[1];
-            tile[2 * TILE + c] = r2 + add[2];
+            tile[LTILE + c] = r1 + add[1];
+            tile[2 * LTILE + c] = r2 + add[2];
             __asm__ volatile("" ::: "memory");
           }
         }
     }
   }
 };
+
+// stages of an integrator (the switch in k_sl_adjoint)
+__device__ __forceinline__ int n_stages(int integ) {
+  return (integ == kEuler || integ == kTrapezoidal) ? 1 : (integ == kRK2 ? 2 : (integ == kRK3 ? 3 : 4));
+}
 
 // One backward semi-Lagrangian sub-cycle in reverse for the bricks of one
 // colour.  lam: adjoint of the new flowmap at the interior nodes (ghosted
@@ -499,17 +535,30 @@ struct WindowScatter {
 // forward kernel read: values or spline coefficients); phase 1 + b: stage
 // velocity buffer b, which receives stage s >= 2 when sbuf[s] == b.  The
 // first stage's nodal adjoint goes to dun / dunp1, split by ct0.
-template <typename T, class A, int BRX, int BRY, int BRZ>
+// TS (tile store): every brick of the grid in one launch, each writing its
+// whole tile to its own slot tiles[phase][brick][component][TILE], which
+// k_adj_gather then adds into the fields; otherwise the bricks of one colour
+// read-modify-write their tiles into the fields directly.
+template <typename T, class A, int BRX, int BRY, int BRZ, bool TS>
 __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, StageParams<T> sp, CF3<T> fmold,
                                                    CF3<T> lam, F3<T> adj_fm, F3<T> adj_v0, F3<T> adj_v1,
                                                    F3<T> adj_v2, F3<T> ovf_fm, F3<T> ovf_v0, F3<T> ovf_v1,
-                                                   F3<T> ovf_v2, int4 sbuf, int nbuf, F3<T> dun, F3<T> dunp1) {
+                                                   F3<T> ovf_v2, int4 sbuf, int nbuf, F3<T> dun, F3<T> dunp1,
+                                                   TileStore<T> tsd) {
   using WS = WindowScatter<T, A, BRX, BRY, BRZ>;
-  __shared__ T tile_mem[3 * WS::TILE];
+  __shared__ T tile_mem[3 * WS::LTILE];
   const A ad{};
   const int b = blockIdx.x;
-  const int ibx = 2 * (b % al.gx) + al.cx, iby = 2 * ((b / al.gx) % al.gy) + al.cy,
-            ibz = 2 * (b / (al.gx * al.gy)) + al.cz;
+  int ibx, iby, ibz;
+  if constexpr (TS) {
+    ibx = b % al.nbx;
+    iby = (b / al.nbx) % al.nby;
+    ibz = b / (al.nbx * al.nby);
+  } else {
+    ibx = 2 * (b % al.gx) + al.cx;
+    iby = 2 * ((b / al.gx) % al.gy) + al.cy;
+    ibz = 2 * (b / (al.gx * al.gy)) + al.cz;
+  }
   const int t = threadIdx.x;
   WS ws_;
   ws_.tile = (lptr<T>)tile_mem;
@@ -518,6 +567,8 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
   ws_.lz = t / (BRX * BRY);
   ws_.N0 = g.N[0];
   ws_.N1 = g.N[1];
+  ws_.oflag = TS ? tsd.flag : nullptr;
+  ws_.oseq = tsd.seq;
   const int i = ibx * BRX + ws_.lx + 1, j = iby * BRY + ws_.ly + 1, k = ibz * BRZ + ws_.lz + 1;
   // tile origin (ghosted index of window cell 0 of the brick's first node)
   const int ox = ibx * BRX + 1 + A::BX, oy = iby * BRY + 1 + A::BY, oz = ibz * BRZ + 1 + A::BZ;
@@ -663,7 +714,7 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
   }
   // ---- scatter phases through the wave's tile ----
   for (int ph = 0; ph <= nbuf; ph++) {
-    for (int e = t; e < 3 * WS::TILE; e += 64) ws_.tile[e] = T(0);
+    for (int e = t; e < 3 * WS::LTILE; e += 64) ws_.tile[e] = T(0);
     __syncthreads();
     const F3<T> dst = ph == 0 ? adj_fm : (ph == 1 ? adj_v0 : (ph == 2 ? adj_v1 : adj_v2));
     ws_.ovf = ph == 0 ? ovf_fm : (ph == 1 ? ovf_v0 : (ph == 2 ? ovf_v1 : ovf_v2));
@@ -672,13 +723,56 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
       const bool use0[1] = {active};
       ws_.template multi<1>(ad, g, &xb, use0, nd, &gl3);
     } else {
-      // stages 1..3 (indices) that feed this buffer, in one window pass
+      // the stages (1..3) that feed this buffer, in one window pass; which
+      // ones they are is the same for every lane of the launch
+      const int nsu = n_stages(sp.integ);
+      int sel[3] = {1, 1, 1}, np = 0;
+#pragma unroll
+      for (int st = 1; st < 4; st++) {
+        const int sb_ = st == 1 ? sbuf.y : (st == 2 ? sbuf.z : sbuf.w);
+        if (st < nsu && sb_ == ph - 1) {
+          if (np == 0) sel[0] = st;
+          else if (np == 1) sel[1] = st;
+          else sel[2] = st;
+          np++;
+        }
+      }
+      T q[3][3], w[3][3];
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          q[p][d] = sel[p] == 1 ? xs[1][d] : (sel[p] == 2 ? xs[2][d] : xs[3][d]);
+          w[p][d] = sel[p] == 1 ? ws[1][d] : (sel[p] == 2 ? ws[2][d] : ws[3][d]);
+        }
       const bool on = active && !stuck;
-      const bool use[3] = {on && 1 < ns && sbuf.y == ph - 1, on && 2 < ns && sbuf.z == ph - 1,
-                           on && 3 < ns && sbuf.w == ph - 1};
-      ws_.template multi<3>(ad, g, xs + 1, use, nd, ws + 1);
+      const bool use[3] = {on, on, on};
+      if (np == 1) ws_.template multi<1>(ad, g, q, use, nd, w);
+      else if (np == 2) ws_.template multi<2>(ad, g, q, use, nd, w);
+      else if (np == 3) ws_.template multi<3>(ad, g, q, use, nd, w);
     }
     __syncthreads();
+    if constexpr (TS) {
+      // the whole tile, zeros included, to this brick's slot (plain stores)
+      T* out = tsd.tiles + ((size_t)ph * al.nbx * al.nby * al.nbz + (size_t)b) * WS::kStore;
+      if constexpr (WS::kOctStore) {
+        // store e: component e / 512, octant (e / 64) % 8, lane position e % 64
+        for (int e = t; e < 3 * kOct; e += 64) {
+          const int m = e >> 9, o = (e >> 6) & 7;
+          const int rx = (o & 1) * 4 + (t & 3), ry = ((o >> 1) & 1) * 4 + ((t >> 2) & 3), rz = (o >> 2) * 4 + (t >> 4);
+          if (rx < WS::TX && ry < WS::TY && rz < WS::TZ)
+            __builtin_nontemporal_store(ws_.tile[m * WS::LTILE + WS::lidx(rx, ry, rz)], out + e);
+        }
+      } else {
+        for (int e = t; e < 3 * WS::TILE; e += 64) {
+          const int m = e / WS::TILE, r = e - m * WS::TILE;
+          const int rx = r % WS::TX, ry = (r / WS::TX) % WS::TY, rz = r / (WS::TX * WS::TY);
+          __builtin_nontemporal_store(ws_.tile[m * WS::LTILE + WS::lidx(rx, ry, rz)], out + e);
+        }
+      }
+      __syncthreads();
+      continue;
+    }
     // flush: batches of 8 independent read-modify-writes per lane, so that
     // the global loads of a batch are in flight together
     constexpr int kFlushBatch = 8;
@@ -691,9 +785,10 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
         d[q] = nullptr;
         v[q] = T(0);
         if (e < 3 * WS::TILE) {
-          v[q] = ws_.tile[e];
           const int m = e / WS::TILE, r = e - m * WS::TILE;
-          const int cx = ox + r % WS::TX, cy = oy + (r / WS::TX) % WS::TY, cz = oz + r / (WS::TX * WS::TY);
+          const int rx = r % WS::TX, ry = (r / WS::TX) % WS::TY, rz = r / (WS::TX * WS::TY);
+          v[q] = ws_.tile[m * WS::LTILE + WS::lidx(rx, ry, rz)];
+          const int cx = ox + rx, cy = oy + ry, cz = oz + rz;
           if (v[q] != T(0) && cx >= 0 && cy >= 0 && cz >= 0 && cx < g.N[0] && cy < g.N[1] && cz < g.N[2])
             d[q] = dst.c[m] + cx + g.N[0] * (cy + g.N[1] * cz);
         }
@@ -706,6 +801,148 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
     }
     __syncthreads();
   }
+}
+
+// Destinations of the tile-store gather.  Phase 0 is the adjoint of the old
+// flowmap (assigned); phase 1 + b is stage velocity buffer b, the blend
+// (1 - c[b]) u_n + c[b] u_np1, whose adjoint goes straight to dn / dnp1 (the
+// transpose of the blend).  The overflow fields are added and cleared when
+// the scatter flagged an overflow (*flag == seq); otherwise they are zero.
+template <typename T>
+struct GatherOut {
+  F3<T> fm, dn, dnp1;
+  F3<T> ovf[4];
+  T c[3];
+  const int* flag;
+  int seq;
+};
+
+template <typename T>
+__device__ __forceinline__ void gather_store(const GatherOut<T>& o, long long l, int nph, T (&sum)[4][3]) {
+  const bool ovf_on = *o.flag == o.seq;
+#pragma unroll
+  for (int ph = 0; ph < 4; ph++)
+    if (ovf_on && ph < nph) {
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        const T ov = o.ovf[ph].c[m][l];
+        if (ov != T(0)) {
+          sum[ph][m] += ov;
+          o.ovf[ph].c[m][l] = T(0);
+        }
+      }
+    }
+#pragma unroll
+  for (int m = 0; m < 3; m++) o.fm.c[m][l] = sum[0][m];
+  if (nph > 1) {
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+      T an = T(0), anp1 = T(0);
+#pragma unroll
+      for (int b = 0; b < 3; b++)
+        if (b + 1 < nph) {
+          an += (T(1) - o.c[b]) * sum[b + 1][m];
+          anp1 += o.c[b] * sum[b + 1][m];
+        }
+      if (an != T(0)) o.dn.c[m][l] += an;
+      if (anp1 != T(0)) o.dnp1.c[m][l] += anp1;
+    }
+  }
+}
+
+// Tile-store gather, any brick shape: thread per ghosted cell; it adds the
+// tiles of the bricks whose tile covers the cell (brick b's tile starts at
+// b BR + 1 + B along each axis and spans TL cells, so at most ceil(TL / BR)
+// bricks per axis).  Tiles are stored row-major [3][TZ][TY][TX].
+__device__ __forceinline__ int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+template <typename T, int BRX, int BRY, int BRZ, int TX, int TY, int TZ, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(kBlock) k_adj_gather(GridDev<T> g, AdjLaunch al, const T* __restrict__ tiles,
+                                                       int nph, GatherOut<T> out) {
+  constexpr int TILE = TX * TY * TZ;
+  constexpr int MX = (TX + BRX - 1) / BRX, MY = (TY + BRY - 1) / BRY, MZ = (TZ + BRZ - 1) / BRZ;
+  const long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= g.nt) return;
+  const int cx = (int)(l % g.N[0]), cy = (int)((l / g.N[0]) % g.N[1]), cz = (int)(l / ((long long)g.N[0] * g.N[1]));
+  // b BR + 1 + B <= c < b BR + 1 + B + TL
+  const int bx0 = floordiv(cx - BX - TX + BRX - 1, BRX), bx1 = min(floordiv(cx - 1 - BX, BRX), al.nbx - 1);
+  const int by0 = floordiv(cy - BY - TY + BRY - 1, BRY), by1 = min(floordiv(cy - 1 - BY, BRY), al.nby - 1);
+  const int bz0 = floordiv(cz - BZ - TZ + BRZ - 1, BRZ), bz1 = min(floordiv(cz - 1 - BZ, BRZ), al.nbz - 1);
+  const size_t nbricks = (size_t)al.nbx * al.nby * al.nbz;
+  T sum[4][3] = {};
+#pragma unroll
+  for (int ph = 0; ph < 4; ph++)
+    if (ph < nph) {
+      const T* tph = tiles + (size_t)ph * nbricks * 3 * TILE;
+#pragma unroll
+      for (int az = 0; az < MZ; az++)
+#pragma unroll
+        for (int ay = 0; ay < MY; ay++)
+#pragma unroll
+          for (int ax = 0; ax < MX; ax++) {
+            const int bx = bx0 + ax, by = by0 + ay, bz = bz0 + az;
+            if (bx >= 0 && by >= 0 && bz >= 0 && bx <= bx1 && by <= by1 && bz <= bz1) {
+              const int rx = cx - (bx * BRX + 1 + BX), ry = cy - (by * BRY + 1 + BY), rz = cz - (bz * BRZ + 1 + BZ);
+              const T* tl =
+                  tph + (bx + (size_t)al.nbx * (by + (size_t)al.nby * bz)) * 3 * TILE + (rz * TY + ry) * TX + rx;
+#pragma unroll
+              for (int m = 0; m < 3; m++) sum[ph][m] += tl[m * TILE];
+            }
+          }
+    }
+  gather_store(out, l, nph, sum);
+}
+
+// Tile-store gather for 4 x 4 x 4 bricks whose tiles fit 8 x 8 x 8 cells,
+// stored as eight 4 x 4 x 4 octants per component ([3][oz][oy][ox][64]).
+// Wave w handles the 4 x 4 x 4 block of cells C, c = 4 C + 1 + B + lane
+// offset along each axis: brick C covers it with octant 0 and brick C - 1
+// with octant 1 along that axis, at the lane's own position in the octant,
+// so every wave reads each (brick, component) as one contiguous 256-byte run.
+template <typename T, int TX, int TY, int TZ, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(256) k_adj_gather_oct(GridDev<T> g, AdjLaunch al, const T* __restrict__ tiles,
+                                                        int nph, GatherOut<T> out) {
+  static_assert(TX <= 8 && TY <= 8 && TZ <= 8, "octant tiles span at most 8 cells per axis");
+  const int lane = threadIdx.x & 63;
+  const long long cb = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int C0x = floordiv(-1 - BX, 4), C0y = floordiv(-1 - BY, 4), C0z = floordiv(-1 - BZ, 4);
+  const int ncx = floordiv(g.N[0] - 2 - BX, 4) - C0x + 1, ncy = floordiv(g.N[1] - 2 - BY, 4) - C0y + 1,
+            ncz = floordiv(g.N[2] - 2 - BZ, 4) - C0z + 1;
+  if (cb >= (long long)ncx * ncy * ncz) return;
+  const int Cx = C0x + (int)(cb % ncx), Cy = C0y + (int)((cb / ncx) % ncy), Cz = C0z + (int)(cb / ((long long)ncx * ncy));
+  const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+  const int cx = 4 * Cx + 1 + BX + lx, cy = 4 * Cy + 1 + BY + ly, cz = 4 * Cz + 1 + BZ + lz;
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= g.N[0] || cy >= g.N[1] || cz >= g.N[2]) return;
+  const long long l = cx + (long long)g.N[0] * (cy + (long long)g.N[1] * cz);
+  const size_t nbricks = (size_t)al.nbx * al.nby * al.nbz;
+  // the eight (brick, octant) slots: every load is issued (a slot outside
+  // the grid of bricks reads brick 0, a padded cell reads its unused slot)
+  // and the ones that do not cover the cell are dropped by a select
+  size_t off[8];
+  bool ok[8];
+#pragma unroll
+  for (int o = 0; o < 8; o++) {
+    const int sx = o & 1, sy = (o >> 1) & 1, sz = o >> 2;
+    const int bx = Cx - sx, by = Cy - sy, bz = Cz - sz;
+    ok[o] = bx >= 0 && by >= 0 && bz >= 0 && bx < al.nbx && by < al.nby && bz < al.nbz && lx + 4 * sx < TX &&
+            ly + 4 * sy < TY && lz + 4 * sz < TZ;
+    off[o] = (ok[o] ? (bx + (size_t)al.nbx * (by + (size_t)al.nby * bz)) * 3 * kOct : 0) + o * 64 + lane;
+  }
+  T sum[4][3] = {};
+#pragma unroll
+  for (int ph = 0; ph < 4; ph++)
+    if (ph < nph) {
+      const T* tph = tiles + (size_t)ph * nbricks * 3 * kOct;
+      T v[8][3];
+#pragma unroll
+      for (int o = 0; o < 8; o++)
+#pragma unroll
+        for (int m = 0; m < 3; m++) v[o][m] = __builtin_nontemporal_load(tph + off[o] + m * kOct);
+#pragma unroll
+      for (int o = 0; o < 8; o++)
+#pragma unroll
+        for (int m = 0; m < 3; m++) sum[ph][m] += ok[o] ? v[o][m] : T(0);
+    }
+  gather_store(out, l, nph, sum);
 }
 
 // dst += ovf, ovf = 0
@@ -909,49 +1146,109 @@ __global__ void __launch_bounds__(kBlock) k_interior(GridDev<T> g, CF3<T> src, F
 static inline dim3 agrid(long long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
 template <typename T, class A, int BRX, int BRY, int BRZ>
-static void sl_adjoint_colours(const GridDev<T>& g, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
+static size_t tile_store_bytes(const GridDev<T>& g, int nbuf) {
+  using WS = WindowScatter<T, A, BRX, BRY, BRZ>;
+  const size_t nbricks = (size_t)((g.n[0] + BRX - 1) / BRX) * ((g.n[1] + BRY - 1) / BRY) * ((g.n[2] + BRZ - 1) / BRZ);
+  return (size_t)(nbuf + 1) * nbricks * WS::kStore * sizeof(T);
+}
+
+template <typename T, class A, int BRX, int BRY, int BRZ>
+static bool sl_adjoint_colours(const GridDev<T>& g, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
                                F3<T> adj_fm, F3<T> const adj_v[3], F3<T> const ovf[4], const int sbuf[4], int nbuf,
-                               F3<T> dun, F3<T> dunp1, hipStream_t s) {
+                               F3<T> dun, F3<T> dunp1, const TileStore<T>* ts, const T bct[3], F3<T> dcn,
+                               F3<T> dcnp1, hipStream_t s) {
   AdjLaunch al;
   al.nbx = (g.n[0] + BRX - 1) / BRX;
   al.nby = (g.n[1] + BRY - 1) / BRY;
   al.nbz = (g.n[2] + BRZ - 1) / BRZ;
   const int4 sb = make_int4(sbuf[0], sbuf[1], sbuf[2], sbuf[3]);
+  if (ts) {
+    // one launch over every brick, then one gather pass (which also merges
+    // the overflow fields)
+    using WS = WindowScatter<T, A, BRX, BRY, BRZ>;
+    al.cx = al.cy = al.cz = 0;
+    al.gx = al.nbx;
+    al.gy = al.nby;
+    hipLaunchKernelGGL((k_sl_adjoint<T, A, BRX, BRY, BRZ, true>), dim3((unsigned)(al.nbx * al.nby * al.nbz)),
+                       dim3(64), 0, s, g, al, sp, fmold, lam, adj_fm, adj_v[0], adj_v[1], adj_v[2], ovf[0], ovf[1],
+                       ovf[2], ovf[3], sb, nbuf, dun, dunp1, *ts);
+    GatherOut<T> go;
+    go.fm = adj_fm;
+    go.dn = dcn;
+    go.dnp1 = dcnp1;
+    for (int p = 0; p < 4; p++) go.ovf[p] = ovf[p];
+    for (int b = 0; b < 3; b++) go.c[b] = bct[b];
+    go.flag = ts->flag;
+    go.seq = ts->seq;
+    if constexpr (WS::kOctStore) {
+      auto cblocks = [](int N, int B) {
+        auto fd = [](int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); };
+        return (long long)(fd(N - 2 - B, 4) - fd(-1 - B, 4) + 1);
+      };
+      const long long ncb = cblocks(g.N[0], A::BX) * cblocks(g.N[1], A::BY) * cblocks(g.N[2], A::BZ);
+      hipLaunchKernelGGL((k_adj_gather_oct<T, WS::TX, WS::TY, WS::TZ, A::BX, A::BY, A::BZ>),
+                         dim3((unsigned)((ncb + 3) / 4)), dim3(256), 0, s, g, al, (const T*)ts->tiles, nbuf + 1,
+                         go);
+    } else {
+      hipLaunchKernelGGL((k_adj_gather<T, BRX, BRY, BRZ, WS::TX, WS::TY, WS::TZ, A::BX, A::BY, A::BZ>), agrid(g.nt),
+                         dim3(kBlock), 0, s, g, al, (const T*)ts->tiles, nbuf + 1, go);
+    }
+    return true;
+  }
   for (int c = 0; c < 8; c++) {
     al.cx = c & 1; al.cy = (c >> 1) & 1; al.cz = c >> 2;
     al.gx = (al.nbx - al.cx + 1) / 2;
     al.gy = (al.nby - al.cy + 1) / 2;
     const int gz = (al.nbz - al.cz + 1) / 2;
     if (al.gx <= 0 || al.gy <= 0 || gz <= 0) continue;
-    hipLaunchKernelGGL((k_sl_adjoint<T, A, BRX, BRY, BRZ>), dim3((unsigned)(al.gx * al.gy * gz)), dim3(64), 0, s, g,
-                       al, sp, fmold, lam, adj_fm, adj_v[0], adj_v[1], adj_v[2], ovf[0], ovf[1], ovf[2], ovf[3], sb,
-                       nbuf, dun, dunp1);
+    hipLaunchKernelGGL((k_sl_adjoint<T, A, BRX, BRY, BRZ, false>), dim3((unsigned)(al.gx * al.gy * gz)), dim3(64), 0,
+                       s, g, al, sp, fmold, lam, adj_fm, adj_v[0], adj_v[1], adj_v[2], ovf[0], ovf[1], ovf[2], ovf[3],
+                       sb, nbuf, dun, dunp1, TileStore<T>{nullptr, nullptr, 0});
   }
   const long long nb = std::min<long long>((g.nt + kBlock - 1) / kBlock, 4096);
   F3<T> dsts[4] = {adj_fm, adj_v[0], adj_v[1], adj_v[2]};
   for (int ph = 0; ph <= nbuf; ph++)
     hipLaunchKernelGGL((k_merge_overflow<T>), dim3((unsigned)nb), dim3(kBlock), 0, s, dsts[ph], ovf[ph], g.nt);
+  return false;
 }
 
 template <typename T>
-void launch_sl_adjoint(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
+bool launch_sl_adjoint(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
                        F3<T> adj_fm, F3<T> const adj_v[3], F3<T> const ovf[4], const int sbuf[4], int nbuf,
-                       F3<T> dun, F3<T> dunp1, hipStream_t s) {
+                       F3<T> dun, F3<T> dunp1, const TileStore<T>* ts, const T bct[3], F3<T> dcn, F3<T> dcnp1,
+                       hipStream_t s) {
   const bool flat = g.n[2] == 1;
-  auto go = [&](auto tag) {
+  auto go = [&](auto tag) -> bool {
     using A = decltype(tag);
     if (flat)
-      sl_adjoint_colours<T, A, 8, 8, 1>(g, sp, fmold, lam, adj_fm, adj_v, ovf, sbuf, nbuf, dun, dunp1, s);
-    else
-      sl_adjoint_colours<T, A, 4, 4, 4>(g, sp, fmold, lam, adj_fm, adj_v, ovf, sbuf, nbuf, dun, dunp1, s);
+      return sl_adjoint_colours<T, A, 8, 8, 1>(g, sp, fmold, lam, adj_fm, adj_v, ovf, sbuf, nbuf, dun, dunp1, ts,
+                                               bct, dcn, dcnp1, s);
+    return sl_adjoint_colours<T, A, 4, 4, 4>(g, sp, fmold, lam, adj_fm, adj_v, ovf, sbuf, nbuf, dun, dunp1, ts,
+                                             bct, dcn, dcnp1, s);
   };
   if (interp == kNearest) return go(AdjNearest<T>{});
   if (interp == kLinear) return go(AdjLinear<T>{});
   const int kx = g.kord[0], kz = g.kord[2];
-  if (kx == 4 && kz == 4) go(AdjSpline<T, 4, 4>{});
-  else if (kx == 4 && kz == 2) go(AdjSpline<T, 4, 2>{});
-  else if (kx == 3 && kz == 3) go(AdjSpline<T, 3, 3>{});
-  else go(AdjSpline<T, 3, 2>{});
+  if (kx == 4 && kz == 4) return go(AdjSpline<T, 4, 4>{});
+  if (kx == 4 && kz == 2) return go(AdjSpline<T, 4, 2>{});
+  if (kx == 3 && kz == 3) return go(AdjSpline<T, 3, 3>{});
+  return go(AdjSpline<T, 3, 2>{});
+}
+
+template <typename T>
+size_t sl_adjoint_tile_bytes(const GridDev<T>& g, int interp, int nbuf) {
+  const bool flat = g.n[2] == 1;
+  auto go = [&](auto tag) -> size_t {
+    using A = decltype(tag);
+    return flat ? tile_store_bytes<T, A, 8, 8, 1>(g, nbuf) : tile_store_bytes<T, A, 4, 4, 4>(g, nbuf);
+  };
+  if (interp == kNearest) return go(AdjNearest<T>{});
+  if (interp == kLinear) return go(AdjLinear<T>{});
+  const int kx = g.kord[0], kz = g.kord[2];
+  if (kx == 4 && kz == 4) return go(AdjSpline<T, 4, 4>{});
+  if (kx == 4 && kz == 2) return go(AdjSpline<T, 4, 2>{});
+  if (kx == 3 && kz == 3) return go(AdjSpline<T, 3, 3>{});
+  return go(AdjSpline<T, 3, 2>{});
 }
 
 template <typename T>
@@ -984,8 +1281,11 @@ void launch_interior(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStream_t s) 
 }
 
 #define INST_ADJ(T, U)                                                                                          \
-  template void launch_sl_adjoint<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, CF3<T>, F3<T>,      \
-                                     F3<T> const*, F3<T> const*, const int*, int, F3<T>, F3<T>, hipStream_t);   \
+  template bool launch_sl_adjoint<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, CF3<T>, F3<T>,      \
+                                     F3<T> const*, F3<T> const*, const int*, int, F3<T>, F3<T>,                 \
+                                     const TileStore<T>*, const T*,                                             \
+                                     F3<T>, F3<T>, hipStream_t);                                                \
+  template size_t sl_adjoint_tile_bytes<T>(const GridDev<T>&, int, int);                                        \
   template void launch_adj_unblend<T>(const GridDev<T>&, F3<T>, T, F3<T>, F3<T>, hipStream_t);                  \
   template void launch_fold_ghost<T>(const GridDev<T>&, F3<T>, int, hipStream_t);                               \
   template void launch_ftle_adjoint<T>(const GridDev<T>&, CF3<T>, double, int, double, const double*, F3<T>,    \

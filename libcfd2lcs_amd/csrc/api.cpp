@@ -39,6 +39,9 @@ using U = lcsdata_t;  // user-facing real type
 
 namespace lcs {
 
+// largest tile-store scratch of the adjoint scatter (bytes)
+constexpr size_t kTileStoreMax = size_t(16) << 30;
+
 // ---------------------------------------------------------------------------
 // options (data_m.f90:15-94) and sticky error
 // ---------------------------------------------------------------------------
@@ -1431,6 +1434,8 @@ struct Engine : EngineBase {
     Blend blends[6];
     // state
     DevBuf fm[2], fmcoef, compfm, dxout, ftle, lam, adjfm, adjv[3], dcn, dcnp1, dun, dunp1, ovf[4];
+    DevBuf tiles, oflag;         // tile-store scratch of the reverse sub-cycle scatter
+    int oseq = 0;                // per-call value of its overflow flag
     int fcur = 0;
     std::vector<DevBuf> ckpts;   // fm (ghosted values) after steps 0, ckpt, 2 ckpt, ...
     std::vector<DevBuf> segf;    // per sub-cycle of a segment: the field k_sl_step read
@@ -1726,13 +1731,34 @@ struct Engine : EngineBase {
             }
             sbuf[a] = f;
           }
-          HIPCHK_R(hipMemsetAsync(tape.adjfm.p, 0, fb, stream), 1);
           const DevBuf& fread = tape.segf[(size_t)(s - s0) * maxsub + k];
           F3<T> av[3] = {f3(tape.adjv[0]), f3(tape.adjv[1]), f3(tape.adjv[2])};
           F3<T> ov[4] = {f3(tape.ovf[0]), f3(tape.ovf[1]), f3(tape.ovf[2]), f3(tape.ovf[3])};
-          launch_sl_adjoint(g, tape.interp, sp, cf3(const_cast<DevBuf&>(fread)), cf3(tape.lam), f3(tape.adjfm), av,
-                            ov, sbuf, nb, f3(tape.dun), f3(tape.dunp1), stream);
-          for (int b = 0; b < nb; b++)
+          // the scatter stores each brick's tile and gathers them in one
+          // pass (8 tiles per node and phase, at most 3 phases) that also
+          // unblends the stage buffers; above kTileStoreMax bytes it adds the
+          // tiles in place, colour by colour, and the buffers are unblended
+          // below
+          TileStore<T> tstore{nullptr, nullptr, 0};
+          const size_t tb = sl_adjoint_tile_bytes(g, tape.interp, nb);
+          if (tb <= kTileStoreMax) {
+            if (!tape.oflag.p) {
+              // -1: a value no call number takes
+              if (!ensure(tape.oflag, sizeof(int))) return 1;
+              HIPCHK_R(hipMemsetAsync(tape.oflag.p, 0xFF, sizeof(int), stream), 1);
+            }
+            if (!ensure(tape.tiles, tb)) return 1;
+            tstore = TileStore<T>{(T*)tape.tiles.p, (int*)tape.oflag.p, tape.oseq};
+            tape.oseq = (tape.oseq + 1) & 0x7FFFFFFF;
+          } else {
+            HIPCHK_R(hipMemsetAsync(tape.adjfm.p, 0, fb, stream), 1);
+          }
+          const T bc[3] = {(T)bct[0], (T)bct[1], (T)bct[2]};
+          const bool unblended =
+              launch_sl_adjoint(g, tape.interp, sp, cf3(const_cast<DevBuf&>(fread)), cf3(tape.lam), f3(tape.adjfm),
+                                av, ov, sbuf, nb, f3(tape.dun), f3(tape.dunp1), tstore.tiles ? &tstore : nullptr,
+                                bc, f3(tape.dcn), f3(tape.dcnp1), stream);
+          for (int b = 0; b < nb && !unblended; b++)
             launch_adj_unblend(g, f3(tape.adjv[b]), (T)bct[b], f3(tape.dcn), f3(tape.dcnp1), stream);
           if (tape.spl) launch_spline_fit_T(g, f3(tape.adjfm), stream);
           launch_fold_ghost(g, f3(tape.adjfm), 1, stream);

@@ -208,13 +208,32 @@ void launch_interior_to_user(const GridDev<T>& g, CF3<T> src, int ghosted, U* a,
                              hipStream_t stream);
 
 // ---- adjoint launchers (adjoint.hip) ----
+// tile-store scratch of the reverse sub-cycle scatter: tiles (of
+// sl_adjoint_tile_bytes bytes) and a flag that the scatter sets to seq when a
+// point's support leaves its window (seq differs between calls, and the flag
+// starts at a value no seq takes)
+template <typename T>
+struct TileStore {
+  T* tiles;
+  int* flag;
+  int seq;
+};
 // one semi-Lagrangian sub-cycle in reverse; stage s >= 2 scatters into
 // adj_v[sbuf[s]]; the first stage's nodal adjoint goes to dun / dunp1; ovf[4]
 // (zero on entry and exit) take the rare scatters outside a node's window
 template <typename T>
-void launch_sl_adjoint(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
+bool launch_sl_adjoint(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, CF3<T> lam,
                        F3<T> adj_fm, F3<T> const adj_v[3], F3<T> const ovf[4], const int sbuf[4], int nbuf,
-                       F3<T> dun, F3<T> dunp1, hipStream_t stream);
+                       F3<T> dun, F3<T> dunp1, const TileStore<T>* ts, const T bct[3], F3<T> dcn,
+                       F3<T> dcnp1, hipStream_t stream);
+// bytes of the tile-store scratch launch_sl_adjoint uses when given one.
+// ts != nullptr: one launch over all bricks plus a gather pass that assigns
+// adj_fm and adds the stage-buffer adjoints straight into dcn / dcnp1
+// (weights 1 - bct[b], bct[b]); adj_v is not touched and it returns true.
+// ts == nullptr: eight colour launches that add their tiles into adj_fm and
+// adj_v in place (the caller unblends adj_v); it returns false.
+template <typename T>
+size_t sl_adjoint_tile_bytes(const GridDev<T>& g, int interp, int nbuf);
 // an += (1-ct) b, anp1 += ct b, b = 0
 template <typename T>
 void launch_adj_unblend(const GridDev<T>& g, F3<T> b, T ct, F3<T> an, F3<T> anp1, hipStream_t stream);
