@@ -602,70 +602,50 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
 #pragma unroll
       for (int c = 0; c < 3; c++) up[c] = w0 * sp.un.c[c][l] + sp.ct0 * sp.unp1.c[c][l];
     }
+    // Every integrator here has the form x_s = x0 + a_s dt u(x_{s-1}),
+    // xp = x0 + sum_s b_s dt u(x_s) (RK2: a = 1/2, b = (0, 1); RK3: a =
+    // (1/2, 2), b = (1/6, 2/3, 1/6); RK4: a = (1/2, 1/2, 1), b = (1/6, 1/3,
+    // 1/3, 1/6); Euler and trapezoidal: one stage), with the increments
+    // summed in the order integrate_inc() uses, so one loop holds every
+    // stage evaluation.
     T a_[4] = {0, 0, 0, 0}, b_[4] = {0, 0, 0, 0};
     switch (sp.integ) {
       case kEuler:
       case kTrapezoidal:
         ns = 1; b_[0] = T(1);
-#pragma unroll
-        for (int d = 0; d < 3; d++) xp[d] = x0[d] + dt * up[d];
         break;
       case kRK2:
         ns = 2; a_[1] = HALF; b_[1] = T(1);
-#pragma unroll
-        for (int d = 0; d < 3; d++) xs[1][d] = x0[d] + HALF * dt * up[d];
-        ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
-#pragma unroll
-        for (int d = 0; d < 3; d++) xp[d] = x0[d] + dt * up[d];
         break;
       case kRK3:
         ns = 3; a_[1] = HALF; a_[2] = TWO; b_[0] = SIXTH; b_[1] = TWOTHIRD; b_[2] = SIXTH;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          kk[d] = dt * up[d];
-          x1[d] = SIXTH * kk[d];
-          xs[1][d] = x0[d] + HALF * kk[d];
-        }
-        ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          kk[d] = dt * up[d];
-          xs[2][d] = x0[d] + TWO * kk[d];
-          x1[d] = x1[d] + TWOTHIRD * kk[d];
-        }
-        ad.eval(g, xs[2], nd, vf[2], up, Js[2]);
-#pragma unroll
-        for (int d = 0; d < 3; d++) xp[d] = x0[d] + (x1[d] + SIXTH * (dt * up[d]));
         break;
       case kRK4:
       default:
         ns = 4; a_[1] = HALF; a_[2] = HALF; a_[3] = T(1);
         b_[0] = SIXTH; b_[1] = THIRD; b_[2] = THIRD; b_[3] = SIXTH;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          kk[d] = dt * up[d];
-          x1[d] = SIXTH * kk[d];
-          xs[1][d] = x0[d] + HALF * kk[d];
-        }
-        ad.eval(g, xs[1], nd, vf[1], up, Js[1]);
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          kk[d] = dt * up[d];
-          x1[d] = x1[d] + THIRD * kk[d];
-          xs[2][d] = x0[d] + HALF * kk[d];
-        }
-        ad.eval(g, xs[2], nd, vf[2], up, Js[2]);
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          kk[d] = dt * up[d];
-          x1[d] = x1[d] + THIRD * kk[d];
-          xs[3][d] = x0[d] + kk[d];
-        }
-        ad.eval(g, xs[3], nd, vf[3], up, Js[3]);
-#pragma unroll
-        for (int d = 0; d < 3; d++) xp[d] = x0[d] + (x1[d] + SIXTH * (dt * up[d]));
         break;
     }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      kk[d] = dt * up[d];
+      x1[d] = b_[0] * kk[d];
+    }
+#pragma unroll
+    for (int st = 1; st < 4; st++) {
+      if (st < ns) {  // the same for every lane
+#pragma unroll
+        for (int d = 0; d < 3; d++) xs[st][d] = x0[d] + a_[st] * kk[d];
+        ad.eval(g, xs[st], nd, vf[st], up, Js[st]);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          kk[d] = dt * up[d];
+          x1[d] = x1[d] + b_[st] * kk[d];
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) xp[d] = x0[d] + x1[d];
     // ---- BC of the departure point ----
     int ndb[3] = {i, j, k};
     xb[0] = xp[0]; xb[1] = xp[1]; xb[2] = xp[2];
@@ -718,15 +698,16 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
     __syncthreads();
     const F3<T> dst = ph == 0 ? adj_fm : (ph == 1 ? adj_v0 : (ph == 2 ? adj_v1 : adj_v2));
     ws_.ovf = ph == 0 ? ovf_fm : (ph == 1 ? ovf_v0 : (ph == 2 ? ovf_v1 : ovf_v2));
-    // every lane calls the scatter (inactive lanes contribute nothing)
+    // the points of this phase, in one window pass: phase 0 the departure
+    // point with the flowmap adjoint, phase 1 + b the stages (1..3) that
+    // feed buffer b with their velocity adjoints; which points they are is
+    // the same for every lane of the launch, and every lane calls the
+    // scatter (inactive lanes contribute nothing)
+    int sel[3] = {0, 0, 0}, np = 0;
     if (ph == 0) {
-      const bool use0[1] = {active};
-      ws_.template multi<1>(ad, g, &xb, use0, nd, &gl3);
+      np = 1;
     } else {
-      // the stages (1..3) that feed this buffer, in one window pass; which
-      // ones they are is the same for every lane of the launch
       const int nsu = n_stages(sp.integ);
-      int sel[3] = {1, 1, 1}, np = 0;
 #pragma unroll
       for (int st = 1; st < 4; st++) {
         const int sb_ = st == 1 ? sbuf.y : (st == 2 ? sbuf.z : sbuf.w);
@@ -737,19 +718,25 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
           np++;
         }
       }
-      T q[3][3], w[3][3];
+    }
+    T q[3][3], w[3][3];
 #pragma unroll
-      for (int p = 0; p < 3; p++)
+    for (int p = 0; p < 3; p++)
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-          q[p][d] = sel[p] == 1 ? xs[1][d] : (sel[p] == 2 ? xs[2][d] : xs[3][d]);
-          w[p][d] = sel[p] == 1 ? ws[1][d] : (sel[p] == 2 ? ws[2][d] : ws[3][d]);
-        }
-      const bool on = active && !stuck;
-      const bool use[3] = {on, on, on};
-      if (np == 1) ws_.template multi<1>(ad, g, q, use, nd, w);
-      else if (np == 2) ws_.template multi<2>(ad, g, q, use, nd, w);
-      else if (np == 3) ws_.template multi<3>(ad, g, q, use, nd, w);
+      for (int d = 0; d < 3; d++) {
+        const int sp_ = sel[p];
+        q[p][d] = sp_ == 0 ? xb[d] : (sp_ == 1 ? xs[1][d] : (sp_ == 2 ? xs[2][d] : xs[3][d]));
+        w[p][d] = sp_ == 0 ? gl3[d] : (sp_ == 1 ? ws[1][d] : (sp_ == 2 ? ws[2][d] : ws[3][d]));
+      }
+    const bool on = ph == 0 ? active : (active && !stuck);
+    const bool use[3] = {on, on, on};
+    // (three points, which needs all stages to share one buffer, take a
+    // two-point and a one-point pass: fewer instantiations, less code)
+    if (np == 1) {
+      ws_.template multi<1>(ad, g, q, use, nd, w);
+    } else if (np >= 2) {
+      ws_.template multi<2>(ad, g, q, use, nd, w);
+      if (np == 3) ws_.template multi<1>(ad, g, q + 2, use + 2, nd, w + 2);
     }
     __syncthreads();
     if constexpr (TS) {
