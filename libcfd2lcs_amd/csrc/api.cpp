@@ -70,6 +70,9 @@ struct Options {
   // AUX_GRID (data_m.f90:39): six auxiliary particle sets per FTLE
   // diagnostic, read when the diagnostic is created
   int aux_grid = 0;
+  // adjoint scatter by in-place colour launches instead of the tile store
+  // (which it falls back to anyway above kTileStoreMax): for tests
+  int adj_colours = 0;
   bool warn_opt = false, warn_par = false;  // SAVEd warn flags (cfd2lcs.f90:444,632)
 };
 
@@ -1741,7 +1744,7 @@ struct Engine : EngineBase {
           // below
           TileStore<T> tstore{nullptr, nullptr, 0};
           const size_t tb = sl_adjoint_tile_bytes(g, tape.interp, nb);
-          if (tb <= kTileStoreMax) {
+          if (tb <= kTileStoreMax && !g_opt.adj_colours) {
             if (!tape.oflag.p) {
               // -1: a value no call number takes
               if (!ensure(tape.oflag, sizeof(int))) return 1;
@@ -2032,6 +2035,7 @@ void cfd2lcs_set_option_c(char option[], int val) {
   else if (o == "WRITE_SLOTS") boolopt(g_opt.write_slots);
   else if (o == "VELOCITY_INVARIANTS") boolopt(g_opt.invariants);
   else if (o == "AUX_GRID") boolopt(g_opt.aux_grid);
+  else if (o == "ADJOINT_COLOURS") boolopt(g_opt.adj_colours);
   else if (o == "KERNEL_TIMING") {
     boolopt(g_opt.kernel_timing);
   } else if (o == "VELOCITY_STORAGE") {

@@ -8,6 +8,7 @@ cfd2lcs_adjoint_backward_c), which has no counterpart in the reference:
   random directions over all snapshots (smooth cubic B-spline path to 1e-6
   relative, linear path to 1e-4, whose bracket switches are kinks);
 * checkpointing every k steps changes nothing but memory;
+* the tile-store and colour-launch scatters agree;
 * the torch.autograd binding carries the gradient to flow parameters
   (double-gyre A, epsilon, omega), checked against finite differences of the
   parameters.
@@ -150,6 +151,32 @@ def test_adjoint_gradient_matches_central_differences(dp, case, interp, integ, n
         errs.append(abs(ad - fd) / max(abs(fd), 1e-30))
         print(f"{case} interp {interp} integ {integ}: adjoint {ad:.10e} fd {fd:.10e} rel {errs[-1]:.2e}")
     assert max(errs) <= tol
+
+
+@pytest.mark.parametrize("case,interp,integ,n", CASES)
+def test_tile_store_and_colour_scatter_agree(dp, case, interp, integ, n):
+    """The two scatter paths of the reverse sub-cycle, tile store + gather
+    (the default) and in-place colour launches (used when the tile store
+    would exceed its size cap), give the same gradient up to the order of
+    summation."""
+    import libcfd2lcs_amd as L
+    X, Y, Z, fl = setup(dp, case, interp, integ, n)
+    times = [0.05 * s for s in range(7)]
+    snaps = gyre_snapshots(X, Y, times) if case == "gyre" else abc_snapshots(X, Y, Z, times)
+    us, vs, ws = (dev(a) for a in snaps)
+    wts = dev(np.random.default_rng(11).standard_normal(us.shape[1:]))
+    out = []
+    try:
+        for colours in (False, True):
+            dp.set_option("ADJOINT_COLOURS", L.LCS_TRUE if colours else L.LCS_FALSE)
+            Lv, g = _loss_and_grad(dp, us, vs, ws, times, wts)
+            out.append((Lv, torch.stack(g).cpu().numpy()))
+    finally:
+        dp.set_option("ADJOINT_COLOURS", L.LCS_FALSE)
+    assert out[0][0] == out[1][0]
+    scale = np.abs(out[0][1]).max()
+    assert scale > 0
+    assert np.abs(out[0][1] - out[1][1]).max() <= 1e-12 * scale
 
 
 def test_checkpointing_changes_nothing(dp):
