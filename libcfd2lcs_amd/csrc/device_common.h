@@ -45,6 +45,41 @@ __device__ __forceinline__ void decode_interior(const GridDev<T>& g, int p, int&
   j = r % g.n[1] + 1;
   k = r / g.n[1] + 1;
 }
+// The same bijection in slab order: blocks of kSlabJ rows of j, each swept
+// through every k, i fastest.  The nodes one XCD has in flight then lie in a
+// slab of kSlabJ rows by some ten planes, whose stencil rows (cubic: four
+// planes and rows around each) stay within that XCD's 4 MB L2 instead of
+// being fetched again for every plane a whole-plane sweep passes.
+constexpr int kSlabJ = 16;
+template <typename T>
+__device__ __forceinline__ void decode_interior_slab(const GridDev<T>& g, int p, int& i, int& j, int& k) {
+  const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  const int full = n1 / kSlabJ;
+  const int slab = kSlabJ * n0 * n2;  // nodes of a full slab
+  int jb, rows, r;
+  if (p < full * slab) {
+    jb = p / slab;
+    r = p - jb * slab;
+    rows = kSlabJ;
+  } else {
+    jb = full;
+    r = p - full * slab;
+    rows = n1 - full * kSlabJ;
+  }
+  const int plane = rows * n0;
+  k = r / plane + 1;
+  r -= (k - 1) * plane;
+  j = jb * kSlabJ + r / n0 + 1;
+  i = r % n0 + 1;
+}
+// inverse of decode_interior_slab: the slab-order index of interior node
+// (i, j, k) of an n0 x n1 x n2 grid
+__device__ __forceinline__ int encode_interior_slab(int n0, int n1, int n2, int i, int j, int k) {
+  const int full = n1 / kSlabJ;
+  const int jb = (j - 1) / kSlabJ, jj = (j - 1) - jb * kSlabJ;
+  const int rows = jb < full ? kSlabJ : n1 - full * kSlabJ;
+  return jb * kSlabJ * n0 * n2 + ((k - 1) * rows + jj) * n0 + (i - 1);
+}
 template <typename T>
 __device__ __forceinline__ int pid(const GridDev<T>& g, int i, int j, int k) {
   return (i - 1) + g.n[0] * ((j - 1) + g.n[1] * (k - 1));

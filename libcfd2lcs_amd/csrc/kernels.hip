@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T>
   const int p = xcd_thread_index();
   if (p >= g.np) return;
   int i, j, k;
-  decode_interior(g, p, i, j, k);
+  decode_interior_slab(g, p, i, j, k);
   sl_node<T, S>(g, sp, interp, fmold, fmnew, i, j, k, st);
 }
 

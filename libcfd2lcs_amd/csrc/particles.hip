@@ -5,27 +5,30 @@
 // started on neighbouring nodes ends up spread over the domain and its
 // interpolation gathers stop sharing cache lines.  Every particle's update
 // depends on its own state only, so the set may be stored in any order: it is
-// radix-sorted by the interior index of its tracked node, which makes the 64
-// particles of a wavefront gather from a few neighbouring grid rows again.
+// radix-sorted by the slab-order index of its tracked node (the order the
+// semi-Lagrangian step walks the nodes in), which makes the 64 particles of a
+// wavefront gather from a few neighbouring grid rows again and keeps what one
+// XCD has in flight within its L2.
 // The origin index (lp%no0, data_m.f90:392-396) travels with each particle,
 // and the displacement is scattered back to origin order wherever the
 // reference reads it by origin (exchange_lpmap, comms_m.f90:1191-1376).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include "device_common.h"
 #include "lcs_device.h"
 
 namespace lcs {
 
 constexpr int kPBlock = 256;
 
-__global__ void __launch_bounds__(kPBlock) k_sort_keys(int n0, int n1, int np, const int16_t* __restrict__ ns0,
+__global__ void __launch_bounds__(kPBlock) k_sort_keys(int n0, int n1, int n2, int np, const int16_t* __restrict__ ns0,
                                                        const int16_t* __restrict__ ns1,
                                                        const int16_t* __restrict__ ns2, unsigned* __restrict__ key,
                                                        unsigned* __restrict__ val) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
-  key[q] = (unsigned)((ns0[q] - 1) + n0 * ((ns1[q] - 1) + n1 * (ns2[q] - 1)));
+  key[q] = (unsigned)encode_interior_slab(n0, n1, n2, ns0[q], ns1[q], ns2[q]);
   val[q] = (unsigned)q;
 }
 
@@ -76,7 +79,7 @@ int launch_particle_sort(const GridDev<T>& g, ParticleSet<T> a, ParticleSet<T> b
                          unsigned* vals, void* temp, size_t temp_bytes, hipStream_t s) {
   const int np = (int)npl;
   const dim3 grid((np + kPBlock - 1) / kPBlock);
-  hipLaunchKernelGGL(k_sort_keys, grid, dim3(kPBlock), 0, s, g.n[0], g.n[1], np, a.n[0], a.n[1], a.n[2], keys,
+  hipLaunchKernelGGL(k_sort_keys, grid, dim3(kPBlock), 0, s, g.n[0], g.n[1], g.n[2], np, a.n[0], a.n[1], a.n[2], keys,
                      vals);
   hipcub::DoubleBuffer<unsigned> kb(keys, keys + np), vb(vals, vals + np);
   size_t tb = temp_bytes;
