@@ -580,10 +580,11 @@ template <typename T, typename S, class I>
 __global__ void __launch_bounds__(kBlock) k_sl_step2(GridDev<T> g, GridDev<T> lg, const int16_t* __restrict__ nodes,
                                                      int fresh, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
                                                      DevStatus* st) {
-  const int p = xcd_thread_index();
-  if (p >= lg.np) return;
+  const int w = xcd_thread_index();
+  if (w >= lg.np) return;
   int i, j, k;
-  decode_interior(lg, p, i, j, k);
+  decode_interior_slab(lg, w, i, j, k);
+  const int p = pid(lg, i, j, k);
   const I interp{};
   const T x0[3] = {lg.gl[0][i], lg.gl[1][j], lg.gl[2][k]};
   int nd[3], nm[3];
