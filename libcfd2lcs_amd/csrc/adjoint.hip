@@ -548,7 +548,10 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
   using WS = WindowScatter<T, A, BRX, BRY, BRZ>;
   __shared__ T tile_mem[3 * WS::LTILE];
   const A ad{};
-  const int b = blockIdx.x;
+  // tile store: each XCD takes a contiguous run of bricks (workgroups are
+  // dealt round-robin over the 8 XCDs), so bricks that share stencil rows
+  // share that XCD's L2
+  const int b = TS ? xcd_brick(blockIdx.x, gridDim.x) : blockIdx.x;
   int ibx, iby, ibz;
   if constexpr (TS) {
     ibx = b % al.nbx;
