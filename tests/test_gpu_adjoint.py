@@ -95,9 +95,16 @@ CASES = [
     ("abc", F.LINEAR, F.RK3, 10),
     ("abc", F.CUBIC, F.RK4, 10),
 ]
+# every tile width of the 3-D scatter: 4 (nearest), 6 (linear), 7 (quadratic
+# B-spline), 8 (cubic)
+CASES_ALL_TILES = CASES + [
+    ("abc", F.QUADRATIC, F.RK4, 10),
+    ("abc", F.NEAREST_NBR, F.RK2, 10),
+    ("gyre", F.QUADRATIC, F.RK3, (24, 12)),
+]
 
 
-@pytest.mark.parametrize("case,interp,integ,n", CASES)
+@pytest.mark.parametrize("case,interp,integ,n", CASES + [("abc", F.QUADRATIC, F.RK4, 10)])
 def test_adjoint_forward_matches_bkwd_diagnostic(dp, case, interp, integ, n):
     import libcfd2lcs_amd as L
     X, Y, Z, fl = setup(dp, case, interp, integ, n)
@@ -153,7 +160,7 @@ def test_adjoint_gradient_matches_central_differences(dp, case, interp, integ, n
     assert max(errs) <= tol
 
 
-@pytest.mark.parametrize("case,interp,integ,n", CASES)
+@pytest.mark.parametrize("case,interp,integ,n", CASES_ALL_TILES)
 def test_tile_store_and_colour_scatter_agree(dp, case, interp, integ, n):
     """The two scatter paths of the reverse sub-cycle, tile store + gather
     (the default) and in-place colour launches (used when the tile store
