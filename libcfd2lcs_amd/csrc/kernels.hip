@@ -384,8 +384,15 @@ __global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b,
 // Stage (a'): forward particle sub-cycle (update_lp, lp_motion_m.f90:15-54):
 // integrate, BC, stick, dx accumulation, one tracking pass, periodic wrap.
 // ---------------------------------------------------------------------------
+// FP16 records on a rectilinear grid: 4 waves per SIMD fit without spilling
+// (114-120 VGPRs) and keep more row loads in flight; the other variants keep
+// the compiler's register budget (FP32 and BF16 would spill)
+template <typename T, typename S, class I>
+constexpr int fwd_min_waves() {
+  return (std::is_same<S, half4_t>::value && !I::kNonRect) ? 4 : 1;
+}
 template <typename T, typename S, class I, bool COMP>
-__global__ void __launch_bounds__(kBlock) k_fwd_step(GridDev<T> g, StageParams<T> sp, ParticleSet<T> ps, int np,
+__global__ void __launch_bounds__(kBlock, (fwd_min_waves<T, S, I>())) k_fwd_step(GridDev<T> g, StageParams<T> sp, ParticleSet<T> ps, int np,
                                                      DevStatus* st) {
   constexpr bool NR = I::kNonRect;
   const int p = xcd_thread_index();
