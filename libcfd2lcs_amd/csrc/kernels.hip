@@ -232,8 +232,22 @@ __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T
   flush_status(st, err, oob);
 }
 
+// FP32 semi-Lagrangian step with FP32 fields (cubic, 3-D/2-D splines,
+// linear): 5 waves per SIMD fit in 96 VGPRs with no extra spilling and keep
+// more row loads in flight; the quadratic spline and the 16-bit records
+// would spill at that budget and keep the compiler's choice
+template <class I>
+struct QuadraticSpline : std::false_type {};
+template <typename T>
+struct QuadraticSpline<IntSpline<T, 3, 3>> : std::true_type {};
 template <typename T, typename S, class I>
-__global__ void __launch_bounds__(kBlock) k_sl_step(GridDev<T> g, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
+constexpr int sl_min_waves() {
+  return (std::is_same<T, float>::value && std::is_same<S, T>::value && !I::kNonRect && !QuadraticSpline<I>::value)
+             ? 5
+             : 1;
+}
+template <typename T, typename S, class I>
+__global__ void __launch_bounds__(kBlock, (sl_min_waves<T, S, I>())) k_sl_step(GridDev<T> g, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
                                                     DevStatus* st) {
   const I interp{};
   const int p = xcd_thread_index();
