@@ -232,10 +232,11 @@ __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T
   flush_status(st, err, oob);
 }
 
-// FP32 semi-Lagrangian step with FP32 fields (cubic, 3-D/2-D splines,
-// linear): 5 waves per SIMD fit in 96 VGPRs with no extra spilling and keep
-// more row loads in flight; the quadratic spline and the 16-bit records
-// would spill at that budget and keep the compiler's choice
+// Waves per SIMD the semi-Lagrangian step asks for, where they fit with no
+// extra spilling and keep more row loads in flight: FP32 fields 5 (96
+// VGPRs; 6 spills and runs 20 % slower), FP16 records 6 (80 VGPRs).  The
+// 3-D quadratic spline, BF16 records and non-rectilinear grids would spill
+// and keep the compiler's choice.
 template <class I>
 struct QuadraticSpline : std::false_type {};
 template <typename T>
@@ -244,7 +245,7 @@ template <typename T, typename S, class I>
 constexpr int sl_min_waves() {
   return (std::is_same<T, float>::value && std::is_same<S, T>::value && !I::kNonRect && !QuadraticSpline<I>::value)
              ? 5
-             : 1;
+             : ((std::is_same<S, half4_t>::value && !I::kNonRect && !QuadraticSpline<I>::value) ? 6 : 1);
 }
 template <typename T, typename S, class I>
 __global__ void __launch_bounds__(kBlock, (sl_min_waves<T, S, I>())) k_sl_step(GridDev<T> g, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
