@@ -753,6 +753,12 @@ struct Engine : EngineBase {
   Blend blends[6];
   unsigned long long blend_clock = 0;
   int vold = 0;  // index of u_n in vel[] during an update
+  double umax[2] = {0.0, 0.0};  // largest |component| of vel[b] (interior)
+  // 16-bit stage fields hold u / vscale, vscale = 2^(e-7) for the largest
+  // |component| 2^e * m (1 <= m < 2) of both snapshots: stored values stay
+  // below 2^8, leaving 2^8 of headroom for spline coefficients under FP16's
+  // 65504, and anything above 2^-14 * vscale stays a normal number
+  double vscale = 1.0;
 
   struct StageField {
     const void* c[3];
@@ -791,7 +797,7 @@ struct Engine : EngineBase {
     }
     char* p = (char*)victim->buf.p;
     void* out[3] = {p, p + eb * g.nt, p + 2 * eb * g.nt};
-    ktimed(14, [&] { launch_blend(g, out, stor, cf3(A), cf3(B), (T)ct, stream); });
+    ktimed(14, [&] { launch_blend(g, out, stor, cf3(A), cf3(B), (T)ct, (T)(1.0 / vscale), st, stream); });
     victim->ct = ct;
     victim->storage = stor;
     victim->spl = spl;
@@ -833,6 +839,7 @@ struct Engine : EngineBase {
         ns = 4;
     }
     sp.storage = g_opt.vel_storage;
+    sp.vscale = (T)vscale;
     for (int a = 0; a < 4; a++) {
       const StageField f = stage_field(c[a < ns ? a : 0], spl);
       for (int d = 0; d < 3; d++) sp.v[a][d] = f.c[d];
@@ -892,7 +899,7 @@ struct Engine : EngineBase {
     }
     F3<T> unp1 = f3(vel[cur]);
     CF3<T> unm = cf3(vel[old]);
-    HIPCHK(hipMemsetAsync(&st->cfl_bits, 0, sizeof(st->cfl_bits), stream));
+    HIPCHK(hipMemsetAsync(&st->cfl_bits, 0, sizeof(st->cfl_bits) + sizeof(st->umax_bits), stream));
     ktimed(13, [&] {
       launch_ingest<U, T>(g, unp1, su, sv, sw, stream);
       launch_vel_ghost(g, unp1, unm, st, stream);
@@ -904,13 +911,23 @@ struct Engine : EngineBase {
     harvest_timers();
     oob_total = st_host->oob;
     if (st_host->err) {
-      fail(st_host->err & kErrBcFlag ? "unknown bcflag" : (st_host->err & kErrTrack ? "particle tracking failed" : "device error"));
+      fail(st_host->err & kErrBcFlag
+               ? "unknown bcflag"
+               : (st_host->err & kErrTrack
+                      ? "particle tracking failed"
+                      : (st_host->err & kErrRange ? "velocity outside the FP16 range of VELOCITY_STORAGE = LCS_FP16"
+                                                  : "device error")));
       return;
     }
     double cfl_rate = 0;
     {
       unsigned long long bits = st_host->cfl_bits;
       std::memcpy(&cfl_rate, &bits, sizeof(double));
+      bits = st_host->umax_bits;
+      std::memcpy(&umax[cur], &bits, sizeof(double));
+      const double m = std::max(umax[0], umax[1]);
+      vscale = 1.0;
+      if (g_opt.vel_storage == kStoreF16 && m > 0.0 && std::isfinite(m)) vscale = std::ldexp(1.0, std::ilogb(m) - 7);
     }
     if (spl) {
       // spline coefficients of u_np1 (and of u_n when they are missing)
@@ -1497,7 +1514,7 @@ struct Engine : EngineBase {
     for (int pass = 0; pass < 2; pass++) {
       const int b = pass == 0 ? old : cur;
       const int snap = pass == 0 ? s - 1 : s;
-      HIPCHK_R(hipMemsetAsync(&st->cfl_bits, 0, sizeof(st->cfl_bits), stream), false);
+      HIPCHK_R(hipMemsetAsync(&st->cfl_bits, 0, sizeof(st->cfl_bits) + sizeof(st->umax_bits), stream), false);
       launch_ingest<U, T>(g, f3(vel[b]), tape.du + snap * np, tape.dv + snap * np, tape.dw + snap * np, stream);
       launch_vel_ghost(g, f3(vel[b]), cf3(vel[1 - b]), st, stream);
       launch_cfl(g, f3(vel[b]), st, stream);

@@ -550,6 +550,15 @@ __device__ __forceinline__ CF3<S> stage(const StageParams<T>& sp, int i) {
   return CF3<S>{{(const S*)sp.v[i][0], (const S*)sp.v[i][1], (const S*)sp.v[i][2]}};
 }
 
+// a velocity gathered from a 16-bit stage field, back to its scale
+template <typename S, typename T>
+__device__ __forceinline__ void stage_scale(const StageParams<T>& sp, T up[3]) {
+  if constexpr (Packed<S>::value) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) up[d] = up[d] * sp.vscale;
+  }
+}
+
 // integrate_lp stages (lp_motion_m.f90:129-328) in increment form:
 // inc = x_new - xn summed from the stage terms alone, so that no term is
 // rounded against |xn|.  RK3 evaluates k3 at xn + 2 k2 as the reference does
@@ -588,6 +597,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
       for (int d = 0; d < 3; d++) k[d] = dt * up[d];
       at(HALF, k);
       vel(g, xp, nd, stage<S>(sp, 1), up, oob, &sq);
+      stage_scale<S>(sp, up);
 #pragma unroll
       for (int d = 0; d < 3; d++) inc[d] = dt * up[d];
       break;
@@ -600,6 +610,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
       }
       at(HALF, k);
       vel(g, xp, nd, stage<S>(sp, 1), up, oob, &sq);
+      stage_scale<S>(sp, up);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
@@ -607,6 +618,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
       }
       at(TWO, k);
       vel(g, xp, nd, stage<S>(sp, 2), up, oob, &sq);
+      stage_scale<S>(sp, up);
 #pragma unroll
       for (int d = 0; d < 3; d++) inc[d] = inc[d] + SIXTH * (dt * up[d]);
       break;
@@ -620,6 +632,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
       }
       at(HALF, k);
       vel(g, xp, nd, stage<S>(sp, 1), up, oob, &sq);
+      stage_scale<S>(sp, up);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
@@ -627,6 +640,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
       }
       at(HALF, k);
       vel(g, xp, nd, stage<S>(sp, 2), up, oob, &sq);
+      stage_scale<S>(sp, up);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         k[d] = dt * up[d];
@@ -634,6 +648,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
       }
       at(T(1), k);
       vel(g, xp, nd, stage<S>(sp, 3), up, oob, &sq);
+      stage_scale<S>(sp, up);
 #pragma unroll
       for (int d = 0; d < 3; d++) inc[d] = inc[d] + SIXTH * (dt * up[d]);
       break;

@@ -69,11 +69,11 @@ __global__ void __launch_bounds__(kBlock) k_vel_ghost(GridDev<T> g, F3<T> u, CF3
 // MASK nodes get zero velocity; cfl = sum_d |u_d / delta_d| over interior
 // nodes (lp_motion_m.f90:90-98).  Grid-stride loop, block max folded into
 // st->cfl_bits (cfl >= 0, so the IEEE bit pattern orders as an unsigned
-// integer).
+// integer); the largest |component| goes to st->umax_bits the same way.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_cfl(GridDev<T> g, F3<T> u, DevStatus* st) {
-  __shared__ double smax[kBlock / 64];
-  double c = 0.0;
+  __shared__ double smax[2][kBlock / 64];
+  double c = 0.0, um = 0.0;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < g.np;
        p += (long long)gridDim.x * blockDim.x) {
     int i, j, k;
@@ -89,14 +89,25 @@ __global__ void __launch_bounds__(kBlock) k_cfl(GridDev<T> g, F3<T> u, DevStatus
     s = s + fabs(b * g.dl[1][p]);
     s = s + fabs(d * g.dl[2][p]);
     c = fmax(c, (double)s);
+    um = fmax(um, (double)fmax(fabs(a), fmax(fabs(b), fabs(d))));
   }
-  for (int o = 32; o > 0; o >>= 1) c = fmax(c, __shfl_down(c, o, 64));
-  if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = c;
+  for (int o = 32; o > 0; o >>= 1) {
+    c = fmax(c, __shfl_down(c, o, 64));
+    um = fmax(um, __shfl_down(um, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    smax[0][threadIdx.x >> 6] = c;
+    smax[1][threadIdx.x >> 6] = um;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double m = smax[0];
-    for (int w = 1; w < kBlock / 64; w++) m = fmax(m, smax[w]);
+    double m = smax[0][0], n = smax[1][0];
+    for (int w = 1; w < kBlock / 64; w++) {
+      m = fmax(m, smax[0][w]);
+      n = fmax(n, smax[1][w]);
+    }
     atomicMax(&st->cfl_bits, (unsigned long long)__double_as_longlong(m));
+    atomicMax(&st->umax_bits, (unsigned long long)__double_as_longlong(n));
   }
 }
 
@@ -375,17 +386,26 @@ __global__ void __launch_bounds__(BX * BY * BZ) k_sl_tiled(GridDev<T> g, StagePa
 }
 
 // time blend u = (1-ct) u_n + ct u_np1 over the ghosted grid, in the
-// reference's expression order (lp_motion_m.f90:171-173)
+// reference's expression order (lp_motion_m.f90:171-173).  16-bit storage
+// keeps the blend times inv, a power of two (exact), so that values and
+// spline coefficients sit in the format's normal range; a value that would
+// still overflow FP16 sets kErrRange instead of turning into inf.
 template <typename T, typename S>
-__global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b, T ct, long long nt) {
+__global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b, T ct, T inv, long long nt,
+                                                  DevStatus* st) {
 #pragma clang fp contract(off)
   const T w0 = T(1) - ct;
+  bool range = false;
   for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < nt; l += (long long)gridDim.x * blockDim.x) {
     if constexpr (Packed<S>::value) {
       using E = typename Packed<S>::E;
       S r;
 #pragma unroll
-      for (int c = 0; c < 3; c++) r.e[c] = narrow<E>(w0 * a.c[c][l] + ct * b.c[c][l]);
+      for (int c = 0; c < 3; c++) {
+        const T x = (w0 * a.c[c][l] + ct * b.c[c][l]) * inv;
+        if constexpr (std::is_same<E, half_t>::value) range |= fabs(x) > T(65504);
+        r.e[c] = narrow<E>(x);
+      }
       r.e[3] = E{0};
       out.c[0][l] = r;
     } else {
@@ -393,6 +413,7 @@ __global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b,
       for (int c = 0; c < 3; c++) out.c[c][l] = narrow<S>(w0 * a.c[c][l] + ct * b.c[c][l]);
     }
   }
+  if (range) atomicOr(&st->err, (int)kErrRange);
 }
 
 // ---------------------------------------------------------------------------
@@ -425,7 +446,10 @@ __global__ void __launch_bounds__(kBlock, (fwd_min_waves<T, S, I>())) k_fwd_step
     for (int d = 0; d < 3; d++) xnl[d] = ps.xl[d][p];
   }
   const Split<T> s0{{xn[0], xn[1], xn[2]}, {xnl[0], xnl[1], xnl[2]}};
-  auto first = [&](T up[3]) { interp(g, xn, nd, stage<S>(sp, 0), up, oob, COMP ? &s0 : nullptr); };
+  auto first = [&](T up[3]) {
+    interp(g, xn, nd, stage<S>(sp, 0), up, oob, COMP ? &s0 : nullptr);
+    stage_scale<S>(sp, up);
+  };
   if constexpr (!COMP) {
     // displacement from the summed stage increments (see sl_node)
     T inc[3], qr[3];
@@ -620,7 +644,10 @@ __global__ void __launch_bounds__(kBlock) k_sl_step2(GridDev<T> g, GridDev<T> lg
   const T mask = g.flag[gid(g, nm[0], nm[1], nm[2])] != kMask ? T(1) : T(0);
   unsigned long long oob = 0;
   int err = 0;
-  auto first = [&](T up[3]) { interp(g, x0, nd, stage<S>(sp, 0), up, oob); };
+  auto first = [&](T up[3]) {
+    interp(g, x0, nd, stage<S>(sp, 0), up, oob);
+    stage_scale<S>(sp, up);
+  };
   T inc[3], xr[3], xp[3];
   integrate_inc<S>(g, sp, interp, first, nd, x0, inc, oob);
 #pragma unroll
@@ -1193,12 +1220,13 @@ bool dispatch_spline(const GridDev<T>& g, F&& f) {
 }
 
 template <typename T>
-void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a, CF3<T> b, T ct, hipStream_t s) {
+void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a, CF3<T> b, T ct, T inv,
+                  DevStatus* st, hipStream_t s) {
   const long long nb = std::min<long long>((g.nt + kBlock - 1) / kBlock, 8192);
   auto go = [&](auto tag) {
     using S = decltype(tag);
     F3<S> o{{(S*)out[0], (S*)out[1], (S*)out[2]}};
-    hipLaunchKernelGGL((k_blend<T, S>), dim3((unsigned)nb), dim3(kBlock), 0, s, o, a, b, ct, g.nt);
+    hipLaunchKernelGGL((k_blend<T, S>), dim3((unsigned)nb), dim3(kBlock), 0, s, o, a, b, ct, inv, g.nt, st);
   };
   if constexpr (std::is_same<T, float>::value) {
     if (storage == kStoreF16) return go(half4_t{});
@@ -1548,7 +1576,7 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
   template void launch_field_from_origin<T>(const GridDev<T>&, CF3<T>, F3<T>, hipStream_t);                   \
   template void launch_spline_fit<T>(const GridDev<T>&, F3<T>, int, hipStream_t);                             \
   template void launch_spline_fit_T<T>(const GridDev<T>&, F3<T>, hipStream_t);                                \
-  template void launch_blend<T>(const GridDev<T>&, void* const*, int, CF3<T>, CF3<T>, T, hipStream_t);     \
+  template void launch_blend<T>(const GridDev<T>&, void* const*, int, CF3<T>, CF3<T>, T, T, DevStatus*, hipStream_t);     \
   template void launch_sl_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, F3<T>, DevStatus*,  \
                                   hipStream_t, int);                                                          \
   template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, ParticleSet<T>, int, int, \

@@ -16,7 +16,7 @@ enum Integrator { kEuler = 0, kTrapezoidal = 1, kRK2 = 2, kRK3 = 3, kRK4 = 4 };
 enum Interpolator { kNearest = 0, kLinear = 1, kQuadratic = 2, kCubic = 3, kTSE = 5, kTSELimit = 6 };
 
 // device error bits (folded into the sticky error flag on the host)
-enum DevErr { kErrBcFlag = 1, kErrTrack = 2, kErrSpline = 4 };
+enum DevErr { kErrBcFlag = 1, kErrTrack = 2, kErrSpline = 4, kErrRange = 8 };
 
 // Grid description passed by value to every kernel.
 template <typename T>
@@ -59,6 +59,7 @@ struct CF3 {
 
 struct DevStatus {
   unsigned long long cfl_bits;  // max CFL over interior nodes (as ordered bits)
+  unsigned long long umax_bits; // max |velocity component| over interior nodes (ordered bits)
   unsigned long long oob;       // spline evaluations outside the knot range
   int err;                      // DevErr bits
   int pad;
@@ -90,6 +91,10 @@ struct StageParams {
   // coefficients for QUADRATIC/CUBIC; element type given by `storage`
   const void* v[4][3];
   int storage;
+  // 16-bit storage holds u / vscale (a power of two chosen per update so
+  // that the stored values and coefficients stay in the format's normal
+  // range); gathered stage velocities are multiplied back by it
+  T vscale;
   // grid values of u_n and u_np1 and the first stage's ct, for the
   // semi-Lagrangian step whose first stage starts on a node
   CF3<T> un, unp1;
@@ -123,10 +128,11 @@ template <typename T>
 void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st, hipStream_t stream);
 template <typename T>
 void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t stream);
-// out (element type per `storage`) = (1-ct) a + ct b
+// out (element type per `storage`) = ((1-ct) a + ct b) * inv, inv a power
+// of two (1 for native storage); a 16-bit value out of range sets kErrRange
 template <typename T>
-void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a, CF3<T> b, T ct,
-                  hipStream_t stream);
+void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a, CF3<T> b, T ct, T inv,
+                  DevStatus* st, hipStream_t stream);
 // tiled: FP32 B-spline steps on 3-D grids stage the gathered coefficients
 // in LDS (bit-identical results)
 template <typename T>

@@ -2149,3 +2149,74 @@ void oc_velocity_invariants(const ocfd *s, const double *u, const double *v, con
       }
   for (int c = 0; c < 3; c++) free(ug[c]);
 }
+
+/* ======================================================================
+ * Single-point probes of the particle operations on the flow grid, for the
+ * known-answer tests of tracking, particle BCs and interpolation.
+ * ====================================================================== */
+
+/* track_lp2node (lp_tracking_m.f90:72-109) of one point from node n3: one
+ * pass, or (recursive) passes until the node stays, as reconstruct_flowmap's
+ * recursive tracking does (without the periodic exchange).  Returns the
+ * number of passes, or -1 after more than 2*max(n) passes. */
+int oc_track_point(const ocfd *s, const double q[3], int n3[3], int recursive) {
+  const ogrid *g = &s->F.g;
+  const int max_track = 2 * imax3(g->n[0], g->n[1], g->n[2]);
+  for (int pass = 1;; pass++) {
+    int moved = track_pass(g, q, n3);
+    if (!recursive || !moved) return pass;
+    if (pass > max_track) return -1;
+  }
+}
+
+/* set_lp_bc (lp_tracking_m.f90:185-277) of one point q tracked to node n3
+ * (clamped in place); returns 1 for LP_STICK, -1 for an unknown flag */
+int oc_lp_bc_point(const ocfd *s, double q[3], int n3[3]) {
+  odiag one;
+  memset(&one, 0, sizeof(one));
+  double x0 = q[0], x1 = q[1], x2 = q[2];
+  int a = n3[0], b = n3[1], c = n3[2], fl = 0;
+  double *xp[3] = {&x0, &x1, &x2};
+  int *nd[3] = {&a, &b, &c};
+  for (int d = 0; d < 3; d++) one.xp[d] = xp[d];
+  one.pflag = &fl;
+  one.np = 1;
+  int e = set_lp_bc(&s->F.g, &one, nd);
+  q[0] = x0; q[1] = x1; q[2] = x2;
+  n3[0] = a; n3[1] = b; n3[2] = c;
+  if (e) return -1;
+  return fl == LP_STICK ? 1 : 0;
+}
+
+/* interp_s2u_r1 (lp_tracking_m.f90:279-662) with interpolator interp of an
+ * interior field (u, v, w: np each, Fortran order) at npt points q (3*npt,
+ * point-major) from nodes n3 (3*npt), after the ghost exchange and
+ * set_velocity_bc that cfd2lcs_update applies to a velocity field. */
+int oc_interp_points(ocfd *s, int interp, const double *u, const double *v, const double *w, long npt,
+                     const double *q, const int *n3, double *out) {
+  const ogrid *g = &s->F.g;
+  double *f[3], *pos[3], *res[3];
+  int *nd[3];
+  const double *src[3] = {u, v, w};
+  for (int d = 0; d < 3; d++) {
+    f[d] = calloc(g->nt, sizeof(double));
+    pos[d] = malloc(npt * sizeof(double));
+    res[d] = malloc(npt * sizeof(double));
+    nd[d] = malloc(npt * sizeof(int));
+    for (long p = 0; p < npt; p++) { pos[d][p] = q[3 * p + d]; nd[d][p] = n3[3 * p + d]; }
+  }
+  for (int k = 1; k <= g->n[2]; k++)
+    for (int j = 1; j <= g->n[1]; j++)
+      for (int i = 1; i <= g->n[0]; i++)
+        for (int d = 0; d < 3; d++) f[d][GID(g, i, j, k)] = src[d][PIDG(g, i, j, k)];
+  for (int d = 0; d < 3; d++) exchange(g, f[d], -1, NULL, 0);
+  int e = set_velocity_bc(g, f);
+  const int save = s->interpolator;
+  s->interpolator = interp;
+  if (!e) e = interp_field(s, &s->F, npt, f, pos, nd, res);
+  s->interpolator = save;
+  for (long p = 0; p < npt; p++)
+    for (int d = 0; d < 3; d++) out[3 * p + d] = res[d][p];
+  for (int d = 0; d < 3; d++) { free(f[d]); free(pos[d]); free(res[d]); free(nd[d]); }
+  return e;
+}

@@ -81,6 +81,19 @@ void oc_cauchy_green(const double *g, double *c);
  * NULL for zero); out: Q, C, D, H, L2, np each */
 void oc_velocity_invariants(const ocfd *s, const double *u, const double *v, const double *w, double *out);
 
+/* ---- single-point probes on the flow grid (known-answer tests) ---- */
+/* track_lp2node from node n3 (updated): one pass, or passes until the node
+ * stays (recursive); returns the passes taken, -1 past 2*max(n) passes */
+int oc_track_point(const ocfd *s, const double q[3], int n3[3], int recursive);
+/* set_lp_bc of point q (updated) tracked to node n3 (clamped in place);
+ * 1 = LP_STICK, 0 = in bounds, -1 = unknown flag */
+int oc_lp_bc_point(const ocfd *s, double q[3], int n3[3]);
+/* interp_s2u_r1 with interpolator interp of an interior field after the
+ * ghost exchange and set_velocity_bc, at npt points q[3*p+d] from nodes
+ * n3[3*p+d]; out[3*p+c] */
+int oc_interp_points(ocfd *s, int interp, const double *u, const double *v, const double *w, long npt,
+                     const double *q, const int *n3, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -124,3 +124,40 @@ class RandomFourierField:
             f = torch.fft.ifftn(self.C[c] * rot).real * (self.n ** 3) * self.scale
             out.append(f.contiguous())
         return out
+
+
+def uniform_grid(n, h=1.0, lo=0.0):
+    """Cell-centred grid with spacing h on every axis: x_i = lo + (i + 1/2) h
+    (exact in binary for h = 1 and lo = 0); all flags INTERNAL."""
+    n = (n, n, n) if np.isscalar(n) else n
+    X, Y, Z = cell_grid(n, (lo, lo, lo), tuple(lo + h * m for m in n))
+    return X, Y, Z, np.full(X.shape, LCS_INTERNAL, dtype=np.int32)
+
+
+def channel_grid(ni, nj, nk=1, mask=True, walls=LCS_WALL):
+    """[0,2]x[0,1](x[0,1]) cell-centred channel: LCS_INFLOW on the i = 1
+    face, LCS_OUTFLOW on the i = ni face, `walls` on the j faces (and on
+    the k faces in 3-D), and with mask an LCS_MASK block around (0.9, 0.55)."""
+    X, Y, Z = cell_grid((ni, nj, nk), (0.0, 0.0, 0.0), (2.0, 1.0, 1.0))
+    fl = np.full(X.shape, LCS_INTERNAL, dtype=np.int32)
+    fl[:, 0, :] = walls
+    fl[:, -1, :] = walls
+    if nk > 1:
+        fl[:, :, 0] = walls
+        fl[:, :, -1] = walls
+    fl[0, :, :] = LCS_INFLOW
+    fl[-1, :, :] = LCS_OUTFLOW
+    if mask:
+        m = (np.abs(X - 0.9) < 0.14) & (np.abs(Y - 0.55) < 0.12)
+        if nk > 1:
+            m &= np.abs(Z - 0.5) < 0.3
+        fl[m] = LCS_MASK
+    return X, Y, Z, fl
+
+
+def channel_velocity(X, Y, Z, t, U=0.3):
+    """Mean flow U along x plus the double gyre; w = 0.1 sin(2 pi z) in 3-D."""
+    u, v, w = double_gyre_velocity(X, Y, t)
+    if X.shape[2] > 1:
+        w = 0.1 * np.sin(2 * np.pi * Z) * np.cos(np.pi * Y)
+    return u + U, v, w

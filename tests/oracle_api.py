@@ -61,6 +61,12 @@ def lib():
         L.oc_eig_sym3_max.argtypes = [_dp]
         L.oc_cauchy_green.argtypes = [_dp, _dp]
         L.oc_velocity_invariants.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _dp]
+        L.oc_track_point.restype = C.c_int
+        L.oc_track_point.argtypes = [C.c_void_p, _dp, _ip, C.c_int]
+        L.oc_lp_bc_point.restype = C.c_int
+        L.oc_lp_bc_point.argtypes = [C.c_void_p, _dp, _ip]
+        L.oc_interp_points.restype = C.c_int
+        L.oc_interp_points.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_long, _dp, _ip, _dp]
         _lib = L
     return _lib
 
@@ -166,6 +172,32 @@ class Oracle:
         assert rc == 0
         rs = lambda a: a.reshape(N, order="F")
         return [rs(a) for a in arrs], rs(fl), [rs(a) for a in bn]
+
+    def track(self, q, node, recursive=True):
+        """track_lp2node of point q from 1-based node (i, j, k) on the flow
+        grid; returns (node, passes)"""
+        n3 = np.ascontiguousarray(node, dtype=np.int32).copy()
+        r = lib().oc_track_point(self.h, np.ascontiguousarray(q, dtype=np.float64), n3, int(recursive))
+        return tuple(int(a) for a in n3), r
+
+    def lp_bc(self, q, node):
+        """set_lp_bc of point q tracked to node; returns (q, node, stick)"""
+        qq = np.ascontiguousarray(q, dtype=np.float64).copy()
+        n3 = np.ascontiguousarray(node, dtype=np.int32).copy()
+        r = lib().oc_lp_bc_point(self.h, qq, n3)
+        assert r >= 0
+        return qq, tuple(int(a) for a in n3), bool(r)
+
+    def interp(self, interp, u, v, w, q, nodes):
+        """interp_s2u_r1 of the interior field (u, v, w) at points q (npt, 3)
+        from nodes (npt, 3), after the velocity ghost exchange and BC"""
+        q = np.ascontiguousarray(np.atleast_2d(q), dtype=np.float64)
+        nd = np.ascontiguousarray(np.atleast_2d(nodes), dtype=np.int32)
+        out = np.zeros(q.shape)
+        rc = lib().oc_interp_points(self.h, int(interp), _f(u), _f(v), _f(w), q.shape[0], q.ravel(),
+                                    nd.ravel(), out.ravel())
+        assert rc == 0
+        return out
 
     def periodic_shift(self):
         ps = np.zeros(81)

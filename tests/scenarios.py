@@ -35,6 +35,44 @@ def make_case(name, n=None):
         X, Y, Z, fl = F.abc_grid(n)
         X, Y, Z = X + 0.05 * np.sin(Y), Y + 0.05 * np.sin(Z), Z + 0.05 * np.sin(X)
         vel = lambda t: F.abc_velocity(X, Y, Z)
+    elif name in ("channel", "channel3d"):
+        # INFLOW / OUTFLOW faces, WALL (2-D) or SLIP (3-D) side faces, an
+        # interior MASK block; mean flow plus the double gyre
+        if name == "channel":
+            ni, nj = n or (64, 32)
+            X, Y, Z, fl = F.channel_grid(ni, nj)
+        else:
+            ni, nj, nk = n or (24, 16, 12)
+            X, Y, Z, fl = F.channel_grid(ni, nj, nk, walls=F.LCS_SLIP)
+        vel = lambda t: F.channel_velocity(X, Y, Z, t)
+    elif name in ("wall_box", "io_box"):
+        # uniform flow u = (1/2, 0, 0) along a box of spacing 1 with WALL
+        # (INFLOW / OUTFLOW) node lines at i = 1 and i = n, y and z periodic
+        X, Y, Z, fl = F.uniform_grid(n or (24, 6, 6))
+        fl[0], fl[-1] = (F.LCS_WALL, F.LCS_WALL) if name == "wall_box" else (F.LCS_INFLOW, F.LCS_OUTFLOW)
+        vel = lambda t: (0.5 + 0 * X, 0 * X, 0 * X)
+    elif name == "abc_slip_y":
+        # ABC box periodic in x and z, SLIP walls on the j faces: the ghost z
+        # coordinates take the reference's periodic_shift(i,k,k,3) reading
+        # (comms_m.f90:750), which here differs from periodic_shift(i,j,k,3)
+        X, Y, Z, fl = F.abc_grid(n or (12, 10, 8))
+        fl[:, 0, :] = F.LCS_SLIP
+        fl[:, -1, :] = F.LCS_SLIP
+        vel = lambda t: F.abc_velocity(X, Y, Z)
+    elif name == "mask_box":
+        # periodic box with a MASK block on nodes 4..8; a sheared mean flow,
+        # so that the FTLE outside the block is well above rounding
+        X, Y, Z, fl = F.uniform_grid(n or 12)
+        fl[3:8, 3:8, 3:8] = F.LCS_MASK
+        L = X.shape[0]
+        vel = lambda t: (0.3 + 0.2 * np.sin(2 * np.pi * Y / L), 0.2 + 0.1 * np.sin(2 * np.pi * Z / L),
+                         -0.1 + 0.1 * np.sin(2 * np.pi * X / L))
+    elif name == "quad_shear":
+        # u = (y^2, 0.05, 0): periodic in x, SLIP walls in y, 2-D
+        X, Y, Z = F.cell_grid(n or (8, 64, 1), (0, 0, 0), (1, 1, 1))
+        fl = np.full(X.shape, F.LCS_INTERNAL, dtype=np.int32)
+        fl[:, 0], fl[:, -1] = F.LCS_SLIP, F.LCS_SLIP
+        vel = lambda t: (Y ** 2, 0.05 + 0 * Y, 0 * Y)
     else:
         raise ValueError(name)
     return X, Y, Z, fl, vel
@@ -42,11 +80,12 @@ def make_case(name, n=None):
 
 def run_both(lib, case, n=None, integrator=F.RK2, interpolator=F.LINEAR, T=1.0, h=0.25, dt=0.05,
              nsteps=30, diags=(F.FTLE_FWD, F.FTLE_BKWD), cfl=None, every_step=None, resolution=0,
-             aux=False, stats=None):
+             aux=False, stats=None, options=()):
     """Returns a list of per-step records {step, fm: {handle: (gpu, oracle)}, ftle ...}.
     resolution / aux: the diagnostics' grid (new_sgrid_from_sgrid) and
     AUX_GRID, both read when the diagnostics are created; stats, when a
-    list, receives the integration counters of both sides at the end."""
+    list, receives the integration counters of both sides at the end;
+    options: further (name, value) set_option calls."""
     X, Y, Z, fl, vel = make_case(case, n)
     o = Oracle(X, Y, Z, fl)
     lib.finalize()
@@ -67,7 +106,7 @@ def run_both(lib, case, n=None, integrator=F.RK2, interpolator=F.LINEAR, T=1.0, 
         for c in range(3):
             assert np.array_equal(xyz_g[c], xyz_o[c]) or lib.dtype != np.float64
         handles.append(hg)
-    for name, val in (("INTEGRATOR", integrator), ("INTERPOLATOR", interpolator)):
+    for name, val in (("INTEGRATOR", integrator), ("INTERPOLATOR", interpolator)) + tuple(options):
         o.set_option(name, val)
         lib.set_option(name, val)
     if cfl is not None:

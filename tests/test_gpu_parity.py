@@ -271,3 +271,70 @@ def test_double_gyre_512x256_walls_rk2_linear_fp64(dp):
     recs, hs = run_both(dp, "gyre", n=(512, 256), integrator=F.RK2, interpolator=F.LINEAR, T=1.0, h=0.5,
                         dt=0.01, nsteps=102, every_step=25)
     assert _check(recs, hs, 1e-10, 1e-9) >= 4
+
+
+def test_double_gyre_512x256_full_window_fwd_fp64(dp):
+    """The reference's CPU configuration over its whole window: double gyre
+    512 x 256 x 1, walls, dt = 0.01, RK2 + LINEAR, forward FTLE at T = 15
+    with h = 1, i.e. a 15-slot ring.  1501 updates reach t = 15, whose output
+    is the first to compose all 15 sub-intervals; every output on the way
+    composes the slots written so far (flowmap_m.f90:253-268).  Output
+    counts and integration counters agree exactly, the FTLE to 1e-9.
+
+    The per-slot flowmaps agree to ~1e-14 (FP64 rounding: the GPU forms
+    displacements from stage increments and interpolates from split
+    coordinates, the oracle from positions), and composing k slots carries
+    that difference through the stretching of k time units, which doubles
+    it per slot on this flow (1.3e-14 with one slot, 1.1e-10 with 14).  So
+    the composed maps are held to 1e-10 through 13 slots and, beyond, to at
+    most 2.5x growth per slot from there."""
+    stats = []
+    recs, hs = run_both(dp, "gyre", n=(512, 256), integrator=F.RK2, interpolator=F.LINEAR, T=15.0, h=1.0,
+                        dt=0.01, nsteps=1501, every_step=500, diags=(F.FTLE_FWD,), stats=stats)
+    (gf, gb), (of_, ob) = stats[0]
+    assert gf == of_ and gb == ob == 0
+    assert dp.noutput(hs[0]) == 15
+    outs = [r["out"][hs[0]] for r in recs if hs[0] in r["out"]]
+    assert len(outs) == 15
+    errs = []
+    for k, (gc, oc, gftle, oftle) in enumerate(outs):
+        errs.append(max(rel_err(gc[c], oc[c]) for c in range(3)))
+        m = np.isfinite(oftle) & (oftle > -50)
+        fe = np.abs(gftle[m] - oftle[m]).max()
+        print(f"output {k + 1:2d} ({k + 1:2d} slots): composed flowmap rel {errs[-1]:.3e}, FTLE abs {fe:.3e}")
+        assert fe <= 1e-9
+    assert errs[0] <= 2e-14
+    assert max(errs[:13]) <= 1e-10
+    for k in (13, 14):
+        assert errs[k] <= 2.5 * errs[k - 1]
+
+
+@pytest.mark.parametrize("k", [17, -20])
+@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC])
+def test_fp16_storage_is_scale_invariant(sp, k, interp):
+    """FP16 stage fields hold u / 2^e with 2^e set per update from the
+    largest |u|.  The velocity times a = 2^k over times divided by a traces
+    the same paths: with a = 2^17 (|u| ~ 3.6e5, past FP16's 65504) and
+    a = 2^-20 (|u| ~ 2.6e-6, FP16 subnormal) the flowmaps are bit-identical
+    to the a = 1 run and the FTLE is a times it."""
+    import libcfd2lcs_amd as L
+    X, Y, Z, fl = F.abc_grid(16)
+    u0 = F.abc_velocity(X, Y, Z)
+    res = []
+    for a in (1.0, 2.0 ** k):
+        sp.finalize()
+        sp.init(X, Y, Z, fl)
+        hs = [sp.diagnostic_init(d, 0, 0.4 / a, 0.2 / a, "x") for d in (F.FTLE_FWD, F.FTLE_BKWD)]
+        for name, val in (("INTEGRATOR", F.RK4), ("INTERPOLATOR", interp), ("VELOCITY_STORAGE", L.LCS_FP16)):
+            sp.set_option(name, val)
+        for s in range(10):
+            sp.update(*(a * c for c in u0), np.float32(0.05 * s) / np.float32(a))
+            assert sp.error == 0
+        res.append([(sp.composed(h), sp.ftle(h), sp.flowmap(h)) for h in hs])
+        assert all(sp.noutput(h) == 2 for h in hs)
+    a = 2.0 ** k
+    for (c1, f1, m1), (c2, f2, m2) in zip(*res):
+        for d in range(3):
+            assert np.array_equal(c1[d], c2[d]) and np.array_equal(m1[d], m2[d])
+        m = np.isfinite(f1) & (f1 > -50)
+        assert np.allclose(f2[m] / a, f1[m], rtol=1e-6, atol=0)
