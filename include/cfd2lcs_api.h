@@ -214,6 +214,21 @@ int cfd2lcs_adjoint_backward_c(const double *d_dftle, lcsdata_t *d_gu,
 void cfd2lcs_adjoint_release_c(void);
 void cfd2lcs_adjoint_stats_c(double out[4]);
 
+/* Allocation-free variant of cfd2lcs_adjoint_forward_c.  It has no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same inputs and FTLE output as
+ * cfd2lcs_adjoint_forward_c, but it keeps no tape (no backward call can
+ * follow it) and it calls no hipMalloc/hipFree: every buffer it needs is
+ * carved from a fixed 64 MiB device workspace that the library holds on each
+ * GPU from load until the process ends.  Grids whose buffers do not fit
+ * (about 74^3 in FP32 with cubic B-splines, half that in FP64) fail with a
+ * message.  Calls that overlap on one GPU share the workspace and race.
+ * cfd2lcs_init_c must have run.  Returns 0 on success. */
+int cfd2lcs_adjoint_forward_noalloc_c(int n[3], int nsnap, const lcsdata_t *d_u,
+                                      const lcsdata_t *d_v, const lcsdata_t *d_w,
+                                      const lcsdata_t *times, double *d_ftle);
+
 /* ---------------------------------------------------------------------
  * Part 3: Fortran-callable entry points.  A Fortran host calls the
  * reference's external subroutines cfd2lcs_init, cfd2lcs_diagnostic_init,

@@ -35,7 +35,8 @@ EXPORTED = (
     "cfd2lcs_get_ftle_device_c", "cfd2lcs_get_noutput_c", "cfd2lcs_get_stats_c",
     "cfd2lcs_get_timers_c", "cfd2lcs_debug_grid_c",
     "cfd2lcs_adjoint_forward_c", "cfd2lcs_adjoint_backward_c", "cfd2lcs_adjoint_release_c",
-    "cfd2lcs_adjoint_stats_c", "cfd2lcs_flush_output_c", "cfd2lcs_format_es_c",
+    "cfd2lcs_adjoint_stats_c", "cfd2lcs_adjoint_forward_noalloc_c", "cfd2lcs_flush_output_c",
+    "cfd2lcs_format_es_c",
     "cfd2lcs_init_", "cfd2lcs_diagnostic_init_", "cfd2lcs_update_",
     "cfd2lcs_diagnostic_destroy_", "cfd2lcs_set_option_", "cfd2lcs_set_param_",
     "cfd2lcs_finalize_",
@@ -89,6 +90,9 @@ def load(variant="dp"):
     L.cfd2lcs_adjoint_forward_c.argtypes = [ip, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, rp, C.c_int,
                                             C.c_void_p]
     L.cfd2lcs_adjoint_forward_c.restype = C.c_int
+    L.cfd2lcs_adjoint_forward_noalloc_c.argtypes = [ip, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, rp,
+                                                    C.c_void_p]
+    L.cfd2lcs_adjoint_forward_noalloc_c.restype = C.c_int
     L.cfd2lcs_adjoint_backward_c.argtypes = [C.c_void_p] * 4
     L.cfd2lcs_adjoint_backward_c.restype = C.c_int
     L.cfd2lcs_adjoint_release_c.argtypes = []
@@ -228,6 +232,14 @@ class Cfd2lcs:
         ts = np.ascontiguousarray(np.asarray(times, dtype=self.dtype))
         return self.L.cfd2lcs_adjoint_forward_c(n, int(nsnap), C.c_void_p(du), C.c_void_p(dv), C.c_void_p(dw),
                                                 self._rp(ts), int(checkpoint), C.c_void_p(dftle))
+
+    def adjoint_forward_noalloc(self, du, dv, dw, nsnap, times, dftle):
+        """The FTLE of adjoint_forward without a tape, computed in the
+        library's fixed device workspace (no device allocation)."""
+        n = (C.c_int * 3)(*self.shape)
+        ts = np.ascontiguousarray(np.asarray(times, dtype=self.dtype))
+        return self.L.cfd2lcs_adjoint_forward_noalloc_c(n, int(nsnap), C.c_void_p(du), C.c_void_p(dv),
+                                                        C.c_void_p(dw), self._rp(ts), C.c_void_p(dftle))
 
     def adjoint_backward(self, ddftle, gu, gv, gw):
         return self.L.cfd2lcs_adjoint_backward_c(C.c_void_p(ddftle), C.c_void_p(gu), C.c_void_p(gv),

@@ -46,6 +46,7 @@ def build(verbose=False):
         "kernels": os.path.join(BUILD, "kernels.o"),
         "particles": os.path.join(BUILD, "particles.o"),
         "adjoint": os.path.join(BUILD, "adjoint.o"),
+        "scratch": os.path.join(BUILD, "scratch.o"),
         "grid": os.path.join(BUILD, "grid_host.o"),
         "io": os.path.join(BUILD, "lcs_io.o"),
         "api_dp": os.path.join(BUILD, "api_dp.o"),
@@ -57,6 +58,7 @@ def build(verbose=False):
         # hardware floating-point atomics for the adjoint scatters
         [HIPCC, *FLAGS, "-munsafe-fp-atomics", *inc, "-c", os.path.join(CSRC, "adjoint.hip"), "-o",
          objs["adjoint"]],
+        [HIPCC, *FLAGS, *inc, "-c", os.path.join(CSRC, "scratch.hip"), "-o", objs["scratch"]],
         [HIPCC, *FLAGS, *inc, "-c", os.path.join(CSRC, "grid_host.cpp"), "-o", objs["grid"]],
         [HIPCC, *FLAGS, *inc, "-pthread", "-c", os.path.join(CSRC, "lcs_io.cpp"), "-o", objs["io"]],
         [HIPCC, *FLAGS, *inc, "-c", os.path.join(CSRC, "api.cpp"), "-o", objs["api_dp"]],
@@ -71,7 +73,7 @@ def build(verbose=False):
         out = os.path.join(LIB, f"libcfd2lcs_{tag}.so")
         tmp = out + ".tmp"
         links.append([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
-                      objs[f"api_{tag}"], objs["kernels"], objs["particles"], objs["adjoint"],
+                      objs[f"api_{tag}"], objs["kernels"], objs["particles"], objs["adjoint"], objs["scratch"],
                       objs["grid"], objs["io"], "-pthread"])
     _run(links)
     for tag in ("dp", "sp"):

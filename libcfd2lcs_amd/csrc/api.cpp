@@ -34,6 +34,7 @@ typedef double lcsdata_t;
 #include "grid_host.h"
 #include "lcs_device.h"
 #include "lcs_io.h"
+#include "scratch.h"
 
 using U = lcsdata_t;  // user-facing real type
 
@@ -102,31 +103,39 @@ static void fail(const std::string& msg) {
     }                                                                                   \
   } while (0)
 
-// device allocation owned by the engine
+// device allocation owned by the engine, or (owned = false) a view of
+// memory that someone else owns (the fixed workspace of scratch.hip)
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), owned(o.owned) {
+    o.p = nullptr;
+    o.bytes = 0;
+    o.owned = true;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
-      p = o.p; bytes = o.bytes;
-      o.p = nullptr; o.bytes = 0;
+      p = o.p; bytes = o.bytes; owned = o.owned;
+      o.p = nullptr; o.bytes = 0; o.owned = true;
     }
     return *this;
   }
   void swap(DevBuf& o) noexcept {
     std::swap(p, o.p);
     std::swap(bytes, o.bytes);
+    std::swap(owned, o.owned);
   }
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    owned = true;
   }
   // at least b bytes (the contents are lost when it grows)
   bool ensure(size_t b) { return (p && bytes >= b) || alloc(b); }
@@ -139,6 +148,12 @@ struct DevBuf {
     }
     bytes = b;
     return true;
+  }
+  void view(void* q, size_t b) {
+    release();
+    p = q;
+    bytes = b;
+    owned = false;
   }
 };
 
@@ -160,6 +175,8 @@ struct EngineBase {
   virtual int adjoint_forward(int nsnap, const U* du, const U* dv, const U* dw, const U* times, int ckpt,
                               double* d_ftle) = 0;
   virtual int adjoint_backward(const double* d_dftle, U* gu, U* gv, U* gw) = 0;
+  virtual int adjoint_forward_noalloc(int nsnap, const U* du, const U* dv, const U* dw, const U* times,
+                                      double* d_ftle) = 0;
   virtual void adjoint_release() = 0;
   virtual void adjoint_stats(double out[4]) = 0;
   virtual int flush_output() = 0;
@@ -178,6 +195,7 @@ struct Engine : EngineBase {
   int spline_k = 0;     // order of the fitted spline state (0 = none)
   DevBuf status_mem;
   DevStatus* st = nullptr;
+  bool ws_ok = false;  // the fixed workspace's address was resolved at init
   DevStatus* st_host = nullptr;
   DevBuf vel[2];        // [u_n, u_np1] ghosted, 3 components each
   DevBuf vcoef[2];      // spline coefficients of vel[]
@@ -468,6 +486,9 @@ struct Engine : EngineBase {
       return false;
     }
     if (!upload_grid(hg, g, grid_mem)) return false;
+    // device address of the allocation-free entry point's fixed workspace on
+    // this device (cfd2lcs_adjoint_forward_noalloc_c reports a failure here)
+    ws_ok = scratch_setup();
     if (!status_mem.alloc(sizeof(DevStatus))) {
       fail("hipMalloc failed (status)");
       return false;
@@ -496,6 +517,30 @@ struct Engine : EngineBase {
   // lp_tracking_m.f90:506-508
   int spline_order_for(int interp) const { return interp == kQuadratic ? 3 : 4; }
   // knots and collocation factors of grid h for the active interpolator
+  // While cfd2lcs_adjoint_forward_noalloc_c runs, every buffer the shared
+  // code paths would allocate is carved instead from the fixed workspace of
+  // scratch.hip (arena != null), and memory is cleared by a fill kernel.
+  struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+  };
+  Arena* arena = nullptr;
+  // b with at least `bytes`: kept if large enough, else allocated (or, in an
+  // allocation-free call, carved from the workspace)
+  bool get_buf(DevBuf& b, size_t bytes) {
+    if (b.p && b.bytes >= bytes) return true;
+    if (!arena) return b.alloc(bytes);
+    const size_t off = (arena->used + 255) & ~size_t(255);
+    if (off + bytes > arena->size) return false;
+    b.view(arena->base + off, bytes);
+    arena->used = off + bytes;
+    return true;
+  }
+  bool zero_async(void* p, size_t bytes) {
+    if (arena) return launch_zero(p, bytes, stream) == hipSuccess;
+    return hipMemsetAsync(p, 0, bytes, stream) == hipSuccess;
+  }
+
   bool grid_factors(const HostGrid& h, GridDev<T>& gd, DevBuf sm[3], int& have, int interp) {
     const int k = spline_order_for(interp);
     if (have == k) return true;
@@ -512,7 +557,7 @@ struct Engine : EngineBase {
       for (size_t a = 0; a < ax.knots.size(); a++) hv[a] = (T)ax.knots[a];
       for (size_t a = 0; a < ax.band.size(); a++) hv[ax.knots.size() + a] = (T)ax.band[a];
       for (size_t a = 0; a < ax.poly.size(); a++) hv[ax.knots.size() + ax.band.size() + a] = (T)ax.poly[a];
-      if (!sm[d].alloc(nr * sizeof(T)) ||
+      if (!get_buf(sm[d], nr * sizeof(T)) ||
           hipMemcpy(sm[d].p, hv.data(), nr * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
         fail("upload of the spline factors failed");
         return false;
@@ -576,7 +621,7 @@ struct Engine : EngineBase {
   // kernel-level timing, on request only (it adds two events per launch)
   template <class F>
   void ktimed(int cat, F&& launch) {
-    if (!g_opt.kernel_timing) {
+    if (!g_opt.kernel_timing || arena) {
       launch();
       return;
     }
@@ -779,19 +824,18 @@ struct Engine : EngineBase {
     // per-component stride of the stage field (0 for the packed records)
     const size_t eb = stor == kStoreNative ? sizeof(T) : 0;
     const size_t fbytes_stage = stage_field_bytes(stor, sizeof(T), (size_t)g.nt);
-    Blend* victim = &blends[0];
+    Blend* victim = nullptr;
     for (auto& b : blends) {
       if (b.valid && b.ct == ct && b.storage == stor && b.spl == spl) {
         b.stamp = ++blend_clock;
         char* p = (char*)b.buf.p;
         return StageField{{p, p + eb * g.nt, p + 2 * eb * g.nt}};
       }
-      if (!b.valid || b.stamp < victim->stamp || !victim->valid) {
-        if (!b.valid && victim->valid) victim = &b;
-        else if (b.valid == victim->valid && b.stamp < victim->stamp) victim = &b;
-      }
+      // an allocation-free call reuses only the entries it gave workspace to
+      if (arena && !b.buf.p) continue;
+      if (!victim || (!b.valid && victim->valid) || (b.valid == victim->valid && b.stamp < victim->stamp)) victim = &b;
     }
-    if ((!victim->buf.p || victim->buf.bytes < fbytes_stage) && !victim->buf.alloc(3 * fbytes())) {
+    if (!victim || (!(victim->buf.p && victim->buf.bytes >= fbytes_stage) && !get_buf(victim->buf, 3 * fbytes()))) {
       fail("hipMalloc failed (blended velocity)");
       return whole(A);
     }
@@ -1024,7 +1068,7 @@ struct Engine : EngineBase {
   }
 
   bool fit_vel(int b) {
-    if (!vcoef[b].p && !vcoef[b].alloc(3 * fbytes())) {
+    if (!get_buf(vcoef[b], 3 * fbytes())) {
       fail("hipMalloc failed (velocity coefficients)");
       return false;
     }
@@ -1486,14 +1530,15 @@ struct Engine : EngineBase {
 
   bool ensure(DevBuf& b, size_t bytes, bool zero = false) {
     if (!b.p || b.bytes < bytes) {
-      if (!b.alloc(bytes)) {
-        fail("hipMalloc failed (adjoint buffers; out of device memory?)");
+      if (!get_buf(b, bytes)) {
+        fail(arena ? "the allocation-free workspace is too small for this call"
+                   : "hipMalloc failed (adjoint buffers; out of device memory?)");
         return false;
       }
       tape.bytes += bytes;
     }
-    if (zero && hipMemsetAsync(b.p, 0, bytes, stream) != hipSuccess) {
-      fail("hipMemset failed");
+    if (zero && !zero_async(b.p, bytes)) {
+      fail("clearing a device buffer failed");
       return false;
     }
     return true;
@@ -1514,7 +1559,10 @@ struct Engine : EngineBase {
     for (int pass = 0; pass < 2; pass++) {
       const int b = pass == 0 ? old : cur;
       const int snap = pass == 0 ? s - 1 : s;
-      HIPCHK_R(hipMemsetAsync(&st->cfl_bits, 0, sizeof(st->cfl_bits) + sizeof(st->umax_bits), stream), false);
+      if (!zero_async(&st->cfl_bits, sizeof(st->cfl_bits) + sizeof(st->umax_bits))) {
+        fail("clearing the status words failed");
+        return false;
+      }
       launch_ingest<U, T>(g, f3(vel[b]), tape.du + snap * np, tape.dv + snap * np, tape.dw + snap * np, stream);
       launch_vel_ghost(g, f3(vel[b]), cf3(vel[1 - b]), st, stream);
       launch_cfl(g, f3(vel[b]), st, stream);
@@ -1570,30 +1618,45 @@ struct Engine : EngineBase {
     return true;
   }
 
-  int adjoint_forward(int nsnap, const U* du, const U* dv, const U* dw, const U* times, int ckpt,
-                      double* d_ftle) override {
+  // arguments of a forward window call (both entry points)
+  bool window_args_ok(int nsnap, const U* du, const U* dv, const U* dw, const U* times, int ckpt, const char* who) {
     const int interp = g_opt.interpolator;
     if (!hg.rect) {
       fail("the adjoint pair on a non-rectilinear grid is not supported by this build");
-      return 1;
+      return false;
     }
     if (nsnap < 2 || !du || !dv || !dw || !times || ckpt < 1) {
-      fail("cfd2lcs_adjoint_forward_c: need at least 2 snapshots, device arrays, times and checkpoint >= 1");
-      return 1;
+      fail(std::string(who) + ": need at least 2 snapshots, device arrays, times and checkpoint >= 1");
+      return false;
     }
     if (g_opt.vel_storage != kStoreNative) {
       fail("the adjoint runs with VELOCITY_STORAGE equal to the compute precision");
-      return 1;
+      return false;
     }
     if (interp != kNearest && interp != kLinear && interp != kQuadratic && interp != kCubic) {
       fail("the adjoint supports INTERPOLATOR NEAREST_NBR, LINEAR, QUADRATIC and CUBIC");
-      return 1;
+      return false;
     }
     for (int s = 1; s < nsnap; s++)
       if (!(times[s] > times[s - 1])) {
-        fail("cfd2lcs_adjoint_forward_c: snapshot times must increase");
-        return 1;
+        fail(std::string(who) + ": snapshot times must increase");
+        return false;
       }
+    return true;
+  }
+
+  int adjoint_forward(int nsnap, const U* du, const U* dv, const U* dw, const U* times, int ckpt,
+                      double* d_ftle) override {
+    if (!window_args_ok(nsnap, du, dv, dw, times, ckpt, "cfd2lcs_adjoint_forward_c")) return 1;
+    return window_forward(nsnap, du, dv, dw, times, ckpt, d_ftle, true);
+  }
+
+  // the BKWD FTLE of one window of device snapshots (the diagnostic with
+  // h = T); keep: record the tape (checkpoints every ckpt steps) for
+  // adjoint_backward, else write the FTLE straight to d_ftle and keep nothing
+  int window_forward(int nsnap, const U* du, const U* dv, const U* dw, const U* times, int ckpt, double* d_ftle,
+                     bool keep) {
+    const int interp = g_opt.interpolator;
     auto t0 = std::chrono::steady_clock::now();
     tape.have = false;
     tape.fwd_done = false;
@@ -1611,12 +1674,12 @@ struct Engine : EngineBase {
     for (int b = 0; b < 2; b++)
       if (!ensure(tape.vel[b], fb, true) || !ensure(tape.fm[b], fb, true)) return 1;
     tape.fcur = 0;
-    const int nseg = (nsnap - 2) / ckpt + 1;
+    const int nseg = keep ? (nsnap - 2) / ckpt + 1 : 0;
     tape.ckpts.resize(nseg);
     swap_vel_state();
     bool ok = true;
     for (int s = 1; s < nsnap && ok; s++) {
-      if ((s - 1) % ckpt == 0) {
+      if (keep && (s - 1) % ckpt == 0) {
         DevBuf& c = tape.ckpts[(s - 1) / ckpt];
         ok = ensure(c, fb);
         if (ok) ok = hipMemcpyAsync(c.p, tape.fm[tape.fcur].p, fb, hipMemcpyDeviceToDevice, stream) == hipSuccess;
@@ -1642,14 +1705,14 @@ struct Engine : EngineBase {
     if (!ok) return 1;
     // FTLE of the window (the composition of a single slot is the slot itself)
     if (!ensure(tape.dxout, 3 * sizeof(T) * (size_t)g.np) || !ensure(tape.compfm, fb, true) ||
-        !ensure(tape.ftle, sizeof(double) * (size_t)g.np))
+        (keep && !ensure(tape.ftle, sizeof(double) * (size_t)g.np)))
       return 1;
+    double* fout = keep ? (double*)tape.ftle.p : d_ftle;
     launch_interior(g, cf3(tape.fm[tape.fcur]), p3(tape.dxout), stream);
     launch_field_from_origin(g, cp3(tape.dxout), f3(tape.compfm), stream);
     const double tiny = sizeof(U) == 8 ? DBL_MIN : (double)FLT_MIN;
-    launch_ftle(g, cf3(tape.compfm), (double)tape.Tint, g_opt.incompressible, tiny, (double*)tape.ftle.p,
-                p3(tape.dxout), stream);
-    if (d_ftle)
+    launch_ftle(g, cf3(tape.compfm), (double)tape.Tint, g_opt.incompressible, tiny, fout, p3(tape.dxout), stream);
+    if (keep && d_ftle)
       HIPCHK_R(hipMemcpyAsync(d_ftle, tape.ftle.p, sizeof(double) * (size_t)g.np, hipMemcpyDeviceToDevice, stream),
                1);
     HIPCHK_R(hipGetLastError(), 1);
@@ -1659,10 +1722,104 @@ struct Engine : EngineBase {
       fail("device error in the adjoint forward pass");
       return 1;
     }
-    tape.have = true;
-    tape.fwd_done = true;
+    tape.have = keep;
+    tape.fwd_done = keep;
     tape.t_fwd = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return 0;
+  }
+
+
+  // cfd2lcs_adjoint_forward_noalloc_c: the FTLE of cfd2lcs_adjoint_forward_c
+  // with every buffer carved from the fixed workspace of scratch.hip instead
+  // of allocated, and no tape kept.  The engine's own buffers (velocity ring,
+  // blend pool, status words, a tape of an earlier adjoint_forward) are set
+  // aside for the call and restored after it; spline factors already set up
+  // are reused, missing ones live in the workspace for the call only.
+  int adjoint_forward_noalloc(int nsnap, const U* du, const U* dv, const U* dw, const U* times,
+                              double* d_ftle) override {
+    if (!window_args_ok(nsnap, du, dv, dw, times, 1, "cfd2lcs_adjoint_forward_noalloc_c")) return 1;
+    if (!d_ftle) {
+      fail("cfd2lcs_adjoint_forward_noalloc_c: null FTLE pointer");
+      return 1;
+    }
+    void* ws = ws_ok ? scratch_ptr() : nullptr;
+    if (!ws) {
+      fail("cfd2lcs_adjoint_forward_noalloc_c: the device workspace was not set up on this device");
+      return 1;
+    }
+    const int interp = g_opt.interpolator;
+    const bool spl = is_spline(interp);
+    // workspace plan: status words, u_n / u_np1 (+ coefficients), three
+    // stage blends, the flowmap pair (+ coefficients), the composed field,
+    // the interior displacement, and spline factors if not yet set up
+    const size_t fb = 3 * fbytes();
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t need = al(sizeof(DevStatus)) + (2 + (spl ? 2 : 0) + 3 + 2 + (spl ? 1 : 0) + 1) * al(fb) +
+                  al(3 * sizeof(T) * (size_t)g.np);
+    const bool own_factors = spl && spline_k != spline_order_for(interp);
+    if (own_factors)
+      for (int d = 0; d < 3; d++) {
+        // the same sizes grid_factors uploads
+        SplineAxis ax;
+        const int kk = std::max(std::min(spline_order_for(interp), hg.N[d] - 1), 2);
+        if (!spline_axis(hg.xl[d], kk, ax).empty()) {
+          fail("cfd2lcs_adjoint_forward_noalloc_c: spline set-up failed");
+          return 1;
+        }
+        need += al(sizeof(T) * (ax.knots.size() + ax.band.size() + ax.poly.size()));
+      }
+    if (need > kScratchBytes) {
+      fail("cfd2lcs_adjoint_forward_noalloc_c: this grid needs " + std::to_string(need >> 20) +
+           " MiB of workspace, more than the fixed " + std::to_string(kScratchBytes >> 20) + " MiB");
+      return 1;
+    }
+    Arena ar;
+    ar.base = (char*)ws;
+    ar.size = kScratchBytes;
+    // set aside what the shared code would touch
+    Tape saved = std::move(tape);
+    tape = Tape();
+    DevStatus* st_saved = st;
+    DevBuf sm_saved[3];
+    GridDev<T> g_saved = g;
+    const int k_saved = spline_k;
+    const bool vc_saved[2] = {vcoef_ok[0], vcoef_ok[1]};
+    arena = &ar;
+    bool ok = true;
+    DevBuf stbuf;
+    ok = get_buf(stbuf, sizeof(DevStatus));
+    if (ok) {
+      st = (DevStatus*)stbuf.p;
+      ok = zero_async(st, sizeof(DevStatus));
+    }
+    for (int b = 0; b < 3 && ok; b++) ok = get_buf(tape.blends[b].buf, fb);
+    for (int b = 0; b < 2 && ok; b++) ok = get_buf(tape.vel[b], fb) && (!spl || get_buf(tape.vcoef[b], fb));
+    if (ok && own_factors) {
+      for (int d = 0; d < 3; d++) sm_saved[d].swap(spline_mem[d]);
+      spline_k = 0;
+    }
+    int rc = 1;
+    if (ok) rc = window_forward(nsnap, du, dv, dw, times, 1, d_ftle, false);
+    else fail("cfd2lcs_adjoint_forward_noalloc_c: workspace plan exceeded");
+    // everything above ran on the stream; drain it before the workspace and
+    // the status words are handed back
+    (void)hipStreamSynchronize(stream);
+    arena = nullptr;
+    st = st_saved;
+    if (own_factors) {
+      for (int d = 0; d < 3; d++) sm_saved[d].swap(spline_mem[d]);
+      g.kord[0] = g_saved.kord[0]; g.kord[1] = g_saved.kord[1]; g.kord[2] = g_saved.kord[2];
+      for (int d = 0; d < 3; d++) {
+        g.kn[d] = g_saved.kn[d];
+        g.band[d] = g_saved.band[d];
+        g.bpoly[d] = g_saved.bpoly[d];
+      }
+      spline_k = k_saved;
+      vcoef_ok[0] = vc_saved[0];
+      vcoef_ok[1] = vc_saved[1];
+    }
+    tape = std::move(saved);
+    return rc;
   }
 
   int adjoint_backward(const double* d_dftle, U* gu, U* gv, U* gw) override {
@@ -2176,6 +2333,22 @@ int cfd2lcs_adjoint_forward_c(int n[3], int nsnap, const lcsdata_t* d_u, const l
     return 1;
   }
   return g_eng->adjoint_forward(nsnap, d_u, d_v, d_w, times, checkpoint_every, d_ftle);
+}
+
+int cfd2lcs_adjoint_forward_noalloc_c(int n[3], int nsnap, const lcsdata_t* d_u, const lcsdata_t* d_v,
+                                      const lcsdata_t* d_w, const lcsdata_t* times, double* d_ftle) {
+  if (g_error) return 1;
+  // the engine (device grid, stream, status words) exists from
+  // cfd2lcs_init_c on; this entry point never creates it
+  if (!g_eng) {
+    fail("cfd2lcs_adjoint_forward_noalloc_c: call cfd2lcs_init_c first");
+    return 1;
+  }
+  if (!n || n[0] != g_grid->n[0] || n[1] != g_grid->n[1] || n[2] != g_grid->n[2]) {
+    fail("received velocity array of incorrect dimension");
+    return 1;
+  }
+  return g_eng->adjoint_forward_noalloc(nsnap, d_u, d_v, d_w, times, d_ftle);
 }
 
 int cfd2lcs_adjoint_backward_c(const double* d_dftle, lcsdata_t* d_gu, lcsdata_t* d_gv, lcsdata_t* d_gw) {

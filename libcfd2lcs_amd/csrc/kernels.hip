@@ -1219,6 +1219,25 @@ bool dispatch_spline(const GridDev<T>& g, F&& f) {
   return false;
 }
 
+// zero fill: 16-byte words, then the tail bytes
+__global__ void __launch_bounds__(kBlock) k_zero(unsigned char* p, size_t bytes) {
+  const size_t nw = bytes / 16;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += stride)
+    reinterpret_cast<uint4*>(p)[w] = make_uint4(0u, 0u, 0u, 0u);
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < bytes - nw * 16) p[nw * 16 + t] = 0;
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+  if (!bytes) return hipSuccess;
+  if ((reinterpret_cast<uintptr_t>(p) & 15) != 0) return hipErrorInvalidValue;
+  const size_t nw = bytes / 16;
+  const unsigned nb = (unsigned)std::max<size_t>(1, std::min<size_t>((nw + kBlock - 1) / kBlock, 4096));
+  hipLaunchKernelGGL(k_zero, dim3(nb), dim3(kBlock), 0, s, (unsigned char*)p, bytes);
+  return hipGetLastError();
+}
+
 template <typename T>
 void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a, CF3<T> b, T ct, T inv,
                   DevStatus* st, hipStream_t s) {

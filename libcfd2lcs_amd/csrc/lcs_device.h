@@ -117,6 +117,8 @@ struct ParticleSet {
 };
 
 // ---- launchers (kernels.hip, particles.hip); all asynchronous on `stream` ----
+// zero `bytes` bytes at p (a fill kernel: usable where a memset node is not)
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t stream);
 template <typename U, typename T>
 void launch_ingest(const GridDev<T>& g, F3<T> dst, const U* u, const U* v, const U* w,
                    hipStream_t stream);
