@@ -26,7 +26,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "flowmap grid-pt updates/s + FTLE fields/s at 256³ (fp32 vs fp64); bwd/fwd ratio"
 HBM_PEAK_GBS = 8000.0
-TIMER_NAMES = {8: "k_fwd_step", 9: "k_sl_step", 10: "k_fit_axis", 11: "k_compose", 12: "k_ftle"}
+TIMER_NAMES = {8: "k_fwd_step", 9: "k_sl_step", 10: "k_fit_axis", 11: "k_compose_slot", 12: "k_ftle",
+               16: "compose_sort"}
 
 
 def abc_grid(n):
@@ -399,11 +400,11 @@ def precision_study(n, steps):
 
 def timers(lib):
     import ctypes as C
-    sec = (C.c_double * 16)()
-    cnt = (C.c_longlong * 16)()
+    sec = (C.c_double * 17)()
+    cnt = (C.c_longlong * 17)()
     lib.L.cfd2lcs_get_timers_c.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]
     lib.L.cfd2lcs_get_timers_c.restype = C.c_int
-    lib.L.cfd2lcs_get_timers_c(sec, cnt, 16)
+    lib.L.cfd2lcs_get_timers_c(sec, cnt, 17)
     return np.array(list(sec)), np.array(list(cnt))
 
 

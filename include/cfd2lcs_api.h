@@ -178,7 +178,8 @@ void cfd2lcs_get_stats_c(double out[8]);
  * and, when the option "KERNEL_TIMING" is LCS_TRUE, per kernel:
  *   8 forward particle step   9 semi-Lagrangian step   10 spline fit
  *   11 composition kernel    12 FTLE kernel           13 ghost/ingest/CFL
- *   14 time blend of the stage velocities  15 forward particle sort */
+ *   14 time blend of the stage velocities  15 forward particle sort
+ *   16 composition sort (between the per-slot composition launches) */
 int cfd2lcs_get_timers_c(double *seconds, long long *counts, int n);
 
 /* Host-side grid set-up, for inspection: lines[N0+N1+N2] (ghosted

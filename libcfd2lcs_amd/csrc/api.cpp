@@ -56,6 +56,9 @@ struct Options {
   int precision = sizeof(U) == 8 ? LCS_FP64 : LCS_FP32;
   int kernel_timing = 0;  // time every hot kernel launch with HIP events
   int particle_sort = 1;  // re-order forward particles by node every n updates (0: never)
+  // composition: one launch per slot with the particles re-ordered by brick
+  // before every n-th slot (0: the single-launch composition)
+  int compose_sort = 1;
   int vel_storage = kStoreNative;  // storage of the gathered stage velocities
   int compensated = 0;    // two-sum position/displacement updates of forward particles
   int sl_tiled = 0;       // LDS-tiled semi-Lagrangian B-spline step (results identical; slower here)
@@ -210,8 +213,8 @@ struct Engine : EngineBase {
   // timers: 0 FWD advect, 1 BKWD advect, 2 composition, 3 FTLE (phases);
   // with KERNEL_TIMING: 8 k_fwd_step, 9 k_sl_step, 10 spline fit,
   // 11 k_compose, 12 k_ftle, 13 ghost/ingest/CFL kernels, 14 time blend,
-  // 15 forward particle sort
-  enum { kTimers = 16 };
+  // 15 forward particle sort, 16 composition sort
+  enum { kTimers = 17 };
   double tsum[kTimers] = {};
   long long tcount[kTimers] = {};
   std::vector<hipEvent_t> evpool;
@@ -295,6 +298,9 @@ struct Engine : EngineBase {
   DevBuf invbuf;  // velocity invariants, 5 * np
   DevBuf uscratch[2];  // velocity at T0 of a diagnostic grid: ghosted field on
                        // the flow grid (and its spline fit), then 3 * np
+  // the re-ordered composition's particle records, brick keys / positions
+  // and sort scratch, shared by all sets (sized for the largest)
+  DevBuf crec[2], ckeys, cvals, ctemp;
 
   ~Engine() override {
     if (writer) writer->flush();
@@ -1131,12 +1137,67 @@ struct Engine : EngineBase {
       if (d.slot_ok[s]) order.push_back(s);
       s += inc;
     }
-    HIPCHK(hipMemcpyAsync(d.order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream));
-    ktimed(11, [&] {
-      launch_compose(lg, interp, (const T*)(spl ? d.slotcoef.p : d.slots.p), nt, (const int*)d.order.p,
-                     (int)order.size(), p3(d.dxout, np), st, stream);
-    });
+    if (g_opt.compose_sort > 0 && !order.empty()) {
+      if (!compose_sorted(d, interp, (const T*)(spl ? d.slotcoef.p : d.slots.p), order)) return;
+    } else {
+      HIPCHK(hipMemcpyAsync(d.order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, stream));
+      ktimed(11, [&] {
+        launch_compose(lg, interp, (const T*)(spl ? d.slotcoef.p : d.slots.p), nt, (const int*)d.order.p,
+                       (int)order.size(), p3(d.dxout, np), st, stream);
+      });
+    }
     ktimed(13, [&] { launch_field_from_origin(lg, cp3(d.dxout, np), f3(d.compfm, nt), stream); });
+  }
+
+  // the composition one slot per launch (k_compose_slot), the particles
+  // re-ordered by the brick of their tracked node before every
+  // COMPOSE_SORT-th slot; same results as k_compose
+  bool compose_sorted(Diag& d, int interp, const T* base, const std::vector<int>& order) {
+    GridDev<T>& lg = G(d);
+    const long long np = NP(d), nt = NT(d);
+    const int ns = (int)order.size();
+    const int every = g_opt.compose_sort;
+    // the brick keys of k_compose_slot (comp_key): 8 x 8 x 4 nodes
+    const long long nbricks = (long long)((lg.n[0] + 7) / 8) * ((lg.n[1] + 7) / 8) * ((lg.n[2] + 3) / 4);
+    const size_t rb = sizeof(CompRec<T>) * (size_t)np;
+    bool ok = ns < 2 || (crec[0].ensure(rb) && crec[1].ensure(rb));
+    const bool sorts = ns > 1 && every <= ns - 1;
+    size_t tb = 0;
+    if (ok && sorts) {
+      tb = compose_sort_temp_bytes(np, nbricks);
+      ok = tb && ckeys.ensure(2 * sizeof(unsigned) * (size_t)np) && cvals.ensure(2 * sizeof(unsigned) * (size_t)np) &&
+           ctemp.ensure(tb);
+    }
+    if (!ok) {
+      fail("hipMalloc failed (composition records)");
+      return false;
+    }
+    int cur = 0;
+    const unsigned* perm = nullptr;
+    for (int s = 0; s < ns; s++) {
+      const T* sb = base + (size_t)order[s] * 3 * (size_t)nt;
+      const CF3<T> f{{sb, sb + nt, sb + 2 * nt}};
+      const bool last = s == ns - 1;
+      const bool sort_next = !last && (s + 1) % every == 0;
+      unsigned* keys = sort_next ? (unsigned*)ckeys.p : nullptr;
+      unsigned* vals = sort_next ? (unsigned*)cvals.p : nullptr;
+      ktimed(11, [&] {
+        launch_compose_slot(lg, interp, f, (const CompRec<T>*)crec[cur].p, perm, (CompRec<T>*)crec[1 - cur].p, keys,
+                            vals, s == 0 ? 1 : 0, last ? 1 : 0, p3(d.dxout, np), st, stream);
+      });
+      if (last) break;
+      perm = nullptr;
+      if (sort_next) {
+        ktimed(16, [&] { perm = launch_compose_sort(keys, vals, np, nbricks, ctemp.p, tb, stream); });
+        if (!perm) {
+          fail("composition sort failed");
+          return false;
+        }
+      }
+      cur = 1 - cur;
+    }
+    HIPCHK_R(hipGetLastError(), false);
+    return true;
   }
 
   // the diagnostic at an h boundary (cfd2lcs.f90:147-213): every set's
@@ -2226,6 +2287,9 @@ void cfd2lcs_set_option_c(char option[], int val) {
   } else if (o == "PARTICLE_SORT") {
     if (val < 0) warn = true;
     else g_opt.particle_sort = val;
+  } else if (o == "COMPOSE_SORT") {
+    if (val < 0) warn = true;
+    else g_opt.compose_sort = val;
   } else if (o == "UPDATE_FREQ") {
   } else if (o == "INTERPOLATOR") {
     if (val != NEAREST_NBR && val != LINEAR && val != QUADRATIC && val != CUBIC && val != TSE && val != TSE_LIMIT)

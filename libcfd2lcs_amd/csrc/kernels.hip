@@ -729,6 +729,110 @@ __global__ void __launch_bounds__(kBlock) k_field_at_nodes(GridDev<T> g, GridDev
 // periodic image / node guess) and track recursively with a periodic wrap
 // after each pass (the direction is FWD during reconstruction).
 // ---------------------------------------------------------------------------
+// One slot of the composition for one particle: the body of
+// reconstruct_flowmap's slot loop (flowmap_m.f90:262-308) -- BC, the slot's
+// displacement at the particle, relocation into the bounding box or its
+// first periodic image (exchange_lp_alltoall, comms_m.f90:1378-1709), the
+// node guess and recursive re-tracking.  q is sq.qb + sq.off on entry and
+// on return.
+template <bool NR, typename T>
+__device__ __forceinline__ void compose_bc(const GridDev<T>& g, T q[3], Split<T>& sq, int nd[3], int& err) {
+  const T qr[3] = {q[0], q[1], q[2]};
+  project_flat<NR>(g, nd, q);
+  set_lp_bc<NR>(g, nd, q, err);
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+    if (q[d] != qr[d]) {
+      sq.qb[d] = q[d];
+      sq.off[d] = T(0);
+    }
+}
+
+template <bool NR, typename T>
+__device__ __forceinline__ void compose_after(const GridDev<T>& g, const T up[3], T q[3], Split<T>& sq, int nd[3],
+                                              T dxx[3], int& err) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    sq.off[d] = sq.off[d] + up[d];
+    q[d] = sq.qb[d] + sq.off[d];
+    dxx[d] = dxx[d] + up[d];
+  }
+  // relocation to the bounding box or its first periodic image
+  const T* bb = g.bb;
+  bool in = q[0] >= bb[0] && q[0] <= bb[3] && q[1] >= bb[1] && q[1] <= bb[4] && q[2] >= bb[2] && q[2] <= bb[5];
+  if (!in) {
+    for (int c = 0; c < 27 && !in; c++) {
+      const T y0 = q[0] - g.ps[c][0], y1 = q[1] - g.ps[c][1], y2 = q[2] - g.ps[c][2];
+      if (y0 >= bb[0] && y0 <= bb[3] && y1 >= bb[1] && y1 <= bb[4] && y2 >= bb[2] && y2 <= bb[5]) {
+        const T y[3] = {y0, y1, y2};
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+          if (y[d] != q[d]) {
+            sq.qb[d] = y[d];
+            sq.off[d] = T(0);
+          }
+        q[0] = y0; q[1] = y1; q[2] = y2;
+        in = true;
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    T sv = clamp01((q[d] - bb[d]) / (bb[3 + d] - bb[d]));
+    T v = (T)(float)g.n[d] * sv;
+    nd[d] = iclamp((int)round(v), 1, g.n[d]);
+  }
+  if constexpr (NR) {
+    // the node guess of a non-rectilinear grid from the positions of the
+    // coordinate extrema (comms_m.f90:1666-1681)
+    T stu[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) stu[d] = clamp01((q[d] - bb[d]) / (bb[3 + d] - bb[d]));
+    const T third = T(1) / T(3);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const T a = third * (g.pmin[0][d] + g.pmin[1][d] + g.pmin[2][d]);
+      const T b = third * (stu[0] * (g.pmax[0][d] - g.pmin[0][d]) + stu[1] * (g.pmax[1][d] - g.pmin[1][d]) +
+                           stu[2] * (g.pmax[2][d] - g.pmin[2][d]));
+      nd[d] = iclamp((int)round(a) + (int)round(b), 1, g.n[d]);
+    }
+  }
+  for (int pass = 1;; pass++) {
+    bool moved = track_pass<NR>(g, q, nd);
+    if (pass > g.max_track) {
+      err |= kErrTrack;
+      break;
+    }
+    T sh[3];
+    wrap(g, nd, q, sh);
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      if (sh[d] != T(0)) {
+        sq.qb[d] = q[d];
+        sq.off[d] = T(0);
+      }
+    if (!moved) break;
+  }
+}
+
+// One slot of the composition for one particle: the body of
+// reconstruct_flowmap's slot loop (flowmap_m.f90:262-308) -- BC, the slot's
+// displacement at the particle, relocation into the bounding box or its
+// first periodic image (exchange_lp_alltoall, comms_m.f90:1378-1709), the
+// node guess and recursive re-tracking.  q is sq.qb + sq.off on entry and
+// on return.
+template <typename T, class I>
+__device__ __forceinline__ void compose_slot(const GridDev<T>& g, const CF3<T>& f, T q[3], Split<T>& sq, int nd[3],
+                                             T dxx[3], unsigned long long& oob, int& err) {
+  constexpr bool NR = I::kNonRect;
+  compose_bc<NR>(g, q, sq, nd, err);
+  T up[3];
+  I{}(g, q, nd, f, up, oob, &sq);
+  compose_after<NR>(g, up, q, sq, nd, dxx, err);
+}
+
+// The whole composition in one launch: one thread follows one origin node
+// through every slot.
 template <typename T, class I>
 __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __restrict__ slots,
                                                     long long stride, const int* __restrict__ order, int nslot,
@@ -752,83 +856,102 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev<T> g, const T* __res
   for (int s = 0; s < nslot; s++) {
     const T* base = slots + (long long)order[s] * 3 * stride;
     const CF3<T> f{{base, base + stride, base + 2 * stride}};
-    const T qr[3] = {q[0], q[1], q[2]};
-    project_flat<NR>(g, nd, q);
-    set_lp_bc<NR>(g, nd, q, err);
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-      if (q[d] != qr[d]) {
-        sq.qb[d] = q[d];
-        sq.off[d] = T(0);
-      }
-    T up[3];
-    I{}(g, q, nd, f, up, oob, &sq);
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      sq.off[d] = sq.off[d] + up[d];
-      q[d] = sq.qb[d] + sq.off[d];
-      dxx[d] = dxx[d] + up[d];
-    }
-    // relocation to the bounding box or its first periodic image
-    const T* bb = g.bb;
-    bool in = q[0] >= bb[0] && q[0] <= bb[3] && q[1] >= bb[1] && q[1] <= bb[4] && q[2] >= bb[2] && q[2] <= bb[5];
-    if (!in) {
-      for (int c = 0; c < 27 && !in; c++) {
-        const T y0 = q[0] - g.ps[c][0], y1 = q[1] - g.ps[c][1], y2 = q[2] - g.ps[c][2];
-        if (y0 >= bb[0] && y0 <= bb[3] && y1 >= bb[1] && y1 <= bb[4] && y2 >= bb[2] && y2 <= bb[5]) {
-          const T y[3] = {y0, y1, y2};
-#pragma unroll
-          for (int d = 0; d < 3; d++)
-            if (y[d] != q[d]) {
-              sq.qb[d] = y[d];
-              sq.off[d] = T(0);
-            }
-          q[0] = y0; q[1] = y1; q[2] = y2;
-          in = true;
-        }
-      }
-    }
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      T sv = clamp01((q[d] - bb[d]) / (bb[3 + d] - bb[d]));
-      T v = (T)(float)g.n[d] * sv;
-      nd[d] = iclamp((int)round(v), 1, g.n[d]);
-    }
-    if constexpr (NR) {
-      // the node guess of a non-rectilinear grid from the positions of the
-      // coordinate extrema (comms_m.f90:1666-1681)
-      T stu[3];
-#pragma unroll
-      for (int d = 0; d < 3; d++) stu[d] = clamp01((q[d] - bb[d]) / (bb[3 + d] - bb[d]));
-      const T third = T(1) / T(3);
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        const T a = third * (g.pmin[0][d] + g.pmin[1][d] + g.pmin[2][d]);
-        const T b = third * (stu[0] * (g.pmax[0][d] - g.pmin[0][d]) + stu[1] * (g.pmax[1][d] - g.pmin[1][d]) +
-                             stu[2] * (g.pmax[2][d] - g.pmin[2][d]));
-        nd[d] = iclamp((int)round(a) + (int)round(b), 1, g.n[d]);
-      }
-    }
-    for (int pass = 1;; pass++) {
-      bool moved = track_pass<NR>(g, q, nd);
-      if (pass > g.max_track) {
-        err |= kErrTrack;
-        break;
-      }
-      T sh[3];
-      wrap(g, nd, q, sh);
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-        if (sh[d] != T(0)) {
-          sq.qb[d] = q[d];
-          sq.off[d] = T(0);
-        }
-      if (!moved) break;
-    }
+    compose_slot<T, I>(g, f, q, sq, nd, dxx, oob, err);
   }
   dxout.c[0][p] = dxx[0];
   dxout.c[1][p] = dxx[1];
   dxout.c[2][p] = dxx[2];
+  flush_status(st, err, oob);
+}
+
+// The composition one slot per launch.  Between slots the particles are
+// re-ordered by the brick (8^3 nodes) of their tracked node, so that the 64
+// particles of a wavefront gather one slot's interpolation stencils from a
+// few neighbouring bricks instead of from wherever their trajectories have
+// dispersed to.  Each particle's arithmetic is exactly that of k_compose (the
+// state between slots is its split position, node, displacement and origin),
+// so the order changes no result.
+// the first slot's particle order: whole bricks of 8 x 8 x 4 nodes (one
+// workgroup each) where the grid divides into them, else slab order
+template <typename T>
+__device__ __forceinline__ void decode_comp_first(const GridDev<T>& g, int t, int& i, int& j, int& k) {
+  if ((g.n[0] & 7) == 0 && (g.n[1] & 7) == 0 && (g.n[2] & 3) == 0) {
+    const int b = t >> 8, l = t & 255, nbx = g.n[0] >> 3, nby = g.n[1] >> 3;
+    i = (b % nbx) * 8 + (l & 7) + 1;
+    j = ((b / nbx) % nby) * 8 + ((l >> 3) & 7) + 1;
+    k = (b / (nbx * nby)) * 4 + (l >> 6) + 1;
+  } else {
+    decode_interior_slab(g, t, i, j, k);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ unsigned comp_key(const GridDev<T>& g, const int nd[3]) {
+  // bricks of 8 x 8 x 4 nodes: about one workgroup's worth of particles
+  const int i = (iclamp(nd[0], 1, g.n[0]) - 1) >> 3, j = (iclamp(nd[1], 1, g.n[1]) - 1) >> 3,
+            k = (iclamp(nd[2], 1, g.n[2]) - 1) >> 2;
+  const int nbx = (g.n[0] + 7) >> 3, nby = (g.n[1] + 7) >> 3;
+  return (unsigned)((k * nby + j) * nbx + i);
+}
+
+template <typename T, class I>
+__global__ void __launch_bounds__(kBlock) k_compose_slot(GridDev<T> g, CF3<T> f, const CompRec<T>* __restrict__ in,
+                                                         const unsigned* __restrict__ perm, CompRec<T>* __restrict__ out,
+                                                         unsigned* __restrict__ keys, unsigned* __restrict__ vals,
+                                                         int first, int last, F3<T> dxout, DevStatus* st) {
+  constexpr bool NR = I::kNonRect;
+  const int t = xcd_thread_index();
+  if (t >= g.np) return;
+  T q[3], dxx[3];
+  int nd[3], org;
+  Split<T> sq;
+  if (first) {
+    int i, j, k;
+    decode_comp_first(g, t, i, j, k);
+    nd[0] = i; nd[1] = j; nd[2] = k;
+    node_pos<NR>(g, i, j, k, q);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      sq.qb[d] = q[d];
+      sq.off[d] = T(0);
+      dxx[d] = T(0);
+    }
+    org = pid(g, i, j, k);
+  } else {
+    const CompRec<T> r = in[perm ? perm[t] : (unsigned)t];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      sq.qb[d] = r.qb[d];
+      sq.off[d] = r.off[d];
+      q[d] = r.qb[d] + r.off[d];
+      dxx[d] = r.dx[d];
+      nd[d] = r.nd[d];
+    }
+    org = r.org;
+  }
+  unsigned long long oob = 0;
+  int err = 0;
+  compose_slot<T, I>(g, f, q, sq, nd, dxx, oob, err);
+  if (last) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) dxout.c[d][org] = dxx[d];
+  } else {
+    CompRec<T> r;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      r.qb[d] = sq.qb[d];
+      r.off[d] = sq.off[d];
+      r.dx[d] = dxx[d];
+      r.nd[d] = (int16_t)nd[d];
+    }
+    r.pad = 0;
+    r.org = org;
+    out[t] = r;
+    if (keys) {
+      keys[t] = comp_key(g, nd);
+      vals[t] = (unsigned)t;
+    }
+  }
   flush_status(st, err, oob);
 }
 
@@ -1464,6 +1587,184 @@ void launch_compose(const GridDev<T>& g, int interp, const T* slots, long long s
   }
 }
 
+// The slot kernel for the B-spline interpolants on a rectilinear grid: the
+// workgroup's particles, sorted by brick, need the coefficients of a small
+// box of nodes, which the workgroup loads into LDS once (coalesced along x)
+// and every particle then sums its stencil from there instead of issuing
+// its K^2 KZ scattered row loads through the texture path.  The box is the
+// union of the particles' stencils; a workgroup whose box exceeds the LDS
+// capacity (particles spread over many bricks) gathers from global memory.
+// The sums run in spline_sum's order: results equal k_compose_slot's.
+template <typename T>
+constexpr int comp_tile_cap() {
+  return sizeof(T) == 4 ? 2048 : 1024;
+}
+
+template <typename T, int KX, int KZ>
+__global__ void __launch_bounds__(kBlock) k_compose_slot_tiled(GridDev<T> g, CF3<T> f, const CompRec<T>* __restrict__ in,
+                                                               const unsigned* __restrict__ perm,
+                                                               CompRec<T>* __restrict__ out, unsigned* __restrict__ keys,
+                                                               unsigned* __restrict__ vals, int first, int last,
+                                                               F3<T> dxout, DevStatus* st) {
+  constexpr int CAP = comp_tile_cap<T>();
+  __shared__ T tile[3 * CAP];
+  __shared__ int box[6];
+  const int t = xcd_thread_index();
+  const bool active = t < g.np;
+  if (threadIdx.x < 3) {
+    box[threadIdx.x] = INT_MAX;
+    box[3 + threadIdx.x] = INT_MIN;
+  }
+  __syncthreads();
+  T q[3] = {T(0), T(0), T(0)}, dxx[3] = {T(0), T(0), T(0)};
+  int nd[3] = {1, 1, 1}, org = 0;
+  Split<T> sq{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
+  unsigned long long oob = 0;
+  int err = 0;
+  SplineW<T, KX, KZ> w;
+  int r = 3;  // 3: no particle
+  if (active) {
+    if (first) {
+      int i, j, k;
+      decode_comp_first(g, t, i, j, k);
+      nd[0] = i; nd[1] = j; nd[2] = k;
+      node_pos<false>(g, i, j, k, q);
+#pragma unroll
+      for (int d = 0; d < 3; d++) sq.qb[d] = q[d];
+      org = pid(g, i, j, k);
+    } else {
+      const CompRec<T> rr = in[perm ? perm[t] : (unsigned)t];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        sq.qb[d] = rr.qb[d];
+        sq.off[d] = rr.off[d];
+        q[d] = rr.qb[d] + rr.off[d];
+        dxx[d] = rr.dx[d];
+        nd[d] = rr.nd[d];
+      }
+      org = rr.org;
+    }
+    compose_bc<false>(g, q, sq, nd, err);
+    r = spline_setup(g, q, nd, w, &sq);
+  }
+  // the box of coefficient indices the workgroup's stencils start in
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    lo[d] = r == 0 ? w.b3[d] : INT_MAX;
+    hi[d] = r == 0 ? w.b3[d] : INT_MIN;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      atomicMin(&box[d], lo[d]);
+      atomicMax(&box[3 + d], hi[d]);
+    }
+  }
+  __syncthreads();
+  const int b0 = box[0], b1 = box[1], b2 = box[2];
+  const int ex = box[3] - b0 + KX, ey = box[4] - b1 + KX, ez = box[5] - b2 + KZ;
+  const bool staged = box[3] >= b0 && (long long)ex * ey * ez <= CAP;
+  if (staged) {
+    const int vol = ex * ey * ez;
+    for (int e = threadIdx.x; e < vol; e += blockDim.x) {
+      const int x = e % ex, yz = e / ex, y = yz % ey, z = yz / ey;
+      const long long gi = (long long)(b0 + x) + (long long)g.N[0] * ((b1 + y) + (long long)g.N[1] * (b2 + z));
+#pragma unroll
+      for (int c = 0; c < 3; c++) tile[c * CAP + e] = f.c[c][gi];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  T up[3];
+  if (r) {
+    if (r == 1) oob++;
+    up[0] = up[1] = up[2] = T(0);
+  } else if (staged) {
+    const int ox = w.b3[0] - b0, oy = w.b3[1] - b1, oz = w.b3[2] - b2;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const T* tc = tile + c * CAP;
+      T acc = T(0);
+#pragma unroll
+      for (int cz = 0; cz < KZ; cz++) {
+        T accy = T(0);
+#pragma unroll
+        for (int by = 0; by < KX; by++) {
+          const T* row = tc + ((oz + cz) * ey + (oy + by)) * ex + ox;
+          T accx = T(0);
+#pragma unroll
+          for (int a = 0; a < KX; a++) accx += w.wx[a] * row[a];
+          accy += w.wy[by] * accx;
+        }
+        acc += w.wz[cz] * accy;
+      }
+      up[c] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; c++) up[c] = spline_sum(g, w, f.c[c]);
+  }
+  compose_after<false>(g, up, q, sq, nd, dxx, err);
+  if (last) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) dxout.c[d][org] = dxx[d];
+  } else {
+    CompRec<T> rr;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      rr.qb[d] = sq.qb[d];
+      rr.off[d] = sq.off[d];
+      rr.dx[d] = dxx[d];
+      rr.nd[d] = (int16_t)nd[d];
+    }
+    rr.pad = 0;
+    rr.org = org;
+    out[t] = rr;
+    if (keys) {
+      keys[t] = comp_key(g, nd);
+      vals[t] = (unsigned)t;
+    }
+  }
+  flush_status(st, err, oob);
+}
+
+template <typename T>
+void launch_compose_slot(const GridDev<T>& g, int interp, CF3<T> f, const CompRec<T>* in, const unsigned* perm,
+                         CompRec<T>* out, unsigned* keys, unsigned* vals, int first, int last, F3<T> dxout,
+                         DevStatus* st, hipStream_t s) {
+  const dim3 grid = grid_for(g.np), blk(kBlock);
+  auto go = [&](auto in_) {
+    using I = decltype(in_);
+    hipLaunchKernelGGL((k_compose_slot<T, I>), grid, blk, 0, s, g, f, in, perm, out, keys, vals, first, last, dxout,
+                       st);
+  };
+  if (dispatch_nonrect(g, interp, go)) {
+  } else if (interp == kLinear) {
+    go(IntLinear<T>{});
+  } else if (interp == kNearest) {
+    go(IntNearest<T>{});
+  } else if (interp == kTSE) {
+    go(IntTSE<T, false>{});
+  } else if (interp == kTSELimit) {
+    go(IntTSE<T, true>{});
+  } else {
+    dispatch_spline(g, [&](auto KX, auto KZ) {
+      constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
+      hipLaunchKernelGGL((k_compose_slot_tiled<T, kx, kz>), grid, blk, 0, s, g, f, in, perm, out, keys, vals, first,
+                         last, dxout, st);
+    });
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Velocity invariants (compute_invariants, invariants_m.f90:15-198) of the
 // velocity at T0 (lp%ugrid), one node per thread.  u holds the interior
@@ -1613,6 +1914,9 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
                                    hipStream_t);                                                              \
   template void launch_compose<T>(const GridDev<T>&, int, const T*, long long, const int*, int, F3<T>,       \
                                   DevStatus*, hipStream_t);                                                   \
+  template void launch_compose_slot<T>(const GridDev<T>&, int, CF3<T>, const CompRec<T>*, const unsigned*,     \
+                                       CompRec<T>*, unsigned*, unsigned*, int, int, F3<T>, DevStatus*,        \
+                                       hipStream_t);                                                          \
   template void launch_ftle<T>(const GridDev<T>&, CF3<T>, double, int, double, double*, F3<T>, hipStream_t); \
   template void launch_invariants<T>(const GridDev<T>&, const T*, T*, hipStream_t);                           \
   template void launch_ingest<float, T>(const GridDev<T>&, F3<T>, const float*, const float*, const float*,  \

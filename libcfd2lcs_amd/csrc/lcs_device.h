@@ -204,6 +204,28 @@ void launch_field_from_origin(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStr
 template <typename T>
 void launch_compose(const GridDev<T>& g, int interp, const T* slots, long long slot_stride,
                     const int* order, int nslot, F3<T> dxout, DevStatus* st, hipStream_t stream);
+// composition state of one particle between slots: split position, tracked
+// node, accumulated displacement, origin (48 bytes in FP32, 96 in FP64)
+template <typename T>
+struct __attribute__((aligned(16))) CompRec {
+  T qb[3], off[3], dx[3];
+  int16_t nd[3];
+  int16_t pad;
+  int org;
+};
+// one slot of the re-ordered composition: particles from in (through perm
+// when given, else in place; first: seeded on the nodes) through the slot
+// field f, written to out with their brick keys and positions (keys may be
+// null), or, on the last slot, their displacement scattered by origin
+template <typename T>
+void launch_compose_slot(const GridDev<T>& g, int interp, CF3<T> f, const CompRec<T>* in, const unsigned* perm,
+                         CompRec<T>* out, unsigned* keys, unsigned* vals, int first, int last, F3<T> dxout,
+                         DevStatus* st, hipStream_t stream);
+// radix sort of np (brick key, position) pairs held in double buffers of 2 np
+// values each; returns the positions in key order (null on failure)
+size_t compose_sort_temp_bytes(long long np, long long nkeys);
+const unsigned* launch_compose_sort(unsigned* keys, unsigned* vals, long long np, long long nkeys, void* temp,
+                                    size_t temp_bytes, hipStream_t stream);
 // velocity invariants Q, C, D, H, L2 (5 * np) of the interior velocity u
 // (3 * np, or null for zero) at T0
 template <typename T>

@@ -88,6 +88,23 @@ int launch_particle_sort(const GridDev<T>& g, ParticleSet<T> a, ParticleSet<T> b
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// brick-key sort of the re-ordered composition (kernels.hip k_compose_slot)
+size_t compose_sort_temp_bytes(long long np, long long nkeys) {
+  size_t bytes = 0;
+  hipcub::DoubleBuffer<unsigned> k(nullptr, nullptr), v(nullptr, nullptr);
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)np, 0, key_bits(nkeys)) != hipSuccess) return 0;
+  return bytes;
+}
+
+const unsigned* launch_compose_sort(unsigned* keys, unsigned* vals, long long npl, long long nkeys, void* temp,
+                                    size_t temp_bytes, hipStream_t s) {
+  const int np = (int)npl;
+  hipcub::DoubleBuffer<unsigned> kb(keys, keys + np), vb(vals, vals + np);
+  size_t tb = temp_bytes;
+  if (hipcub::DeviceRadixSort::SortPairs(temp, tb, kb, vb, np, 0, key_bits(nkeys), s) != hipSuccess) return nullptr;
+  return vb.Current();
+}
+
 template <typename T>
 void launch_unpermute(long long npl, const int* org, CF3<T> src, const T* const lo[3], F3<T> dst, hipStream_t s) {
   const int np = (int)npl;
