@@ -59,6 +59,9 @@ struct Options {
   // composition: one launch per slot with the particles re-ordered by brick
   // before every n-th slot (0: the single-launch composition)
   int compose_sort = 1;
+  // B-spline steps stage the basis tables of their workgroup's intervals in
+  // LDS (0: every table read from global memory; same results)
+  int spline_lds = 1;
   int vel_storage = kStoreNative;  // storage of the gathered stage velocities
   int compensated = 0;    // two-sum position/displacement updates of forward particles
   int sl_tiled = 0;       // LDS-tiled semi-Lagrangian B-spline step (results identical; slower here)
@@ -890,6 +893,7 @@ struct Engine : EngineBase {
     }
     sp.storage = g_opt.vel_storage;
     sp.vscale = (T)vscale;
+    sp.lds = g_opt.spline_lds;
     for (int a = 0; a < 4; a++) {
       const StageField f = stage_field(c[a < ns ? a : 0], spl);
       for (int d = 0; d < 3; d++) sp.v[a][d] = f.c[d];
@@ -2290,6 +2294,9 @@ void cfd2lcs_set_option_c(char option[], int val) {
   } else if (o == "COMPOSE_SORT") {
     if (val < 0) warn = true;
     else g_opt.compose_sort = val;
+  } else if (o == "SPLINE_LDS") {
+    if (val != LCS_TRUE && val != LCS_FALSE) warn = true;
+    else g_opt.spline_lds = val;
   } else if (o == "UPDATE_FREQ") {
   } else if (o == "INTERPOLATOR") {
     if (val != NEAREST_NBR && val != LINEAR && val != QUADRATIC && val != CUBIC && val != TSE && val != TSE_LIMIT)

@@ -186,30 +186,88 @@ __device__ __forceinline__ T trilin(const LinW<T>& w, T f0, T f1, T f2, T f3, T 
 // Returns 0 = ok, 1 = outside the knot range (db3val returns 0 and reports),
 // 2 = y/z at the last knot (db3val returns 0 silently,
 // bspline_sub_module.f90:634-636).
+// LDS pointers (address space 3): data a workgroup stages once and its lanes
+// read many times
+template <typename T>
+using lcptr = const __attribute__((address_space(3))) T*;
+template <typename T>
+using lptr = __attribute__((address_space(3))) T*;
+
+// A workgroup's LDS copy of rows [r0, r0 + nr) of one axis's per-interval
+// basis table (K*K+1 values per row) and of the knots t(r0-1) .. t(r0+nr)
+// (0-based), covering the intervals its lanes' points fall in; nr = 0:
+// nothing staged.  The B-spline set-up reads a row or knot from here when it
+// is staged and from global memory otherwise, so staging changes no value.
+template <typename T>
+struct AxisLds {
+  lcptr<T> poly = nullptr;
+  lcptr<T> knot = nullptr;
+  int r0 = 0, nr = 0;
+};
+template <typename T>
+struct SplineLds {
+  AxisLds<T> ax[3];
+};
+
+// interval of x: the largest L in [K, n] with t(L) <= x, probing from a hint
+// (0 = ok, 1 = outside the knot range, 2 = y/z at the last knot)
 template <typename T, int K>
-__device__ __forceinline__ int bs_setup(const T* __restrict__ t, const T* __restrict__ poly, int n, T x, bool xaxis,
-                                        int hint, int& base, T v[K], const T* xs = nullptr) {
+__device__ __forceinline__ int bs_locate(const T* __restrict__ t, int n, T x, bool xaxis, int hint, int& left,
+                                         const AxisLds<T>* al = nullptr) {
   const int L = n + K;
   if (!(x >= t[0]) || x > t[L - 1]) return 1;
-  int left;
+  const bool st = al && al->nr > 0;
+  const int klo = st ? al->r0 - 1 : 0, khi = st ? al->r0 + al->nr : -1;
+  auto kt = [&](int a) -> T { return (a >= klo && a <= khi) ? al->knot[a - klo] : t[a]; };
   if (x >= t[L - 1]) {
     if (!xaxis) return 2;
     left = n;
   } else {
     left = iclamp(hint, K, n);
-    while (left < n && x >= t[left]) left++;
-    while (left > K && x < t[left - 1]) left--;
+    while (left < n && x >= kt(left)) left++;
+    while (left > K && x < kt(left - 1)) left--;
   }
-  const T* P = poly + left * (K * K + 1);
-  // xs: {base coordinate, offset} of x (Split)
-  const T s = (xs ? (xs[0] - t[left - 1]) + xs[1] : x - t[left - 1]) * P[K * K];
+  return 0;
+}
+
+// the K nonzero basis values at x in interval `left`, from the interval's
+// polynomial table (exact piecewise form of the Cox-de Boor recurrence,
+// Horner in the normalised interval coordinate)
+template <typename T, int K>
+__device__ __forceinline__ void bs_weights(const T* __restrict__ t, const T* __restrict__ poly, int left, T x, T v[K],
+                                           const T* xs = nullptr, const AxisLds<T>* al = nullptr) {
+  auto eval = [&](auto P, T tl) {
+    // xs: {base coordinate, offset} of x (Split)
+    const T s = (xs ? (xs[0] - tl) + xs[1] : x - tl) * P[K * K];
 #pragma unroll
-  for (int a = 0; a < K; a++) {
-    T w = P[a * K + K - 1];
+    for (int a = 0; a < K; a++) {
+      T w = P[a * K + K - 1];
 #pragma unroll
-    for (int m = K - 2; m >= 0; m--) w = w * s + P[a * K + m];
-    v[a] = w;
-  }
+      for (int m = K - 2; m >= 0; m--) w = w * s + P[a * K + m];
+      v[a] = w;
+    }
+  };
+  if (al && al->nr > 0 && left >= al->r0 && left < al->r0 + al->nr)
+    eval(al->poly + (left - al->r0) * (K * K + 1), al->knot[left - al->r0]);
+  else
+    eval(poly + left * (K * K + 1), t[left - 1]);
+}
+
+// B-spline basis of order K at x (bspline_sub_module.f90: dintrv 2558-2634,
+// dbspvn 2288-2370).  The interval index is the largest L in [K, n] with
+// t(L) <= x (dintrv; dbvalu's left limit at x = t(n+1)), found by probing from
+// a hint next to the tracked node; the K nonzero basis values come from the
+// interval's polynomial table.  Returns 0 = ok, 1 = outside the knot range
+// (db3val returns 0 and reports), 2 = y/z at the last knot (db3val returns 0
+// silently, bspline_sub_module.f90:634-636).
+template <typename T, int K>
+__device__ __forceinline__ int bs_setup(const T* __restrict__ t, const T* __restrict__ poly, int n, T x, bool xaxis,
+                                        int hint, int& base, T v[K], const T* xs = nullptr,
+                                        const AxisLds<T>* al = nullptr) {
+  int left = 0;
+  const int r = bs_locate<T, K>(t, n, x, xaxis, hint, left, al);
+  if (r) return r;
+  bs_weights<T, K>(t, poly, left, x, v, xs, al);
   base = left - K;
   return 0;
 }
@@ -271,10 +329,12 @@ struct SplineW {
   T wx[KX], wy[KX], wz[KZ];
 };
 
-// nd: node next to q (tracked node), used as the interval hint
+// nd: node next to q (tracked node), used as the interval hint; sl: the
+// workgroup's staged tables, if any
 template <typename T, int KX, int KZ>
 __device__ __forceinline__ int spline_setup(const GridDev<T>& g, const T q[3], const int nd[3],
-                                            SplineW<T, KX, KZ>& s, const Split<T>* sq = nullptr) {
+                                            SplineW<T, KX, KZ>& s, const Split<T>* sq = nullptr,
+                                            const SplineLds<T>* sl = nullptr) {
   int bx, by, bz;
   T xs[3][2];
   if (sq) {
@@ -285,16 +345,82 @@ __device__ __forceinline__ int spline_setup(const GridDev<T>& g, const T q[3], c
     }
   }
   int rx = bs_setup<T, KX>(g.kn[0], g.bpoly[0], g.N[0], q[0], true, nd[0] + (KX + 2) / 2, bx, s.wx,
-                           sq ? xs[0] : nullptr);
+                           sq ? xs[0] : nullptr, sl ? &sl->ax[0] : nullptr);
   int ry = bs_setup<T, KX>(g.kn[1], g.bpoly[1], g.N[1], q[1], false, nd[1] + (KX + 2) / 2, by, s.wy,
-                           sq ? xs[1] : nullptr);
+                           sq ? xs[1] : nullptr, sl ? &sl->ax[1] : nullptr);
   int rz = bs_setup<T, KZ>(g.kn[2], g.bpoly[2], g.N[2], q[2], false, nd[2] + (KZ + 2) / 2, bz, s.wz,
-                           sq ? xs[2] : nullptr);
+                           sq ? xs[2] : nullptr, sl ? &sl->ax[2] : nullptr);
   if (rx == 1 || ry == 1 || rz == 1) return 1;
   if (ry || rz) return 2;
   s.base = bx + g.N[0] * (by + g.N[1] * bz);
   s.b3[0] = bx; s.b3[1] = by; s.b3[2] = bz;
   return 0;
+}
+
+// Stages into LDS the basis-table rows and knots of the intervals a
+// workgroup's points can fall in: nodes [lo[d], hi[d]] (reduced over the
+// workgroup's lanes, inactive lanes passing INT_MAX / INT_MIN) give hints
+// node + (K+2)/2, and points within a cell of their node fall at most two
+// intervals from the hint.  Axis capacities cap[d] rows; every thread of the
+// block calls it (it synchronises).
+template <typename T, int KX, int KZ>
+__device__ __forceinline__ void stage_spline_tables(const GridDev<T>& g, const int nd[3], bool active, lptr<T> buf,
+                                                    const int cap[3], lptr<int> red, SplineLds<T>& sl) {
+  if (threadIdx.x < 3) {
+    red[threadIdx.x] = INT_MAX;
+    red[3 + threadIdx.x] = INT_MIN;
+  }
+  __syncthreads();
+  int lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    lo[d] = active ? nd[d] : INT_MAX;
+    hi[d] = active ? nd[d] : INT_MIN;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      atomicMin((int*)&red[d], lo[d]);
+      atomicMax((int*)&red[3 + d], hi[d]);
+    }
+  }
+  __syncthreads();
+  size_t off = 0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int K = d < 2 ? KX : KZ, RW = K * K + 1;
+    AxisLds<T>& al = sl.ax[d];
+    int r0 = red[d] + (K + 2) / 2 - 2, r1 = red[3 + d] + (K + 2) / 2 + 2;
+    r0 = max(r0, K);
+    r1 = min(r1, g.N[d]);
+    int nr = r1 - r0 + 1;
+    nr = nr < 0 ? 0 : (nr > cap[d] ? cap[d] : nr);
+    lptr<T> pb = buf + off;
+    lptr<T> kb = buf + off + (size_t)cap[d] * RW;
+    off += (size_t)cap[d] * RW + cap[d] + 2;
+    al.poly = pb;
+    al.knot = kb;
+    al.r0 = r0;
+    al.nr = nr;
+    const T* P = g.bpoly[d] + (size_t)r0 * RW;
+    for (int e = threadIdx.x; e < nr * RW; e += blockDim.x) pb[e] = P[e];
+    const T* t = g.kn[d] + (r0 - 1);
+    for (int e = threadIdx.x; e < nr + 2; e += blockDim.x) kb[e] = t[e];
+  }
+  __syncthreads();
+}
+// LDS values stage_spline_tables needs for capacities cap (rows per axis)
+template <int KX, int KZ>
+constexpr int spline_tables_lds(int cx, int cy, int cz) {
+  return cx * (KX * KX + 1) + cx + 2 + cy * (KX * KX + 1) + cy + 2 + cz * (KZ * KZ + 1) + cz + 2;
 }
 
 template <typename T, int KX, int KZ, typename S>
@@ -499,11 +625,14 @@ struct IntLinear {
 template <typename T, int KX, int KZ>
 struct IntSpline {
   static constexpr bool kNonRect = false;
+  static constexpr int kKX = KX, kKZ = KZ;
+  SplineLds<T> lds;      // the workgroup's staged basis tables
+  bool use_lds = false;  // read them (else every table read goes to global memory)
   template <typename S>
   __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& f, T out[3],
                              unsigned long long& oob, const Split<T>* sq = nullptr) const {
     SplineW<T, KX, KZ> s;
-    int r = spline_setup(g, q, nd, s, sq);
+    int r = spline_setup(g, q, nd, s, sq, use_lds ? &lds : nullptr);
     if (r) {
       if (r == 1) oob++;
       out[0] = out[1] = out[2] = T(0);
@@ -540,6 +669,15 @@ struct IntSpline {
     }
   }
 };
+
+// the B-spline interpolants (whose kernels stage basis tables in LDS)
+template <class I>
+struct SplineInterp : std::false_type {};
+template <typename T, int KX, int KZ>
+struct SplineInterp<IntSpline<T, KX, KZ>> : std::true_type {};
+// LDS rows per axis of the staged basis tables: x covers a workgroup's 256
+// nodes of one row plus the stencil reach, y and z a few rows
+constexpr int kTabX = 264, kTabY = 24;
 
 // ---------------------------------------------------------------------------
 // Particle operations

@@ -99,6 +99,8 @@ struct StageParams {
   // semi-Lagrangian step whose first stage starts on a node
   CF3<T> un, unp1;
   T ct0;
+  // B-spline steps stage the basis tables of their intervals in LDS
+  int lds;
 };
 
 // The forward particle set in one storage order: position, displacement since

@@ -369,3 +369,38 @@ def test_compose_reordering_is_bit_exact(dp, case, n, interp):
                 for d in range(3):
                     assert np.array_equal(ca[d], cb[d])
                 assert np.array_equal(fa, fb, equal_nan=True)
+
+
+@pytest.mark.parametrize("variant,case,n,interp,storage", [
+    ("dp", "abc", 20, F.CUBIC, None), ("sp", "abc", 24, F.CUBIC, None), ("sp", "gyre_slip", (48, 24), F.QUADRATIC, None),
+    ("sp", "abc", 24, F.CUBIC, "FP16"), ("sp", "channel3d", (24, 16, 12), F.CUBIC, None)])
+def test_spline_table_staging_is_bit_exact(variant, case, n, interp, storage):
+    """The B-spline steps read their basis tables and knots from a
+    workgroup's LDS copy (SPLINE_LDS = 1, the default) or from global memory
+    (0): the same values, so the flowmaps agree bit for bit."""
+    import libcfd2lcs_amd as L
+    import scenarios as S
+    lib = L.Cfd2lcs(variant)
+    out = []
+    for lds in (0, 1):
+        X, Y, Z, fl, vel = S.make_case(case, n)
+        lib.finalize()
+        lib.init(X, Y, Z, fl)
+        hs = [lib.diagnostic_init(d, 0, 0.6, 0.2, "x") for d in (F.FTLE_FWD, F.FTLE_BKWD)]
+        opts = [("INTEGRATOR", F.RK4), ("INTERPOLATOR", interp), ("SPLINE_LDS", lds)]
+        if storage:
+            opts.append(("VELOCITY_STORAGE", L.LCS_FP16))
+        for k, v in opts:
+            lib.set_option(k, v)
+        res = []
+        for s in range(10):
+            lib.update(*vel(0.05 * s), 0.05 * s)
+            assert lib.error == 0
+            res.append([lib.flowmap(h) for h in hs])
+        res.append([lib.composed(h) for h in hs])
+        out.append(res)
+    lib.finalize()
+    for a, b in zip(*out):
+        for fa, fb in zip(a, b):
+            for d in range(3):
+                assert np.array_equal(fa[d], fb[d])
