@@ -54,6 +54,9 @@ def run_gpu(variant, n, steps, warmup, dt, T, h, u2=True, opts=None, resolution=
         raise RuntimeError("cfd2lcs_init failed")
     if aux:
         lib.set_option("AUX_GRID", L.LCS_TRUE)
+    # options that take effect at a reset (storage modes) apply from the first
+    for k, v in (opts or {}).items():
+        lib.set_option(k, v)
     hf = lib.diagnostic_init(L.FTLE_FWD, resolution, T, h, "fwdFTLE")
     hb = lib.diagnostic_init(L.FTLE_BKWD, resolution, T, h, "bkwdFTLE")
     if lib.error:

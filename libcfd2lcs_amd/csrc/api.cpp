@@ -63,6 +63,10 @@ struct Options {
   // LDS (0: every table read from global memory; same results)
   int spline_lds = 1;
   int vel_storage = kStoreNative;  // storage of the gathered stage velocities
+  // storage of the backward semi-Lagrangian flowmap and its coefficients
+  // (kStoreF32: FP32 under FP64 compute); takes effect at the next h-interval
+  // reset
+  int fm_storage = kStoreNative;
   int compensated = 0;    // two-sum position/displacement updates of forward particles
   int sl_tiled = 0;       // LDS-tiled semi-Lagrangian B-spline step (results identical; slower here)
   // output files: FLOWMAP_IO, VELOCITY_IO, BCFLAG_IO (data_m.f90:44-54) pick
@@ -206,6 +210,11 @@ struct Engine : EngineBase {
   DevBuf vel[2];        // [u_n, u_np1] ghosted, 3 components each
   DevBuf vcoef[2];      // spline coefficients of vel[]
   bool vcoef_ok[2] = {false, false};
+  // vcoef[b] holds FP32 coefficients (FP64 compute with FP32 stage storage:
+  // the blends then read half the bytes, and ct = 0 or 1 reads them as they
+  // are)
+  bool vcoef32[2] = {false, false};
+  bool want_vcoef32() const { return sizeof(T) == 8 && g_opt.vel_storage == kStoreF32; }
   int cur = 1;          // index of u_np1 in vel[]
   DevBuf stage_u;       // staging of user velocity (lcsdata_t), 3*np
   DevBuf stage_out;     // staging for getters (lcsdata_t), 3*np
@@ -263,6 +272,8 @@ struct Engine : EngineBase {
     size_t stemp_bytes = 0;
     DevBuf fm[2];          // BKWD semi-Lagrangian flowmap ping-pong (ghosted)
     int fcur = 0;
+    bool fm32 = false;     // fm and fmcoef hold FP32 in this h interval (FP64 compute)
+    size_t fm_elem() const { return fm32 ? sizeof(float) : sizeof(T); }
     DevBuf fmcoef;         // spline coefficients of fm[fcur]
     DevBuf slots, slotcoef;  // nslot * 3 ghosted fields
     DevBuf order;          // slot order of the composition
@@ -345,6 +356,17 @@ struct Engine : EngineBase {
   }
   F3<T> f3(DevBuf& b) { return f3(b, g.nt); }
   CF3<T> cf3(DevBuf& b) { return cf3(b, g.nt); }
+  // the three component arrays of b as element type E
+  template <typename E>
+  static F3<E> f3e(DevBuf& b, long long nt) {
+    E* p = (E*)b.p;
+    return F3<E>{{p, p + nt, p + 2 * nt}};
+  }
+  template <typename E>
+  static CF3<E> cf3e(DevBuf& b, long long nt) {
+    const E* p = (const E*)b.p;
+    return CF3<E>{{p, p + nt, p + 2 * nt}};
+  }
   // particle-set layout inside one allocation: x[3], d[3] (T), n[3]
   // (int16), org (int), each array aligned to 256 bytes
   size_t pset_bytes(long long np) const {
@@ -701,6 +723,7 @@ struct Engine : EngineBase {
     if (dg->dir > 0) {
       ok = ok && dg->part[0].alloc(pset_bytes(np));
     } else {
+      // sized for T: the storage may change at any h-interval reset
       ok = ok && dg->fm[0].alloc(3 * fb) && dg->fm[1].alloc(3 * fb);
     }
     ok = ok && dg->slots.alloc((size_t)dg->nslot * 3 * fb) && dg->order.alloc(sizeof(int) * dg->nslot) &&
@@ -787,8 +810,9 @@ struct Engine : EngineBase {
       dg.since_sort = 0;
       dg.cmode = g_opt.compensated;
     } else {
-      (void)hipMemsetAsync(dg.fm[0].p, 0, 3 * fbytes(NT(dg)), stream);
-      (void)hipMemsetAsync(dg.fm[1].p, 0, 3 * fbytes(NT(dg)), stream);
+      dg.fm32 = sizeof(T) == 8 && g_opt.fm_storage == kStoreF32;
+      (void)hipMemsetAsync(dg.fm[0].p, 0, 3 * dg.fm_elem() * (size_t)NT(dg), stream);
+      (void)hipMemsetAsync(dg.fm[1].p, 0, 3 * dg.fm_elem() * (size_t)NT(dg), stream);
       dg.fresh_nodes = true;
     }
   }
@@ -822,20 +846,32 @@ struct Engine : EngineBase {
     DevBuf& A = spl ? vcoef[vold] : vel[vold];
     DevBuf& B = spl ? vcoef[cur] : vel[cur];
     const int stor = g_opt.vel_storage;
+    // the two snapshots' fields as the blend reads them
+    const bool in32 = spl && vcoef32[vold];
     auto whole = [&](DevBuf& b) {
+      if (in32) {
+        const CF3<float> f = cf3e<float>(b, g.nt);
+        return StageField{{f.c[0], f.c[1], f.c[2]}};
+      }
       CF3<T> f = cf3(b);
       return StageField{{f.c[0], f.c[1], f.c[2]}};
     };
-    if (stor == kStoreNative) {
+    if (stor == kStoreNative || (stor == kStoreF32 && in32)) {
       if (ct == (U)0) return whole(A);
       if (ct == (U)1) return whole(B);
     }
     // per-component stride of the stage field (0 for the packed records)
-    const size_t eb = stor == kStoreNative ? sizeof(T) : 0;
+    const size_t eb = stor == kStoreNative ? sizeof(T) : (stor == kStoreF32 ? sizeof(float) : 0);
     const size_t fbytes_stage = stage_field_bytes(stor, sizeof(T), (size_t)g.nt);
+    // A rounded storage format reuses a blend whose weight is within 2^-10
+    // of the format's unit roundoff of ct: the forward and backward
+    // sub-cycles reach the same stage times by sums rounded differently, and
+    // the velocity such a weight difference makes is far below the rounding
+    // of the stored value.  Native storage matches ct exactly.
+    const double tol = stor == kStoreF32 ? 0x1p-34 : (stor == kStoreF16 ? 0x1p-21 : (stor == kStoreBF16 ? 0x1p-18 : 0.0));
     Blend* victim = nullptr;
     for (auto& b : blends) {
-      if (b.valid && b.ct == ct && b.storage == stor && b.spl == spl) {
+      if (b.valid && std::fabs((double)b.ct - (double)ct) <= tol && b.storage == stor && b.spl == spl) {
         b.stamp = ++blend_clock;
         char* p = (char*)b.buf.p;
         return StageField{{p, p + eb * g.nt, p + 2 * eb * g.nt}};
@@ -850,7 +886,17 @@ struct Engine : EngineBase {
     }
     char* p = (char*)victim->buf.p;
     void* out[3] = {p, p + eb * g.nt, p + 2 * eb * g.nt};
-    ktimed(14, [&] { launch_blend(g, out, stor, cf3(A), cf3(B), (T)ct, (T)(1.0 / vscale), st, stream); });
+    if constexpr (sizeof(T) == 8) {
+      if (in32) {
+        ktimed(14, [&] {
+          launch_blend(g, out, stor, cf3e<float>(A, g.nt), cf3e<float>(B, g.nt), (T)ct, (T)(1.0 / vscale), st, stream);
+        });
+      } else {
+        ktimed(14, [&] { launch_blend(g, out, stor, cf3(A), cf3(B), (T)ct, (T)(1.0 / vscale), st, stream); });
+      }
+    } else {
+      ktimed(14, [&] { launch_blend(g, out, stor, cf3(A), cf3(B), (T)ct, (T)(1.0 / vscale), st, stream); });
+    }
     victim->ct = ct;
     victim->storage = stor;
     victim->spl = spl;
@@ -906,7 +952,7 @@ struct Engine : EngineBase {
 
   void update(const U* u, const U* v, const U* w, bool device, U time) override {
     int interp = g_opt.interpolator;
-    if (sizeof(T) == 8 && g_opt.vel_storage != kStoreNative) {
+    if (sizeof(T) == 8 && (g_opt.vel_storage == kStoreF16 || g_opt.vel_storage == kStoreBF16)) {
       fail("VELOCITY_STORAGE = LCS_FP16 / LCS_BF16 needs PRECISION = LCS_FP32");
       return;
     }
@@ -984,9 +1030,10 @@ struct Engine : EngineBase {
       if (g_opt.vel_storage == kStoreF16 && m > 0.0 && std::isfinite(m)) vscale = std::ldexp(1.0, std::ilogb(m) - 7);
     }
     if (spl) {
-      // spline coefficients of u_np1 (and of u_n when they are missing)
+      // spline coefficients of u_np1 (and of u_n when they are missing or in
+      // the other element type)
       if (!fit_vel(cur)) return;
-      if (!vcoef_ok[old] && !fit_vel(old)) return;
+      if ((!vcoef_ok[old] || vcoef32[old] != want_vcoef32()) && !fit_vel(old)) return;
     }
     const U dt_f = t_np1 - t_n;
     for (auto& dgp : diags) {
@@ -1049,28 +1096,38 @@ struct Engine : EngineBase {
       }
       // the MASK test reads lp%no_scfd as it stands at the start of the update
       const bool fresh_at_start = dg.fresh_nodes;
-      for (int s = 0; s < nsub; s++) {
-        DevBuf& a = dg.fm[dg.fcur];
-        DevBuf& b = dg.fm[1 - dg.fcur];
-        CF3<T> src = cf3(a, nt);
-        if (spl) {
-          HIPCHK(hipMemcpyAsync(dg.fmcoef.p, a.p, 3 * fbytes(nt), hipMemcpyDeviceToDevice, stream));
-          ktimed(10, [&] { launch_spline_fit(lg, f3(dg.fmcoef, nt), 3, stream); });
-          src = cf3(dg.fmcoef, nt);
+      auto sub_cycles = [&](auto tag) {
+        using E = decltype(tag);
+        for (int s = 0; s < nsub; s++) {
+          DevBuf& a = dg.fm[dg.fcur];
+          DevBuf& b = dg.fm[1 - dg.fcur];
+          CF3<E> src = cf3e<E>(a, nt);
+          if (spl) {
+            // coefficients of fm into fmcoef (the copy rides on the first pass)
+            ktimed(10, [&] { launch_spline_fit(lg, f3e<E>(dg.fmcoef, nt), 3, stream, src.c); });
+            src = cf3e<E>(dg.fmcoef, nt);
+          }
+          const StageParams<T> sp = stage_params(t, dt, spl);
+          if (dg.sg) {
+            const int fresh = (dg.fresh_nodes ? 1 : 0) | (fresh_at_start ? 2 : 0);
+            ktimed(9, [&] {
+              launch_sl_step2(g, lg, (const int16_t*)dg.sg->nodes.p, fresh, interp, sp, src, f3e<E>(b, nt), st,
+                              stream);
+            });
+            dg.fresh_nodes = false;
+          } else {
+            ktimed(9, [&] { launch_sl_step(g, interp, sp, src, f3e<E>(b, nt), st, stream, g_opt.sl_tiled); });
+          }
+          ktimed(13, [&] { launch_fm_ghost(lg, f3e<E>(b, nt), cf3e<E>(a, nt), st, stream); });
+          dg.fcur = 1 - dg.fcur;
+          t = t + dt;
         }
-        const StageParams<T> sp = stage_params(t, dt, spl);
-        if (dg.sg) {
-          const int fresh = (dg.fresh_nodes ? 1 : 0) | (fresh_at_start ? 2 : 0);
-          ktimed(9, [&] {
-            launch_sl_step2(g, lg, (const int16_t*)dg.sg->nodes.p, fresh, interp, sp, src, f3(b, nt), st, stream);
-          });
-          dg.fresh_nodes = false;
-        } else {
-          ktimed(9, [&] { launch_sl_step(g, interp, sp, src, f3(b), st, stream, g_opt.sl_tiled); });
-        }
-        ktimed(13, [&] { launch_fm_ghost(lg, f3(b, nt), cf3(a, nt), st, stream); });
-        dg.fcur = 1 - dg.fcur;
-        t = t + dt;
+      };
+      if constexpr (sizeof(T) == 8) {
+        if (dg.fm32) sub_cycles(float{});
+        else sub_cycles(T{});
+      } else {
+        sub_cycles(T{});
       }
     }
     timed_end(tm);
@@ -1082,12 +1139,16 @@ struct Engine : EngineBase {
       fail("hipMalloc failed (velocity coefficients)");
       return false;
     }
-    if (hipMemcpyAsync(vcoef[b].p, vel[b].p, 3 * fbytes(), hipMemcpyDeviceToDevice, stream) != hipSuccess) {
-      fail("copy failed");
-      return false;
+    const CF3<T> u = cf3(vel[b]);
+    const bool w32 = want_vcoef32();
+    if constexpr (sizeof(T) == 8) {
+      if (w32) ktimed(10, [&] { launch_spline_fit(g, f3e<float>(vcoef[b], g.nt), 3, stream, u.c); });
+      else ktimed(10, [&] { launch_spline_fit(g, f3(vcoef[b]), 3, stream, u.c); });
+    } else {
+      ktimed(10, [&] { launch_spline_fit(g, f3(vcoef[b]), 3, stream, u.c); });
     }
-    ktimed(10, [&] { launch_spline_fit(g, f3(vcoef[b]), 3, stream); });
     vcoef_ok[b] = true;
+    vcoef32[b] = w32;
     return true;
   }
 
@@ -1101,6 +1162,8 @@ struct Engine : EngineBase {
     F3<T> slot{{sp, sp + nt, sp + 2 * nt}};
     if (d.dir > 0) {
       launch_field_from_origin(G(d), fwd_dx_by_origin(d), slot, stream);
+    } else if (d.fm32) {
+      launch_widen((const float*)d.fm[d.fcur].p, sp, 3 * nt, stream);
     } else {
       HIPCHK(hipMemcpyAsync(sp, d.fm[d.fcur].p, 3 * fbytes(nt), hipMemcpyDeviceToDevice, stream));
     }
@@ -1124,8 +1187,9 @@ struct Engine : EngineBase {
       for (int s = 0; s < d.nslot; s++) {
         if (!d.slot_ok[s] || d.slot_coef_ok[s]) continue;
         T* cp = slot_ptr(d.slotcoef, s, nt);
-        HIPCHK(hipMemcpyAsync(cp, slot_ptr(d.slots, s, nt), 3 * fbytes(nt), hipMemcpyDeviceToDevice, stream));
-        ktimed(10, [&] { launch_spline_fit(lg, F3<T>{{cp, cp + nt, cp + 2 * nt}}, 3, stream); });
+        const T* sp = slot_ptr(d.slots, s, nt);
+        const T* in[3] = {sp, sp + nt, sp + 2 * nt};
+        ktimed(10, [&] { launch_spline_fit(lg, F3<T>{{cp, cp + nt, cp + 2 * nt}}, 3, stream, in); });
         d.slot_coef_ok[s] = 1;
       }
     }
@@ -1558,6 +1622,7 @@ struct Engine : EngineBase {
     // swapped with the streaming state while the tape runs
     DevBuf vel[2], vcoef[2];
     bool vcoef_ok[2] = {false, false};
+    bool vcoef32[2] = {false, false};
     int cur = 1, vold = 0;
     U t_n = 0, t_np1 = 0;
     Blend blends[6];
@@ -1578,6 +1643,7 @@ struct Engine : EngineBase {
       vel[b].swap(tape.vel[b]);
       vcoef[b].swap(tape.vcoef[b]);
       std::swap(vcoef_ok[b], tape.vcoef_ok[b]);
+      std::swap(vcoef32[b], tape.vcoef32[b]);
     }
     std::swap(cur, tape.cur);
     std::swap(vold, tape.vold);
@@ -1668,8 +1734,7 @@ struct Engine : EngineBase {
     const size_t fb = 3 * fbytes();
     if (tape.spl) {
       if (!ensure(tape.fmcoef, fb)) return false;
-      HIPCHK_R(hipMemcpyAsync(tape.fmcoef.p, a.p, fb, hipMemcpyDeviceToDevice, stream), false);
-      launch_spline_fit(g, f3(tape.fmcoef), 3, stream);
+      launch_spline_fit(g, f3(tape.fmcoef), 3, stream, src.c);
       src = cf3(tape.fmcoef);
     }
     if (keep) {
@@ -2067,6 +2132,13 @@ struct Engine : EngineBase {
     Diag* dg = diag(h);
     if (!dg) return 1;
     if (dg->dir > 0) return copy_out(G(*dg), fwd_dx_by_origin(*dg), 0, a, b, c);
+    if (dg->fm32) {
+      // widened into a temporary ghosted field first
+      DevBuf w;
+      if (!w.alloc(3 * fbytes(NT(*dg)))) return 1;
+      launch_widen((const float*)dg->fm[dg->fcur].p, (T*)w.p, 3 * NT(*dg), stream);
+      return copy_out(G(*dg), cf3(w, NT(*dg)), 1, a, b, c);
+    }
     return copy_out(G(*dg), cf3(dg->fm[dg->fcur], NT(*dg)), 1, a, b, c);
   }
   int get_composed(int h, U* a, U* b, U* c) override {
@@ -2278,11 +2350,18 @@ void cfd2lcs_set_option_c(char option[], int val) {
   else if (o == "KERNEL_TIMING") {
     boolopt(g_opt.kernel_timing);
   } else if (o == "VELOCITY_STORAGE") {
-    // storage of the gathered stage velocities: the compute precision, or
-    // 16-bit (FP32 compute only)
+    // storage of the gathered stage velocities: the compute precision,
+    // 16-bit (FP32 compute only) or FP32 under FP64 compute
     if (val == LCS_FP16) g_opt.vel_storage = kStoreF16;
     else if (val == LCS_BF16) g_opt.vel_storage = kStoreBF16;
-    else if (val == LCS_FP32 || val == LCS_FP64) g_opt.vel_storage = kStoreNative;
+    else if (val == LCS_FP32) g_opt.vel_storage = sizeof(U) == 8 ? kStoreF32 : kStoreNative;
+    else if (val == LCS_FP64) g_opt.vel_storage = kStoreNative;
+    else warn = true;
+  } else if (o == "FLOWMAP_STORAGE") {
+    // storage of the backward flowmap from the next h-interval reset on: the
+    // compute precision, or FP32 under FP64 compute
+    if (val == LCS_FP32) g_opt.fm_storage = sizeof(U) == 8 ? kStoreF32 : kStoreNative;
+    else if (val == LCS_FP64) g_opt.fm_storage = kStoreNative;
     else warn = true;
   } else if (o == "COMPENSATED") {
     boolopt(g_opt.compensated);

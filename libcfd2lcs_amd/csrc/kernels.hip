@@ -114,8 +114,8 @@ __global__ void __launch_bounds__(kBlock) k_cfl(GridDev<T> g, F3<T> u, DevStatus
 // ghost cells of a flowmap: periodic copies then set_flowmap_bc's ghost
 // pass (flowmap_m.f90:362-390).  Interior zeroing of MASK/INFLOW/OUTFLOW
 // nodes is done by the producer.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_fm_ghost(GridDev<T> g, F3<T> fm, CF3<T> fmold, DevStatus* st) {
+template <typename T, typename E>
+__global__ void __launch_bounds__(kBlock) k_fm_ghost(GridDev<T> g, F3<E> fm, CF3<E> fmold, DevStatus* st) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (t >= ghost_count(g) || !decode_ghost(g, t, i, j, k)) return;
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(kBlock) k_fm_ghost(GridDev<T> g, F3<T> fm, CF3
     case kMask:
     case kInflow:
     case kOutflow:
-      for (int c = 0; c < 3; c++) fm.c[c][l] = T(0);
+      for (int c = 0; c < 3; c++) fm.c[c][l] = E(0);
       break;
     case kSlip:
     case k2D: {
@@ -188,9 +188,9 @@ __global__ void __launch_bounds__(kBlock) k_field_from_origin(GridDev<T> g, CF3<
 // velocity (bracket of the origin node, lp_motion_m.f90:171-327), particle
 // BC, old flowmap at the departure point, displacement form write.
 // ---------------------------------------------------------------------------
-template <typename T, typename S, class I>
+template <typename T, typename S, class I, typename FM>
 __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T>& sp, const I& interp,
-                                        const CF3<T>& fmold, const F3<T>& fmnew, int i, int j, int k,
+                                        const CF3<FM>& fmold, const F3<FM>& fmnew, int i, int j, int k,
                                         DevStatus* st) {
   constexpr bool NR = I::kNonRect;
   int nd[3] = {i, j, k};
@@ -238,7 +238,7 @@ __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T
     // a boundary projection or stick moved the point: its displacement
     const T disp = xp[d] == xr[d] ? inc[d] : xp[d] - x0[d];
     T v = (fmp[d] + disp) * mask;
-    fmnew.c[d][l] = zero ? T(0) : v;
+    fmnew.c[d][l] = zero ? FM(0) : FM(v);
   }
   flush_status(st, err, oob);
 }
@@ -250,22 +250,28 @@ __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T
 // and keep the compiler's choice.
 template <class I>
 struct QuadraticSpline : std::false_type {};
+// FP32 stage fields under FP64 compute
+template <typename T, typename S>
+struct MixedF32 : std::integral_constant<bool, std::is_same<T, double>::value && std::is_same<S, float>::value> {};
 template <typename T>
 struct QuadraticSpline<IntSpline<T, 3, 3>> : std::true_type {};
 template <typename T, typename S, class I>
 constexpr int sl_min_waves() {
   return (std::is_same<T, float>::value && std::is_same<S, T>::value && !I::kNonRect && !QuadraticSpline<I>::value)
              ? 5
-             : ((std::is_same<S, half4_t>::value && !I::kNonRect && !QuadraticSpline<I>::value) ? 6 : 1);
+             : ((std::is_same<S, half4_t>::value && !I::kNonRect && !QuadraticSpline<I>::value)
+                    ? 6
+                    : ((MixedF32<T, S>::value && !I::kNonRect && !QuadraticSpline<I>::value) ? 4 : 1));
 }
-template <typename T, typename S, class I>
-__global__ void __launch_bounds__(kBlock, (sl_min_waves<T, S, I>())) k_sl_step(GridDev<T> g, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
+template <typename T, typename S, class I, typename FM>
+__global__ void __launch_bounds__(kBlock, (sl_min_waves<T, S, I>())) k_sl_step(GridDev<T> g, StageParams<T> sp, CF3<FM> fmold, F3<FM> fmnew,
                                                     DevStatus* st) {
   const int p = xcd_thread_index();
-  // 16-bit stage records: the basis tables are a large share of what the
-  // step reads, and staging them in LDS pays (2.21 -> 1.80 ms at 256^3); with
-  // FP32 component fields the staging costs more than it saves
-  if constexpr (SplineInterp<I>::value && Packed<S>::value) {
+  // 16-bit stage records and FP32 fields under FP64 compute: the basis
+  // tables are a large share of what the step reads, and staging them in LDS
+  // pays (2.21 -> 1.80 ms and 5.24 -> 3.47 ms at 256^3); with fields in the
+  // compute type (FP32 or FP64) the staging costs more than it saves
+  if constexpr (SplineInterp<I>::value && (Packed<S>::value || MixedF32<T, S>::value)) {
     constexpr int KX = I::kKX, KZ = I::kKZ;
     __shared__ T tbuf[spline_tables_lds<KX, KZ>(kTabX, kTabY, kTabY)];
     __shared__ int tred[6];
@@ -408,8 +414,8 @@ __global__ void __launch_bounds__(BX * BY * BZ) k_sl_tiled(GridDev<T> g, StagePa
 // keeps the blend times inv, a power of two (exact), so that values and
 // spline coefficients sit in the format's normal range; a value that would
 // still overflow FP16 sets kErrRange instead of turning into inf.
-template <typename T, typename S>
-__global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b, T ct, T inv, long long nt,
+template <typename T, typename S, typename A>
+__global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<A> a, CF3<A> b, T ct, T inv, long long nt,
                                                   DevStatus* st) {
 #pragma clang fp contract(off)
   const T w0 = T(1) - ct;
@@ -420,7 +426,7 @@ __global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b,
       S r;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const T x = (w0 * a.c[c][l] + ct * b.c[c][l]) * inv;
+        const T x = (w0 * T(a.c[c][l]) + ct * T(b.c[c][l])) * inv;
         if constexpr (std::is_same<E, half_t>::value) range |= fabs(x) > T(65504);
         r.e[c] = narrow<E>(x);
       }
@@ -428,7 +434,7 @@ __global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b,
       out.c[0][l] = r;
     } else {
 #pragma unroll
-      for (int c = 0; c < 3; c++) out.c[c][l] = narrow<S>(w0 * a.c[c][l] + ct * b.c[c][l]);
+      for (int c = 0; c < 3; c++) out.c[c][l] = narrow<S>(w0 * T(a.c[c][l]) + ct * T(b.c[c][l]));
     }
   }
   if (range) atomicOr(&st->err, (int)kErrRange);
@@ -443,7 +449,7 @@ __global__ void __launch_bounds__(kBlock) k_blend(F3<S> out, CF3<T> a, CF3<T> b,
 // the compiler's register budget (FP32 and BF16 would spill)
 template <typename T, typename S, class I>
 constexpr int fwd_min_waves() {
-  return (std::is_same<S, half4_t>::value && !I::kNonRect) ? 4 : 1;
+  return ((std::is_same<S, half4_t>::value || MixedF32<T, S>::value) && !I::kNonRect) ? 4 : 1;
 }
 template <typename T, typename S, class I, bool COMP>
 __global__ void __launch_bounds__(kBlock, (fwd_min_waves<T, S, I>())) k_fwd_step(GridDev<T> g, StageParams<T> sp, ParticleSet<T> ps, int np,
@@ -658,9 +664,9 @@ __global__ void __launch_bounds__(kBlock) k_fwd_place(GridDev<T> g, GridDev<T> l
 // sub-cycle after a reset: bits of fresh), and the old flowmap is
 // interpolated on lg through the node itself.  The flowmap BC of lg zeroes
 // MASK/INFLOW/OUTFLOW nodes of lg (set_flowmap_bc, flowmap_m.f90:345-360).
-template <typename T, typename S, class I>
+template <typename T, typename S, class I, typename FM>
 __global__ void __launch_bounds__(kBlock) k_sl_step2(GridDev<T> g, GridDev<T> lg, const int16_t* __restrict__ nodes,
-                                                     int fresh, StageParams<T> sp, CF3<T> fmold, F3<T> fmnew,
+                                                     int fresh, StageParams<T> sp, CF3<FM> fmold, F3<FM> fmnew,
                                                      DevStatus* st) {
   const int w = xcd_thread_index();
   if (w >= lg.np) return;
@@ -707,7 +713,7 @@ __global__ void __launch_bounds__(kBlock) k_sl_step2(GridDev<T> g, GridDev<T> lg
   for (int d = 0; d < 3; d++) {
     const T disp = xp[d] == xr[d] ? inc[d] : xp[d] - x0[d];
     const T v = (fmp[d] + disp) * mask;
-    fmnew.c[d][l] = zero ? T(0) : v;
+    fmnew.c[d][l] = zero ? FM(0) : FM(v);
   }
   flush_status(st, err, oob);
 }
@@ -1096,8 +1102,8 @@ __global__ void __launch_bounds__(kBlock) k_ftle_aux(GridDev<T> lg, AuxSets<T> a
 // (dbintk/dbnfac).  Forward and backward substitution in the order of
 // dbnslv (bspline_sub_module.f90:2202-2259), in place.
 // ---------------------------------------------------------------------------
-// One line of the banded solve through the accessor at(r) (a reference to
-// row r).  The rows a recurrence step needs are independent of the recurrence
+// One line of the banded solve through the accessors ld(r) and st(r, v) (row
+// r of the line, held in the element type of the array, computed in T).  The rows a recurrence step needs are independent of the recurrence
 // itself, so they are loaded kFitPf rows ahead (double-buffered) and the
 // dependent chain runs on registers only; the arithmetic and its order are
 // those of the unpipelined loop.
@@ -1125,27 +1131,29 @@ __device__ __forceinline__ T bwd_elim(int r, int n, T v, const T* win, const T* 
   return v;
 }
 
-template <typename T, int K, bool TR, class At>
-__device__ __forceinline__ void band_solve(At at, const T* __restrict__ W, int n) {
+// The prefetched rows stay in the element type E until used, so that their
+// loads are still in flight while the recurrence runs.
+template <typename T, int K, bool TR, typename E, class Ld, class St>
+__device__ __forceinline__ void band_solve(Ld ld, St st, const T* __restrict__ W, int n) {
   constexpr int M = K - 1, PF = kFitPf;
   T win[M > 0 ? M : 1];  // forward: win[0] = b(r-1), ...; backward: win[0] = b(r+1), ...
 #pragma unroll
   for (int m = 0; m < M; m++) win[m] = T(0);
-  T cur[PF], nxt[PF];
+  E cur[PF], nxt[PF];
 #pragma unroll
-  for (int u = 0; u < PF; u++) cur[u] = (u < n) ? at(u) : T(0);
+  for (int u = 0; u < PF; u++) cur[u] = (u < n) ? ld(u) : E(0);
   for (int r0 = 0; r0 < n; r0 += PF) {
 #pragma unroll
-    for (int u = 0; u < PF; u++) nxt[u] = (r0 + PF + u < n) ? at(r0 + PF + u) : T(0);
+    for (int u = 0; u < PF; u++) nxt[u] = (r0 + PF + u < n) ? ld(r0 + PF + u) : E(0);
 #pragma unroll
     for (int u = 0; u < PF; u++) {
       const int r = r0 + u;
       if (r < n) {
-        const T v = fwd_elim<T, K, TR>(r, cur[u], win, W);
+        const T v = fwd_elim<T, K, TR>(r, T(cur[u]), win, W);
 #pragma unroll
         for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
         win[0] = v;
-        at(r) = v;
+        st(r, v);
       }
     }
 #pragma unroll
@@ -1154,19 +1162,19 @@ __device__ __forceinline__ void band_solve(At at, const T* __restrict__ W, int n
 #pragma unroll
   for (int m = 0; m < M; m++) win[m] = T(0);
 #pragma unroll
-  for (int u = 0; u < PF; u++) cur[u] = (n - 1 - u >= 0) ? at(n - 1 - u) : T(0);
+  for (int u = 0; u < PF; u++) cur[u] = (n - 1 - u >= 0) ? ld(n - 1 - u) : E(0);
   for (int r0 = n - 1; r0 >= 0; r0 -= PF) {
 #pragma unroll
-    for (int u = 0; u < PF; u++) nxt[u] = (r0 - PF - u >= 0) ? at(r0 - PF - u) : T(0);
+    for (int u = 0; u < PF; u++) nxt[u] = (r0 - PF - u >= 0) ? ld(r0 - PF - u) : E(0);
 #pragma unroll
     for (int u = 0; u < PF; u++) {
       const int r = r0 - u;
       if (r >= 0) {
-        const T v = bwd_elim<T, K, TR>(r, n, cur[u], win, W);
+        const T v = bwd_elim<T, K, TR>(r, n, T(cur[u]), win, W);
 #pragma unroll
         for (int m = M - 1; m >= 1; m--) win[m] = win[m - 1];
         win[0] = v;
-        at(r) = v;
+        st(r, v);
       }
     }
 #pragma unroll
@@ -1174,8 +1182,8 @@ __device__ __forceinline__ void band_solve(At at, const T* __restrict__ W, int n
   }
 }
 
-template <typename T, int K, bool TR>
-__global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restrict__ W, int n, int axis, int N0,
+template <typename T, int K, bool TR, typename E>
+__global__ void __launch_bounds__(kBlock) k_fit_axis(F3<E> a, const T* __restrict__ W, int n, int axis, int N0,
                                                      int N1, int nlines) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   const int comp = blockIdx.y;
@@ -1189,8 +1197,9 @@ __global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restric
   } else {
     start = id; stride = (long long)N0 * N1;
   }
-  T* b = a.c[comp] + start;
-  band_solve<T, K, TR>([&](int r) -> T& { return b[r * stride]; }, W, n);
+  E* b = a.c[comp] + start;
+  band_solve<T, K, TR, E>([&](int r) -> E { return b[r * stride]; }, [&](int r, T v) { b[r * stride] = E(v); }, W,
+                          n);
 }
 
 // x-axis pass of the fit: the lines are contiguous rows.  One wavefront
@@ -1200,48 +1209,59 @@ __global__ void __launch_bounds__(kBlock) k_fit_axis(F3<T> a, const T* __restric
 // same way; the next chunk's loads are in flight during the recurrence.  The
 // forward sweep leaves its result in place and the backward sweep reads it
 // back, as the strided passes do.  Same arithmetic, in the same order, as
-// band_solve.
+// band_solve.  With src, the forward sweep reads the data from src instead
+// of a (a copy of the data into a fused with the fit); E is the element type
+// of the arrays, the arithmetic is in T.
 constexpr int kXC = 32;
-template <typename T, int K, bool TR>
-__global__ void __launch_bounds__(64) k_fit_x(F3<T> a, const T* __restrict__ W, int n, int nlines) {
+template <typename T, int K, bool TR, typename E, typename ES>
+__global__ void __launch_bounds__(64) k_fit_x(F3<E> a, CF3<ES> src, const T* __restrict__ W, int n, int nlines) {
   __shared__ T tile[64][kXC + 1];
   constexpr int M = K - 1;
   const int lane = threadIdx.x;
   const long long line0 = (long long)blockIdx.x * 64;
   const int nl = (int)min((long long)64, nlines - line0);
-  T* base = a.c[blockIdx.y] + line0 * n;
+  E* base = a.c[blockIdx.y] + line0 * n;
   const int nch = (n + kXC - 1) / kXC;
-  T buf[kXC];
+  // raw elements (when source and result share a type): converted when
+  // written to LDS, so that the next chunk's loads stay in flight during
+  // the recurrence
+  using B = typename std::conditional<std::is_same<E, ES>::value, E, T>::type;
+  B buf[kXC];
   // element it*64+lane of a chunk: row (it*64+lane)/kXC, column (it*64+lane)%kXC
-  auto load = [&](int ch) {
+  auto load = [&](const auto* in, int ch) {
 #pragma unroll
     for (int it = 0; it < kXC; it++) {
       const int e = it * 64 + lane, row = e / kXC, col = ch * kXC + e % kXC;
-      buf[it] = (row < nl && col < n) ? base[(long long)row * n + col] : T(0);
+      buf[it] = (row < nl && col < n) ? B(in[(long long)row * n + col]) : B(0);
     }
   };
   auto to_lds = [&]() {
 #pragma unroll
     for (int it = 0; it < kXC; it++) {
       const int e = it * 64 + lane;
-      tile[e / kXC][e % kXC] = buf[it];
+      tile[e / kXC][e % kXC] = T(buf[it]);
     }
   };
   auto store = [&](int ch) {
 #pragma unroll
     for (int it = 0; it < kXC; it++) {
       const int e = it * 64 + lane, row = e / kXC, col = ch * kXC + e % kXC;
-      if (row < nl && col < n) base[(long long)row * n + col] = tile[e / kXC][e % kXC];
+      if (row < nl && col < n) base[(long long)row * n + col] = E(tile[e / kXC][e % kXC]);
     }
   };
   T win[M > 0 ? M : 1];
   // forward substitution (unit lower factor)
 #pragma unroll
   for (int m = 0; m < M; m++) win[m] = T(0);
-  load(0);
+  const ES* from = src.c[0] ? src.c[blockIdx.y] + line0 * n : nullptr;
+  auto load_fwd = [&](int ch) {
+    if constexpr (std::is_same<E, ES>::value) load(from ? from : (const ES*)base, ch);
+    else load(from, ch);
+  };
+  load_fwd(0);
   for (int ch = 0; ch < nch; ch++) {
     to_lds();
-    if (ch + 1 < nch) load(ch + 1);
+    if (ch + 1 < nch) load_fwd(ch + 1);
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kXC; u++) {
@@ -1261,10 +1281,10 @@ __global__ void __launch_bounds__(64) k_fit_x(F3<T> a, const T* __restrict__ W, 
   // backward substitution (upper factor), last chunk first
 #pragma unroll
   for (int m = 0; m < M; m++) win[m] = T(0);
-  load(nch - 1);
+  load((const E*)base, nch - 1);
   for (int ch = nch - 1; ch >= 0; ch--) {
     to_lds();
-    if (ch > 0) load(ch - 1);
+    if (ch > 0) load((const E*)base, ch - 1);
     __syncthreads();
 #pragma unroll
     for (int u = kXC - 1; u >= 0; u--) {
@@ -1321,10 +1341,10 @@ void launch_cfl(const GridDev<T>& g, F3<T> u, DevStatus* st, hipStream_t s) {
   hipLaunchKernelGGL((k_cfl<T>), dim3((unsigned)nb), dim3(kBlock), 0, s, g, u, st);
 }
 
-template <typename T>
-void launch_fm_ghost_impl(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st, hipStream_t s) {
+template <typename T, typename E>
+void launch_fm_ghost_impl(const GridDev<T>& g, F3<E> fm, CF3<E> fmold, DevStatus* st, hipStream_t s) {
   long long ng = 2LL * g.N[0] * g.N[1] + (2LL * g.N[0] + 2LL * g.n[1]) * g.n[2];
-  hipLaunchKernelGGL((k_fm_ghost<T>), grid_for(ng), dim3(kBlock), 0, s, g, fm, fmold, st);
+  hipLaunchKernelGGL((k_fm_ghost<T, E>), grid_for(ng), dim3(kBlock), 0, s, g, fm, fmold, st);
 }
 
 template <typename T>
@@ -1332,38 +1352,43 @@ void launch_field_from_origin(const GridDev<T>& g, CF3<T> src, F3<T> dst, hipStr
   hipLaunchKernelGGL((k_field_from_origin<T>), grid_for(g.nt), dim3(kBlock), 0, s, g, src, dst);
 }
 
-template <typename T, bool TR>
-static void fit_pass(const GridDev<T>& g, F3<T> coef, int ncomp, int axis, hipStream_t s) {
+template <typename T, bool TR, typename E, typename ES>
+static void fit_pass(const GridDev<T>& g, F3<E> coef, CF3<ES> src, int ncomp, int axis, hipStream_t s) {
   const int n = g.N[axis];
   const int nlines = (int)(g.nt / n);
   if (axis == 0) {
     dim3 gx((nlines + 63) / 64, ncomp);
     switch (g.kord[0]) {
-      case 2: hipLaunchKernelGGL((k_fit_x<T, 2, TR>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
-      case 3: hipLaunchKernelGGL((k_fit_x<T, 3, TR>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
-      default: hipLaunchKernelGGL((k_fit_x<T, 4, TR>), gx, dim3(64), 0, s, coef, g.band[0], n, nlines); break;
+      case 2: hipLaunchKernelGGL((k_fit_x<T, 2, TR, E, ES>), gx, dim3(64), 0, s, coef, src, g.band[0], n, nlines); break;
+      case 3: hipLaunchKernelGGL((k_fit_x<T, 3, TR, E, ES>), gx, dim3(64), 0, s, coef, src, g.band[0], n, nlines); break;
+      default: hipLaunchKernelGGL((k_fit_x<T, 4, TR, E, ES>), gx, dim3(64), 0, s, coef, src, g.band[0], n, nlines); break;
     }
     return;
   }
   dim3 grid((nlines + kBlock - 1) / kBlock, ncomp);
   switch (g.kord[axis]) {
-    case 2: hipLaunchKernelGGL((k_fit_axis<T, 2, TR>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
-    case 3: hipLaunchKernelGGL((k_fit_axis<T, 3, TR>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
-    default: hipLaunchKernelGGL((k_fit_axis<T, 4, TR>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+    case 2: hipLaunchKernelGGL((k_fit_axis<T, 2, TR, E>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+    case 3: hipLaunchKernelGGL((k_fit_axis<T, 3, TR, E>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
+    default: hipLaunchKernelGGL((k_fit_axis<T, 4, TR, E>), grid, dim3(kBlock), 0, s, coef, g.band[axis], n, axis, g.N[0], g.N[1], nlines); break;
   }
 }
 
-// fits in place: coef holds the data on entry and the coefficients on exit
-template <typename T>
-void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t s) {
-  for (int axis = 0; axis < 3; axis++) fit_pass<T, false>(g, coef, ncomp, axis, s);
+// coef = the coefficients of the data in src (or in coef itself, in place,
+// when src is null); the element type E of both is T or, under FP64
+// compute, FP32 (the solve runs in T)
+template <typename T, typename E, typename ES>
+void launch_spline_fit(const GridDev<T>& g, F3<E> coef, int ncomp, hipStream_t s, const ES* const* src) {
+  const CF3<ES> in{{src ? src[0] : nullptr, src ? src[1] : nullptr, src ? src[2] : nullptr}};
+  fit_pass<T, false, E, ES>(g, coef, in, ncomp, 0, s);
+  const CF3<E> none{{nullptr, nullptr, nullptr}};
+  for (int axis = 1; axis < 3; axis++) fit_pass<T, false, E, E>(g, coef, none, ncomp, axis, s);
 }
 
 // transpose of launch_spline_fit (three components, in place): A^T solves in
 // the reverse order of the passes
 template <typename T>
 void launch_spline_fit_T(const GridDev<T>& g, F3<T> a, hipStream_t s) {
-  for (int axis = 2; axis >= 0; axis--) fit_pass<T, true>(g, a, 3, axis, s);
+  for (int axis = 2; axis >= 0; axis--) fit_pass<T, true, T, T>(g, a, CF3<T>{{nullptr, nullptr, nullptr}}, 3, axis, s);
 }
 
 // dispatch over interpolator and spline orders (KX = orders of x and y, KZ of z)
@@ -1397,18 +1422,20 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T>
-void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a, CF3<T> b, T ct, T inv,
+template <typename T, typename A>
+void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<A> a, CF3<A> b, T ct, T inv,
                   DevStatus* st, hipStream_t s) {
   const long long nb = std::min<long long>((g.nt + kBlock - 1) / kBlock, 8192);
   auto go = [&](auto tag) {
     using S = decltype(tag);
     F3<S> o{{(S*)out[0], (S*)out[1], (S*)out[2]}};
-    hipLaunchKernelGGL((k_blend<T, S>), dim3((unsigned)nb), dim3(kBlock), 0, s, o, a, b, ct, inv, g.nt, st);
+    hipLaunchKernelGGL((k_blend<T, S, A>), dim3((unsigned)nb), dim3(kBlock), 0, s, o, a, b, ct, inv, g.nt, st);
   };
   if constexpr (std::is_same<T, float>::value) {
     if (storage == kStoreF16) return go(half4_t{});
     if (storage == kStoreBF16) return go(bf164_t{});
+  } else {
+    if (storage == kStoreF32) return go(float{});
   }
   go(T{});
 }
@@ -1419,6 +1446,8 @@ void with_storage(int storage, F&& f) {
   if constexpr (std::is_same<T, float>::value) {
     if (storage == kStoreF16) return f(half4_t{});
     if (storage == kStoreBF16) return f(bf164_t{});
+  } else {
+    if (storage == kStoreF32) return f(float{});
   }
   f(T{});
 }
@@ -1443,17 +1472,17 @@ bool with_storage_nonrect(const GridDev<T>& g, int interp, int storage, F&& f) {
   return true;
 }
 
-template <typename T>
-void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold, F3<T> fmnew,
+template <typename T, typename FM>
+void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<FM> fmold, F3<FM> fmnew,
                     DevStatus* st, hipStream_t s, int tiled) {
   const dim3 grid = grid_for(g.np), blk(kBlock);
   if (with_storage_nonrect(g, interp, sp.storage, [&](auto tag, auto in) {
         using S = decltype(tag);
         using I = decltype(in);
-        hipLaunchKernelGGL((k_sl_step<T, S, I>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+        hipLaunchKernelGGL((k_sl_step<T, S, I, FM>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
       }))
     return;
-  if constexpr (std::is_same<T, float>::value) {
+  if constexpr (std::is_same<T, float>::value && std::is_same<FM, T>::value) {
     // FP32 B-spline step on a 3-D grid with native stage storage: LDS tiles
     if (tiled && (interp == kQuadratic || interp == kCubic) && sp.storage == kStoreNative && g.flat < 0 &&
         g.kord[0] == g.kord[1] && g.kord[1] == g.kord[2]) {
@@ -1473,17 +1502,17 @@ void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, C
   with_storage<T>(sp.storage, [&](auto tag) {
     using S = decltype(tag);
     if (interp == kLinear) {
-      hipLaunchKernelGGL((k_sl_step<T, S, IntLinear<T>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+      hipLaunchKernelGGL((k_sl_step<T, S, IntLinear<T>, FM>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
     } else if (interp == kNearest) {
-      hipLaunchKernelGGL((k_sl_step<T, S, IntNearest<T>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+      hipLaunchKernelGGL((k_sl_step<T, S, IntNearest<T>, FM>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
     } else if (interp == kTSE) {
-      hipLaunchKernelGGL((k_sl_step<T, S, IntTSE<T, false>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+      hipLaunchKernelGGL((k_sl_step<T, S, IntTSE<T, false>, FM>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
     } else if (interp == kTSELimit) {
-      hipLaunchKernelGGL((k_sl_step<T, S, IntTSE<T, true>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+      hipLaunchKernelGGL((k_sl_step<T, S, IntTSE<T, true>, FM>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
     } else {
       dispatch_spline(g, [&](auto KX, auto KZ) {
         constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
-        hipLaunchKernelGGL((k_sl_step<T, S, IntSpline<T, kx, kz>>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
+        hipLaunchKernelGGL((k_sl_step<T, S, IntSpline<T, kx, kz>, FM>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
       });
     }
   });
@@ -1536,28 +1565,28 @@ void launch_fwd_place(const GridDev<T>& g, const GridDev<T>& lg, ParticleSet<T> 
   hipLaunchKernelGGL((k_fwd_place<T>), grid_for(lg.np), dim3(kBlock), 0, s, g, lg, ps, st);
 }
 
-template <typename T>
+template <typename T, typename FM>
 void launch_sl_step2(const GridDev<T>& g, const GridDev<T>& lg, const int16_t* nodes, int fresh, int interp,
-                     const StageParams<T>& sp, CF3<T> fmold, F3<T> fmnew, DevStatus* st, hipStream_t s) {
+                     const StageParams<T>& sp, CF3<FM> fmold, F3<FM> fmnew, DevStatus* st, hipStream_t s) {
   const dim3 grid = grid_for(lg.np), blk(kBlock);
   with_storage<T>(sp.storage, [&](auto tag) {
     using S = decltype(tag);
     if (interp == kLinear) {
-      hipLaunchKernelGGL((k_sl_step2<T, S, IntLinear<T>>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold, fmnew,
+      hipLaunchKernelGGL((k_sl_step2<T, S, IntLinear<T>, FM>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold, fmnew,
                          st);
     } else if (interp == kNearest) {
-      hipLaunchKernelGGL((k_sl_step2<T, S, IntNearest<T>>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold, fmnew,
+      hipLaunchKernelGGL((k_sl_step2<T, S, IntNearest<T>, FM>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold, fmnew,
                          st);
     } else if (interp == kTSE) {
-      hipLaunchKernelGGL((k_sl_step2<T, S, IntTSE<T, false>>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold,
+      hipLaunchKernelGGL((k_sl_step2<T, S, IntTSE<T, false>, FM>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold,
                          fmnew, st);
     } else if (interp == kTSELimit) {
-      hipLaunchKernelGGL((k_sl_step2<T, S, IntTSE<T, true>>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold,
+      hipLaunchKernelGGL((k_sl_step2<T, S, IntTSE<T, true>, FM>), grid, blk, 0, s, g, lg, nodes, fresh, sp, fmold,
                          fmnew, st);
     } else {
       dispatch_spline(g, [&](auto KX, auto KZ) {
         constexpr int kx = decltype(KX)::value, kz = decltype(KZ)::value;
-        hipLaunchKernelGGL((k_sl_step2<T, S, IntSpline<T, kx, kz>>), grid, blk, 0, s, g, lg, nodes, fresh, sp,
+        hipLaunchKernelGGL((k_sl_step2<T, S, IntSpline<T, kx, kz>, FM>), grid, blk, 0, s, g, lg, nodes, fresh, sp,
                            fmold, fmnew, st);
       });
     }
@@ -1919,21 +1948,33 @@ template <typename T>
 void launch_vel_ghost(const GridDev<T>& g, F3<T> u, CF3<T> uold, DevStatus* st, hipStream_t s) {
   launch_vel_ghost_impl(g, u, uold, st, s);
 }
-template <typename T>
-void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st, hipStream_t s) {
+template <typename T, typename E>
+void launch_fm_ghost(const GridDev<T>& g, F3<E> fm, CF3<E> fmold, DevStatus* st, hipStream_t s) {
   launch_fm_ghost_impl(g, fm, fmold, st, s);
+}
+
+// dst = src widened to T (n values)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_widen(const float* __restrict__ src, T* __restrict__ dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = T(src[i]);
+}
+template <typename T>
+void launch_widen(const float* src, T* dst, long long n, hipStream_t s) {
+  const long long nb = std::min<long long>((n + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL((k_widen<T>), dim3((unsigned)std::max<long long>(nb, 1)), dim3(kBlock), 0, s, src, dst, n);
 }
 
 // explicit instantiations
 #define INST_T(T)                                                                                             \
   template void launch_vel_ghost<T>(const GridDev<T>&, F3<T>, CF3<T>, DevStatus*, hipStream_t);               \
   template void launch_cfl<T>(const GridDev<T>&, F3<T>, DevStatus*, hipStream_t);                             \
-  template void launch_fm_ghost<T>(const GridDev<T>&, F3<T>, CF3<T>, DevStatus*, hipStream_t);                \
+  template void launch_fm_ghost<T, T>(const GridDev<T>&, F3<T>, CF3<T>, DevStatus*, hipStream_t);             \
   template void launch_field_from_origin<T>(const GridDev<T>&, CF3<T>, F3<T>, hipStream_t);                   \
-  template void launch_spline_fit<T>(const GridDev<T>&, F3<T>, int, hipStream_t);                             \
+  template void launch_spline_fit<T, T, T>(const GridDev<T>&, F3<T>, int, hipStream_t, const T* const*);      \
   template void launch_spline_fit_T<T>(const GridDev<T>&, F3<T>, hipStream_t);                                \
-  template void launch_blend<T>(const GridDev<T>&, void* const*, int, CF3<T>, CF3<T>, T, T, DevStatus*, hipStream_t);     \
-  template void launch_sl_step<T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, F3<T>, DevStatus*,  \
+  template void launch_blend<T, T>(const GridDev<T>&, void* const*, int, CF3<T>, CF3<T>, T, T, DevStatus*, hipStream_t);  \
+  template void launch_sl_step<T, T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, F3<T>, DevStatus*, \
                                   hipStream_t, int);                                                          \
   template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, ParticleSet<T>, int, int, \
                                    DevStatus*, hipStream_t);                                                   \
@@ -1941,7 +1982,7 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
   template void launch_track_nodes<T>(const GridDev<T>&, const GridDev<T>&, int16_t*, DevStatus*, hipStream_t); \
   template void launch_fwd_place<T>(const GridDev<T>&, const GridDev<T>&, ParticleSet<T>, DevStatus*,          \
                                     hipStream_t);                                                             \
-  template void launch_sl_step2<T>(const GridDev<T>&, const GridDev<T>&, const int16_t*, int, int,             \
+  template void launch_sl_step2<T, T>(const GridDev<T>&, const GridDev<T>&, const int16_t*, int, int,          \
                                    const StageParams<T>&, CF3<T>, F3<T>, DevStatus*, hipStream_t);            \
   template void launch_expand_periodic<T>(const GridDev<T>&, const T*, F3<T>, hipStream_t);                    \
   template void launch_field_at_nodes<T>(const GridDev<T>&, const GridDev<T>&, const int16_t*, int, CF3<T>,   \
@@ -1966,5 +2007,20 @@ void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st,
 
 INST_T(float)
 INST_T(double)
+// FP64 compute over FP32 flowmap storage
+template void launch_fm_ghost<double, float>(const GridDev<double>&, F3<float>, CF3<float>, DevStatus*, hipStream_t);
+template void launch_spline_fit<double, float, float>(const GridDev<double>&, F3<float>, int, hipStream_t,
+                                                      const float* const*);
+template void launch_spline_fit<double, float, double>(const GridDev<double>&, F3<float>, int, hipStream_t,
+                                                       const double* const*);
+template void launch_blend<double, float>(const GridDev<double>&, void* const*, int, CF3<float>, CF3<float>, double,
+                                          double, DevStatus*, hipStream_t);
+template void launch_sl_step<double, float>(const GridDev<double>&, int, const StageParams<double>&, CF3<float>,
+                                            F3<float>, DevStatus*, hipStream_t, int);
+template void launch_sl_step2<double, float>(const GridDev<double>&, const GridDev<double>&, const int16_t*, int, int,
+                                             const StageParams<double>&, CF3<float>, F3<float>, DevStatus*,
+                                             hipStream_t);
+template void launch_widen<double>(const float*, double*, long long, hipStream_t);
+template void launch_widen<float>(const float*, float*, long long, hipStream_t);
 
 }  // namespace lcs

@@ -70,15 +70,17 @@ struct DevStatus {
 // pack the three components of a node into one 8-byte record (u, v, w, 0),
 // so a node's velocity is a single aligned load and a 4-node spline row two
 // 16-byte loads (against three for the FP32 component arrays); values are
-// widened to FP32 in registers.
-enum Storage { kStoreNative = 0, kStoreF16 = 1, kStoreBF16 = 2 };
+// widened to FP32 in registers.  kStoreF32 (FP64 compute only) keeps one
+// FP32 array per component: the gathers move FP32 rows and the sums, the
+// positions and the flowmap stay FP64.
+enum Storage { kStoreNative = 0, kStoreF16 = 1, kStoreBF16 = 2, kStoreF32 = 3 };
 struct half_t { uint16_t u; };  // IEEE binary16
 struct bf16_t { uint16_t u; };  // bfloat16 (upper half of a binary32)
 struct __attribute__((aligned(8))) half4_t { half_t e[4]; };
 struct __attribute__((aligned(8))) bf164_t { bf16_t e[4]; };
 // bytes of one stage velocity field (all three components) over nt nodes
 inline size_t stage_field_bytes(int storage, size_t native, size_t nt) {
-  return storage == kStoreNative ? 3 * native * nt : 8 * nt;
+  return storage == kStoreNative ? 3 * native * nt : (storage == kStoreF32 ? 3 * sizeof(float) * nt : 8 * nt);
 }
 
 // Per-subcycle integration parameters (lp_motion_m.f90:129-360).
@@ -128,20 +130,27 @@ template <typename T>
 void launch_vel_ghost(const GridDev<T>& g, F3<T> u, CF3<T> uold, DevStatus* st, hipStream_t stream);
 template <typename T>
 void launch_cfl(const GridDev<T>& g, F3<T> u, DevStatus* st, hipStream_t stream);
+// flowmap fields hold E = T or, under FP64 compute, FP32 (FLOWMAP_STORAGE)
+template <typename T, typename E>
+void launch_fm_ghost(const GridDev<T>& g, F3<E> fm, CF3<E> fmold, DevStatus* st, hipStream_t stream);
+// coef = spline coefficients of the data in src[0..2] (element type ES), or
+// of coef itself (in place) when src is null; the solve runs in T
+template <typename T, typename E, typename ES = E>
+void launch_spline_fit(const GridDev<T>& g, F3<E> coef, int ncomp, hipStream_t stream,
+                       const ES* const* src = nullptr);
+// dst = src widened to T (n values)
 template <typename T>
-void launch_fm_ghost(const GridDev<T>& g, F3<T> fm, CF3<T> fmold, DevStatus* st, hipStream_t stream);
-template <typename T>
-void launch_spline_fit(const GridDev<T>& g, F3<T> coef, int ncomp, hipStream_t stream);
+void launch_widen(const float* src, T* dst, long long n, hipStream_t stream);
 // out (element type per `storage`) = ((1-ct) a + ct b) * inv, inv a power
 // of two (1 for native storage); a 16-bit value out of range sets kErrRange
-template <typename T>
-void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<T> a, CF3<T> b, T ct, T inv,
+template <typename T, typename A>
+void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<A> a, CF3<A> b, T ct, T inv,
                   DevStatus* st, hipStream_t stream);
 // tiled: FP32 B-spline steps on 3-D grids stage the gathered coefficients
 // in LDS (bit-identical results)
-template <typename T>
-void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<T> fmold,
-                    F3<T> fmnew, DevStatus* st, hipStream_t stream, int tiled = 0);
+template <typename T, typename FM>
+void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, CF3<FM> fmold,
+                    F3<FM> fmnew, DevStatus* st, hipStream_t stream, int tiled = 0);
 // compensated: two-sum updates of position and displacement (uses xl, dl);
 // np particles
 template <typename T>
@@ -165,9 +174,9 @@ void launch_fwd_place(const GridDev<T>& g, const GridDev<T>& lg, ParticleSet<T> 
 // one semi-Lagrangian sub-cycle of the flowmap on lg; fresh bit 0: the stage
 // velocities and the particle BC use the tracked nodes as they are (else
 // clamped to the interior), bit 1: the MASK test does
-template <typename T>
+template <typename T, typename FM>
 void launch_sl_step2(const GridDev<T>& g, const GridDev<T>& lg, const int16_t* nodes, int fresh, int interp,
-                     const StageParams<T>& sp, CF3<T> fmold, F3<T> fmnew, DevStatus* st, hipStream_t stream);
+                     const StageParams<T>& sp, CF3<FM> fmold, F3<FM> fmnew, DevStatus* st, hipStream_t stream);
 // ghosted field on g from an interior one: interior copied, periodic ghosts
 // from their images, other ghosts zero (exchange_sdata of a zeroed field)
 template <typename T>

@@ -194,6 +194,9 @@ def _vs_oracle(lib, opts, interp, n=20, T=0.4, h=0.2, nsteps=10):
     lib.finalize()
     lib.init(X, Y, Z, fl)
     o = Oracle(X, Y, Z, fl)
+    # storage options that take effect at a reset apply from the first one
+    for k, v in opts.items():
+        lib.set_option(k, v)
     hs = [lib.diagnostic_init(d, 0, T, h, "x") for d in (F.FTLE_FWD, F.FTLE_BKWD)]
     for d in (F.FTLE_FWD, F.FTLE_BKWD):
         o.diagnostic_init(d, 0, T, h, "x")
@@ -247,6 +250,27 @@ def test_compensated_positions_against_fp64_oracle(sp, interp):
     print(f"compensated interp {interp}: flowmap {ec:.3e} (plain {ep:.3e}), ftle {fc:.3e} (plain {fp:.3e})")
     assert ec <= 2e-5 and fc <= 1e-3
     assert ec <= 1.5 * ep
+
+
+@pytest.mark.parametrize("interp", [F.LINEAR, F.CUBIC])
+@pytest.mark.parametrize("which", ["velocity", "flowmap", "both"])
+def test_fp32_storage_under_fp64_compute_against_oracle(dp, interp, which):
+    """FP64 compute with FP32 storage of the gathered stage velocities
+    (VELOCITY_STORAGE = LCS_FP32) and/or of the backward flowmap and its
+    coefficients (FLOWMAP_STORAGE = LCS_FP32): flowmaps and FTLE stay within
+    a few FP32 roundoffs of the FP64 oracle, and differ measurably from the
+    all-FP64 run (the FP32 storage path is the one that ran)."""
+    import libcfd2lcs_amd as L
+    opts = {}
+    if which in ("velocity", "both"):
+        opts["VELOCITY_STORAGE"] = L.LCS_FP32
+    if which in ("flowmap", "both"):
+        opts["FLOWMAP_STORAGE"] = L.LCS_FP32
+    e32, f32 = _vs_oracle(dp, opts, interp)
+    e64, f64 = _vs_oracle(dp, {}, interp)
+    print(f"{which} interp {interp}: flowmap {e32:.3e} (fp64 {e64:.3e}), ftle {f32:.3e} (fp64 {f64:.3e})")
+    assert e32 <= 2e-6 and f32 <= 2e-5
+    assert e32 > 100 * e64
 
 
 def test_16bit_storage_needs_fp32_compute(dp):
