@@ -327,7 +327,7 @@ def run_adjoint(variant, n, steps, ckpt, dt=0.02):
                        "snapshots": "ABC + seeded Gaussian field (sigma 0.05, smoothed over 4 cells)"}}
 
 
-def run_turbulence(variant, opts, n, steps, dt=0.005, T=1.0, h=0.1, kmax=42, seed=0):
+def run_turbulence(variant, opts, n, steps, dt=0.005, T=1.0, h=0.1, kmax=42, seed=0, field=None):
     """Precision-study workload: forward FTLE (T = 1, h = 0.1, RK3, cubic
     B-spline, CFL 0.5) of the random-Fourier turbulent field at n^3 over
     `steps` steps of dt; the velocity is generated on the device each step
@@ -337,13 +337,17 @@ def run_turbulence(variant, opts, n, steps, dt=0.005, T=1.0, h=0.1, kmax=42, see
     import libcfd2lcs_amd as L
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from flows import RandomFourierField
-    field = RandomFourierField(n, kmax=kmax, seed=seed)
+    if field is None:
+        field = RandomFourierField(n, kmax=kmax, seed=seed)
     lib = L.Cfd2lcs(variant)
     lib.finalize()
     hgrid = 2 * np.pi / n
     a = 0.5 * hgrid + np.arange(n) * hgrid
     X, Y, Z = np.meshgrid(a, a, a, indexing="ij")
     lib.init(X, Y, Z, np.zeros(X.shape, dtype=np.int32))
+    # storage options take effect at a reset: set them before the diagnostic
+    for k, v in (opts or {}).items():
+        lib.set_option(k, v)
     hd = lib.diagnostic_init(L.FTLE_FWD, 0, T, h, "fwdFTLE")
     lib.set_option("INTEGRATOR", L.RK3)
     lib.set_option("INTERPOLATOR", L.CUBIC)
@@ -373,14 +377,23 @@ def run_turbulence(variant, opts, n, steps, dt=0.005, T=1.0, h=0.1, kmax=42, see
     return {"ftle": ftle, "updates_per_s": upd / busy, "outputs": nout}
 
 
-def precision_study(n, steps):
+def precision_study(n, steps, T=1.0):
+    """Velocity-storage sweep of the turbulence workload: every mode runs the
+    same snapshots; FTLE error of the last output against the FP64 run,
+    relative to the largest |FTLE| of the FP64 run."""
     import libcfd2lcs_amd as L
-    runs = [("fp64", "dp", {}), ("fp32", "sp", {}),
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from flows import RandomFourierField
+    runs = [("fp64", "dp", {}),
+            ("fp32_storage_fp64_compute", "dp", {"VELOCITY_STORAGE": L.LCS_FP32, "FLOWMAP_STORAGE": L.LCS_FP32}),
+            ("fp32", "sp", {}),
             ("fp16", "sp", {"VELOCITY_STORAGE": L.LCS_FP16, "COMPENSATED": L.LCS_TRUE}),
             ("bf16", "sp", {"VELOCITY_STORAGE": L.LCS_BF16, "COMPENSATED": L.LCS_TRUE})]
+    field = RandomFourierField(n, kmax=42, seed=0)
     res = {}
     for name, var, opts in runs:
-        res[name] = run_turbulence(var, opts, n, steps)
+        res[name] = run_turbulence(var, opts, n, steps, T=T, field=field)
+    del field
     ref = res["fp64"]["ftle"]
     scale = float(np.abs(ref).max())
     out = {}
@@ -396,8 +409,9 @@ def precision_study(n, steps):
             "data": "synthetic: divergence-free random Fourier field, |k| <= 42, E(k) ~ k^4 exp(-2 (k/4)^2), "
                     "unit RMS, phases advancing at |k|^(2/3), seed 0",
             "config": {"workload": f"turbulence{n}_rk3_cubic_fwd", "grid": [n] * 3, "dt": 0.005, "steps": steps,
-                       "T": 1.0, "h": 0.1, "integrator": "RK3", "interpolator": "CUBIC",
+                       "T": T, "h": 0.1, "integrator": "RK3", "interpolator": "CUBIC",
                        "compare": "FTLE of the last output"},
+            "normalisation": "max |FTLE - FTLE_fp64| / max |FTLE_fp64| over all nodes",
             "ftle_scale": scale, "by_storage": out}
 
 
@@ -411,17 +425,67 @@ def timers(lib):
     return np.array(list(sec)), np.array(list(cnt))
 
 
-def kernel_bytes_per_point(name, s):
-    """Algorithmic HBM bytes per grid point per launch of each hot kernel, for
-    storage size s (4 or 8): the semi-Lagrangian step reads both velocity
-    snapshots and the old flowmap and writes the new one (6 s + 6 s); the
-    forward step reads both snapshots and reads+writes position,
-    displacement and the tracked node (6 s + 2 (6 s + 6))."""
+def canonical_bytes_per_update(s_v, s_fm):
+    """Algorithmic HBM bytes of one grid-point update: both velocity
+    snapshots read once (6 s_v) and the flowmap read and written once
+    (6 s_fm), for stored velocity size s_v and flowmap (or particle state)
+    size s_fm; a compensated FP32 hi + lo flowmap counts s_fm = 8.  FP64 96 B,
+    FP32 48 B, 16-bit velocity + FP32 flowmap 36 B, 16-bit velocity +
+    compensated positions 60 B."""
+    return 6 * s_v + 6 * s_fm
+
+
+def layout_bytes_per_point(name, s):
+    """Bytes the kernel's own layout moves per point and launch (storage
+    size s), beside the canonical count: the forward step also reads and
+    writes position, displacement and the int16 tracked node (6 s + 2 (6 s + 6))."""
     if name == "k_sl_step":
         return 12 * s
     if name == "k_fwd_step":
         return 6 * s + 2 * (6 * s + 6)
     return None
+
+
+# FP32 VALU peak of MI355X (dense, packed FMA) and the flops of one
+# grid-point update: about 2.8 kflop for RK4 + cubic B-spline (four
+# 64-tap, 3-component stencils with their basis evaluations) and 190 for
+# RK2 + linear
+VALU_PEAK_TFLOPS = 157.3
+FLOPS_PER_UPDATE = {"cubic_rk4": 2800.0, "linear_rk2": 190.0}
+
+
+def source_digest():
+    """SHA-256 over the HIP/C++ sources of the library: counter figures
+    taken from a profile apply only to the build they were measured on."""
+    import hashlib
+    d = os.path.join(ROOT, "libcfd2lcs_amd", "csrc")
+    hs = hashlib.sha256()
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".hip", ".h", ".cpp")):
+            hs.update(f.encode())
+            with open(os.path.join(d, f), "rb") as fh:
+                hs.update(fh.read())
+    return hs.hexdigest()
+
+
+def profile_figures(name, n):
+    """HBM bytes per launch (FETCH_SIZE / WRITE_SIZE counter passes scaled by
+    the copy-kernel calibration) and VALU busy fraction of kernel `name` at
+    n^3 from profiles/, with the source digest they were measured on; None
+    when no profile of the current sources exists."""
+    path = os.path.join(ROOT, "profiles", "abc256_rk4_cubic_fp32_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            tr = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    k = tr.get("kernels", {}).get(name)
+    if not k or k.get("n") != n:
+        return None
+    return {"source_sha256": tr.get("source_sha256"), "bytes_per_launch": k.get("bytes_per_launch"),
+            "valu_busy": k.get("valu_busy"), "ta_busy": k.get("ta_busy"), "file": os.path.relpath(path, ROOT)}
 
 
 def cpu_baseline(n_cpu, budget_s):
@@ -499,6 +563,14 @@ def main():
                     help="flow grid of the diagnostic-grid workloads (resolution 1; AUX_GRID); 0 skips them")
     ap.add_argument("--no-nonrect", dest="nonrect", action="store_false",
                     help="skip the non-rectilinear (warped) grid workload")
+    ap.add_argument("--no-precise", dest="precise", action="store_false",
+                    help="skip the FP64-compute run with FP32 velocity and flowmap storage")
+    ap.add_argument("--no-linear", dest="linear", action="store_false",
+                    help="skip the RK2 + linear runs (the HBM-bound scheme) at the headline size")
+    ap.add_argument("--study-n", type=int, default=256,
+                    help="grid of the reduced precision study folded into the default run; 0 skips it")
+    ap.add_argument("--study-steps", type=int, default=100,
+                    help="steps of the reduced precision study (forward FTLE at T = study-steps * 0.005)")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -537,6 +609,28 @@ def main():
         print(f"[bench] mixed ({args.mixed} velocity, compensated FP32 positions): "
               f"{rmix['updates'] / rmix['elapsed']:.4e} updates/s", file=sys.stderr, flush=True)
 
+    # FP64 compute with FP32 velocity and flowmap storage: every gathered
+    # stage velocity, spline coefficient and backward flowmap is stored in
+    # FP32, positions, sums, fits, composition and FTLE stay FP64
+    rprec = None
+    if args.precise:
+        import libcfd2lcs_amd as L
+        rprec = run_gpu("dp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle,
+                        {"VELOCITY_STORAGE": L.LCS_FP32, "FLOWMAP_STORAGE": L.LCS_FP32})
+        print(f"[bench] fp64 compute, fp32 storage: {rprec['updates'] / rprec['elapsed']:.4e} updates/s",
+              file=sys.stderr, flush=True)
+    # RK2 + linear at the headline size: the scheme whose update is bound by
+    # HBM rather than by the gathers' VALU work
+    rlin = {}
+    if args.linear:
+        import libcfd2lcs_amd as L
+        for key, var, o in (("fp32", "sp", {}), ("fp16_compensated", "sp", {"VELOCITY_STORAGE": L.LCS_FP16,
+                                                                             "COMPENSATED": L.LCS_TRUE})):
+            o = dict(o, INTEGRATOR=L.RK2, INTERPOLATOR=L.LINEAR)
+            rlin[key] = run_gpu(var, args.n, args.steps, args.warmup, dt, T, h, False, o)
+            print(f"[bench] rk2 + linear {key}: {rlin[key]['updates'] / rlin[key]['elapsed']:.4e} updates/s",
+                  file=sys.stderr, flush=True)
+
     # diagnostic grids: a refined LCS grid (resolution 1: twice the nodes per
     # axis, backward sub-cycles halved by dt_factor) and AUX_GRID (seven
     # particle sets per direction), FP32, same scheme, one h interval
@@ -553,6 +647,11 @@ def main():
         rwarp = run_gpu("sp", args.n, args.steps, args.warmup, dt, T, h, False, warp=0.05)
         print(f"[bench] non-rectilinear: {rwarp['updates'] / rwarp['elapsed']:.4e} updates/s", file=sys.stderr,
               flush=True)
+
+    rstudy = None
+    if args.study_n > 0 and args.study_steps > 0:
+        rstudy = precision_study(args.study_n, args.study_steps, T=round(args.study_steps * 0.005, 6))
+        print(f"[bench] precision study: fp32 max rel {rstudy['value']:.3e}", file=sys.stderr, flush=True)
 
     radj = None
     if args.adjoint_steps > 0:
@@ -584,29 +683,48 @@ def main():
                       "ftle64_at_max": float(b.ravel()[k]), "ftle64_min": float(b.min()),
                       "frac_above_1e-4_rel": float((e > 1e-4 * float(np.abs(b).max())).mean())}
         return {"max_abs": mx, "rms": (ss / cnt) ** 0.5, "max_rel": mx / scale, "ftle_scale": scale,
+                "normalisation": "max |FTLE - FTLE_fp64| / max |FTLE_fp64| over both directions",
                 "per_direction": per}
+    def roofline(r, s_v, s_fm, flops_key, layout_s=None, s_pos=None):
+        """HBM roofline of the run's dominant hot kernel at the canonical
+        bytes per update (velocity size s_v; flowmap size s_fm for the
+        semi-Lagrangian step, particle state size s_pos, default s_fm, for the
+        forward step), with the VALU fraction at the scheme's flops per
+        update (both from the kernel's mean launch time, HIP events on the
+        library's stream)."""
+        ks_ = r["kernel_sec"]
+        dk = max(("k_sl_step", "k_fwd_step"), key=lambda k: ks_[k])
+        avg_ = ks_[dk] / max(r["kernel_cnt"][dk], 1)
+        bpu = canonical_bytes_per_update(s_v, s_fm if dk == "k_sl_step" or s_pos is None else s_pos)
+        ach = bpu * r["np"] / avg_ / 1e9
+        tfl = FLOPS_PER_UPDATE[flops_key] * r["np"] / avg_ / 1e12
+        out_ = {"kernel": dk, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_ * 1e3, 4),
+                "bytes_per_update": bpu,
+                "valu": {"flops_per_update": FLOPS_PER_UPDATE[flops_key], "achieved_tflops": round(tfl, 3),
+                         "peak_tflops": VALU_PEAK_TFLOPS, "frac": round(tfl / VALU_PEAK_TFLOPS, 4)}}
+        if layout_s is not None:
+            out_["layout_bytes_per_point"] = layout_bytes_per_point(dk, layout_s)
+        return out_
+
     # dominant kernel of the FP32 run
-    ks = r32["kernel_sec"]
-    dom = max(("k_sl_step", "k_fwd_step"), key=lambda k: ks[k])
-    cnt = max(r32["kernel_cnt"][dom], 1)
-    avg = ks[dom] / cnt
-    bpp = kernel_bytes_per_point(dom, 4)
-    achieved = bpp * r32["np"] / avg / 1e9
-    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "avg_launch_ms": round(avg * 1e3, 4), "bytes_per_point": bpp,
-            "note": "gather kernels: limited by the L1 texture-address path (TA busy ~72 %, VALU ~33 %, "
-                    "counter passes in profiles/), not by HBM"}
-    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter passes of
-    # the same workload, scaled by the copy-kernel calibration
-    traffic_file = os.path.join(ROOT, "profiles", "abc256_rk4_cubic_fp32_traffic.json")
-    if os.path.exists(traffic_file):
-        try:
-            tr = json.load(open(traffic_file))
-            if dom in tr and tr[dom].get("n") == args.n:
-                roof["traffic"] = tr[dom]["bytes_per_launch"]
-        except Exception:
-            pass
+    roof = roofline(r32, 4, 4, "cubic_rk4", layout_s=4)
+    dom = roof["kernel"]
+    roof["note"] = ("gather kernels: limited by the texture-address path (one wave-level row load costs about "
+                    "16 TA cycles), not by HBM; counter passes in profiles/")
+    # HBM bytes per launch and VALU busy from the counter passes of the same
+    # workload, when they were measured on these sources
+    pf = profile_figures(dom, args.n)
+    digest = source_digest()
+    if pf is not None:
+        if pf["source_sha256"] == digest:
+            roof["traffic"] = pf["bytes_per_launch"]
+            roof["valu"]["measured_busy"] = pf["valu_busy"]
+            roof["ta_busy"] = pf["ta_busy"]
+            roof["traffic_source"] = pf["file"]
+        else:
+            roof["traffic_note"] = ("counter profile %s was measured on other sources (digest %s, current %s); "
+                                    "traffic not reported" % (pf["file"], str(pf["source_sha256"])[:12], digest[:12]))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_n, args.cpu_budget)
@@ -644,6 +762,7 @@ def main():
             "value": rmix["updates"] / rmix["elapsed"], "ms_per_step": 1e3 * rmix["elapsed"] / args.steps,
             "over_fp64": (rmix["updates"] / rmix["elapsed"]) / (r64["updates"] / r64["elapsed"]) if r64 else None,
             "ftle_fields_per_s": u2(rmix), "ftle_error_vs_fp64": ftle_err(rmix),
+            "roofline": roofline(rmix, {"fp16": 2, "bf16": 2, "fp32": 4}[args.mixed], 4, "cubic_rk4", s_pos=8),
             "kernel_ms": {k: round(1e3 * s_ / max(rmix["kernel_cnt"][k], 1), 4)
                           for k, s_ in rmix["kernel_sec"].items()}},
         "diagnostic_grids": {
@@ -665,10 +784,31 @@ def main():
             "ms_per_step": 1e3 * rwarp["elapsed"] / args.steps,
             "kernel_ms": {k: round(1e3 * s_ / max(rwarp["kernel_cnt"][k], 1), 4)
                           for k, s_ in rwarp["kernel_sec"].items()}},
+        "fp64_compute_fp32_storage": None if rprec is None else {
+            "velocity_storage": "f32", "flowmap_storage": "f32", "compute": "f64",
+            "value": rprec["updates"] / rprec["elapsed"], "ms_per_step": 1e3 * rprec["elapsed"] / args.steps,
+            "over_fp64": (rprec["updates"] / rprec["elapsed"]) / (r64["updates"] / r64["elapsed"]) if r64 else None,
+            "ftle_fields_per_s": u2(rprec), "ftle_error_vs_fp64": ftle_err(rprec),
+            "roofline": roofline(rprec, 4, 4, "cubic_rk4"),
+            "kernel_ms": {k: round(1e3 * s_ / max(rprec["kernel_cnt"][k], 1), 4)
+                          for k, s_ in rprec["kernel_sec"].items()}},
+        "linear_rk2": {
+            key: {"value": r["updates"] / r["elapsed"], "unit": "grid-pt updates/s",
+                  "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+                  "roofline": roofline(r, 2 if key.startswith("fp16") else 4, 4, "linear_rk2",
+                                       s_pos=8 if key.startswith("fp16") else 4),
+                  "kernel_ms": {k: round(1e3 * s_ / max(r["kernel_cnt"][k], 1), 4)
+                                for k, s_ in r["kernel_sec"].items()}}
+            for key, r in rlin.items()} or None,
+        "linear_rk2_note": "ABC %d^3, RK2 + linear, FWD + BKWD, CFL 0.5; canonical bytes per update 48 (fp32); fp16 "
+                           "velocity: 36 (semi-Lagrangian step) and 60 (forward step, compensated FP32 "
+                           "positions)" % args.n,
+        "precision_study": rstudy,
         "bwd_fwd_ratio": None if radj is None else radj["bwd_fwd_ratio"],
         "adjoint": radj,
-        "kernel_ms": {k: round(1e3 * s / max(r32["kernel_cnt"][k], 1), 4) for k, s in ks.items()},
+        "kernel_ms": {k: round(1e3 * s / max(r32["kernel_cnt"][k], 1), 4) for k, s in r32["kernel_sec"].items()},
         "roofline": roof,
+        "source_sha256": digest,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -68,7 +68,7 @@ struct Options {
   // reset
   int fm_storage = kStoreNative;
   int compensated = 0;    // two-sum position/displacement updates of forward particles
-  int sl_tiled = 0;       // LDS-tiled semi-Lagrangian B-spline step (results identical; slower here)
+  int sl_tiled = 1;       // LDS bricks for the FP32 cubic semi-Lagrangian step (results identical)
   // output files: FLOWMAP_IO, VELOCITY_IO, BCFLAG_IO (data_m.f90:44-54) pick
   // the variables of the Tecplot FTLE file; write_lcs switches that file
   // (always written by the reference) and write_slots the flowmap/velocity

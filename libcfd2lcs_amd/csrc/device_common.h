@@ -235,10 +235,11 @@ __device__ __forceinline__ int bs_locate(const T* __restrict__ t, int n, T x, bo
 // Horner in the normalised interval coordinate)
 template <typename T, int K>
 __device__ __forceinline__ void bs_weights(const T* __restrict__ t, const T* __restrict__ poly, int left, T x, T v[K],
-                                           const T* xs = nullptr, const AxisLds<T>* al = nullptr) {
+                                           bool hx = false, T xb = T(0), T xo = T(0),
+                                           const AxisLds<T>* al = nullptr) {
   auto eval = [&](auto P, T tl) {
-    // xs: {base coordinate, offset} of x (Split)
-    const T s = (xs ? (xs[0] - tl) + xs[1] : x - tl) * P[K * K];
+    // hx: x is given as base coordinate xb plus offset xo (Split)
+    const T s = (hx ? (xb - tl) + xo : x - tl) * P[K * K];
 #pragma unroll
     for (int a = 0; a < K; a++) {
       T w = P[a * K + K - 1];
@@ -262,12 +263,12 @@ __device__ __forceinline__ void bs_weights(const T* __restrict__ t, const T* __r
 // silently, bspline_sub_module.f90:634-636).
 template <typename T, int K>
 __device__ __forceinline__ int bs_setup(const T* __restrict__ t, const T* __restrict__ poly, int n, T x, bool xaxis,
-                                        int hint, int& base, T v[K], const T* xs = nullptr,
+                                        int hint, int& base, T v[K], bool hx = false, T xb = T(0), T xo = T(0),
                                         const AxisLds<T>* al = nullptr) {
   int left = 0;
   const int r = bs_locate<T, K>(t, n, x, xaxis, hint, left, al);
   if (r) return r;
-  bs_weights<T, K>(t, poly, left, x, v, xs, al);
+  bs_weights<T, K>(t, poly, left, x, v, hx, xb, xo, al);
   base = left - K;
   return 0;
 }
@@ -336,25 +337,56 @@ __device__ __forceinline__ int spline_setup(const GridDev<T>& g, const T q[3], c
                                             SplineW<T, KX, KZ>& s, const Split<T>* sq = nullptr,
                                             const SplineLds<T>* sl = nullptr) {
   int bx, by, bz;
-  T xs[3][2];
-  if (sq) {
+  const bool h = sq != nullptr;
+  T xb[3] = {T(0), T(0), T(0)}, xo[3] = {T(0), T(0), T(0)};
+  if (h) {
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      xs[d][0] = sq->qb[d];
-      xs[d][1] = sq->off[d];
+      xb[d] = sq->qb[d];
+      xo[d] = sq->off[d];
     }
   }
-  int rx = bs_setup<T, KX>(g.kn[0], g.bpoly[0], g.N[0], q[0], true, nd[0] + (KX + 2) / 2, bx, s.wx,
-                           sq ? xs[0] : nullptr, sl ? &sl->ax[0] : nullptr);
-  int ry = bs_setup<T, KX>(g.kn[1], g.bpoly[1], g.N[1], q[1], false, nd[1] + (KX + 2) / 2, by, s.wy,
-                           sq ? xs[1] : nullptr, sl ? &sl->ax[1] : nullptr);
-  int rz = bs_setup<T, KZ>(g.kn[2], g.bpoly[2], g.N[2], q[2], false, nd[2] + (KZ + 2) / 2, bz, s.wz,
-                           sq ? xs[2] : nullptr, sl ? &sl->ax[2] : nullptr);
+  int rx = bs_setup<T, KX>(g.kn[0], g.bpoly[0], g.N[0], q[0], true, nd[0] + (KX + 2) / 2, bx, s.wx, h, xb[0], xo[0],
+                           sl ? &sl->ax[0] : nullptr);
+  int ry = bs_setup<T, KX>(g.kn[1], g.bpoly[1], g.N[1], q[1], false, nd[1] + (KX + 2) / 2, by, s.wy, h, xb[1], xo[1],
+                           sl ? &sl->ax[1] : nullptr);
+  int rz = bs_setup<T, KZ>(g.kn[2], g.bpoly[2], g.N[2], q[2], false, nd[2] + (KZ + 2) / 2, bz, s.wz, h, xb[2], xo[2],
+                           sl ? &sl->ax[2] : nullptr);
   if (rx == 1 || ry == 1 || rz == 1) return 1;
   if (ry || rz) return 2;
   s.base = bx + g.N[0] * (by + g.N[1] * bz);
   s.b3[0] = bx; s.b3[1] = by; s.b3[2] = bz;
   return 0;
+}
+
+// Copies into LDS the basis-table rows and knots of the intervals of points
+// within a cell of nodes [lo[d], hi[d]] (hints node + (K+2)/2, at most two
+// intervals away), at most cap[d] rows per axis; no synchronisation.
+template <typename T, int KX, int KZ>
+__device__ __forceinline__ void copy_spline_tables(const GridDev<T>& g, const int lo[3], const int hi[3], lptr<T> buf,
+                                                   const int cap[3], SplineLds<T>& sl) {
+  size_t off = 0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int K = d < 2 ? KX : KZ, RW = K * K + 1;
+    AxisLds<T>& al = sl.ax[d];
+    int r0 = lo[d] + (K + 2) / 2 - 2, r1 = hi[d] + (K + 2) / 2 + 2;
+    r0 = max(r0, K);
+    r1 = min(r1, g.N[d]);
+    int nr = r1 - r0 + 1;
+    nr = nr < 0 ? 0 : (nr > cap[d] ? cap[d] : nr);
+    lptr<T> pb = buf + off;
+    lptr<T> kb = buf + off + (size_t)cap[d] * RW;
+    off += (size_t)cap[d] * RW + cap[d] + 2;
+    al.poly = pb;
+    al.knot = kb;
+    al.r0 = r0;
+    al.nr = nr;
+    const T* P = g.bpoly[d] + (size_t)r0 * RW;
+    for (int e = threadIdx.x; e < nr * RW; e += blockDim.x) pb[e] = P[e];
+    const T* t = g.kn[d] + (r0 - 1);
+    for (int e = threadIdx.x; e < nr + 2; e += blockDim.x) kb[e] = t[e];
+  }
 }
 
 // Stages into LDS the basis-table rows and knots of the intervals a
@@ -393,28 +425,8 @@ __device__ __forceinline__ void stage_spline_tables(const GridDev<T>& g, const i
     }
   }
   __syncthreads();
-  size_t off = 0;
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const int K = d < 2 ? KX : KZ, RW = K * K + 1;
-    AxisLds<T>& al = sl.ax[d];
-    int r0 = red[d] + (K + 2) / 2 - 2, r1 = red[3 + d] + (K + 2) / 2 + 2;
-    r0 = max(r0, K);
-    r1 = min(r1, g.N[d]);
-    int nr = r1 - r0 + 1;
-    nr = nr < 0 ? 0 : (nr > cap[d] ? cap[d] : nr);
-    lptr<T> pb = buf + off;
-    lptr<T> kb = buf + off + (size_t)cap[d] * RW;
-    off += (size_t)cap[d] * RW + cap[d] + 2;
-    al.poly = pb;
-    al.knot = kb;
-    al.r0 = r0;
-    al.nr = nr;
-    const T* P = g.bpoly[d] + (size_t)r0 * RW;
-    for (int e = threadIdx.x; e < nr * RW; e += blockDim.x) pb[e] = P[e];
-    const T* t = g.kn[d] + (r0 - 1);
-    for (int e = threadIdx.x; e < nr + 2; e += blockDim.x) kb[e] = t[e];
-  }
+  const int lo3[3] = {red[0], red[1], red[2]}, hi3[3] = {red[3], red[4], red[5]};
+  copy_spline_tables<T, KX, KZ>(g, lo3, hi3, buf, cap, sl);
   __syncthreads();
 }
 // LDS values stage_spline_tables needs for capacities cap (rows per axis)
@@ -704,10 +716,21 @@ __device__ __forceinline__ void stage_scale(const StageParams<T>& sp, T up[3]) {
 // (lp_motion_m.f90:239-247).  The first stage of every scheme samples the
 // velocity at the start position xn, which first(up) supplies (an
 // interpolation, or the nodal value when xn is a node).
-template <typename S, typename T, class V, class V0>
+// FIX >= 0 fixes the scheme at compile time (the kernel then holds one
+// scheme's stages only).  hook(std::integral_constant<int, k>) runs before
+// the gather of stage field k (1..3).
+struct NoHook {
+  template <class K>
+  __device__ __forceinline__ void operator()(K) const {}
+};
+template <typename S, int FIX = -1, typename T, class V, class V0, class H = NoHook>
 __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StageParams<T>& sp, const V& vel,
                                               const V0& first, const int nd[3], const T xn[3], T inc[3],
-                                              unsigned long long& oob, const T* xnl = nullptr) {
+                                              unsigned long long& oob, const T* xnl = nullptr,
+                                              const H& hook = H{}) {
+  using K1 = std::integral_constant<int, 1>;
+  using K2 = std::integral_constant<int, 2>;
+  using K3 = std::integral_constant<int, 3>;
   const T HALF = T(1) / T(2), THIRD = T(1) / T(3), TWOTHIRD = T(2) / T(3), SIXTH = T(1) / T(6), TWO = T(2);
   const T dt = sp.dt;
   T up[3], k[3], xp[3];
@@ -723,7 +746,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
       sq.off[d] = xnl ? xnl[d] + o : o;
     }
   };
-  switch (sp.integ) {
+  switch (FIX >= 0 ? FIX : sp.integ) {
     case kEuler:
     case kTrapezoidal:
       first(up);
@@ -735,6 +758,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
 #pragma unroll
       for (int d = 0; d < 3; d++) k[d] = dt * up[d];
       at(HALF, k);
+      hook(K1{});
       vel(g, xp, nd, stage<S>(sp, 1), up, oob, &sq);
       stage_scale<S>(sp, up);
 #pragma unroll
@@ -748,6 +772,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
         inc[d] = SIXTH * k[d];
       }
       at(HALF, k);
+      hook(K1{});
       vel(g, xp, nd, stage<S>(sp, 1), up, oob, &sq);
       stage_scale<S>(sp, up);
 #pragma unroll
@@ -756,6 +781,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
         inc[d] = inc[d] + TWOTHIRD * k[d];
       }
       at(TWO, k);
+      hook(K2{});
       vel(g, xp, nd, stage<S>(sp, 2), up, oob, &sq);
       stage_scale<S>(sp, up);
 #pragma unroll
@@ -770,6 +796,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
         inc[d] = SIXTH * k[d];
       }
       at(HALF, k);
+      hook(K1{});
       vel(g, xp, nd, stage<S>(sp, 1), up, oob, &sq);
       stage_scale<S>(sp, up);
 #pragma unroll
@@ -778,6 +805,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
         inc[d] = inc[d] + THIRD * k[d];
       }
       at(HALF, k);
+      hook(K2{});
       vel(g, xp, nd, stage<S>(sp, 2), up, oob, &sq);
       stage_scale<S>(sp, up);
 #pragma unroll
@@ -786,6 +814,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
         inc[d] = inc[d] + THIRD * k[d];
       }
       at(T(1), k);
+      hook(K3{});
       vel(g, xp, nd, stage<S>(sp, 3), up, oob, &sq);
       stage_scale<S>(sp, up);
 #pragma unroll

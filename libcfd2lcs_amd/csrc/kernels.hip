@@ -188,10 +188,10 @@ __global__ void __launch_bounds__(kBlock) k_field_from_origin(GridDev<T> g, CF3<
 // velocity (bracket of the origin node, lp_motion_m.f90:171-327), particle
 // BC, old flowmap at the departure point, displacement form write.
 // ---------------------------------------------------------------------------
-template <typename T, typename S, class I, typename FM>
+template <typename T, typename S, int FIX = -1, class I, typename FM, class H = NoHook>
 __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T>& sp, const I& interp,
                                         const CF3<FM>& fmold, const F3<FM>& fmnew, int i, int j, int k,
-                                        DevStatus* st) {
+                                        DevStatus* st, bool active = true, const H& hook = H{}) {
   constexpr bool NR = I::kNonRect;
   int nd[3] = {i, j, k};
   T x0[3];
@@ -213,7 +213,7 @@ __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T
   // displacement keeps the increments' precision instead of that of a
   // difference of two rounded |x|-sized positions
   T inc[3], xr[3];
-  integrate_inc<S>(g, sp, interp, first, nd, x0, inc, oob);
+  integrate_inc<S, FIX>(g, sp, interp, first, nd, x0, inc, oob, (const T*)nullptr, hook);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     xr[d] = x0[d] + inc[d];
@@ -226,13 +226,18 @@ __device__ __forceinline__ void sl_node(const GridDev<T>& g, const StageParams<T
   int no[3] = {i, j, k};
   T fmp[3];
   // the old flowmap at the departure point, located from the node and the
-  // increment unless a boundary condition moved the point
-  const Split<T> sq{{x0[0], x0[1], x0[2]}, {inc[0], inc[1], inc[2]}};
+  // increment unless a boundary condition moved the point (then from the
+  // moved point plus a zero offset, which interpolates exactly as the
+  // point itself: (q - x) + 0 = q - x)
   const bool split = xp[0] == xr[0] && xp[1] == xr[1] && xp[2] == xr[2];
-  interp(g, xp, no, fmold, fmp, oob, split ? &sq : nullptr);
+  const Split<T> sq{{split ? x0[0] : xp[0], split ? x0[1] : xp[1], split ? x0[2] : xp[2]},
+                    {split ? inc[0] : T(0), split ? inc[1] : T(0), split ? inc[2] : T(0)}};
+  hook(std::integral_constant<int, 4>{});  // before the flowmap gather
+  interp(g, xp, no, fmold, fmp, oob, &sq);
   const int f = g.flag[l];
   const T mask = (f != kMask) ? T(1) : T(0);
   const bool zero = (f == kMask || f == kInflow || f == kOutflow);
+  if (!active) return;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     // a boundary projection or stick moved the point: its displacement
@@ -296,117 +301,207 @@ __global__ void __launch_bounds__(kBlock, (sl_min_waves<T, S, I>())) k_sl_step(G
 }
 
 // ---------------------------------------------------------------------------
-// Stage (a) with the brick's coefficient tiles in LDS (B-spline, 3-D): a
-// workgroup owns a BX x BY x BZ brick of nodes and stages, for every field
-// the step gathers from (the old flowmap's coefficients and the distinct
-// stage-velocity coefficient fields), the coefficients from kTileHalo cells
-// below to kTileHalo cells above the brick.  A point within about half a cell
-// of its node (the stage and departure points under CFL <= 0.5) finds its
-// whole support in the tile; any other point gathers from global memory.
-// The sums run in spline_sum's order, so the results equal the untiled
-// kernel's bit for bit.  Measured on ABC 256^3 (RK4, cubic) it is slower than
-// the gather kernel (4.0 vs 2.6 ms per sub-cycle): with a 12-float tile pitch
-// two lanes of each row pair share an LDS bank (bank-conflict cycles are
-// about half of the LDS-active cycles), the staging index arithmetic adds two
-// thirds to the VALU count, and 148 VGPRs leave 3 waves per SIMD, while the
-// gather kernel's float4 row loads already run the texture path at about
-// 70 % busy.  It is kept as the "SL_TILED" option.
+// Stage (a) with the brick's coefficient tiles in LDS (cubic B-spline, 3-D
+// grid, FP32): a workgroup owns a brick of kBrickX x kBrickY x kBrickZ nodes.
+// The fields the step gathers from are staged into one LDS tile in the order
+// the scheme first needs them (RK4: the half-step stage field, the
+// full-step stage field, the old flowmap's coefficients), each from two
+// cells below to two cells above the brick, with coalesced loads.  While a
+// stage computes from the tile, the next field's loads are already in flight
+// into registers; at the next field's first use the workgroup synchronises,
+// writes them into the tile and synchronises again.  One 3-component tile
+// (21 KB) leaves room for four or more workgroups per CU, which hide those
+// synchronisations.  A point within half a cell of its node (every stage and
+// departure point under CFL <= 0.5) finds its whole 4 x 4 x 4 support in the
+// tile and sums it with LDS reads at compile-time offsets from one address;
+// any other point (or field) gathers from global memory.  A wave is one
+// z-layer of the brick, two rows of 32 nodes, so each half-wave's reads at
+// one stencil offset touch 32 consecutive words: 32 distinct banks.  The basis
+// tables and knots of the brick's intervals are staged too.  The sums run in
+// spline_sum's order, so results equal k_sl_step's bit for bit; what changes
+// is where the bytes come from: the texture path moves about 20 staged
+// values per node and field instead of about 200 gathered ones.
 // ---------------------------------------------------------------------------
-constexpr int kTileHalo = 2;
+constexpr int kBrickX = 32, kBrickY = 2, kBrickZ = 4;
 
-template <typename T, int KX, int KZ, int TX, int TY, int TZ>
-struct IntSplineTiled {
+// a point whose support leaves the tile (or a field not staged): the
+// global-memory sums of spline_sum, kept out of line so that the kernel's
+// interpolation sites share one copy of this rare path (weights passed as
+// scalars, so that nothing goes through the stack)
+template <typename T>
+struct Sum3 {
+  T v[3];
+};
+template <typename T>
+__device__ __noinline__ Sum3<T> spline_sum3_global(const T* __restrict__ c0, const T* __restrict__ c1,
+                                                   const T* __restrict__ c2, int base, int sy, int sz, T x0, T x1,
+                                                   T x2, T x3, T y0, T y1, T y2, T y3, T z0, T z1, T z2, T z3) {
+  Sum3<T> r;
+  const T wx[4] = {x0, x1, x2, x3}, wy[4] = {y0, y1, y2, y3}, wz[4] = {z0, z1, z2, z3};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const T* p = (c == 0 ? c0 : (c == 1 ? c1 : c2)) + base;
+    T acc = T(0);
+#pragma unroll
+    for (int cz = 0; cz < 4; cz++) {
+      T accy = T(0);
+#pragma unroll
+      for (int by = 0; by < 4; by++) {
+        const T* row = p + by * sy + cz * sz;
+        T accx = T(0);
+#pragma unroll
+        for (int a = 0; a < 4; a++) accx += wx[a] * row[a];
+        accy += wy[by] * accx;
+      }
+      acc += wz[cz] * accy;
+    }
+    r.v[c] = acc;
+  }
+  return r;
+}
+
+// adapts the brick kernel's interpolation lambda to the interpolant
+// interface sl_node and integrate_inc call
+template <class F>
+struct BrickInterp {
   static constexpr bool kNonRect = false;
-  static constexpr int TILE = TX * TY * TZ;
-  const T* key[3];  // component-0 pointer of each staged field (null: unused)
-  lcptr<T> tile;    // [field][component][TZ][TY][TX]
-  int o[3];         // ghosted index of tile cell (0, 0, 0)
-  __device__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
-                             unsigned long long& oob, const Split<T>* sq = nullptr) const {
-    SplineW<T, KX, KZ> s;
-    const int r = spline_setup(g, q, nd, s, sq);
+  const F& f;
+  template <typename T, typename S>
+  __device__ __forceinline__ void operator()(const GridDev<T>& g, const T q[3], const int nd[3], const CF3<S>& fld,
+                                             T out[3], unsigned long long& oob, const Split<T>* sq = nullptr) const {
+    f(g, q, nd, fld, out, oob, sq);
+  }
+};
+
+// LDS rows of the staged basis tables per axis: a brick's nodes plus the
+// reach of points within a cell of them
+constexpr int kBrickTabX = kBrickX + 8, kBrickTabY = kBrickY + 8, kBrickTabZ = kBrickZ + 8;
+
+template <typename T, int BX, int BY, int BZ, int INTEG>
+__global__ void __launch_bounds__(BX * BY * BZ, 4) k_sl_brick(GridDev<T> g, StageParams<T> sp, CF3<T> fmold,
+                                                              F3<T> fmnew, DevStatus* st, int nbx, int nby) {
+  // stencil reach: a cubic point's support starts 2 nodes below its node
+  // at the lowest and ends 2 above at the highest
+  constexpr int TX = BX + 4, TY = BY + 4, TZ = BZ + 4, TILE = TX * TY * TZ;
+  constexpr int NT = BX * BY * BZ, PER = (TILE + NT - 1) / NT;
+  // fields in the order the scheme first gathers them: RK4 stage 2 (its
+  // stage 3 reads the same half-step field), stage 4, then the flowmap;
+  // RK3 stages 2 and 3; RK2 stage 2
+  constexpr int NSEQ = INTEG == kRK2 ? 2 : 3;
+  __shared__ T tile[3 * TILE];
+  __shared__ T tbuf[spline_tables_lds<4, 4>(kBrickTabX, kBrickTabY, kBrickTabZ)];
+  const int b = xcd_brick(blockIdx.x, gridDim.x);
+  const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+  const int t = threadIdx.x;
+  const int o0 = bx * BX - 1, o1 = by * BY - 1, o2 = bz * BZ - 1;  // ghosted index of tile cell (0, 0, 0)
+  constexpr int A1 = NSEQ == 3 ? (INTEG == kRK4 ? 3 : 2) : 1;
+  const T* f0[3] = {(const T*)sp.v[1][0], (const T*)sp.v[1][1], (const T*)sp.v[1][2]};
+  const T* f1[3] = {NSEQ == 3 ? (const T*)sp.v[A1][0] : fmold.c[0], NSEQ == 3 ? (const T*)sp.v[A1][1] : fmold.c[1],
+                    NSEQ == 3 ? (const T*)sp.v[A1][2] : fmold.c[2]};
+  const T* f2[3] = {fmold.c[0], fmold.c[1], fmold.c[2]};
+  // component-0 pointer of field q of the sequence (its identity)
+
+  // this thread's tile cells; cells outside the ghosted grid hold zero (no
+  // point's support reaches them)
+  int gidx[PER];
+  bool cell[PER];
+#pragma unroll
+  for (int it = 0; it < PER; it++) {
+    const int e = t + it * NT;
+    const int x = e % TX, y = (e / TX) % TY, z = e / (TX * TY);
+    const int gx = o0 + x, gy = o1 + y, gz = o2 + z;
+    cell[it] = e < TILE && (unsigned)gx < (unsigned)g.N[0] && (unsigned)gy < (unsigned)g.N[1] &&
+               (unsigned)gz < (unsigned)g.N[2];
+    gidx[it] = cell[it] ? gx + g.N[0] * (gy + g.N[1] * gz) : 0;
+  }
+  T pre[PER][3];
+  auto prefetch = [&](const T* a, const T* b2, const T* c2) __attribute__((always_inline)) {
+#pragma unroll
+    for (int it = 0; it < PER; it++) {
+      pre[it][0] = a[gidx[it]];
+      pre[it][1] = b2[gidx[it]];
+      pre[it][2] = c2[gidx[it]];
+    }
+  };
+  prefetch(f0[0], f0[1], f0[2]);
+  // basis tables of the brick's intervals (visible after the first field's
+  // synchronisation, before any point is located)
+  SplineLds<T> lds;
+  const int lo[3] = {bx * BX + 1, by * BY + 1, bz * BZ + 1};
+  const int hi[3] = {min(lo[0] + BX - 1, g.n[0]), min(lo[1] + BY - 1, g.n[1]), min(lo[2] + BZ - 1, g.n[2])};
+  const int cap[3] = {kBrickTabX, kBrickTabY, kBrickTabZ};
+  copy_spline_tables<T, 4, 4>(g, lo, hi, (lptr<T>)tbuf, cap, lds);
+  // The fields change at fixed points of the step, the same for every
+  // thread: the hook runs before each gather with the stage number as a
+  // type (1..3 the scheme's stages, 4 the flowmap), so which field is staged
+  // where is fixed at compile time, and every thread reaches the
+  // synchronisations.  RK4's stage 3 reads stage 2's half-step field.
+  const void* cur_key = nullptr;  // component-0 pointer of the staged field
+  auto hook = [&](auto K) __attribute__((always_inline)) {
+    constexpr int k = decltype(K)::value;
+    constexpr int need = k == 4 ? NSEQ - 1 : (INTEG == kRK4 ? (k <= 2 ? 0 : 1) : k - 1);
+    if constexpr (!(INTEG == kRK4 && k == 2)) {
+      __syncthreads();  // every thread is done with the previous field
+#pragma unroll
+      for (int it = 0; it < PER; it++) {
+        const int e = t + it * NT;
+        if (it < PER - 1 || e < TILE) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) tile[c * TILE + e] = cell[it] ? pre[it][c] : T(0);
+        }
+      }
+      __syncthreads();
+      if constexpr (need == 0) prefetch(f1[0], f1[1], f1[2]);
+      if constexpr (need == 1 && NSEQ == 3) prefetch(f2[0], f2[1], f2[2]);
+      cur_key = need == 0 ? (const void*)f0[0] : (need == 1 ? (const void*)f1[0] : (const void*)f2[0]);
+    }
+  };
+  auto interp = [&](const GridDev<T>& gg, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+                    unsigned long long& oob, const Split<T>* sq) __attribute__((always_inline)) {
+    const void* k0 = f.c[0];
+    const void* kst = cur_key;
+    SplineW<T, 4, 4> s;
+    const int r = spline_setup(gg, q, nd, s, sq, &lds);
     if (r) {
       if (r == 1) oob++;
       out[0] = out[1] = out[2] = T(0);
       return;
     }
-    const int slot = f.c[0] == key[0] ? 0 : (f.c[0] == key[1] ? 1 : (f.c[0] == key[2] ? 2 : -1));
-    const int lx = s.b3[0] - o[0], ly = s.b3[1] - o[1], lz = s.b3[2] - o[2];
-    if (slot >= 0 && lx >= 0 && lx <= TX - KX && ly >= 0 && ly <= TY - KX && lz >= 0 && lz <= TZ - KZ) {
+    const unsigned lx = (unsigned)(s.b3[0] - o0), ly = (unsigned)(s.b3[1] - o1), lz = (unsigned)(s.b3[2] - o2);
+    if (k0 == kst && lx <= (unsigned)(TX - 4) && ly <= (unsigned)(TY - 4) &&
+        lz <= (unsigned)(TZ - 4)) {
+      const lcptr<T> tb = (lcptr<T>)tile + ((int)lz * TY + (int)ly) * TX + (int)lx;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        lcptr<T> tc = tile + (slot * 3 + c) * TILE + (lz * TY + ly) * TX + lx;
         T acc = T(0);
 #pragma unroll
-        for (int cz = 0; cz < KZ; cz++) {
+        for (int cz = 0; cz < 4; cz++) {
           T accy = T(0);
 #pragma unroll
-          for (int by = 0; by < KX; by++) {
+          for (int by2 = 0; by2 < 4; by2++) {
             T accx = T(0);
 #pragma unroll
-            for (int a = 0; a < KX; a++) accx += s.wx[a] * tc[(cz * TY + by) * TX + a];
-            accy += s.wy[by] * accx;
+            for (int a = 0; a < 4; a++) accx += s.wx[a] * tb[c * TILE + (cz * TY + by2) * TX + a];
+            accy += s.wy[by2] * accx;
           }
           acc += s.wz[cz] * accy;
         }
         out[c] = acc;
       }
     } else {
+      const Sum3<T> r3 = spline_sum3_global<T>(f.c[0], f.c[1], f.c[2], s.base, gg.N[0], gg.N[0] * gg.N[1], s.wx[0],
+                                               s.wx[1], s.wx[2], s.wx[3], s.wy[0], s.wy[1], s.wy[2], s.wy[3], s.wz[0],
+                                               s.wz[1], s.wz[2], s.wz[3]);
 #pragma unroll
-      for (int c = 0; c < 3; c++) out[c] = spline_sum(g, s, f.c[c]);
-    }
-  }
-};
-
-template <typename T, int KX, int KZ, int BX, int BY, int BZ>
-__global__ void __launch_bounds__(BX * BY * BZ) k_sl_tiled(GridDev<T> g, StageParams<T> sp, CF3<T> fmold,
-                                                           F3<T> fmnew, DevStatus* st, int nbx, int nby) {
-  constexpr int NT = BX * BY * BZ;
-  constexpr int TX = BX + 2 * kTileHalo, TY = BY + 2 * kTileHalo, TZ = BZ + 2 * kTileHalo;
-  using I = IntSplineTiled<T, KX, KZ, TX, TY, TZ>;
-  __shared__ T tile_mem[9 * I::TILE];
-  const int b = xcd_brick(blockIdx.x, gridDim.x);
-  const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
-  const int t = threadIdx.x;
-  I interp;
-  interp.tile = (lcptr<T>)tile_mem;
-  interp.o[0] = bx * BX + 1 - kTileHalo;
-  interp.o[1] = by * BY + 1 - kTileHalo;
-  interp.o[2] = bz * BZ + 1 - kTileHalo;
-  // the distinct gathered fields: the flowmap, then the stage fields in use
-  const int ns = sp.integ == kRK4 ? 4 : (sp.integ == kRK3 ? 3 : (sp.integ == kRK2 ? 2 : 1));
-  CF3<T> fb{{nullptr, nullptr, nullptr}}, fc{{nullptr, nullptr, nullptr}};
-#pragma unroll
-  for (int a = 1; a < 4; a++) {
-    const CF3<T> v{{(const T*)sp.v[a][0], (const T*)sp.v[a][1], (const T*)sp.v[a][2]}};
-    if (a >= ns || v.c[0] == fmold.c[0] || v.c[0] == fb.c[0] || v.c[0] == fc.c[0]) continue;
-    if (!fb.c[0]) fb = v;
-    else if (!fc.c[0]) fc = v;
-  }
-  interp.key[0] = fmold.c[0];
-  interp.key[1] = fb.c[0];
-  interp.key[2] = fc.c[0];
-  auto stage_in = [&](int f, const CF3<T>& src) {
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const T* S = src.c[c];
-      for (int e = t; e < I::TILE; e += NT) {
-        const int x = e % TX, y = (e / TX) % TY, z = e / (TX * TY);
-        const int gx = interp.o[0] + x, gy = interp.o[1] + y, gz = interp.o[2] + z;
-        T v = T(0);
-        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < g.N[0] && gy < g.N[1] && gz < g.N[2])
-          v = S[gx + g.N[0] * (gy + g.N[1] * gz)];
-        tile_mem[(f * 3 + c) * I::TILE + e] = v;
-      }
+      for (int c = 0; c < 3; c++) out[c] = r3.v[c];
     }
   };
-  stage_in(0, fmold);
-  if (fb.c[0]) stage_in(1, fb);
-  if (fc.c[0]) stage_in(2, fc);
-  __syncthreads();
-  const int i = bx * BX + t % BX + 1, j = by * BY + (t / BX) % BY + 1, k = bz * BZ + t / (BX * BY) + 1;
-  if (i > g.n[0] || j > g.n[1] || k > g.n[2]) return;
-  sl_node<T, T>(g, sp, interp, fmold, fmnew, i, j, k, st);
+  const BrickInterp<decltype(interp)> bi{interp};
+  // every thread runs the node (clamped into the grid) so that each reaches
+  // the field synchronisations; only nodes of the grid write
+  const int i = lo[0] + t % BX, j = lo[1] + (t / BX) % BY, k = lo[2] + t / (BX * BY);
+  const bool active = i <= g.n[0] && j <= g.n[1] && k <= g.n[2];
+  sl_node<T, T, INTEG>(g, sp, bi, fmold, fmnew, min(i, g.n[0]), min(j, g.n[1]), min(k, g.n[2]), st, active, hook);
 }
 
 // time blend u = (1-ct) u_n + ct u_np1 over the ghosted grid, in the
@@ -1483,20 +1578,21 @@ void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, C
       }))
     return;
   if constexpr (std::is_same<T, float>::value && std::is_same<FM, T>::value) {
-    // FP32 B-spline step on a 3-D grid with native stage storage: LDS tiles
-    if (tiled && (interp == kQuadratic || interp == kCubic) && sp.storage == kStoreNative && g.flat < 0 &&
-        g.kord[0] == g.kord[1] && g.kord[1] == g.kord[2]) {
-      constexpr int BX = 8, BY = 8, BZ = 4;
+    // FP32 cubic step on a 3-D grid with native stage storage: LDS bricks
+    if (tiled && interp == kCubic && sp.storage == kStoreNative && g.flat < 0 && g.kord[0] == 4 && g.kord[1] == 4 &&
+        g.kord[2] == 4) {
+      constexpr int BX = kBrickX, BY = kBrickY, BZ = kBrickZ;
       const int nbx = (g.n[0] + BX - 1) / BX, nby = (g.n[1] + BY - 1) / BY, nbz = (g.n[2] + BZ - 1) / BZ;
       const dim3 tgrid((unsigned)(nbx * nby * nbz)), tblk(BX * BY * BZ);
-      if (g.kord[0] == 4) {
-        hipLaunchKernelGGL((k_sl_tiled<T, 4, 4, BX, BY, BZ>), tgrid, tblk, 0, s, g, sp, fmold, fmnew, st, nbx, nby);
-        return;
-      }
-      if (g.kord[0] == 3) {
-        hipLaunchKernelGGL((k_sl_tiled<T, 3, 3, BX, BY, BZ>), tgrid, tblk, 0, s, g, sp, fmold, fmnew, st, nbx, nby);
-        return;
-      }
+      auto go = [&](auto K) {
+        hipLaunchKernelGGL((k_sl_brick<T, BX, BY, BZ, decltype(K)::value>), tgrid, tblk, 0, s, g, sp, fmold, fmnew,
+                           st, nbx, nby);
+      };
+      // the schemes with stages to gather; Euler and trapezoidal read the
+      // node only and take the gather kernel
+      if (sp.integ == kRK4) { go(std::integral_constant<int, kRK4>{}); return; }
+      if (sp.integ == kRK3) { go(std::integral_constant<int, kRK3>{}); return; }
+      if (sp.integ == kRK2) { go(std::integral_constant<int, kRK2>{}); return; }
     }
   }
   with_storage<T>(sp.storage, [&](auto tag) {
