@@ -728,6 +728,10 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
                                               const V0& first, const int nd[3], const T xn[3], T inc[3],
                                               unsigned long long& oob, const T* xnl = nullptr,
                                               const H& hook = H{}) {
+  // no contraction of the stage combinations: their products feed more than
+  // one sum, and whether the compiler fuses them would otherwise depend on
+  // the kernel around them (every kernel must round a step alike)
+#pragma clang fp contract(off)
   using K1 = std::integral_constant<int, 1>;
   using K2 = std::integral_constant<int, 2>;
   using K3 = std::integral_constant<int, 3>;
@@ -738,6 +742,7 @@ __device__ __forceinline__ void integrate_inc(const GridDev<T>& g, const StagePa
   // (plus xnl, the low part of a compensated position)
   Split<T> sq;
   auto at = [&](T c, const T kk[3]) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       const T o = c * kk[d];
@@ -839,16 +844,32 @@ template <bool NR = false, typename T>
 __device__ __forceinline__ void project_flat(const GridDev<T>& g, int nd[3], T q[3]) {
   if (g.flat < 0) return;
   const int f = g.flat;
-  nd[f] = 1;
-  if constexpr (!NR) {
-    q[f] = q[f] - (q[f] - g.gl[f][1]);
-  } else {
+  // axis f by unrolled selects: a runtime index would put nd and q in
+  // scratch memory
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (d == f) {
+      nd[d] = 1;
+      if constexpr (!NR) q[d] = q[d] - (q[d] - g.gl[d][1]);
+    }
+  }
+  if constexpr (NR) {
     // project_lp2plane (lp_m.f90:450-536): the plane through the node with
     // the normal v1 x v2 of the in-plane central differences
     const int i = iclamp(nd[0], 1, g.n[0]), j = iclamp(nd[1], 1, g.n[1]), k = iclamp(nd[2], 1, g.n[2]);
     const int a0 = f == 0 ? 1 : 0, a1 = f == 2 ? 1 : 2;
     int u0[3] = {i, j, k}, u1[3] = {i, j, k}, w0[3] = {i, j, k}, w1[3] = {i, j, k};
-    u0[a0] -= 1; u1[a0] += 1; w0[a1] -= 1; w1[a1] += 1;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if (d == a0) {
+        u0[d] -= 1;
+        u1[d] += 1;
+      }
+      if (d == a1) {
+        w0[d] -= 1;
+        w1[d] += 1;
+      }
+    }
     T v1[3], v2[3], x[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {

@@ -546,31 +546,12 @@ template <typename T, typename S, class I>
 constexpr int fwd_min_waves() {
   return ((std::is_same<S, half4_t>::value || MixedF32<T, S>::value) && !I::kNonRect) ? 4 : 1;
 }
-template <typename T, typename S, class I, bool COMP>
-__global__ void __launch_bounds__(kBlock, (fwd_min_waves<T, S, I>())) k_fwd_step(GridDev<T> g, StageParams<T> sp, ParticleSet<T> ps, int np,
-                                                     DevStatus* st) {
+// One forward particle sub-cycle (update_lp): integrate, BC, stick, dx
+// accumulation, one tracking pass, periodic wrap.
+template <typename T, typename S, bool COMP, class I>
+__device__ __forceinline__ void fwd_particle(const GridDev<T>& g, const StageParams<T>& sp, const I& interp,
+                                             const ParticleSet<T>& ps, int p, DevStatus* st) {
   constexpr bool NR = I::kNonRect;
-  const int p = xcd_thread_index();
-  I interp{};
-  if constexpr (SplineInterp<I>::value) {
-    // the basis tables of the intervals the workgroup's particles are in
-    constexpr int KX = I::kKX, KZ = I::kKZ;
-    __shared__ T tbuf[spline_tables_lds<KX, KZ>(kTabX, kTabY, kTabY)];
-    __shared__ int tred[6];
-    if (sp.lds) {
-      const bool active = p < np;
-      int n0[3] = {1, 1, 1};
-      if (active) {
-        n0[0] = ps.n[0][p];
-        n0[1] = ps.n[1][p];
-        n0[2] = ps.n[2][p];
-      }
-      const int cap[3] = {kTabX, kTabY, kTabY};
-      stage_spline_tables<T, KX, KZ>(g, n0, active, (lptr<T>)tbuf, cap, (lptr<int>)tred, interp.lds);
-      interp.use_lds = true;
-    }
-  }
-  if (p >= np) return;
   T q[3] = {ps.x[0][p], ps.x[1][p], ps.x[2][p]};
   int nd[3] = {ps.n[0][p], ps.n[1][p], ps.n[2][p]};
   const T xn[3] = {q[0], q[1], q[2]};
@@ -651,6 +632,33 @@ __global__ void __launch_bounds__(kBlock, (fwd_min_waves<T, S, I>())) k_fwd_step
   for (int d = 0; d < 3; d++) ps.x[d][p] = q[d];
   ps.n[0][p] = (int16_t)nd[0]; ps.n[1][p] = (int16_t)nd[1]; ps.n[2][p] = (int16_t)nd[2];
   flush_status(st, err, oob);
+}
+
+template <typename T, typename S, class I, bool COMP>
+__global__ void __launch_bounds__(kBlock, (fwd_min_waves<T, S, I>())) k_fwd_step(GridDev<T> g, StageParams<T> sp, ParticleSet<T> ps, int np,
+                                                     DevStatus* st) {
+  const int p = xcd_thread_index();
+  I interp{};
+  if constexpr (SplineInterp<I>::value) {
+    // the basis tables of the intervals the workgroup's particles are in
+    constexpr int KX = I::kKX, KZ = I::kKZ;
+    __shared__ T tbuf[spline_tables_lds<KX, KZ>(kTabX, kTabY, kTabY)];
+    __shared__ int tred[6];
+    if (sp.lds) {
+      const bool active = p < np;
+      int n0[3] = {1, 1, 1};
+      if (active) {
+        n0[0] = ps.n[0][p];
+        n0[1] = ps.n[1][p];
+        n0[2] = ps.n[2][p];
+      }
+      const int cap[3] = {kTabX, kTabY, kTabY};
+      stage_spline_tables<T, KX, KZ>(g, n0, active, (lptr<T>)tbuf, cap, (lptr<int>)tred, interp.lds);
+      interp.use_lds = true;
+    }
+  }
+  if (p >= np) return;
+  fwd_particle<T, S, COMP>(g, sp, interp, ps, p, st);
 }
 
 // reset_lp for forward particles (lp_m.f90:228-294, resolution 0)
