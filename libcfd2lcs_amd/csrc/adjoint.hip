@@ -671,7 +671,11 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
 #pragma unroll
       for (int d = 0; d < 3; d++) xbar[d] -= nrm[d] * dn;
     }
-    if (g.flat >= 0) xbar[g.flat] = T(0);
+    // the flat axis by unrolled selects (a run-time index would put xbar in
+    // scratch memory)
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+      if (d == g.flat) xbar[d] = T(0);
     T kb[4][3];
 #pragma unroll
     for (int s = 0; s < 4; s++)
@@ -780,7 +784,7 @@ __global__ void __launch_bounds__(64) k_sl_adjoint(GridDev<T> g, AdjLaunch al, S
           v[q] = ws_.tile[m * WS::LTILE + WS::lidx(rx, ry, rz)];
           const int cx = ox + rx, cy = oy + ry, cz = oz + rz;
           if (v[q] != T(0) && cx >= 0 && cy >= 0 && cz >= 0 && cx < g.N[0] && cy < g.N[1] && cz < g.N[2])
-            d[q] = dst.c[m] + cx + g.N[0] * (cy + g.N[1] * cz);
+            d[q] = (m == 0 ? dst.c[0] : (m == 1 ? dst.c[1] : dst.c[2])) + cx + g.N[0] * (cy + g.N[1] * cz);
         }
       }
 #pragma unroll

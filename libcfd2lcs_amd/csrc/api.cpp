@@ -452,6 +452,15 @@ struct Engine : EngineBase {
     gd.max_track = 2 * std::max(h.n[0], std::max(h.n[1], h.n[2]));
     for (int c = 0; c < 27; c++)
       for (int d = 0; d < 3; d++) gd.ps[c][d] = (T)h.ps[c][d];
+    gd.ps_sep = 1;
+    for (int k = -1; k <= 1; k++)
+      for (int j = -1; j <= 1; j++)
+        for (int i = -1; i <= 1; i++) {
+          const int c = (i + 1) + 3 * (j + 1) + 9 * (k + 1);
+          const int a[3] = {(i + 1), 3 * (j + 1), 9 * (k + 1)};
+          for (int d = 0; d < 3; d++)
+            if (gd.ps[c][d] != gd.ps[a[d] + (d == 0 ? 3 + 9 : (d == 1 ? 1 + 9 : 1 + 3))][d]) gd.ps_sep = 0;
+        }
     for (int d = 0; d < 6; d++) gd.bb[d] = (T)h.bb[d];
     const size_t nl = (size_t)h.N[0] + h.N[1] + h.N[2];
     // a non-rectilinear grid adds its full coordinates and face weights
@@ -1677,7 +1686,10 @@ struct Engine : EngineBase {
 
   // snapshots s-1 and s into the ring as u_n and u_np1 (ghosts, MASK zero,
   // spline coefficients); returns the CFL rate of u_np1
-  bool tape_load_step(int s, double& cfl_rate) {
+  // readback = false: the caller needs no CFL rate (the backward pass takes
+  // the sub-cycle counts from the tape), so the step is queued without
+  // waiting for the device; device errors surface at the pass's final check
+  bool tape_load_step(int s, double& cfl_rate, bool readback = true) {
     const int old = cur;
     cur = 1 - cur;
     vold = old;
@@ -1698,14 +1710,17 @@ struct Engine : EngineBase {
       launch_vel_ghost(g, f3(vel[b]), cf3(vel[1 - b]), st, stream);
       launch_cfl(g, f3(vel[b]), st, stream);
     }
-    HIPCHK_R(hipMemcpyAsync(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream), false);
-    HIPCHK_R(hipStreamSynchronize(stream), false);
-    if (st_host->err) {
-      fail("device error while loading snapshots");
-      return false;
+    cfl_rate = 0;
+    if (readback) {
+      HIPCHK_R(hipMemcpyAsync(st_host, st, sizeof(DevStatus), hipMemcpyDeviceToHost, stream), false);
+      HIPCHK_R(hipStreamSynchronize(stream), false);
+      if (st_host->err) {
+        fail("device error while loading snapshots");
+        return false;
+      }
+      unsigned long long bits = st_host->cfl_bits;
+      std::memcpy(&cfl_rate, &bits, sizeof(double));
     }
-    unsigned long long bits = st_host->cfl_bits;
-    std::memcpy(&cfl_rate, &bits, sizeof(double));
     if (tape.spl) {
       if (!fit_vel(cur)) return false;
       if (!fit_vel(old)) return false;
@@ -1733,13 +1748,15 @@ struct Engine : EngineBase {
     CF3<T> src = cf3(a);
     const size_t fb = 3 * fbytes();
     if (tape.spl) {
-      if (!ensure(tape.fmcoef, fb)) return false;
-      launch_spline_fit(g, f3(tape.fmcoef), 3, stream, src.c);
-      src = cf3(tape.fmcoef);
-    }
-    if (keep) {
+      // the coefficients the step reads, fitted straight into the kept
+      // field when the sub-cycle's field is kept
+      DevBuf& cb = keep ? *keep : tape.fmcoef;
+      if (!ensure(cb, fb)) return false;
+      launch_spline_fit(g, f3(cb), 3, stream, src.c);
+      src = cf3(cb);
+    } else if (keep) {
       if (!ensure(*keep, fb)) return false;
-      HIPCHK_R(hipMemcpyAsync(keep->p, tape.spl ? tape.fmcoef.p : a.p, fb, hipMemcpyDeviceToDevice, stream), false);
+      HIPCHK_R(hipMemcpyAsync(keep->p, a.p, fb, hipMemcpyDeviceToDevice, stream), false);
     }
     const StageParams<T> sp = stage_params(t, dt, tape.spl);
     launch_sl_step(g, tape.interp, sp, src, f3(b), st, stream, g_opt.sl_tiled);
@@ -1988,7 +2005,7 @@ struct Engine : EngineBase {
       if (tape.segf.size() < need) tape.segf.resize(need);
       for (int s = s0; s <= s1 && ok; s++) {
         double cfl_rate;
-        if (!(ok = tape_load_step(s, cfl_rate))) break;
+        if (!(ok = tape_load_step(s, cfl_rate, false))) break;
         std::vector<U> ts;
         U dt;
         step_times(s, ts, dt);
@@ -1998,7 +2015,7 @@ struct Engine : EngineBase {
       // reverse sweep over the segment
       for (int s = s1; s >= s0 && ok; s--) {
         double cfl_rate;
-        if (!(ok = tape_load_step(s, cfl_rate))) break;
+        if (!(ok = tape_load_step(s, cfl_rate, false))) break;
         std::vector<U> ts;
         U dt;
         step_times(s, ts, dt);

@@ -46,6 +46,16 @@ __device__ __forceinline__ void decode_interior(const GridDev<T>& g, int p, int&
   j = r % g.n[1] + 1;
   k = r / g.n[1] + 1;
 }
+// floor(e / d) for 0 <= e < 2^20 and 0 < d < 2^20, from rd = 1/d estimated
+// in FP32 and corrected by one step either way (a few instructions where an
+// integer division by a run-time divisor takes dozens)
+__device__ __forceinline__ int small_div(int e, int d, float rd) {
+  int q = (int)((float)e * rd);
+  if (q * d > e) q--;
+  if ((q + 1) * d <= e) q++;
+  return q;
+}
+
 // The same bijection in slab order: blocks of kSlabJ rows of j, each swept
 // through every k, i fastest.  The nodes one XCD has in flight then lie in a
 // slab of kSlabJ rows by a few planes, whose stencil rows (cubic: four

@@ -806,9 +806,12 @@ __global__ void __launch_bounds__(kBlock) k_sl_step2(GridDev<T> g, GridDev<T> lg
   }
   int no[3] = {i, j, k};
   T fmp[3];
-  const Split<T> sq{{x0[0], x0[1], x0[2]}, {inc[0], inc[1], inc[2]}};
+  // as in sl_node: a point a boundary condition moved is the moved point
+  // plus a zero offset
   const bool split = xp[0] == xr[0] && xp[1] == xr[1] && xp[2] == xr[2];
-  interp(lg, xp, no, fmold, fmp, oob, split ? &sq : nullptr);
+  const Split<T> sq{{split ? x0[0] : xp[0], split ? x0[1] : xp[1], split ? x0[2] : xp[2]},
+                    {split ? inc[0] : T(0), split ? inc[1] : T(0), split ? inc[2] : T(0)}};
+  interp(lg, xp, no, fmold, fmp, oob, &sq);
   const int l = gid(lg, i, j, k);
   const int f = lg.flag[l];
   const bool zero = (f == kMask || f == kInflow || f == kOutflow);
@@ -905,7 +908,35 @@ __device__ __forceinline__ void compose_after(const GridDev<T>& g, const T up[3]
   // relocation to the bounding box or its first periodic image
   const T* bb = g.bb;
   bool in = q[0] >= bb[0] && q[0] <= bb[3] && q[1] >= bb[1] && q[1] <= bb[4] && q[2] >= bb[2] && q[2] <= bb[5];
-  if (!in) {
+  if (!in && g.ps_sep) {
+    // the shifts are separable: an image is inside the box iff each axis'
+    // shifted coordinate is, and the first direction (i fastest, then j, k)
+    // with an inside image takes the first inside index of every axis
+    int first[3];
+    bool any = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      first[d] = -2;
+#pragma unroll
+      for (int a = -1; a <= 1; a++) {
+        const int c = d == 0 ? (a + 1) + 12 : (d == 1 ? 3 * (a + 1) + 10 : 9 * (a + 1) + 4);
+        const T y = q[d] - g.ps[c][d];
+        if (first[d] == -2 && y >= bb[d] && y <= bb[3 + d]) first[d] = a;
+      }
+      any = any && first[d] != -2;
+    }
+    if (any) {
+      const int c = (first[0] + 1) + 3 * (first[1] + 1) + 9 * (first[2] + 1);
+      const T y[3] = {q[0] - g.ps[c][0], q[1] - g.ps[c][1], q[2] - g.ps[c][2]};
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        if (y[d] != q[d]) {
+          sq.qb[d] = y[d];
+          sq.off[d] = T(0);
+        }
+      q[0] = y[0]; q[1] = y[1]; q[2] = y[2];
+    }
+  } else if (!in) {
     for (int c = 0; c < 27 && !in; c++) {
       const T y0 = q[0] - g.ps[c][0], y1 = q[1] - g.ps[c][1], y2 = q[2] - g.ps[c][2];
       if (y0 >= bb[0] && y0 <= bb[3] && y1 >= bb[1] && y1 <= bb[4] && y2 >= bb[2] && y2 <= bb[5]) {
@@ -1843,10 +1874,11 @@ __global__ void __launch_bounds__(kBlock) k_compose_slot_tiled(GridDev<T> g, CF3
   const int ex = box[3] - b0 + KX, ey = box[4] - b1 + KX, ez = box[5] - b2 + KZ;
   const bool staged = box[3] >= b0 && (long long)ex * ey * ez <= CAP;
   if (staged) {
-    const int vol = ex * ey * ez;
+    const int vol = ex * ey * ez, pl = ex * ey;
+    const float rex = 1.0f / (float)ex, rpl = 1.0f / (float)pl;
     for (int e = threadIdx.x; e < vol; e += blockDim.x) {
-      const int x = e % ex, yz = e / ex, y = yz % ey, z = yz / ey;
-      const long long gi = (long long)(b0 + x) + (long long)g.N[0] * ((b1 + y) + (long long)g.N[1] * (b2 + z));
+      const int z = small_div(e, pl, rpl), r = e - z * pl, y = small_div(r, ex, rex), x = r - y * ex;
+      const int gi = (b0 + x) + g.N[0] * ((b1 + y) + g.N[1] * (b2 + z));
 #pragma unroll
       for (int c = 0; c < 3; c++) tile[c * CAP + e] = f.c[c][gi];
     }
@@ -1859,16 +1891,17 @@ __global__ void __launch_bounds__(kBlock) k_compose_slot_tiled(GridDev<T> g, CF3
     up[0] = up[1] = up[2] = T(0);
   } else if (staged) {
     const int ox = w.b3[0] - b0, oy = w.b3[1] - b1, oz = w.b3[2] - b2;
+    const int pl = ex * ey, o0 = (oz * ey + oy) * ex + ox;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const T* tc = tile + c * CAP;
+      const T* tc = tile + c * CAP + o0;
       T acc = T(0);
 #pragma unroll
       for (int cz = 0; cz < KZ; cz++) {
         T accy = T(0);
 #pragma unroll
         for (int by = 0; by < KX; by++) {
-          const T* row = tc + ((oz + cz) * ey + (oy + by)) * ex + ox;
+          const T* row = tc + cz * pl + by * ex;
           T accx = T(0);
 #pragma unroll
           for (int a = 0; a < KX; a++) accx += w.wx[a] * row[a];

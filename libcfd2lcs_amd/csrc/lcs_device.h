@@ -33,6 +33,9 @@ struct GridDev {
   int flat;          // the axis with n == 1, or -1
   int max_track;     // 2*max(gn) (lp_tracking_m.f90:127)
   T ps[27][3];       // periodic_shift(i,j,k,:) at (i+1)+3(j+1)+9(k+1)
+  // ps[.][d] depends on the direction's axis-d index only (every rectilinear
+  // grid): the first periodic image of a point is then found axis by axis
+  int ps_sep;
   T bb[6];           // ghost-inclusive bounding box
   int kord[3];       // spline orders
   const T* kn[3];    // spline knots, N[d] + kord[d] each
