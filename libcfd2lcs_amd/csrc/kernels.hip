@@ -332,25 +332,25 @@ template <typename T>
 struct Sum3 {
   T v[3];
 };
-template <typename T>
-__device__ __noinline__ Sum3<T> spline_sum3_global(const T* __restrict__ c0, const T* __restrict__ c1,
-                                                   const T* __restrict__ c2, int base, int sy, int sz, T x0, T x1,
+template <typename T, typename E>
+__device__ __noinline__ Sum3<T> spline_sum3_global(const E* __restrict__ c0, const E* __restrict__ c1,
+                                                   const E* __restrict__ c2, int base, int sy, int sz, T x0, T x1,
                                                    T x2, T x3, T y0, T y1, T y2, T y3, T z0, T z1, T z2, T z3) {
   Sum3<T> r;
   const T wx[4] = {x0, x1, x2, x3}, wy[4] = {y0, y1, y2, y3}, wz[4] = {z0, z1, z2, z3};
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const T* p = (c == 0 ? c0 : (c == 1 ? c1 : c2)) + base;
+    const E* p = (c == 0 ? c0 : (c == 1 ? c1 : c2)) + base;
     T acc = T(0);
 #pragma unroll
     for (int cz = 0; cz < 4; cz++) {
       T accy = T(0);
 #pragma unroll
       for (int by = 0; by < 4; by++) {
-        const T* row = p + by * sy + cz * sz;
+        const E* row = p + by * sy + cz * sz;
         T accx = T(0);
 #pragma unroll
-        for (int a = 0; a < 4; a++) accx += wx[a] * row[a];
+        for (int a = 0; a < 4; a++) accx += wx[a] * T(row[a]);
         accy += wy[by] * accx;
       }
       acc += wz[cz] * accy;
@@ -377,9 +377,12 @@ struct BrickInterp {
 // reach of points within a cell of them
 constexpr int kBrickTabX = kBrickX + 8, kBrickTabY = kBrickY + 8, kBrickTabZ = kBrickZ + 8;
 
-template <typename T, int BX, int BY, int BZ, int INTEG>
-__global__ void __launch_bounds__(BX * BY * BZ, 4) k_sl_brick(GridDev<T> g, StageParams<T> sp, CF3<T> fmold,
-                                                              F3<T> fmnew, DevStatus* st, int nbx, int nby) {
+// Tiles of the fields' element type E: the compute type T (FP32 or FP64),
+// or FP32 under FP64 compute (FP64 arithmetic needs about twice the
+// registers: two waves per SIMD)
+template <typename T, typename E, int BX, int BY, int BZ, int INTEG>
+__global__ void __launch_bounds__(BX * BY * BZ, sizeof(T) == 8 ? 2 : 4)
+    k_sl_brick(GridDev<T> g, StageParams<T> sp, CF3<E> fmold, F3<E> fmnew, DevStatus* st, int nbx, int nby) {
   // stencil reach: a cubic point's support starts 2 nodes below its node
   // at the lowest and ends 2 above at the highest
   constexpr int TX = BX + 4, TY = BY + 4, TZ = BZ + 4, TILE = TX * TY * TZ;
@@ -388,17 +391,17 @@ __global__ void __launch_bounds__(BX * BY * BZ, 4) k_sl_brick(GridDev<T> g, Stag
   // stage 3 reads the same half-step field), stage 4, then the flowmap;
   // RK3 stages 2 and 3; RK2 stage 2
   constexpr int NSEQ = INTEG == kRK2 ? 2 : 3;
-  __shared__ T tile[3 * TILE];
+  __shared__ E tile[3 * TILE];
   __shared__ T tbuf[spline_tables_lds<4, 4>(kBrickTabX, kBrickTabY, kBrickTabZ)];
   const int b = xcd_brick(blockIdx.x, gridDim.x);
   const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
   const int t = threadIdx.x;
   const int o0 = bx * BX - 1, o1 = by * BY - 1, o2 = bz * BZ - 1;  // ghosted index of tile cell (0, 0, 0)
   constexpr int A1 = NSEQ == 3 ? (INTEG == kRK4 ? 3 : 2) : 1;
-  const T* f0[3] = {(const T*)sp.v[1][0], (const T*)sp.v[1][1], (const T*)sp.v[1][2]};
-  const T* f1[3] = {NSEQ == 3 ? (const T*)sp.v[A1][0] : fmold.c[0], NSEQ == 3 ? (const T*)sp.v[A1][1] : fmold.c[1],
-                    NSEQ == 3 ? (const T*)sp.v[A1][2] : fmold.c[2]};
-  const T* f2[3] = {fmold.c[0], fmold.c[1], fmold.c[2]};
+  const E* f0[3] = {(const E*)sp.v[1][0], (const E*)sp.v[1][1], (const E*)sp.v[1][2]};
+  const E* f1[3] = {NSEQ == 3 ? (const E*)sp.v[A1][0] : fmold.c[0], NSEQ == 3 ? (const E*)sp.v[A1][1] : fmold.c[1],
+                    NSEQ == 3 ? (const E*)sp.v[A1][2] : fmold.c[2]};
+  const E* f2[3] = {fmold.c[0], fmold.c[1], fmold.c[2]};
   // component-0 pointer of field q of the sequence (its identity)
 
   // this thread's tile cells; cells outside the ghosted grid hold zero (no
@@ -414,8 +417,8 @@ __global__ void __launch_bounds__(BX * BY * BZ, 4) k_sl_brick(GridDev<T> g, Stag
                (unsigned)gz < (unsigned)g.N[2];
     gidx[it] = cell[it] ? gx + g.N[0] * (gy + g.N[1] * gz) : 0;
   }
-  T pre[PER][3];
-  auto prefetch = [&](const T* a, const T* b2, const T* c2) __attribute__((always_inline)) {
+  E pre[PER][3];
+  auto prefetch = [&](const E* a, const E* b2, const E* c2) __attribute__((always_inline)) {
 #pragma unroll
     for (int it = 0; it < PER; it++) {
       pre[it][0] = a[gidx[it]];
@@ -447,7 +450,7 @@ __global__ void __launch_bounds__(BX * BY * BZ, 4) k_sl_brick(GridDev<T> g, Stag
         const int e = t + it * NT;
         if (it < PER - 1 || e < TILE) {
 #pragma unroll
-          for (int c = 0; c < 3; c++) tile[c * TILE + e] = cell[it] ? pre[it][c] : T(0);
+          for (int c = 0; c < 3; c++) tile[c * TILE + e] = cell[it] ? pre[it][c] : E(0);
         }
       }
       __syncthreads();
@@ -456,7 +459,7 @@ __global__ void __launch_bounds__(BX * BY * BZ, 4) k_sl_brick(GridDev<T> g, Stag
       cur_key = need == 0 ? (const void*)f0[0] : (need == 1 ? (const void*)f1[0] : (const void*)f2[0]);
     }
   };
-  auto interp = [&](const GridDev<T>& gg, const T q[3], const int nd[3], const CF3<T>& f, T out[3],
+  auto interp = [&](const GridDev<T>& gg, const T q[3], const int nd[3], const CF3<E>& f, T out[3],
                     unsigned long long& oob, const Split<T>* sq) __attribute__((always_inline)) {
     const void* k0 = f.c[0];
     const void* kst = cur_key;
@@ -470,7 +473,7 @@ __global__ void __launch_bounds__(BX * BY * BZ, 4) k_sl_brick(GridDev<T> g, Stag
     const unsigned lx = (unsigned)(s.b3[0] - o0), ly = (unsigned)(s.b3[1] - o1), lz = (unsigned)(s.b3[2] - o2);
     if (k0 == kst && lx <= (unsigned)(TX - 4) && ly <= (unsigned)(TY - 4) &&
         lz <= (unsigned)(TZ - 4)) {
-      const lcptr<T> tb = (lcptr<T>)tile + ((int)lz * TY + (int)ly) * TX + (int)lx;
+      const lcptr<E> tb = (lcptr<E>)tile + ((int)lz * TY + (int)ly) * TX + (int)lx;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         T acc = T(0);
@@ -481,7 +484,7 @@ __global__ void __launch_bounds__(BX * BY * BZ, 4) k_sl_brick(GridDev<T> g, Stag
           for (int by2 = 0; by2 < 4; by2++) {
             T accx = T(0);
 #pragma unroll
-            for (int a = 0; a < 4; a++) accx += s.wx[a] * tb[c * TILE + (cz * TY + by2) * TX + a];
+            for (int a = 0; a < 4; a++) accx += s.wx[a] * T(tb[c * TILE + (cz * TY + by2) * TX + a]);
             accy += s.wy[by2] * accx;
           }
           acc += s.wz[cz] * accy;
@@ -489,7 +492,7 @@ __global__ void __launch_bounds__(BX * BY * BZ, 4) k_sl_brick(GridDev<T> g, Stag
         out[c] = acc;
       }
     } else {
-      const Sum3<T> r3 = spline_sum3_global<T>(f.c[0], f.c[1], f.c[2], s.base, gg.N[0], gg.N[0] * gg.N[1], s.wx[0],
+      const Sum3<T> r3 = spline_sum3_global<T, E>(f.c[0], f.c[1], f.c[2], s.base, gg.N[0], gg.N[0] * gg.N[1], s.wx[0],
                                                s.wx[1], s.wx[2], s.wx[3], s.wy[0], s.wy[1], s.wy[2], s.wy[3], s.wz[0],
                                                s.wz[1], s.wz[2], s.wz[3]);
 #pragma unroll
@@ -501,7 +504,7 @@ __global__ void __launch_bounds__(BX * BY * BZ, 4) k_sl_brick(GridDev<T> g, Stag
   // the field synchronisations; only nodes of the grid write
   const int i = lo[0] + t % BX, j = lo[1] + (t / BX) % BY, k = lo[2] + t / (BX * BY);
   const bool active = i <= g.n[0] && j <= g.n[1] && k <= g.n[2];
-  sl_node<T, T, INTEG>(g, sp, bi, fmold, fmnew, min(i, g.n[0]), min(j, g.n[1]), min(k, g.n[2]), st, active, hook);
+  sl_node<T, E, INTEG>(g, sp, bi, fmold, fmnew, min(i, g.n[0]), min(j, g.n[1]), min(k, g.n[2]), st, active, hook);
 }
 
 // time blend u = (1-ct) u_n + ct u_np1 over the ghosted grid, in the
@@ -1616,16 +1619,19 @@ void launch_sl_step(const GridDev<T>& g, int interp, const StageParams<T>& sp, C
         hipLaunchKernelGGL((k_sl_step<T, S, I, FM>), grid, blk, 0, s, g, sp, fmold, fmnew, st);
       }))
     return;
-  if constexpr (std::is_same<T, float>::value && std::is_same<FM, T>::value) {
-    // FP32 cubic step on a 3-D grid with native stage storage: LDS bricks
-    if (tiled && interp == kCubic && sp.storage == kStoreNative && g.flat < 0 && g.kord[0] == 4 && g.kord[1] == 4 &&
+  // cubic step on a 3-D grid: LDS bricks with fields in the compute type
+  // (native stage storage), or FP32 stage fields and flowmap under FP64
+  // compute
+  if constexpr (std::is_same<FM, float>::value || std::is_same<FM, T>::value) {
+    const int want = std::is_same<FM, T>::value ? kStoreNative : kStoreF32;
+    if (tiled && interp == kCubic && sp.storage == want && g.flat < 0 && g.kord[0] == 4 && g.kord[1] == 4 &&
         g.kord[2] == 4) {
       constexpr int BX = kBrickX, BY = kBrickY, BZ = kBrickZ;
       const int nbx = (g.n[0] + BX - 1) / BX, nby = (g.n[1] + BY - 1) / BY, nbz = (g.n[2] + BZ - 1) / BZ;
       const dim3 tgrid((unsigned)(nbx * nby * nbz)), tblk(BX * BY * BZ);
       auto go = [&](auto K) {
-        hipLaunchKernelGGL((k_sl_brick<T, BX, BY, BZ, decltype(K)::value>), tgrid, tblk, 0, s, g, sp, fmold, fmnew,
-                           st, nbx, nby);
+        hipLaunchKernelGGL((k_sl_brick<T, FM, BX, BY, BZ, decltype(K)::value>), tgrid, tblk, 0, s, g, sp, fmold,
+                           fmnew, st, nbx, nby);
       };
       // the schemes with stages to gather; Euler and trapezoidal read the
       // node only and take the gather kernel

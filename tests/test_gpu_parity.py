@@ -103,7 +103,7 @@ def test_abc_fp32_against_fp64_oracle(sp, interp):
     assert _check(recs, hs, 2e-5, 1e-3) >= 2
 
 
-def _sl_run(lib, tiled, integ, interp, shape, cfl, walls, nsteps=8):
+def _sl_run(lib, tiled, integ, interp, shape, cfl, walls, nsteps=8, opts=None):
     import libcfd2lcs_amd as L
     X, Y, Z = F.cell_grid(shape, (0, 0, 0), (2 * np.pi, 2 * np.pi, 2 * np.pi))
     fl = np.full(X.shape, F.LCS_INTERNAL, dtype=np.int32)
@@ -112,6 +112,9 @@ def _sl_run(lib, tiled, integ, interp, shape, cfl, walls, nsteps=8):
         fl[:, :, 0], fl[:, :, -1] = F.LCS_SLIP, F.LCS_SLIP
     lib.finalize()
     lib.init(X, Y, Z, fl)
+    # storage options take effect at a reset: before the diagnostic
+    for k, v in (opts or {}).items():
+        lib.set_option(k, v)
     hd = lib.diagnostic_init(L.FTLE_BKWD, 0, 0.3, 0.15, "b")
     lib.set_option("INTEGRATOR", integ)
     lib.set_option("INTERPOLATOR", interp)
@@ -139,6 +142,30 @@ def test_sl_lds_tiles_are_bit_exact(sp, integ, interp, shape, cfl, walls):
     untiled gather kernel's results bit for bit: every flowmap and FTLE."""
     a_fm, a_out = _sl_run(sp, True, integ, interp, shape, cfl, walls)
     b_fm, b_out = _sl_run(sp, False, integ, interp, shape, cfl, walls)
+    assert len(a_out) == len(b_out) >= 2
+    for x, y in zip(a_fm, b_fm):
+        for c in range(3):
+            assert np.array_equal(x[c], y[c])
+    for (ca, fa), (cb, fb) in zip(a_out, b_out):
+        for c in range(3):
+            assert np.array_equal(ca[c], cb[c])
+        assert np.array_equal(fa, fb, equal_nan=True)
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32"])
+@pytest.mark.parametrize("integ,shape,cfl,walls", [
+    (F.RK4, (40, 18, 13), 0.5, False),     # partial bricks in every direction
+    (F.RK3, (32, 16, 12), 0.5, True),      # walls and slip faces
+    (F.RK4, (16, 16, 16), 2.0, False),     # displacements beyond the tile halo
+])
+def test_sl_lds_tiles_are_bit_exact_fp64_compute(dp, integ, shape, cfl, walls, storage):
+    """Under FP64 compute, with FP64 fields or with FP32 velocity and flowmap
+    storage, the LDS-tiled semi-Lagrangian step gives the gather kernel's
+    results bit for bit."""
+    import libcfd2lcs_amd as L
+    opts = {"VELOCITY_STORAGE": L.LCS_FP32, "FLOWMAP_STORAGE": L.LCS_FP32} if storage == "fp32" else {}
+    a_fm, a_out = _sl_run(dp, True, integ, F.CUBIC, shape, cfl, walls, opts=opts)
+    b_fm, b_out = _sl_run(dp, False, integ, F.CUBIC, shape, cfl, walls, opts=opts)
     assert len(a_out) == len(b_out) >= 2
     for x, y in zip(a_fm, b_fm):
         for c in range(3):
