@@ -571,6 +571,8 @@ def main():
                     help="grid of the reduced precision study folded into the default run; 0 skips it")
     ap.add_argument("--study-steps", type=int, default=100,
                     help="steps of the reduced precision study (forward FTLE at T = study-steps * 0.005)")
+    ap.add_argument("--scheme", choices=["rk4_cubic", "rk2_linear"], default="rk4_cubic",
+                    help="scheme of the FP32 / FP64 runs (rk2_linear: for profiling the HBM-bound scheme alone)")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -593,12 +595,16 @@ def main():
         torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
         del a, b
-    r32 = run_gpu("sp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle)
+    scheme_opts = {}
+    if args.scheme == "rk2_linear":
+        import libcfd2lcs_amd as L
+        scheme_opts = {"INTEGRATOR": L.RK2, "INTERPOLATOR": L.LINEAR}
+    r32 = run_gpu("sp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle, dict(scheme_opts))
     print(f"[bench] fp32: {r32['updates'] / r32['elapsed']:.4e} updates/s, "
           f"{1e3 * r32['elapsed'] / args.steps:.1f} ms/step", file=sys.stderr, flush=True)
     r64 = None
     if not args.no_fp64:
-        r64 = run_gpu("dp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle)
+        r64 = run_gpu("dp", args.n, args.steps, args.warmup, dt, T, h, not args.no_ftle, dict(scheme_opts))
         print(f"[bench] fp64: {r64['updates'] / r64['elapsed']:.4e} updates/s", file=sys.stderr, flush=True)
     rmix = None
     if args.mixed != "none":
@@ -708,7 +714,7 @@ def main():
         return out_
 
     # dominant kernel of the FP32 run
-    roof = roofline(r32, 4, 4, "cubic_rk4", layout_s=4)
+    roof = roofline(r32, 4, 4, "cubic_rk4" if args.scheme == "rk4_cubic" else "linear_rk2", layout_s=4)
     dom = roof["kernel"]
     roof["note"] = ("gather kernels: limited by the texture-address path (one wave-level row load costs about "
                     "16 TA cycles), not by HBM; counter passes in profiles/")
@@ -744,8 +750,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: analytic steady ABC velocity, resident in HBM",
-        "config": {"workload": f"abc{args.n}_rk4_cubic_fwd_bkwd", "grid": [args.n] * 3, "dt": dt, "T": T,
-                   "h": h, "integrator": "RK4", "interpolator": "CUBIC", "cfl": 0.5, "n_sub": r32["nsub"],
+        "config": {"workload": f"abc{args.n}_{args.scheme}_fwd_bkwd", "grid": [args.n] * 3, "dt": dt, "T": T,
+                   "h": h, "integrator": "RK4" if args.scheme == "rk4_cubic" else "RK2",
+                   "interpolator": "CUBIC" if args.scheme == "rk4_cubic" else "LINEAR", "cfl": 0.5, "n_sub": r32["nsub"],
                    "directions": ["FTLE_FWD", "FTLE_BKWD"]},
         "ftle_fields_per_s": u2(r32),
         "ftle_fields_per_s_pipeline": r32.get("u2_pipeline"),
