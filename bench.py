@@ -468,11 +468,12 @@ def source_digest():
     return hs.hexdigest()
 
 
-def profile_figures(name, n):
+def profile_figures(name, n, block="kernels"):
     """HBM bytes per launch (FETCH_SIZE / WRITE_SIZE counter passes scaled by
     the copy-kernel calibration) and VALU busy fraction of kernel `name` at
     n^3 from profiles/, with the source digest they were measured on; None
-    when no profile of the current sources exists."""
+    when no profile of the current sources exists.  block: "kernels" (RK4 +
+    cubic B-spline) or "linear_rk2" (RK2 + linear)."""
     path = os.path.join(ROOT, "profiles", "abc256_rk4_cubic_fp32_traffic.json")
     if not os.path.exists(path):
         return None
@@ -481,7 +482,7 @@ def profile_figures(name, n):
             tr = json.load(fh)
     except (OSError, ValueError):
         return None
-    k = tr.get("kernels", {}).get(name)
+    k = tr.get(block, {}).get(name)
     if not k or k.get("n") != n:
         return None
     return {"source_sha256": tr.get("source_sha256"), "bytes_per_launch": k.get("bytes_per_launch"),
@@ -720,17 +721,23 @@ def main():
                     "16 TA cycles), not by HBM; counter passes in profiles/")
     # HBM bytes per launch and VALU busy from the counter passes of the same
     # workload, when they were measured on these sources
-    pf = profile_figures(dom, args.n)
     digest = source_digest()
-    if pf is not None:
+
+    def stamp_counters(rf, block):
+        pf = profile_figures(rf["kernel"], args.n, block)
+        if pf is None:
+            return rf
         if pf["source_sha256"] == digest:
-            roof["traffic"] = pf["bytes_per_launch"]
-            roof["valu"]["measured_busy"] = pf["valu_busy"]
-            roof["ta_busy"] = pf["ta_busy"]
-            roof["traffic_source"] = pf["file"]
+            rf["traffic"] = pf["bytes_per_launch"]
+            rf["valu"]["measured_busy"] = pf["valu_busy"]
+            rf["ta_busy"] = pf["ta_busy"]
+            rf["traffic_source"] = pf["file"]
         else:
-            roof["traffic_note"] = ("counter profile %s was measured on other sources (digest %s, current %s); "
-                                    "traffic not reported" % (pf["file"], str(pf["source_sha256"])[:12], digest[:12]))
+            rf["traffic_note"] = ("counter profile %s was measured on other sources (digest %s, current %s); "
+                                  "traffic not reported" % (pf["file"], str(pf["source_sha256"])[:12], digest[:12]))
+        return rf
+
+    stamp_counters(roof, "kernels" if args.scheme == "rk4_cubic" else "linear_rk2")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_n, args.cpu_budget)
@@ -802,8 +809,8 @@ def main():
         "linear_rk2": {
             key: {"value": r["updates"] / r["elapsed"], "unit": "grid-pt updates/s",
                   "ms_per_step": 1e3 * r["elapsed"] / args.steps,
-                  "roofline": roofline(r, 2 if key.startswith("fp16") else 4, 4, "linear_rk2",
-                                       s_pos=8 if key.startswith("fp16") else 4),
+                  "roofline": (roofline(r, 2, 4, "linear_rk2", s_pos=8) if key.startswith("fp16") else
+                               stamp_counters(roofline(r, 4, 4, "linear_rk2"), "linear_rk2")),
                   "kernel_ms": {k: round(1e3 * s_ / max(r["kernel_cnt"][k], 1), 4)
                                 for k, s_ in r["kernel_sec"].items()}}
             for key, r in rlin.items()} or None,
