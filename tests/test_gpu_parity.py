@@ -194,6 +194,42 @@ def test_sl_lds_tiles_are_bit_exact_at_256(sp, dp, prec):
     assert moved > 0.01  # the flowmap did move
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_uniform_flow_closed_form_at_256(sp, dp, prec):
+    """Periodic box at the headline size (256^3), uniform velocity U, RK4 +
+    cubic B-spline, both directions: every forward particle and every
+    backward departure point moves by exactly U t, periodic wraps included,
+    so the composed T-flowmaps at a full ring are +U T (FWD) and -U T (BKWD)
+    at every node, and the displacement gradient, hence sigma, vanishes to
+    rounding (FTLE far below any resolved value)."""
+    import libcfd2lcs_amd as L
+    lib = sp if prec == "fp32" else dp
+    # times exact in FP32 too, so that both libraries trigger the same outputs
+    n, T, h, dt = 256, 0.25, 0.125, 0.0625
+    X, Y, Z, fl = F.abc_grid(n)
+    lib.finalize()
+    lib.init(X, Y, Z, fl)
+    hf = lib.diagnostic_init(L.FTLE_FWD, 0, T, h, "fwd")
+    hb = lib.diagnostic_init(L.FTLE_BKWD, 0, T, h, "bkwd")
+    lib.set_option("INTEGRATOR", L.RK4)
+    lib.set_option("INTERPOLATOR", L.CUBIC)
+    U = (0.3, -0.2, 0.1)
+    vel = [np.full(X.shape, c) for c in U]
+    for s in range(10):  # outputs at t = 1/8, 1/4, 3/8, 1/2; the last composes a full ring
+        lib.update(*vel, dt * s)
+        assert lib.error == 0
+    assert lib.noutput(hf) == lib.noutput(hb) == 4
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    for hd, sign in ((hf, 1.0), (hb, -1.0)):
+        comp = lib.composed(hd)
+        for c in range(3):
+            err = float(np.abs(np.asarray(comp[c], dtype=np.float64) - sign * U[c] * T).max())
+            assert err <= tol, f"handle {hd} component {c}: {err:.3e}"
+        ftle = np.asarray(lib.ftle(hd), dtype=np.float64)
+        # log(sigma)/T with sigma below 1e-9 (FP64) or 1e-4 (FP32)
+        assert np.all(ftle <= np.log(1e-9 if prec == "fp64" else 1e-4) / T)
+
+
 def _fwd_run(lib, sort_every, n=24, nsteps=14):
     import libcfd2lcs_amd as L
     X, Y, Z, fl = F.abc_grid(n)
