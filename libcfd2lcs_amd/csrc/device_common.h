@@ -974,7 +974,9 @@ __device__ __forceinline__ bool track_pass(const GridDev<T>& g, const T q[3], in
   const bool fx = g.n[0] == 1, fy = g.n[1] == 1, fz = g.n[2] == 1;
   const T huge = sizeof(T) == 8 ? T(DBL_MAX) : T(FLT_MAX);
   T rmin = huge;
-  int oi = 0, oj = 0, ok = 0;
+  // the chosen neighbour as one code (di + 1) + 3 (dj + 1) + 9 (dk + 1): one
+  // select per candidate instead of three
+  int code = 13;
   constexpr int8_t nb[27][3] = {
       {0, 0, 0},   {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},  {1, 0, 0},   {0, 1, 0},  {0, 0, 1},
       {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {-1, 0, -1}, {1, 0, -1}, {-1, 1, -1}, {0, 1, -1},
@@ -996,11 +998,12 @@ __device__ __forceinline__ bool track_pass(const GridDev<T>& g, const T q[3], in
     }
     if (rsq < T(0.99) * rmin) {
       rmin = rsq;
-      oi = di; oj = dj; ok = dk;
+      code = (di + 1) + 3 * (dj + 1) + 9 * (dk + 1);
     }
   }
-  nd[0] += oi; nd[1] += oj; nd[2] += ok;
-  return oi != 0 || oj != 0 || ok != 0;
+  const int ok = code / 9, r = code - 9 * ok, oj = r / 3, oi = r - 3 * oj;
+  nd[0] += oi - 1; nd[1] += oj - 1; nd[2] += ok - 1;
+  return code != 13;
 }
 
 // exchange_lpdata on one rank (comms_m.f90:872-1189): periodic wrap of a

@@ -1860,13 +1860,11 @@ __global__ void __launch_bounds__(kBlock) k_compose_slot_tiled(GridDev<T> g, CF3
     lo[d] = r == 0 ? w.b3[d] : INT_MAX;
     hi[d] = r == 0 ? w.b3[d] : INT_MIN;
   }
+  // wave reductions (every lane of the wave takes part)
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
-      hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
-    }
+  for (int d = 0; d < 3; d++) {
+    lo[d] = __reduce_min_sync(~0ull, lo[d]);
+    hi[d] = __reduce_max_sync(~0ull, hi[d]);
   }
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
