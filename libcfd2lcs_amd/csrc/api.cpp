@@ -837,7 +837,11 @@ struct Engine : EngineBase {
     bool valid = false;
     unsigned long long stamp = 0;
   };
-  Blend blends[6];
+  // the pool: six entries, or as many as an update's planned blends need
+  // (plan_blends), up to kMaxBlends
+  static constexpr int kMaxBlends = 48;
+  Blend blends[kMaxBlends];
+  int nblend = 6;
   unsigned long long blend_clock = 0;
   int vold = 0;  // index of u_n in vel[] during an update
   double umax[2] = {0.0, 0.0};  // largest |component| of vel[b] (interior)
@@ -879,7 +883,8 @@ struct Engine : EngineBase {
     // of the stored value.  Native storage matches ct exactly.
     const double tol = stor == kStoreF32 ? 0x1p-34 : (stor == kStoreF16 ? 0x1p-21 : (stor == kStoreBF16 ? 0x1p-18 : 0.0));
     Blend* victim = nullptr;
-    for (auto& b : blends) {
+    for (int q = 0; q < nblend; q++) {
+      Blend& b = blends[q];
       if (b.valid && std::fabs((double)b.ct - (double)ct) <= tol && b.storage == stor && b.spl == spl) {
         b.stamp = ++blend_clock;
         char* p = (char*)b.buf.p;
@@ -918,12 +923,11 @@ struct Engine : EngineBase {
   // clamps to 1 as with IEEE minNum/maxNum
   static U ctf(U tt, U t0, U t1) { return std::fmax(std::fmin((tt - t0) / (t1 - t0), (U)1), (U)0); }
 
-  StageParams<T> stage_params(U t, U dt, bool spl) {
-    StageParams<T> sp{};
-    sp.integ = g_opt.integrator;
-    sp.dt = (T)dt;
+  // the blend weights ct of the stages of a sub-cycle from t by dt (c[0] at
+  // the start; RK4's c[2] is its c[1]); returns the stage count
+  int stage_cts(U t, U dt, U c[4]) const {
     const U half = (U)1 / (U)2;
-    U c[4] = {0, 0, 0, 0};
+    c[0] = c[1] = c[2] = c[3] = 0;
     int ns = 1;
     switch (g_opt.integrator) {
       case kEuler: c[0] = ctf(t, t_n, t_np1); break;
@@ -946,6 +950,15 @@ struct Engine : EngineBase {
         c[3] = ctf(t + dt, t_n, t_np1);
         ns = 4;
     }
+    return ns;
+  }
+
+  StageParams<T> stage_params(U t, U dt, bool spl) {
+    StageParams<T> sp{};
+    sp.integ = g_opt.integrator;
+    sp.dt = (T)dt;
+    U c[4];
+    const int ns = stage_cts(t, dt, c);
     sp.storage = g_opt.vel_storage;
     sp.vscale = (T)vscale;
     sp.lds = g_opt.spline_lds;
@@ -1045,6 +1058,8 @@ struct Engine : EngineBase {
       if ((!vcoef_ok[old] || vcoef32[old] != want_vcoef32()) && !fit_vel(old)) return;
     }
     const U dt_f = t_np1 - t_n;
+    plan_blends(spl, cfl_rate, dt_f);
+    if (g_error) return;
     for (auto& dgp : diags) {
       for_sets(*dgp, [&](Diag& dg) {
         if (!g_error) advect(dg, interp, spl, cfl_rate, dt_f);
@@ -1062,18 +1077,94 @@ struct Engine : EngineBase {
 
   // one particle set through the CFD step: set_dt (lp_motion_m.f90:56-127),
   // then update_lp (forward) or update_flowmap_sl (backward)
-  void advect(Diag& dg, int interp, bool spl, double cfl_rate, U dt_f) {
+  // set_dt (lp_motion_m.f90:90-101): the sub-cycle count of a set through
+  // the CFD step (0 when above 1e6) and its signed sub-cycle step
+  int sub_cycle_count(const Diag& dg, double cfl_rate, U dt_f, U& dt) const {
     const U cmax = (U)((T)cfl_rate) * dt_f;
     const double r = std::ceil((double)(cmax / (g_opt.cfl_max * dg.dt_factor)));
-    int nsub = 1;
-    if (r > 1e6) {
+    if (r > 1e6) return 0;
+    const int nsub = r >= 1 ? (int)r : 1;
+    dt = dt_f / (U)nsub;
+    if (dg.dir < 0) dt = (U)-1 * dt;
+    return nsub;
+  }
+
+  // Every stage blend the update's sub-cycles will ask for, formed in
+  // batches that read the two fields once per kBlendBatch blends, before the
+  // sweeps: the weights come from stage_cts() along the same sub-cycle times
+  // the sweeps step through, so the sweeps find them in the pool by the
+  // pool's own match.  A weight this misses is blended on demand, as is
+  // every weight of the packed 16-bit records.
+  void plan_blends(bool spl, double cfl_rate, U dt_f) {
+    nblend = 6;
+    const int stor = g_opt.vel_storage;
+    if (arena || (stor != kStoreNative && stor != kStoreF32)) return;
+    const bool in32 = spl && vcoef32[vold];
+    const bool snap01 = stor == kStoreNative || (stor == kStoreF32 && in32);
+    const double tol = stor == kStoreF32 ? 0x1p-34 : 0.0;
+    std::vector<U> cts;
+    for (auto& dgp : diags) {
+      U dt = 0;
+      const int nsub = sub_cycle_count(*dgp, cfl_rate, dt_f, dt);
+      U t = dgp->dir > 0 ? t_n : t_np1;
+      for (int s = 0; s < nsub; s++) {
+        U c[4];
+        const int ns = stage_cts(t, dt, c);
+        for (int a = 0; a < 4; a++) {
+          const U ct = c[a < ns ? a : 0];
+          if (snap01 && (ct == (U)0 || ct == (U)1)) continue;
+          bool have = false;
+          for (U x : cts) have = have || std::fabs((double)x - (double)ct) <= tol;
+          if (!have) cts.push_back(ct);
+        }
+        t = t + dt;
+      }
+    }
+    // at most kMaxBlends - 2 planned entries (two stay for on-demand
+    // blends) and at most 16 GiB of them
+    const size_t eb = 3 * fbytes();
+    const size_t fit = std::min<size_t>(kMaxBlends - 2, ((size_t)16 << 30) / std::max<size_t>(eb, 1));
+    if (cts.size() < 2 || cts.size() > fit) return;
+    nblend = std::max(6, (int)cts.size() + 2);
+    const size_t sb = stor == kStoreNative ? sizeof(T) : sizeof(float);
+    DevBuf& A = spl ? vcoef[vold] : vel[vold];
+    DevBuf& B = spl ? vcoef[cur] : vel[cur];
+    for (size_t q0 = 0; q0 < cts.size(); q0 += kBlendBatch) {
+      BlendBatch<T> bb{};
+      bb.n = (int)std::min<size_t>(kBlendBatch, cts.size() - q0);
+      for (int q = 0; q < bb.n; q++) {
+        Blend& e = blends[q0 + q];
+        if (!(e.buf.p && e.buf.bytes >= eb) && !get_buf(e.buf, eb)) {
+          fail("hipMalloc failed (blended velocity)");
+          return;
+        }
+        char* p = (char*)e.buf.p;
+        for (int d = 0; d < 3; d++) bb.out[q][d] = p + d * sb * (size_t)g.nt;
+        bb.ct[q] = (T)cts[q0 + q];
+        e.ct = cts[q0 + q];
+        e.storage = stor;
+        e.spl = spl;
+        e.valid = true;
+        e.stamp = ++blend_clock;
+      }
+      if constexpr (sizeof(T) == 8) {
+        if (in32) {
+          ktimed(14, [&] { launch_blend_batch(g, bb, stor, cf3e<float>(A, g.nt), cf3e<float>(B, g.nt), stream); });
+          continue;
+        }
+      }
+      ktimed(14, [&] { launch_blend_batch(g, bb, stor, cf3(A), cf3(B), stream); });
+    }
+  }
+
+  void advect(Diag& dg, int interp, bool spl, double cfl_rate, U dt_f) {
+    U dt = 0;
+    const int nsub = sub_cycle_count(dg, cfl_rate, dt_f, dt);
+    if (nsub == 0) {
       fail("CFL sub-cycle count above 1e6 (velocity or time step out of range)");
       return;
     }
-    if (r >= 1) nsub = (int)r;
     nsub_last = nsub;
-    U dt = dt_f / (U)nsub;
-    if (dg.dir < 0) dt = (U)-1 * dt;
     const long long np = NP(dg), nt = NT(dg);
     if (std::fabs(dt) > 0) {
       if (dg.dir > 0) integ_fwd += np * nsub;
@@ -1635,6 +1726,7 @@ struct Engine : EngineBase {
     int cur = 1, vold = 0;
     U t_n = 0, t_np1 = 0;
     Blend blends[6];
+    int nblend = 6;
     // state
     DevBuf fm[2], fmcoef, compfm, dxout, ftle, lam, adjfm, adjv[3], dcn, dcnp1, dun, dunp1, ovf[4];
     DevBuf tiles, oflag;         // tile-store scratch of the reverse sub-cycle scatter
@@ -1658,6 +1750,7 @@ struct Engine : EngineBase {
     std::swap(vold, tape.vold);
     std::swap(t_n, tape.t_n);
     std::swap(t_np1, tape.t_np1);
+    std::swap(nblend, tape.nblend);
     for (int b = 0; b < 6; b++) {
       blends[b].buf.swap(tape.blends[b].buf);
       std::swap(blends[b].ct, tape.blends[b].ct);
