@@ -1093,15 +1093,15 @@ struct Engine : EngineBase {
   // batches that read the two fields once per kBlendBatch blends, before the
   // sweeps: the weights come from stage_cts() along the same sub-cycle times
   // the sweeps step through, so the sweeps find them in the pool by the
-  // pool's own match.  A weight this misses is blended on demand, as is
-  // every weight of the packed 16-bit records.
+  // pool's own match (stage_field).  A weight this misses is blended on
+  // demand.
   void plan_blends(bool spl, double cfl_rate, U dt_f) {
     nblend = 6;
     const int stor = g_opt.vel_storage;
-    if (arena || (stor != kStoreNative && stor != kStoreF32)) return;
+    if (arena) return;
     const bool in32 = spl && vcoef32[vold];
     const bool snap01 = stor == kStoreNative || (stor == kStoreF32 && in32);
-    const double tol = stor == kStoreF32 ? 0x1p-34 : 0.0;
+    const double tol = stor == kStoreF32 ? 0x1p-34 : (stor == kStoreF16 ? 0x1p-21 : (stor == kStoreBF16 ? 0x1p-18 : 0.0));
     std::vector<U> cts;
     for (auto& dgp : diags) {
       U dt = 0;
@@ -1126,11 +1126,13 @@ struct Engine : EngineBase {
     const size_t fit = std::min<size_t>(kMaxBlends - 2, ((size_t)16 << 30) / std::max<size_t>(eb, 1));
     if (cts.size() < 2 || cts.size() > fit) return;
     nblend = std::max(6, (int)cts.size() + 2);
-    const size_t sb = stor == kStoreNative ? sizeof(T) : sizeof(float);
+    // per-component stride of the stage field (0: one array of packed records)
+    const size_t sb = stor == kStoreNative ? sizeof(T) : (stor == kStoreF32 ? sizeof(float) : 0);
     DevBuf& A = spl ? vcoef[vold] : vel[vold];
     DevBuf& B = spl ? vcoef[cur] : vel[cur];
     for (size_t q0 = 0; q0 < cts.size(); q0 += kBlendBatch) {
       BlendBatch<T> bb{};
+      bb.inv = (T)(1.0 / vscale);
       bb.n = (int)std::min<size_t>(kBlendBatch, cts.size() - q0);
       for (int q = 0; q < bb.n; q++) {
         Blend& e = blends[q0 + q];
@@ -1149,11 +1151,11 @@ struct Engine : EngineBase {
       }
       if constexpr (sizeof(T) == 8) {
         if (in32) {
-          ktimed(14, [&] { launch_blend_batch(g, bb, stor, cf3e<float>(A, g.nt), cf3e<float>(B, g.nt), stream); });
+          ktimed(14, [&] { launch_blend_batch(g, bb, stor, cf3e<float>(A, g.nt), cf3e<float>(B, g.nt), st, stream); });
           continue;
         }
       }
-      ktimed(14, [&] { launch_blend_batch(g, bb, stor, cf3(A), cf3(B), stream); });
+      ktimed(14, [&] { launch_blend_batch(g, bb, stor, cf3(A), cf3(B), st, stream); });
     }
   }
 

@@ -1581,8 +1581,10 @@ void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<A> a
 // output is k_blend's expression in k_blend's order, so its values are the
 // same; the fields are read once per batch instead of once per blend.
 template <typename T, typename S, typename A>
-__global__ void __launch_bounds__(kBlock) k_blend_batch(BlendBatch<T> bb, CF3<A> a, CF3<A> b, long long nt) {
+__global__ void __launch_bounds__(kBlock) k_blend_batch(BlendBatch<T> bb, CF3<A> a, CF3<A> b, long long nt,
+                                                        DevStatus* st) {
 #pragma clang fp contract(off)
+  bool range = false;
   for (long long l = (long long)blockIdx.x * blockDim.x + threadIdx.x; l < nt; l += (long long)gridDim.x * blockDim.x) {
     T va[3], vb[3];
 #pragma unroll
@@ -1594,24 +1596,42 @@ __global__ void __launch_bounds__(kBlock) k_blend_batch(BlendBatch<T> bb, CF3<A>
     for (int q = 0; q < kBlendBatch; q++) {
       if (q < bb.n) {
         const T ct = bb.ct[q], w0 = T(1) - ct;
+        if constexpr (Packed<S>::value) {
+          using E = typename Packed<S>::E;
+          S r;
 #pragma unroll
-        for (int c = 0; c < 3; c++) ((S*)bb.out[q][c])[l] = narrow<S>(w0 * va[c] + ct * vb[c]);
+          for (int c = 0; c < 3; c++) {
+            const T x = (w0 * va[c] + ct * vb[c]) * bb.inv;
+            if constexpr (std::is_same<E, half_t>::value) range |= fabs(x) > T(65504);
+            r.e[c] = narrow<E>(x);
+          }
+          r.e[3] = E{0};
+          ((S*)bb.out[q][0])[l] = r;
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; c++) ((S*)bb.out[q][c])[l] = narrow<S>(w0 * va[c] + ct * vb[c]);
+        }
       }
     }
   }
+  if (range) atomicOr(&st->err, (int)kErrRange);
 }
 
 template <typename T, typename A>
 void launch_blend_batch(const GridDev<T>& g, const BlendBatch<T>& bb, int storage, CF3<A> a, CF3<A> b,
-                        hipStream_t s) {
+                        DevStatus* st, hipStream_t s) {
   const long long nb = std::min<long long>((g.nt + kBlock - 1) / kBlock, 8192);
-  if constexpr (std::is_same<T, double>::value) {
-    if (storage == kStoreF32) {
-      hipLaunchKernelGGL((k_blend_batch<T, float, A>), dim3((unsigned)nb), dim3(kBlock), 0, s, bb, a, b, g.nt);
-      return;
-    }
+  auto go = [&](auto tag) {
+    using S = decltype(tag);
+    hipLaunchKernelGGL((k_blend_batch<T, S, A>), dim3((unsigned)nb), dim3(kBlock), 0, s, bb, a, b, g.nt, st);
+  };
+  if constexpr (std::is_same<T, float>::value) {
+    if (storage == kStoreF16) return go(half4_t{});
+    if (storage == kStoreBF16) return go(bf164_t{});
+  } else {
+    if (storage == kStoreF32) return go(float{});
   }
-  hipLaunchKernelGGL((k_blend_batch<T, T, A>), dim3((unsigned)nb), dim3(kBlock), 0, s, bb, a, b, g.nt);
+  go(T{});
 }
 
 // storage dispatch: the element type of the stage velocity fields
@@ -2152,7 +2172,8 @@ void launch_widen(const float* src, T* dst, long long n, hipStream_t s) {
   template void launch_spline_fit<T, T, T>(const GridDev<T>&, F3<T>, int, hipStream_t, const T* const*);      \
   template void launch_spline_fit_T<T>(const GridDev<T>&, F3<T>, hipStream_t);                                \
   template void launch_blend<T, T>(const GridDev<T>&, void* const*, int, CF3<T>, CF3<T>, T, T, DevStatus*, hipStream_t);  \
-  template void launch_blend_batch<T, T>(const GridDev<T>&, const BlendBatch<T>&, int, CF3<T>, CF3<T>, hipStream_t); \
+  template void launch_blend_batch<T, T>(const GridDev<T>&, const BlendBatch<T>&, int, CF3<T>, CF3<T>, DevStatus*, \
+                                         hipStream_t);                                                         \
   template void launch_sl_step<T, T>(const GridDev<T>&, int, const StageParams<T>&, CF3<T>, F3<T>, DevStatus*, \
                                   hipStream_t, int);                                                          \
   template void launch_fwd_step<T>(const GridDev<T>&, int, const StageParams<T>&, ParticleSet<T>, int, int, \
@@ -2195,7 +2216,7 @@ template void launch_spline_fit<double, float, double>(const GridDev<double>&, F
 template void launch_blend<double, float>(const GridDev<double>&, void* const*, int, CF3<float>, CF3<float>, double,
                                           double, DevStatus*, hipStream_t);
 template void launch_blend_batch<double, float>(const GridDev<double>&, const BlendBatch<double>&, int, CF3<float>,
-                                                CF3<float>, hipStream_t);
+                                                CF3<float>, DevStatus*, hipStream_t);
 template void launch_sl_step<double, float>(const GridDev<double>&, int, const StageParams<double>&, CF3<float>,
                                             F3<float>, DevStatus*, hipStream_t, int);
 template void launch_sl_step2<double, float>(const GridDev<double>&, const GridDev<double>&, const int16_t*, int, int,

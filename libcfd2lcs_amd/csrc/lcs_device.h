@@ -149,19 +149,19 @@ void launch_widen(const float* src, T* dst, long long n, hipStream_t stream);
 template <typename T, typename A>
 void launch_blend(const GridDev<T>& g, void* const out[3], int storage, CF3<A> a, CF3<A> b, T ct, T inv,
                   DevStatus* st, hipStream_t stream);
-// up to kBlendBatch blends of the same two fields in one pass (component
-// arrays: native storage, or FP32 under FP64 compute), each as launch_blend
-// forms it
+// up to kBlendBatch blends of the same two fields in one pass, each as
+// launch_blend forms it (packed 16-bit records: out[q][0])
 constexpr int kBlendBatch = 8;
 template <typename T>
 struct BlendBatch {
   void* out[kBlendBatch][3];
   T ct[kBlendBatch];
+  T inv;
   int n;
 };
 template <typename T, typename A>
 void launch_blend_batch(const GridDev<T>& g, const BlendBatch<T>& bb, int storage, CF3<A> a, CF3<A> b,
-                        hipStream_t stream);
+                        DevStatus* st, hipStream_t stream);
 // tiled: FP32 B-spline steps on 3-D grids stage the gathered coefficients
 // in LDS (bit-identical results)
 template <typename T, typename FM>
